@@ -1,0 +1,255 @@
+/*
+ * rayna_gpu.h — C ABI of the MI355X (gfx950) renderer for rayna_engine's render hot path.
+ *
+ * The ABI replaces `rayna_engine::render::renderer::Renderer` (rayna_engine/src/render/renderer.rs:36-264)
+ * for the scene graph that the engine's `StandardScene` presets build: a flat list of objects
+ * (`Vec<SimpleObject|VolumetricObject>.into()` -> `ObjectList`, rayna_engine/src/object/list.rs:43-129),
+ * each a mesh plus a material, with a `SkyboxInstance`.  Scenes cross the ABI as plain arrays: there are
+ * no closures, no `Dynamic*` plug-ins and no non-identity transforms (the renderer returns
+ * RAYNA_UNSUPPORTED for those).
+ *
+ * All geometry is given in double precision, as the engine's `Number = f64`
+ * (rayna_engine/src/core/types.rs:7); colours are single precision (`Channel = f32`, types.rs:3).
+ *
+ * Threading mirrors `Renderer::render(&mut self)`: one handle is not thread-safe; separate handles are.
+ * The caller owns every host buffer it passes in.
+ */
+#ifndef RAYNA_GPU_H
+#define RAYNA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+
+typedef enum rayna_status {
+    RAYNA_OK = 0,
+    RAYNA_INVALID_ARGUMENT = 1, /* a bad index, size or null pointer in the request */
+    RAYNA_UNSUPPORTED = 2,      /* valid for the engine but not for this renderer (e.g. ray_branching > 1) */
+    RAYNA_DEVICE_ERROR = 3,     /* a HIP call failed; rayna_last_error() names it */
+    RAYNA_OUT_OF_MEMORY = 4
+} rayna_status;
+
+/* ---------------------------------------------------------------- materials
+ * rayna_engine/src/material/mod.rs:168-175 (MaterialInstance) with SolidTexture albedo
+ * (rayna_engine/src/texture/solid.rs:29).
+ *   LAMBERTIAN  lambertian.rs:28-48   colour = albedo
+ *   METAL       metal.rs:65-95        colour = albedo, fuzz
+ *   DIELECTRIC  dielectric.rs:113-175 colour = albedo, refractive_index, density
+ *   ISOTROPIC   isotropic.rs:203-228  colour = albedo, density
+ *   LIGHT       light.rs:244-260      colour = emissive
+ */
+enum {
+    RAYNA_MAT_LAMBERTIAN = 0,
+    RAYNA_MAT_METAL = 1,
+    RAYNA_MAT_DIELECTRIC = 2,
+    RAYNA_MAT_ISOTROPIC = 3,
+    RAYNA_MAT_LIGHT = 4
+};
+
+typedef struct rayna_material {
+    int32_t kind;
+    float colour[3];
+    double fuzz;
+    double refractive_index;
+    double density;
+} rayna_material;
+
+/* ---------------------------------------------------------------- meshes
+ * rayna_engine/src/mesh/mod.rs:68-85 (MeshInstance).
+ *   SPHERE         SphereMesh::new(centre, radius)              primitive/sphere.rs:122-133
+ *   TRIANGLE       one Triangle::new(vertices, normals)         primitive/triangle.rs:240-256
+ *   TRIANGLE_LIST  MeshList of Triangles (its own SAH BVH)      advanced/list.rs:161-192
+ *   PARALLELOGRAM  ParallelogramMesh::new(Planar::new(p,u,v))   planar/parallelogram.rs:444-458
+ *   AXIS_BOX       AxisBoxMesh::new(a, b)                       primitive/axis_box.rs:28-37
+ *   SDF            RaymarchedIsosurfaceMesh::new_custom         isosurface/raymarched.rs:51-57
+ * `index` selects the element of the kind's array; TRIANGLE_LIST uses triangles [index, index+count).
+ */
+enum {
+    RAYNA_MESH_SPHERE = 0,
+    RAYNA_MESH_TRIANGLE = 1,
+    RAYNA_MESH_TRIANGLE_LIST = 2,
+    RAYNA_MESH_PARALLELOGRAM = 3,
+    RAYNA_MESH_AXIS_BOX = 4,
+    RAYNA_MESH_SDF = 5
+};
+
+typedef struct rayna_mesh {
+    int32_t kind;
+    int32_t index;
+    int32_t count;
+    int32_t reserved;
+} rayna_mesh;
+
+/* ---------------------------------------------------------------- signed distance functions
+ * The engine takes any `Fn(Point3) -> Number` (isosurface/mod.rs:7-9); closures cannot cross a C ABI,
+ * so the SDF is one of an enumerated set:
+ *   SPHERE       |p - c| - r of sphere `first_sphere`
+ *   SMIN_SPHERES polynomial smooth-min (parameter k) folded over spheres [first_sphere, +n_spheres)
+ *   MANDELBULB   power-`power` Mandelbulb distance estimate about `centre`, `bulb_iterations`
+ *                iterations, bail-out radius `bailout`
+ * `max_iterations` and `epsilon` are the march limits of RaymarchedIsosurfaceMesh.
+ */
+enum {
+    RAYNA_SDF_SPHERE = 0,
+    RAYNA_SDF_SMIN_SPHERES = 1,
+    RAYNA_SDF_MANDELBULB = 2
+};
+
+typedef struct rayna_sdf {
+    int32_t kind;
+    int32_t first_sphere;
+    int32_t n_spheres;
+    int32_t max_iterations;
+    int32_t bulb_iterations;
+    int32_t reserved;
+    double epsilon;
+    double k;
+    double power;
+    double bailout;
+    double centre[3];
+} rayna_sdf;
+
+/* ---------------------------------------------------------------- objects
+ * rayna_engine/src/object/mod.rs:47-52 (ObjectInstance), identity transform only.
+ *   SIMPLE      SimpleObject::new(mesh, material, None)                    object/simple.rs:175-200
+ *   VOLUMETRIC  VolumetricObject::new(mesh, material, density, None)       object/volumetric.rs:42-72
+ */
+enum {
+    RAYNA_OBJ_SIMPLE = 0,
+    RAYNA_OBJ_VOLUMETRIC = 1
+};
+
+typedef struct rayna_object {
+    int32_t kind;
+    int32_t mesh;
+    int32_t material;
+    int32_t reserved;
+    double density;
+} rayna_object;
+
+/* ---------------------------------------------------------------- skybox (skybox/mod.rs:196-204) */
+enum {
+    RAYNA_SKY_NONE = 0,   /* NoSkybox     skybox/none.rs:8-13 */
+    RAYNA_SKY_SIMPLE = 1, /* SimpleSkybox skybox/simple.rs:12-22 */
+    RAYNA_SKY_WHITE = 2   /* WhiteSkybox  skybox/simple.rs:26-30 */
+};
+
+/* The scene: the object list in insertion order (the order `ObjectList::new_uncorrected` sees), and
+ * flat parameter arrays.  spheres [n][4] = centre xyz, radius; tri_vertices and tri_normals [n][3][3];
+ * parallelograms [n][9] = p, u, v; boxes [n][6] = corner a, corner b. */
+typedef struct rayna_scene_desc {
+    const rayna_object *objects;
+    int32_t n_objects;
+    const rayna_mesh *meshes;
+    int32_t n_meshes;
+    const rayna_material *materials;
+    int32_t n_materials;
+    const double *spheres;
+    int32_t n_spheres;
+    const double *tri_vertices;
+    const double *tri_normals;
+    int32_t n_triangles;
+    const double *parallelograms;
+    int32_t n_parallelograms;
+    const double *boxes;
+    int32_t n_boxes;
+    const rayna_sdf *sdfs;
+    int32_t n_sdfs;
+    int32_t skybox;
+} rayna_scene_desc;
+
+/* ---------------------------------------------------------------- camera (scene/camera.rs:10-38); angles in radians */
+typedef struct rayna_camera {
+    double pos[3];
+    double fwd[3];
+    double v_fov;
+    double focus_dist;
+    double defocus_angle;
+} rayna_camera;
+
+/* ---------------------------------------------------------------- render options (render/render_opts.rs:8-29)
+ * `seed` keys the counter-based RNG that replaces the engine's entropy-seeded generators
+ * (render/renderer.rs:207); every (pixel, sample, bounce, draw slot) has its own stream.  Only
+ * RAYNA_MODE_PBR is rendered; ray_branching must be 1. */
+enum { RAYNA_MODE_PBR = 0 };
+
+typedef struct rayna_render_opts {
+    uint32_t width;
+    uint32_t height;
+    uint32_t samples;
+    uint32_t ray_depth;
+    uint32_t ray_branching;
+    int32_t mode;
+    uint64_t seed;
+} rayna_render_opts;
+
+/* ---------------------------------------------------------------- stats (render/render.rs:4-15, extended) */
+typedef struct rayna_render_stats {
+    double duration_s;       /* host wall time of the render call */
+    double kernel_ms;        /* sum of device time of all kernels of the frame (HIP events) */
+    double extend_ms;        /* device time of the extend (closest-hit) kernels */
+    double shade_ms;         /* device time of the shade kernels */
+    uint64_t accum_frames;   /* frames in the accumulation buffer, as AccumulationBuffer::frame_count */
+    uint64_t samples;        /* camera samples traced this frame (width * height * samples) */
+    uint64_t segments;       /* closest-hit queries this frame (one per path vertex) */
+    uint64_t extend_launches;
+} rayna_render_stats;
+
+/* Output layouts of rayna_render. */
+enum {
+    RAYNA_LAYOUT_XMAJOR = 0,  /* [x][y][rgb], the engine's Image shape (w, h) (core/image.rs:24-26) */
+    RAYNA_LAYOUT_ROWMAJOR = 1 /* [y][x][rgb] */
+};
+
+typedef struct rayna_renderer rayna_renderer;
+
+/* Renderer::new_from (render/renderer.rs:86-107).  Builds the SAH BVHs on the host with the engine's
+ * algorithm (shared/generic_bvh.rs:45-290), uploads the scene and allocates the path queues. */
+rayna_status rayna_create(const rayna_scene_desc *scene, const rayna_camera *camera,
+                          const rayna_render_opts *opts, rayna_renderer **out);
+
+/* Renderer::set_scene / set_camera / set_options (render/renderer.rs:158-176): each clears accumulation. */
+rayna_status rayna_set_scene(rayna_renderer *r, const rayna_scene_desc *scene);
+rayna_status rayna_set_camera(rayna_renderer *r, const rayna_camera *camera);
+rayna_status rayna_set_options(rayna_renderer *r, const rayna_render_opts *opts);
+
+/* Renderer::clear_accumulation (render/renderer.rs:153). */
+rayna_status rayna_clear_accumulation(rayna_renderer *r);
+
+/* Renderer::render (render/renderer.rs:224-264): one frame of `samples` samples per pixel, accumulated
+ * into the running per-pixel mean (render/accum_buffer.rs:31-45).  out_rgb receives width*height*3
+ * floats in `layout`, or may be NULL to leave the image on the device (benchmarks).  An invalid camera
+ * yields the engine's magenta/black checkerboard of ilog2(w) x ilog2(h) pixels (renderer.rs:266-292):
+ * then `out_dims` (if not NULL) receives that size and out_rgb must hold width*height*3 floats. */
+rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uint32_t out_dims[2],
+                          rayna_render_stats *stats);
+
+/* Parity hook (no engine counterpart; the engine's RenderMode debug views are the closest,
+ * renderer.rs:428-490): closest hit of the camera ray of sample `sample_index` of every pixel, row-major.
+ * prim_rank receives the primitive's position in the depth-first order of the engine's BVH
+ * (-1 on a miss, -2 - i for the i-th unbounded object), t the hit distance. */
+rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_t *prim_rank, float *t);
+
+/* Position of each bounded primitive in depth-first order: object index and index within the object's
+ * mesh (triangle index within a TRIANGLE_LIST, else 0).  Returns the count via n_out. */
+rayna_status rayna_dfs_order(rayna_renderer *r, int32_t *object, int32_t *local, int32_t capacity,
+                             int32_t *n_out);
+
+/* FNV-1a hash of the engine BVH topology the renderer built (TLAS and every BLAS). */
+rayna_status rayna_tree_hash(rayna_renderer *r, uint64_t *hash);
+
+/* Renderer drop. */
+void rayna_destroy(rayna_renderer *r);
+
+/* Text of the last error on this handle (or of the last failed rayna_create when r is NULL). */
+const char *rayna_last_error(const rayna_renderer *r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAYNA_GPU_H */

@@ -1,0 +1,1303 @@
+// oracle/twin.hpp — CPU restatement ("twin") of rayna_engine's render hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This is the oracle: the checker that tests/, __graft_entry__.smoke() and
+// bench.py's CPU-baseline leg run next to the GPU renderer.  The product (rayna_amd/) never includes,
+// links or calls it.
+//
+// The twin follows the engine's code path function by function (file:line citations below) and is
+// templated on the geometry precision R:
+//   R = double : the engine's own precision (`Number = f64`, core/types.rs:7) and its recursive integrator
+//                (render/renderer.rs:508-572) -- the reference semantics, the CPU baseline, the RMSE and
+//                finite-difference references;
+//   R = float  : the same formulas in single precision with the GPU's evaluation order (forward
+//                throughput accumulation), so that the GPU path can be compared bit for bit.
+// Colours are single precision in both (`Channel = f32`, core/types.rs:3).
+//
+// Documented deviations from the engine, shared with the GPU path:
+//  (1) counter-based RNG and closed-form disk/ball samplers (twin_rng.hpp);
+//  (2) the SAH builder sorts by (aabb.min[axis], input index) instead of an unstable sort
+//      (shared/generic_bvh.rs:67-86), which makes the tree a pure function of the input;
+//  (3) SDFs are an enumerated set instead of closures (mesh/isosurface/mod.rs:7-9);
+//  (4) "tight" box mode: the BVH topology is the engine's, but boxes are refitted without the origin
+//      that Aabb::encompass_iter folds in (shared/aabb.rs:117-120) and tested conservatively (relative
+//      slack 2^-16).  Culling then never removes the closest primitive, and the closest hit with ties
+//      going to the later primitive in depth-first order -- what the engine's shrinking inclusive
+//      interval computes (mesh/advanced/bvh.rs:77-91) -- no longer depends on the traversal order;
+//  (5) a volume's free-flight candidate is kept when it lies inside the volume, without clamping the
+//      exit to the current (order-dependent) shrunk interval end (object/volumetric.rs:112-123); the
+//      closest-hit selection then discards it exactly when the engine would, up to rounding at a tie.
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+#include "../include/rayna_gpu.h"
+#include "twin_math.hpp"
+#include "twin_rng.hpp"
+
+namespace twin {
+
+// ================================================================================================
+// Vectors, with glam's operation order (dot = (x*x' + y*y') + z*z', normalize = v * (1/length)).
+// ================================================================================================
+
+template <class R>
+struct V3 {
+    R x, y, z;
+    V3 operator+(const V3 &b) const { return {x + b.x, y + b.y, z + b.z}; }
+    V3 operator-(const V3 &b) const { return {x - b.x, y - b.y, z - b.z}; }
+    V3 operator*(R s) const { return {x * s, y * s, z * s}; }
+    V3 operator/(R s) const { return {x / s, y / s, z / s}; }
+    V3 operator-() const { return {-x, -y, -z}; }
+    V3 mul(const V3 &b) const { return {x * b.x, y * b.y, z * b.z}; }
+    R operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+template <class R>
+inline R dot(const V3<R> &a, const V3<R> &b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class R>
+inline V3<R> cross(const V3<R> &a, const V3<R> &b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R>
+inline R length(const V3<R> &a) { return Math<R>::sqrt(dot(a, a)); }
+template <class R>
+inline V3<R> normalize(const V3<R> &a) { return a * (R(1) / length(a)); }
+// glam try_normalize: Some(v * rcp) when rcp = 1/length is finite and positive.
+template <class R>
+inline bool try_normalize(const V3<R> &a, V3<R> *out) {
+    const R rcp = R(1) / length(a);
+    if (std::isfinite(rcp) && rcp > R(0)) {
+        *out = a * rcp;
+        return true;
+    }
+    return false;
+}
+template <class R>
+inline V3<R> recip(const V3<R> &a) { return {R(1) / a.x, R(1) / a.y, R(1) / a.z}; }
+template <class R, class S>
+inline V3<R> cast3(const S *p) { return {(R)p[0], (R)p[1], (R)p[2]}; }
+
+// shared/math.rs:26 and :28-33
+template <class R>
+inline V3<R> reflect(const V3<R> &d, const V3<R> &n) { return d - n * (R(2) * dot(d, n)); }
+template <class R>
+inline V3<R> refract(const V3<R> &v, const V3<R> &n, R ratio) {
+    const R cos_theta = Math<R>::fmin(dot(-v, n), R(1));
+    const V3<R> perp = (v + n * cos_theta) * ratio;
+    const V3<R> par = n * -Math<R>::sqrt(std::fabs(R(1) - dot(perp, perp)));
+    return perp + par;
+}
+
+// shared/ray.rs:15-23: direction normalised, inverse cached.
+template <class R>
+struct Ray {
+    V3<R> pos, dir, inv_dir;
+    static Ray make(const V3<R> &p, const V3<R> &d) {
+        Ray r;
+        r.pos = p;
+        r.dir = normalize(d);
+        r.inv_dir = recip(r.dir);
+        return r;
+    }
+    V3<R> at(R t) const { return pos + dir * t; }
+};
+
+// shared/interval.rs: optional bounds, inclusive.
+template <class R>
+struct Interval {
+    bool has_start, has_end;
+    R start, end;
+    bool contains(R v) const { return (!has_start || start <= v) && (!has_end || v <= end); }
+};
+
+// Colour (core/colour.rs:183-199): component-wise f32.
+struct Col {
+    float r, g, b;
+    Col operator+(const Col &o) const { return {r + o.r, g + o.g, b + o.b}; }
+    Col operator*(const Col &o) const { return {r * o.r, g * o.g, b * o.b}; }
+    Col operator*(float s) const { return {r * s, g * s, b * s}; }
+    Col operator/(float s) const { return {r / s, g / s, b / s}; }
+};
+
+// ================================================================================================
+// AABB in double precision, as the engine builds it (shared/aabb.rs:83-143).
+// ================================================================================================
+
+struct Aabb {
+    double mn[3], mx[3];
+    double area() const {
+        const double sx = mx[0] - mn[0], sy = mx[1] - mn[1], sz = mx[2] - mn[2];
+        return ((sx * sy) + (sy * sz) + (sz * sx)) * 2.0;
+    }
+};
+
+inline Aabb aabb_new(const double a[3], const double b[3]) {
+    Aabb r;
+    for (int i = 0; i < 3; ++i) {
+        r.mn[i] = std::fmin(a[i], b[i]);
+        r.mx[i] = std::fmax(a[i], b[i]);
+    }
+    return r;
+}
+// Aabb::default(): both corners at the origin.
+inline Aabb aabb_origin() { return Aabb{{0, 0, 0}, {0, 0, 0}}; }
+// Aabb::encompass (aabb.rs:109-114)
+inline Aabb encompass(const Aabb &a, const Aabb &b) {
+    double mn[3], mx[3];
+    for (int i = 0; i < 3; ++i) {
+        mn[i] = std::fmin(a.mn[i], b.mn[i]);
+        mx[i] = std::fmax(a.mx[i], b.mx[i]);
+    }
+    return aabb_new(mn, mx);
+}
+// Aabb::encompass_points (aabb.rs:123-132)
+inline Aabb encompass_points(const double (*pts)[3], int n) {
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < n; ++k)
+        for (int i = 0; i < 3; ++i) {
+            mn[i] = std::fmin(mn[i], pts[k][i]);
+            mx[i] = std::fmax(mx[i], pts[k][i]);
+        }
+    return aabb_new(mn, mx);
+}
+// Aabb::min_padded (aabb.rs:137-142)
+inline Aabb min_padded(const Aabb &b, double thresh) {
+    double dims[3], centre[3], mn[3], mx[3];
+    for (int i = 0; i < 3; ++i) {
+        dims[i] = b.mx[i] - b.mn[i];
+        centre[i] = b.mn[i] + dims[i] / 2.0;
+        dims[i] = std::fmax(dims[i], thresh);
+    }
+    for (int i = 0; i < 3; ++i) {
+        mn[i] = centre[i] - dims[i] / 2.0;
+        mx[i] = centre[i] + dims[i] / 2.0;
+    }
+    return aabb_new(mn, mx);
+}
+
+// ================================================================================================
+// GenericBvh (shared/generic_bvh.rs:45-290): the engine's SAH builder.
+// ================================================================================================
+
+struct BvhNode {
+    int item;                   // >= 0: an object leaf (GenericBvhNode::Object); -1: GenericBvhNode::Nested
+    Aabb box;                   // Nested: encompass_iter of the items below (origin included)
+    std::vector<int> children;  // in insertion order
+};
+
+struct Bvh {
+    std::vector<BvhNode> nodes;
+    int root = -1;
+};
+
+class BvhBuilder {
+   public:
+    // literal = true: every candidate split re-unions its slices, exactly as calculate_optimal_split
+    // (generic_bvh.rs:203-274), O(n^2) per node.  literal = false: prefix/suffix sweeps of the same unions
+    // (min/max are exact, so every cost is bit-identical), O(n log n) per node.
+    BvhBuilder(const std::vector<Aabb> &boxes, bool literal) : boxes_(boxes), literal_(literal) {}
+
+    Bvh build() {
+        Bvh bvh;
+        const int n = (int)boxes_.size();
+        if (n == 0) return bvh;  // generic_bvh.rs:54-56: root None
+        std::vector<int> ids(n);
+        for (int i = 0; i < n; ++i) ids[i] = i;
+        bvh_ = &bvh;
+        bvh.root = generate(ids);
+        return bvh;
+    }
+
+   private:
+    const std::vector<Aabb> &boxes_;
+    bool literal_;
+    Bvh *bvh_ = nullptr;
+
+    int new_node(int item, const Aabb &box) {
+        bvh_->nodes.push_back(BvhNode{item, box, {}});
+        return (int)bvh_->nodes.size() - 1;
+    }
+
+    Aabb union_of(const int *ids, int n) const {  // Aabb::encompass_iter: fold from Aabb::default()
+        Aabb a = aabb_origin();
+        for (int i = 0; i < n; ++i) a = encompass(a, boxes_[ids[i]]);
+        return a;
+    }
+
+    // generic_bvh.rs:67-86, with the input index as the tie-break (deviation 2).
+    void sort_axis(std::vector<int> &ids, int axis) const {
+        std::sort(ids.begin(), ids.end(), [&](int a, int b) {
+            const double ka = boxes_[a].mn[axis], kb = boxes_[b].mn[axis];
+            if (ka < kb) return true;
+            if (kb < ka) return false;
+            return a < b;
+        });
+    }
+
+    struct Split {
+        bool ok;
+        int axis;
+        int left;
+        double cost;
+    };
+
+    // calculate_optimal_split::<1, 2> (generic_bvh.rs:173-290)
+    Split optimal_split(std::vector<int> &ids) const {
+        const int n = (int)ids.size();
+        Split best{false, 0, 0, 0.0};
+        if (n <= 2) return best;
+        std::vector<Aabb> pre, suf;
+        for (int axis = 0; axis < 3; ++axis) {
+            sort_axis(ids, axis);
+            if (!literal_) {
+                pre.assign(n + 1, aabb_origin());
+                suf.assign(n + 1, aabb_origin());
+                for (int i = 0; i < n; ++i) pre[i + 1] = encompass(pre[i], boxes_[ids[i]]);
+                for (int i = n - 1; i >= 0; --i) suf[i] = encompass(suf[i + 1], boxes_[ids[i]]);
+            }
+            for (int p = 1; p <= n - 2; ++p) {  // combinator_values = 1..len-1
+                double a_left, a_right;
+                if (literal_) {
+                    a_left = union_of(ids.data(), p).area();
+                    a_right = union_of(ids.data() + p, n - p).area();
+                } else {
+                    a_left = pre[p].area();
+                    a_right = suf[p].area();
+                }
+                const double cost = (double)p * a_left + (double)(n - p) * a_right;
+                if (!best.ok || cost < best.cost) best = Split{true, axis, p, cost};  // min_by keeps the first
+            }
+        }
+        return best;
+    }
+
+    // split_objects (generic_bvh.rs:156-168)
+    void split(std::vector<int> &ids, const Split &s, std::vector<int> &a, std::vector<int> &b) const {
+        sort_axis(ids, s.axis);
+        a.assign(ids.begin(), ids.begin() + s.left);
+        b.assign(ids.begin() + s.left, ids.end());
+    }
+
+    // generate_nodes_sah (generic_bvh.rs:97-154)
+    int generate(std::vector<int> &ids) {
+        const int n = (int)ids.size();
+        if (n == 1) return new_node(ids[0], boxes_[ids[0]]);
+        if (n <= 8) {
+            const int node = new_node(-1, union_of(ids.data(), n));
+            for (int i = 0; i < n; ++i) {
+                const int c = new_node(ids[i], boxes_[ids[i]]);
+                bvh_->nodes[node].children.push_back(c);
+            }
+            return node;
+        }
+        const int main = new_node(-1, union_of(ids.data(), n));
+        Split outer = optimal_split(ids);
+        std::vector<int> halves[2];
+        split(ids, outer, halves[0], halves[1]);
+        for (int h = 0; h < 2; ++h) {
+            Split inner = optimal_split(halves[h]);
+            if (inner.ok) {
+                std::vector<int> chunks[2];
+                split(halves[h], inner, chunks[0], chunks[1]);
+                for (int c = 0; c < 2; ++c) {
+                    const int child = generate(chunks[c]);
+                    bvh_->nodes[main].children.push_back(child);
+                }
+            } else {
+                const int child = generate(halves[h]);
+                bvh_->nodes[main].children.push_back(child);
+            }
+        }
+        return main;
+    }
+};
+
+// FNV-1a over the depth-first walk: (item or -1, child count) per node.
+inline void hash_bvh(const Bvh &b, int node, uint64_t &h) {
+    auto mix = [&](int64_t v) {
+        for (int i = 0; i < 8; ++i) {
+            h ^= (uint64_t)((v >> (8 * i)) & 0xff);
+            h *= 1099511628211ull;
+        }
+    };
+    const BvhNode &nd = b.nodes[node];
+    mix(nd.item);
+    mix((int64_t)nd.children.size());
+    for (int c : nd.children) hash_bvh(b, c, h);
+}
+
+// ================================================================================================
+// The scene: an ObjectList of the input objects (object/list.rs:297-316).
+// ================================================================================================
+
+enum BoxMode { BOX_REFERENCE = 0, BOX_TIGHT = 1 };
+
+// Relative slack of the conservative slab test (deviation 4).
+constexpr double BOX_SLACK = 0x1.0p-16;
+
+struct NodeBoxes {
+    double ref_d[6], tight_d[6];
+    float ref_f[6], tight_f[6];
+};
+
+inline float round_down_f(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+inline float round_up_f(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+inline void fill_boxes(NodeBoxes &nb, const Aabb &ref, const Aabb &tight) {
+    for (int i = 0; i < 3; ++i) {
+        nb.ref_d[i] = ref.mn[i];
+        nb.ref_d[3 + i] = ref.mx[i];
+        nb.tight_d[i] = tight.mn[i];
+        nb.tight_d[3 + i] = tight.mx[i];
+        nb.ref_f[i] = (float)ref.mn[i];
+        nb.ref_f[3 + i] = (float)ref.mx[i];
+        nb.tight_f[i] = round_down_f(tight.mn[i]);
+        nb.tight_f[3 + i] = round_up_f(tight.mx[i]);
+    }
+}
+
+struct Scene {
+    // input copy
+    std::vector<rayna_object> objects;
+    std::vector<rayna_mesh> meshes;
+    std::vector<rayna_material> materials;
+    std::vector<double> spheres, tri_v, tri_n, paras, boxes;
+    std::vector<rayna_sdf> sdfs;
+    int skybox = RAYNA_SKY_NONE;
+
+    // object graph
+    std::vector<int> bounded, unbounded;  // object indices, input order
+    std::vector<Aabb> obj_ref_box, obj_tight_box;
+    std::vector<char> obj_bounded;
+    Bvh tlas;                             // items index `bounded`
+    std::vector<NodeBoxes> tlas_boxes;    // per TLAS node
+    std::vector<Bvh> blas;                // one per TRIANGLE_LIST mesh
+    std::vector<std::vector<NodeBoxes>> blas_boxes;
+    std::vector<std::vector<int>> blas_dfs;  // per blas: item -> depth-first position
+    std::vector<int> mesh_blas;           // mesh -> blas index or -1
+    std::vector<int> obj_rank;            // object -> first depth-first rank (bounded objects), else -1
+    std::vector<int> rank_object, rank_local;  // depth-first rank -> (object, local)
+    uint64_t tree_hash = 0;
+
+    bool ok = false;
+    const char *error = "";
+};
+
+// ---- mesh AABBs (reference and tight) -------------------------------------------------------------
+
+inline bool mesh_aabb(const Scene &s, int mesh, Aabb *ref, Aabb *tight) {
+    const rayna_mesh &m = s.meshes[mesh];
+    switch (m.kind) {
+        case RAYNA_MESH_SPHERE: {  // sphere.rs:123-132
+            const double *sp = &s.spheres[4 * m.index];
+            double a[3], b[3];
+            for (int i = 0; i < 3; ++i) {
+                a[i] = sp[i] - sp[3];
+                b[i] = sp[i] + sp[3];
+            }
+            *ref = *tight = aabb_new(a, b);
+            return true;
+        }
+        case RAYNA_MESH_TRIANGLE: {  // triangle.rs:253
+            *ref = *tight = encompass_points((const double(*)[3]) & s.tri_v[9 * m.index], 3);
+            return true;
+        }
+        case RAYNA_MESH_TRIANGLE_LIST: {  // list.rs:176-181: encompass_iter over the triangles
+            if (m.count <= 0) return false;
+            Aabb r = aabb_origin();
+            Aabb t = encompass_points((const double(*)[3]) & s.tri_v[9 * m.index], 3);
+            for (int i = 0; i < m.count; ++i) {
+                const Aabb tb = encompass_points((const double(*)[3]) & s.tri_v[9 * (m.index + i)], 3);
+                r = encompass(r, tb);
+                t = encompass(t, tb);
+            }
+            *ref = r;
+            *tight = t;
+            return true;
+        }
+        case RAYNA_MESH_PARALLELOGRAM: {  // parallelogram.rs:445-457
+            const double *pp = &s.paras[9 * m.index];
+            double pts[4][3];
+            for (int i = 0; i < 3; ++i) {
+                pts[0][i] = pp[i];
+                pts[1][i] = pp[i] + pp[3 + i];
+                pts[2][i] = pp[i] + pp[6 + i];
+                pts[3][i] = pp[i] + pp[3 + i] + pp[6 + i];
+            }
+            *ref = *tight = min_padded(encompass_points(pts, 4), 1e-6);
+            return true;
+        }
+        case RAYNA_MESH_AXIS_BOX: {  // axis_box.rs:29-37
+            const double *bp = &s.boxes[6 * m.index];
+            *ref = *tight = aabb_new(bp, bp + 3);
+            return true;
+        }
+        default:  // raymarched.rs:65: unbounded
+            return false;
+    }
+}
+
+// Build the object graph: bounded/unbounded split, TLAS and BLASes, depth-first ranks, node boxes.
+inline void build_scene(Scene &s, const rayna_scene_desc &d, bool literal) {
+    s.objects.assign(d.objects, d.objects + d.n_objects);
+    s.meshes.assign(d.meshes, d.meshes + d.n_meshes);
+    s.materials.assign(d.materials, d.materials + d.n_materials);
+    s.spheres.assign(d.spheres, d.spheres + 4 * (size_t)d.n_spheres);
+    s.tri_v.assign(d.tri_vertices, d.tri_vertices + 9 * (size_t)d.n_triangles);
+    s.tri_n.assign(d.tri_normals, d.tri_normals + 9 * (size_t)d.n_triangles);
+    s.paras.assign(d.parallelograms, d.parallelograms + 9 * (size_t)d.n_parallelograms);
+    s.boxes.assign(d.boxes, d.boxes + 6 * (size_t)d.n_boxes);
+    s.sdfs.assign(d.sdfs, d.sdfs + d.n_sdfs);
+    s.skybox = d.skybox;
+
+    const int no = d.n_objects;
+    s.obj_ref_box.assign(no, aabb_origin());
+    s.obj_tight_box.assign(no, aabb_origin());
+    s.obj_bounded.assign(no, 0);
+    for (int o = 0; o < no; ++o) {
+        Aabb r, t;
+        if (mesh_aabb(s, s.objects[o].mesh, &r, &t)) {
+            s.obj_ref_box[o] = r;
+            s.obj_tight_box[o] = t;
+            s.obj_bounded[o] = 1;
+            s.bounded.push_back(o);
+        } else {
+            s.unbounded.push_back(o);
+        }
+    }
+
+    // BLAS per triangle-list mesh (MeshList -> BvhMesh -> GenericBvh over the triangles, list.rs:162-190)
+    s.mesh_blas.assign(s.meshes.size(), -1);
+    for (size_t m = 0; m < s.meshes.size(); ++m) {
+        const rayna_mesh &me = s.meshes[m];
+        if (me.kind != RAYNA_MESH_TRIANGLE_LIST || me.count <= 0) continue;
+        std::vector<Aabb> tb(me.count);
+        for (int i = 0; i < me.count; ++i)
+            tb[i] = encompass_points((const double(*)[3]) & s.tri_v[9 * (me.index + i)], 3);
+        BvhBuilder bb(tb, literal);
+        Bvh b = bb.build();
+        // node boxes + depth-first positions
+        std::vector<NodeBoxes> nbx(b.nodes.size());
+        std::vector<Aabb> tight(b.nodes.size());
+        std::vector<int> dfs(me.count, -1);
+        int counter = 0;
+        // post-order tight refit, pre-order depth-first numbering
+        std::vector<int> order;
+        std::vector<int> stack{b.root};
+        while (!stack.empty()) {
+            const int n = stack.back();
+            stack.pop_back();
+            order.push_back(n);
+            const BvhNode &nd = b.nodes[n];
+            for (int c = (int)nd.children.size() - 1; c >= 0; --c) stack.push_back(nd.children[c]);
+        }
+        for (int n : order)
+            if (b.nodes[n].item >= 0) dfs[b.nodes[n].item] = counter++;
+        for (int k = (int)order.size() - 1; k >= 0; --k) {
+            const int n = order[k];
+            const BvhNode &nd = b.nodes[n];
+            if (nd.item >= 0) tight[n] = nd.box;
+            else {
+                Aabb t = tight[nd.children[0]];
+                for (int c : nd.children) t = encompass(t, tight[c]);
+                tight[n] = t;
+            }
+            fill_boxes(nbx[n], nd.box, tight[n]);
+        }
+        s.mesh_blas[m] = (int)s.blas.size();
+        s.blas.push_back(std::move(b));
+        s.blas_boxes.push_back(std::move(nbx));
+        s.blas_dfs.push_back(std::move(dfs));
+    }
+
+    // TLAS (BvhObject::new_uncorrected over the bounded objects, object/bvh.rs:422-431)
+    {
+        std::vector<Aabb> ob(s.bounded.size());
+        for (size_t i = 0; i < s.bounded.size(); ++i) ob[i] = s.obj_ref_box[s.bounded[i]];
+        BvhBuilder bb(ob, literal);
+        s.tlas = bb.build();
+    }
+    s.tlas_boxes.assign(s.tlas.nodes.size(), NodeBoxes{});
+    s.obj_rank.assign(no, -1);
+    s.tree_hash = 1469598103934665603ull;
+    if (s.tlas.root >= 0) {
+        std::vector<int> order;
+        std::vector<int> stack{s.tlas.root};
+        while (!stack.empty()) {
+            const int n = stack.back();
+            stack.pop_back();
+            order.push_back(n);
+            const BvhNode &nd = s.tlas.nodes[n];
+            for (int c = (int)nd.children.size() - 1; c >= 0; --c) stack.push_back(nd.children[c]);
+        }
+        int rank = 0;
+        for (int n : order) {
+            const BvhNode &nd = s.tlas.nodes[n];
+            if (nd.item < 0) continue;
+            const int o = s.bounded[nd.item];
+            s.obj_rank[o] = rank;
+            const int bi = s.mesh_blas[s.objects[o].mesh];
+            if (s.objects[o].kind == RAYNA_OBJ_SIMPLE && bi >= 0) {
+                const std::vector<int> &dfs = s.blas_dfs[bi];
+                std::vector<int> inv(dfs.size());
+                for (size_t i = 0; i < dfs.size(); ++i) inv[dfs[i]] = (int)i;
+                for (size_t p = 0; p < inv.size(); ++p) {
+                    s.rank_object.push_back(o);
+                    s.rank_local.push_back(inv[p]);
+                }
+                rank += (int)inv.size();
+            } else {
+                s.rank_object.push_back(o);
+                s.rank_local.push_back(0);
+                rank += 1;
+            }
+        }
+        std::vector<Aabb> tight(s.tlas.nodes.size());
+        for (int k = (int)order.size() - 1; k >= 0; --k) {
+            const int n = order[k];
+            const BvhNode &nd = s.tlas.nodes[n];
+            if (nd.item >= 0) tight[n] = s.obj_tight_box[s.bounded[nd.item]];
+            else {
+                Aabb t = tight[nd.children[0]];
+                for (int c : nd.children) t = encompass(t, tight[c]);
+                tight[n] = t;
+            }
+            fill_boxes(s.tlas_boxes[n], nd.box, tight[n]);
+        }
+        hash_bvh(s.tlas, s.tlas.root, s.tree_hash);
+    }
+    for (const Bvh &b : s.blas) hash_bvh(b, b.root, s.tree_hash);
+    s.ok = true;
+}
+
+// ================================================================================================
+// Intersection
+// ================================================================================================
+
+template <class R>
+struct Hit {
+    R t;
+    V3<R> pos, normal, ray_normal;
+    bool front_face;
+    int object;
+    int rank;  // depth-first rank of a bounded primitive; -2 - i for the i-th unbounded object
+    int material;
+};
+
+template <class R>
+struct Ctx {
+    const Scene *s;
+    RngKey key;
+    uint32_t pixel, sample, bounce;
+    int box_mode;
+    uint64_t *prim_tests;
+};
+
+// Aabb::hit (aabb.rs:170-206) in the engine's form, or the conservative tight form (deviation 4).
+template <class R>
+inline bool box_hit(const Ctx<R> &c, const NodeBoxes &nb, const Ray<R> &ray, const Interval<R> &iv) {
+    const bool tight = c.box_mode == BOX_TIGHT;
+    R b[6];
+    for (int i = 0; i < 6; ++i) {
+        if (sizeof(R) == 4) b[i] = (R)(tight ? nb.tight_f[i] : nb.ref_f[i]);
+        else b[i] = (R)(tight ? nb.tight_d[i] : nb.ref_d[i]);
+    }
+    const R tx1 = (b[0] - ray.pos.x) * ray.inv_dir.x, tx2 = (b[3] - ray.pos.x) * ray.inv_dir.x;
+    R tmin = Math<R>::fmin(tx1, tx2), tmax = Math<R>::fmax(tx1, tx2);
+    const R ty1 = (b[1] - ray.pos.y) * ray.inv_dir.y, ty2 = (b[4] - ray.pos.y) * ray.inv_dir.y;
+    tmin = Math<R>::fmax(tmin, Math<R>::fmin(ty1, ty2));
+    tmax = Math<R>::fmin(tmax, Math<R>::fmax(ty1, ty2));
+    const R tz1 = (b[2] - ray.pos.z) * ray.inv_dir.z, tz2 = (b[5] - ray.pos.z) * ray.inv_dir.z;
+    tmin = Math<R>::fmax(tmin, Math<R>::fmin(tz1, tz2));
+    tmax = Math<R>::fmin(tmax, Math<R>::fmax(tz1, tz2));
+    if (!tight) {
+        // Interval::range_overlaps (interval.rs:298-324)
+        R low = tmin, high = tmax;
+        if (iv.has_start) low = (tmin > iv.start) ? tmin : iv.start;
+        if (iv.has_end) high = (tmax < iv.end) ? tmax : iv.end;
+        return low <= high;
+    }
+    const R lo = iv.has_start ? Math<R>::fmax(tmin, iv.start) : tmin;
+    const R hi = iv.has_end ? Math<R>::fmin(tmax, iv.end) : tmax;
+    return lo <= hi * (R)(1.0 + BOX_SLACK);
+}
+
+// SphereMesh::intersect (sphere.rs:140-195)
+template <class R>
+inline bool sphere_intersect(const double *sp, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
+    const V3<R> c = cast3<R>(sp);
+    const R radius = (R)sp[3];
+    const R r2 = radius * radius;  // radius_sqr, precomputed in the engine (sphere.rs:128)
+    const V3<R> oc = ray.pos - c;
+    const R half_b = dot(oc, ray.dir);
+    const R cc = dot(oc, oc) - r2;
+    const R disc = (half_b * half_b) - cc;
+    if (disc < R(0)) return false;
+    const R sq = Math<R>::sqrt(disc);
+    R root = -half_b - sq;
+    if (!iv.contains(root)) {
+        root = -half_b + sq;
+        if (!iv.contains(root)) return false;
+    }
+    h.t = root;
+    h.pos = ray.at(root);
+    const V3<R> local = (h.pos - c) / radius;
+    h.normal = local;
+    const bool inside = dot(ray.dir, local) > R(0);
+    h.ray_normal = inside ? -local : local;
+    h.front_face = !inside;
+    return true;
+}
+
+// Triangle::intersect (triangle.rs:272-328), Moller-Trumbore.
+template <class R>
+inline bool triangle_intersect(const double *tv, const double *tn, const Ray<R> &ray, const Interval<R> &iv,
+                               Hit<R> &h) {
+    const V3<R> v0 = cast3<R>(tv), v1 = cast3<R>(tv + 3), v2 = cast3<R>(tv + 6);
+    const V3<R> e1 = v1 - v0, e2 = v2 - v0;
+    const V3<R> p = cross(ray.dir, e2);
+    const R det = dot(e1, p);
+    if (det == R(0)) return false;
+    const R inv_det = R(1) / det;
+    const V3<R> tvec = ray.pos - v0;
+    const R u = dot(tvec, p) * inv_det;
+    if (u < R(0) || u > R(1)) return false;
+    const V3<R> q = cross(tvec, e1);
+    const R v = dot(ray.dir, q) * inv_det;
+    if (v < R(0) || u + v > R(1)) return false;
+    const R t = dot(e2, q) * inv_det;
+    if (!iv.contains(t)) return false;
+    // interpolate_normals (triangle.rs:333-338): fold from zero, try_normalize
+    const R b0 = (R(1) - u) - v;
+    const V3<R> n0 = cast3<R>(tn), n1 = cast3<R>(tn + 3), n2 = cast3<R>(tn + 6);
+    const V3<R> zero{R(0), R(0), R(0)};
+    const V3<R> sum = ((zero + n0 * b0) + n1 * u) + n2 * v;
+    V3<R> n;
+    if (!try_normalize(sum, &n)) return false;
+    h.t = t;
+    h.pos = ray.at(t);
+    h.normal = n;
+    h.front_face = std::signbit(det);
+    h.ray_normal = n * -signum(det);
+    return true;
+}
+
+// Planar::new (planar/mod.rs:303-314) + ParallelogramMesh::intersect (parallelogram.rs:469-477).
+template <class R>
+inline bool parallelogram_intersect(const double *pp, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
+    // plane constants in double (computed once at construction in the engine), then cast
+    const V3<double> P{pp[0], pp[1], pp[2]}, U{pp[3], pp[4], pp[5]}, Vv{pp[6], pp[7], pp[8]};
+    const V3<double> n_raw = cross(U, Vv);
+    const V3<double> nd = normalize(n_raw);
+    const double dd = -dot(nd, P);
+    const V3<double> wd = n_raw / dot(n_raw, n_raw);
+    const V3<R> n{(R)nd.x, (R)nd.y, (R)nd.z}, w{(R)wd.x, (R)wd.y, (R)wd.z};
+    const V3<R> p{(R)P.x, (R)P.y, (R)P.z}, u{(R)U.x, (R)U.y, (R)U.z}, v{(R)Vv.x, (R)Vv.y, (R)Vv.z};
+    const R d = (R)dd;
+    const R denom = dot(n, ray.dir);
+    if (denom == R(0)) return false;
+    const R t = -(dot(n, ray.pos) + d) / denom;
+    if (!iv.contains(t)) return false;
+    const V3<R> pos = ray.at(t);
+    const V3<R> pl = pos - p;
+    const R alpha = dot(w, cross(pl, v));
+    const R beta = dot(w, cross(u, pl));
+    if (!(alpha <= R(1) && beta <= R(1) && alpha >= R(0) && beta >= R(0))) return false;
+    h.t = t;
+    h.pos = pos;
+    h.normal = n;
+    h.front_face = std::signbit(denom);
+    h.ray_normal = (-n) * signum(denom);
+    return true;
+}
+
+// AxisBoxMesh::intersect (axis_box.rs:64-145), Majercik et al. 2018.
+template <class R>
+inline bool axis_box_intersect(const double *bp, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
+    double mn[3], mx[3], centre_d[3], radius_d[3], inv_radius_d[3];
+    for (int i = 0; i < 3; ++i) {
+        mn[i] = std::fmin(bp[i], bp[3 + i]);
+        mx[i] = std::fmax(bp[i], bp[3 + i]);
+        centre_d[i] = (mn[i] + mx[i]) / 2.0;
+        radius_d[i] = (mx[i] - mn[i]) / 2.0;
+        inv_radius_d[i] = 1.0 / radius_d[i];
+    }
+    const V3<R> centre = cast3<R>(centre_d), radius = cast3<R>(radius_d), inv_radius = cast3<R>(inv_radius_d);
+    const V3<R> ro = ray.pos - centre;
+    const V3<R> rd = ray.dir;
+    const V3<R> ar{std::fabs(ro.x) * inv_radius.x, std::fabs(ro.y) * inv_radius.y, std::fabs(ro.z) * inv_radius.z};
+    const R winding = signum(std::fmax(std::fmax(ar.x, ar.y), ar.z) - R(1));
+    const V3<R> sgn{-signum(rd.x), -signum(rd.y), -signum(rd.z)};
+    V3<R> pd = (radius * winding).mul(sgn) - ro;
+    pd = pd.mul(ray.inv_dir);
+    const int U[3] = {0, 1, 2}, V1[3] = {1, 2, 0}, W1[3] = {2, 0, 1};
+    for (int k = 0; k < 3; ++k) {
+        const int a = U[k], b = V1[k], c = W1[k];
+        const R dist = pd[a];
+        if (!iv.contains(dist)) continue;
+        const R ub = ro[b] + rd[b] * dist, uc = ro[c] + rd[c] * dist;
+        if (std::fabs(ub) < radius[b] && std::fabs(uc) < radius[c]) {
+            V3<R> rn{R(0), R(0), R(0)};
+            if (a == 0) rn.x = sgn.x;
+            if (a == 1) rn.y = sgn.y;
+            if (a == 2) rn.z = sgn.z;
+            h.t = dist;
+            h.pos = ray.at(dist);
+            h.ray_normal = rn;
+            h.normal = rn * winding;
+            h.front_face = !std::signbit(winding);
+            return true;
+        }
+    }
+    return false;
+}
+
+// ---- signed distance functions (enumerated, deviation 3) ----
+
+template <class R>
+inline R ipow(R x, int n) {  // binary powering, the order of compiler-rt's __powidf2
+    R r = R(1);
+    R a = x;
+    while (n > 0) {
+        if (n & 1) r = r * a;
+        n >>= 1;
+        if (n) a = a * a;
+    }
+    return r;
+}
+
+template <class R>
+inline R sdf_eval(const Scene &s, const rayna_sdf &f, const V3<R> &p) {
+    switch (f.kind) {
+        case RAYNA_SDF_SPHERE: {
+            const double *sp = &s.spheres[4 * f.first_sphere];
+            return length(p - cast3<R>(sp)) - (R)sp[3];
+        }
+        case RAYNA_SDF_SMIN_SPHERES: {
+            const R k = (R)f.k;
+            const double *sp = &s.spheres[4 * f.first_sphere];
+            R d = length(p - cast3<R>(sp)) - (R)sp[3];
+            for (int i = 1; i < f.n_spheres; ++i) {
+                const double *q = &s.spheres[4 * (f.first_sphere + i)];
+                const R di = length(p - cast3<R>(q)) - (R)q[3];
+                const R h = std::max(k - std::fabs(d - di), R(0)) / k;
+                d = std::min(d, di) - h * h * k * R(0.25);
+            }
+            return d;
+        }
+        default: {  // RAYNA_SDF_MANDELBULB
+            const int power = (int)f.power;
+            const R bail = (R)f.bailout;
+            const V3<R> c = cast3<R>(f.centre);
+            const V3<R> p0 = p - c;
+            V3<R> z = p0;
+            R dr = R(1), r = R(0);
+            for (int i = 0; i < f.bulb_iterations; ++i) {
+                r = length(z);
+                if (r > bail) break;
+                const R theta = Math<R>::acos(z.z / r);
+                const R phi = Math<R>::atan2(z.y, z.x);
+                dr = ipow(r, power - 1) * (R)power * dr + R(1);
+                const R zr = ipow(r, power);
+                R st, ct, sp, cp;
+                Math<R>::sincos(theta * (R)power, &st, &ct);
+                Math<R>::sincos(phi * (R)power, &sp, &cp);
+                z = V3<R>{st * cp, sp * st, ct} * zr + p0;
+            }
+            return R(0.5) * Math<R>::log(r) * r / dr;
+        }
+    }
+}
+
+// RaymarchedIsosurfaceMesh::intersect (raymarched.rs:228-277)
+template <class R>
+inline bool sdf_intersect(const Scene &s, const rayna_sdf &f, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
+    const R eps = (R)f.epsilon;
+    R total = iv.has_start ? iv.start : R(0);
+    V3<R> point = ray.at(total);
+    int i = 0;
+    for (;;) {
+        const R dist = sdf_eval<R>(s, f, point);
+        total = total + std::fabs(dist);
+        point = ray.at(total);
+        if (std::fabs(dist) < eps && iv.contains(total)) {
+            const V3<R> p = point;
+            const V3<R> g{sdf_eval<R>(s, f, V3<R>{p.x + eps, p.y, p.z}) - sdf_eval<R>(s, f, V3<R>{p.x - eps, p.y, p.z}),
+                          sdf_eval<R>(s, f, V3<R>{p.x, p.y + eps, p.z}) - sdf_eval<R>(s, f, V3<R>{p.x, p.y - eps, p.z}),
+                          sdf_eval<R>(s, f, V3<R>{p.x, p.y, p.z + eps}) - sdf_eval<R>(s, f, V3<R>{p.x, p.y, p.z - eps})};
+            const V3<R> n = normalize(g);
+            h.t = total;
+            h.pos = p;
+            h.normal = n;
+            h.ray_normal = n;
+            h.front_face = !std::signbit(dist);
+            return true;
+        }
+        if (i > f.max_iterations) return false;
+        ++i;
+    }
+}
+
+// Mesh::intersect for one primitive mesh (not a triangle list).
+template <class R>
+inline bool prim_mesh_intersect(const Scene &s, const rayna_mesh &m, const Ray<R> &ray, const Interval<R> &iv,
+                                Hit<R> &h) {
+    switch (m.kind) {
+        case RAYNA_MESH_SPHERE: return sphere_intersect(&s.spheres[4 * m.index], ray, iv, h);
+        case RAYNA_MESH_TRIANGLE: return triangle_intersect(&s.tri_v[9 * m.index], &s.tri_n[9 * m.index], ray, iv, h);
+        case RAYNA_MESH_PARALLELOGRAM: return parallelogram_intersect(&s.paras[9 * m.index], ray, iv, h);
+        case RAYNA_MESH_AXIS_BOX: return axis_box_intersect(&s.boxes[6 * m.index], ray, iv, h);
+        case RAYNA_MESH_SDF: return sdf_intersect(s, s.sdfs[m.index], ray, iv, h);
+        default: return false;
+    }
+}
+
+// BvhMesh::bvh_node_intersect (mesh/advanced/bvh.rs:63-102) over a triangle list.
+template <class R>
+inline bool blas_node(const Ctx<R> &c, int bi, int node, int first_tri, int rank_base, const Ray<R> &ray,
+                      const Interval<R> &iv, Hit<R> &out) {
+    const Scene &s = *c.s;
+    const BvhNode &nd = s.blas[bi].nodes[node];
+    if (!box_hit(c, s.blas_boxes[bi][node], ray, iv)) return false;
+    if (nd.item >= 0) {
+        const int tri = first_tri + nd.item;
+        if (c.prim_tests) ++*c.prim_tests;
+        if (!triangle_intersect(&s.tri_v[9 * tri], &s.tri_n[9 * tri], ray, iv, out)) return false;
+        out.rank = rank_base + s.blas_dfs[bi][nd.item];
+        return true;
+    }
+    Interval<R> shrunk = iv;
+    bool found = false;
+    for (int ch : nd.children) {
+        Hit<R> h;
+        if (blas_node(c, bi, ch, first_tri, rank_base, ray, shrunk, h)) {
+            shrunk.has_end = true;
+            shrunk.end = h.t;
+            out = h;
+            found = true;
+        }
+    }
+    return found;
+}
+
+// Object::full_intersect of one object of the list (SimpleObject simple.rs:215-225,
+// VolumetricObject volumetric.rs:86-160).
+template <class R>
+inline bool object_intersect(const Ctx<R> &c, int o, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
+    const Scene &s = *c.s;
+    const rayna_object &obj = s.objects[o];
+    const rayna_mesh &m = s.meshes[obj.mesh];
+    bool hit = false;
+    if (obj.kind == RAYNA_OBJ_SIMPLE) {
+        if (m.kind == RAYNA_MESH_TRIANGLE_LIST) {
+            const int bi = s.mesh_blas[obj.mesh];
+            if (bi < 0) return false;
+            hit = blas_node(c, bi, s.blas[bi].root, m.index, s.obj_rank[o], ray, iv, h);
+        } else {
+            if (c.prim_tests) ++*c.prim_tests;
+            hit = prim_mesh_intersect(s, m, ray, iv, h);
+            h.rank = s.obj_rank[o];
+        }
+    } else {
+        // volume: entry with the full interval, exit after entry + 0.001, free flight -ln(U)/density
+        Interval<R> full{false, false, R(0), R(0)};
+        Hit<R> he, hx;
+        if (m.kind == RAYNA_MESH_TRIANGLE_LIST) return false;  // not supported as a volume boundary
+        if (c.prim_tests) ++*c.prim_tests;
+        if (!prim_mesh_intersect(s, m, ray, full, he)) return false;
+        R entry = he.t;
+        if (iv.has_start) entry = Math<R>::fmax(entry, iv.start);
+        Interval<R> exit_iv{true, false, entry + R(0.001), R(0)};
+        if (!prim_mesh_intersect(s, m, ray, exit_iv, hx)) return false;
+        const R exit_d = hx.t;
+        const R inside = exit_d - entry;
+        R u[4];
+        draw4<R>(c.key, c.pixel, c.sample, c.bounce, SLOT_MEDIUM_BASE + (uint32_t)o, u);
+        const R neg_inv_density = R(-1) / (R)obj.density;
+        const R hit_dist = neg_inv_density * Math<R>::log(u[0]);
+        const R dist = entry + hit_dist;
+        if (hit_dist > inside) return false;
+        R a[4];
+        draw4<R>(c.key, c.pixel, c.sample, c.bounce, SLOT_MEDIUM_AUX, a);
+        h.t = dist;
+        h.pos = ray.at(dist);
+        // arbitrary unit normals (volumetric.rs:151-152), from their own draw site
+        auto unit = [](R r1, R r2) {
+            R sn, cs;
+            Math<R>::sincos_turns(r1, &sn, &cs);
+            const R q = Math<R>::sqrt(r2 * (R(1) - r2));
+            return V3<R>{(cs * R(2)) * q, (sn * R(2)) * q, R(1) - R(2) * r2};
+        };
+        h.normal = unit(a[0], a[1]);
+        h.ray_normal = unit(a[2], a[3]);
+        h.front_face = true;
+        h.rank = s.obj_rank[o];
+        hit = true;
+    }
+    if (hit) {
+        h.object = o;
+        h.material = obj.material;
+    }
+    return hit;
+}
+
+// BvhObject::bvh_node_intersect (object/bvh.rs:443-481)
+template <class R>
+inline bool tlas_node(const Ctx<R> &c, int node, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &out) {
+    const Scene &s = *c.s;
+    const BvhNode &nd = s.tlas.nodes[node];
+    if (!box_hit(c, s.tlas_boxes[node], ray, iv)) return false;
+    if (nd.item >= 0) return object_intersect(c, s.bounded[nd.item], ray, iv, out);
+    Interval<R> shrunk = iv;
+    bool found = false;
+    for (int ch : nd.children) {
+        Hit<R> h;
+        if (tlas_node(c, ch, ray, shrunk, h)) {
+            shrunk.has_end = true;
+            shrunk.end = h.t;
+            out = h;
+            found = true;
+        }
+    }
+    return found;
+}
+
+// ObjectList::full_intersect (object/list.rs:350-367): the BVH result, then the unbounded objects with the
+// caller's interval; Iterator::min keeps the first of equal minima.
+template <class R>
+inline bool scene_intersect(const Ctx<R> &c, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &out) {
+    const Scene &s = *c.s;
+    bool found = false;
+    if (s.tlas.root >= 0) found = tlas_node(c, s.tlas.root, ray, iv, out);
+    for (size_t i = 0; i < s.unbounded.size(); ++i) {
+        Hit<R> h;
+        if (object_intersect(c, s.unbounded[i], ray, iv, h)) {
+            h.rank = -2 - (int)i;
+            if (!found || h.t < out.t) {
+                out = h;
+                found = true;
+            }
+        }
+    }
+    return found;
+}
+
+// ================================================================================================
+// Materials (material/*.rs) and skyboxes (skybox/*.rs)
+// ================================================================================================
+
+template <class R>
+inline V3<R> unit_sphere(R r1, R r2) {  // rng::normal_on_unit_sphere (shared/rng.rs:75-87)
+    R sn, cs;
+    Math<R>::sincos_turns(r1, &sn, &cs);
+    const R q = Math<R>::sqrt(r2 * (R(1) - r2));
+    return V3<R>{(cs * R(2)) * q, (sn * R(2)) * q, R(1) - R(2) * r2};
+}
+
+// exp of a colour channel: the engine calls f32::exp (dielectric.rs:164, isotropic.rs:226).
+template <class R>
+inline float chan_exp(float x) {
+    if (sizeof(R) == 4) return exp_f32(x);
+    return std::exp(x);
+}
+
+// Material::scatter.  Returns false for "no scatter" (None).
+template <class R>
+inline bool scatter(const rayna_material &m, const Ray<R> &ray, const Hit<R> &h, const R u[4], V3<R> *dir) {
+    switch (m.kind) {
+        case RAYNA_MAT_LAMBERTIAN: {  // lambertian.rs:29-36
+            const V3<R> rnd = unit_sphere(u[0], u[1]) * Math<R>::cbrt01(u[2]);
+            const V3<R> v = h.ray_normal + rnd;
+            if (!try_normalize(v, dir)) *dir = h.ray_normal;
+            return true;
+        }
+        case RAYNA_MAT_METAL: {  // metal.rs:66-82
+            const V3<R> refl = reflect(ray.dir, h.ray_normal);
+            const V3<R> rnd = unit_sphere(u[0], u[1]);
+            const V3<R> v = refl + rnd * (R)m.fuzz;
+            if (dot(v, h.ray_normal) > R(0)) {
+                *dir = normalize(v);
+                return true;
+            }
+            return false;
+        }
+        case RAYNA_MAT_DIELECTRIC: {  // dielectric.rs:114-134, reflectance :169-174
+            const R ior = (R)m.refractive_index;
+            const R ratio = h.front_face ? R(1) / ior : ior;
+            const R cos_t = Math<R>::fmin(dot(-ray.dir, h.ray_normal), R(1));
+            const R sin_t = Math<R>::sqrt(R(1) - cos_t * cos_t);
+            const bool tir = ratio * sin_t > R(1);
+            const R r0 = (R(1) - ratio) / (R(1) + ratio);
+            const R r0s = r0 * r0;
+            const R refl = r0s + (R(1) - r0s) * ipow(R(1) - cos_t, 5);
+            const bool schlick = refl > u[0];
+            *dir = (tir || schlick) ? reflect(ray.dir, h.ray_normal) : refract(ray.dir, h.ray_normal, ratio);
+            return true;
+        }
+        case RAYNA_MAT_ISOTROPIC:  // isotropic.rs:204-206
+            *dir = unit_sphere(u[0], u[1]);
+            return true;
+        default:  // LIGHT (light.rs:245)
+            return false;
+    }
+}
+
+// Material::emitted_light (light.rs:247-249; default black mod.rs:100-103)
+inline Col emitted(const rayna_material &m) {
+    if (m.kind == RAYNA_MAT_LIGHT) return Col{m.colour[0], m.colour[1], m.colour[2]};
+    return Col{0, 0, 0};
+}
+
+// Material::reflected_light as applied to the future colour: out = future * albedo [* exp(-density d)].
+template <class R>
+inline Col reflected(const rayna_material &m, const Ray<R> &ray, const Hit<R> &h, const Col &future) {
+    const Col albedo{m.colour[0], m.colour[1], m.colour[2]};
+    switch (m.kind) {
+        case RAYNA_MAT_LAMBERTIAN:
+        case RAYNA_MAT_METAL: return future * albedo;
+        case RAYNA_MAT_DIELECTRIC: {
+            if (h.front_face) return future;
+            const R dist = length(h.pos - ray.pos);
+            const float tr = (float)(-(R)m.density * dist);
+            return (future * albedo) * chan_exp<R>(tr);
+        }
+        case RAYNA_MAT_ISOTROPIC: {
+            const R dist = length(h.pos - ray.pos);
+            const float tr = (float)(-(R)m.density * dist);
+            return (future * albedo) * chan_exp<R>(tr);
+        }
+        default: return Col{0, 0, 0};
+    }
+}
+
+template <class R>
+inline Col sky(int kind, const Ray<R> &ray) {
+    switch (kind) {
+        case RAYNA_SKY_WHITE: return Col{1, 1, 1};
+        case RAYNA_SKY_SIMPLE: {  // simple.rs:140-148: lerp(white, (0.5, 0.7, 1), a)
+            const float a = (float)(R(0.5) * ray.dir.y + R(0.5));
+            return Col{1.0f + (0.5f - 1.0f) * a, 1.0f + (0.7f - 1.0f) * a, 1.0f + (1.0f - 1.0f) * a};
+        }
+        default: return Col{0, 0, 0};
+    }
+}
+
+// ================================================================================================
+// Camera (scene/camera.rs:285-384)
+// ================================================================================================
+
+struct Viewport {
+    double pos[3], pixel_center[3], vu[3], vv[3], du[3], dv[3];
+    bool valid;
+};
+
+inline Viewport calculate_viewport(const rayna_camera &cam) {
+    Viewport vp{};
+    vp.valid = false;
+    const double focal = cam.focus_dist;
+    if (cam.v_fov == 0.0) return vp;
+    if (focal == 0.0) return vp;
+    V3<double> fwd{cam.fwd[0], cam.fwd[1], cam.fwd[2]};
+    V3<double> wn;
+    if (!try_normalize(fwd, &wn)) return vp;
+    const V3<double> w = -wn;
+    V3<double> u;
+    if (!try_normalize(cross(V3<double>{0, 1, 0}, w), &u)) return vp;
+    const V3<double> v = cross(w, u);
+    const V3<double> pos{cam.pos[0], cam.pos[1], cam.pos[2]};
+    const V3<double> pc = pos - w * focal;
+    const double defocus_radius = focal * std::tan(cam.defocus_angle / 2.0);
+    const V3<double> du = u * defocus_radius, dv = v * defocus_radius;
+    const double hh = std::tan(cam.v_fov / 2.0);
+    const double size = 2.0 * hh * focal;
+    const V3<double> vu = u * size, vv = (-v) * size;
+    auto put = [](double *d, const V3<double> &a) {
+        d[0] = a.x;
+        d[1] = a.y;
+        d[2] = a.z;
+    };
+    put(vp.pos, pos);
+    put(vp.pixel_center, pc);
+    put(vp.vu, vu);
+    put(vp.vv, vv);
+    put(vp.du, du);
+    put(vp.dv, dv);
+    vp.valid = true;
+    return vp;
+}
+
+// The camera ray of sample k of pixel (x, y): render_px_msaa's stratified coordinate (renderer.rs:370-389)
+// and Viewport::calc_ray (camera.rs:362-384).
+template <class R>
+inline Ray<R> camera_ray(const Viewport &vp, const rayna_render_opts &o, const RngKey &key, uint32_t x, uint32_t y,
+                         uint32_t k) {
+    const uint32_t pixel = y * o.width + x;
+    R u[4];
+    draw4<R>(key, pixel, k, 0, SLOT_CAMERA, u);
+    const uint32_t sdim = (uint32_t)std::sqrt((double)o.samples);
+    uint32_t s = sdim;
+    while (s * s > o.samples) --s;
+    while ((s + 1) * (s + 1) <= o.samples) ++s;
+    const R jx = u[0] - R(0.5), jy = u[1] - R(0.5);
+    R cx, cy;
+    if (k < s * s) {
+        const R inv = R(1) / (R)s;
+        const uint32_t i = k % s, j = k / s;
+        cx = ((R)x + jx * inv) + (R)i * inv;
+        cy = ((R)y + jy * inv) + (R)j * inv;
+    } else {
+        cx = (R)x + jx;
+        cy = (R)y + jy;
+    }
+    const R W = (R)o.width, H = (R)o.height;
+    const R uu = (cx - W / R(2)) / H;
+    const R vv = (cy - H / R(2)) / H;
+    const V3<R> pix = (cast3<R>(vp.pixel_center) + cast3<R>(vp.vu) * uu) + cast3<R>(vp.vv) * vv;
+    // lens: sqrt(u) * (cos 2pi v, sin 2pi v) replaces rng::vector_in_unit_circle
+    R sn, cs;
+    Math<R>::sincos_turns(u[3], &sn, &cs);
+    const R rr = Math<R>::sqrt(u[2]);
+    const R dx = rr * cs, dy = rr * sn;
+    const V3<R> org = (cast3<R>(vp.pos) + cast3<R>(vp.du) * dx) + cast3<R>(vp.dv) * dy;
+    return Ray<R>::make(org, pix - org);
+}
+
+// ================================================================================================
+// Integrators
+// ================================================================================================
+
+template <class R>
+struct Tracer {
+    const Scene *s;
+    RngKey key;
+    uint32_t pixel, sample, depth_max;
+    int box_mode;
+    uint64_t segments = 0;
+    uint64_t prim_tests = 0;
+
+    Ctx<R> ctx(uint32_t bounce) { return Ctx<R>{s, key, pixel, sample, bounce, box_mode, &prim_tests}; }
+
+    bool intersect(const Ray<R> &ray, uint32_t bounce, Hit<R> &h) {
+        ++segments;
+        const Interval<R> iv{true, true, R(1e-3), Math<R>::big()};  // renderer.rs:239
+        return scene_intersect(ctx(bounce), ray, iv, h);
+    }
+
+    // ray_colour_recursive (renderer.rs:508-572), ray_branching = 1
+    Col recursive(const Ray<R> &ray, uint32_t depth) {
+        if (depth > depth_max) return Col{0, 0, 0};
+        Hit<R> h;
+        if (!intersect(ray, depth, h)) return sky<R>(s->skybox, ray);
+        const rayna_material &m = s->materials[h.material];
+        const Col le = emitted(m);
+        R u[4];
+        draw4<R>(key, pixel, sample, depth, SLOT_SCATTER, u);
+        V3<R> dir;
+        Col scattered{0, 0, 0};
+        if (scatter(m, ray, h, u, &dir)) {
+            const Ray<R> next = Ray<R>::make(h.pos, dir);
+            const Col future = recursive(next, depth + 1);
+            scattered = reflected(m, ray, h, future);
+        }
+        const Col mean = (Col{0, 0, 0} + scattered) / 1.0f;
+        return le + mean;
+    }
+
+    // The same estimator with forward throughput: L += T*Le at each vertex, T *= attenuation.
+    Col forward(Ray<R> ray) {
+        Col L{0, 0, 0}, T{1, 1, 1};
+        for (uint32_t depth = 0; depth <= depth_max; ++depth) {
+            Hit<R> h;
+            if (!intersect(ray, depth, h)) {
+                L = L + T * sky<R>(s->skybox, ray);
+                break;
+            }
+            const rayna_material &m = s->materials[h.material];
+            L = L + T * emitted(m);
+            R u[4];
+            draw4<R>(key, pixel, sample, depth, SLOT_SCATTER, u);
+            V3<R> dir;
+            if (!scatter(m, ray, h, u, &dir)) break;
+            T = reflected(m, ray, h, T);
+            ray = Ray<R>::make(h.pos, dir);
+        }
+        return L;
+    }
+};
+
+enum Integrator { INTEGRATOR_RECURSIVE = 0, INTEGRATOR_FORWARD = 1 };
+
+struct RenderArgs {
+    const Scene *scene;
+    const rayna_camera *cam;
+    const rayna_render_opts *opts;
+    uint32_t frame;
+    int box_mode;
+    int integrator;
+    uint32_t row_start, row_step;  // rows row_start, row_start + row_step, ...
+    int threads;
+};
+
+// One frame of per-pixel sample means (render_px_msaa's mean, renderer.rs:398-402), row-major, for the
+// selected rows (out has n_rows * width * 3 floats).  Returns the number of closest-hit queries.
+template <class R>
+inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_tests_out) {
+    const rayna_render_opts &o = *a.opts;
+    const Viewport vp = calculate_viewport(*a.cam);
+    const RngKey key = make_key(o.seed, a.frame);
+    const uint32_t n_rows = (o.height - a.row_start + a.row_step - 1) / a.row_step;
+    std::atomic<uint32_t> next{0};
+    std::atomic<uint64_t> segs{0}, tests{0};
+    auto worker = [&]() {
+        uint64_t my_segs = 0, my_tests = 0;
+        for (;;) {
+            const uint32_t r = next.fetch_add(1);
+            if (r >= n_rows) break;
+            const uint32_t y = a.row_start + r * a.row_step;
+            for (uint32_t x = 0; x < o.width; ++x) {
+                Tracer<R> tr{a.scene, key, y * o.width + x, 0, o.ray_depth, a.box_mode};
+                Col sum{0, 0, 0};
+                for (uint32_t k = 0; k < o.samples; ++k) {
+                    tr.sample = k;
+                    const Ray<R> ray = camera_ray<R>(vp, o, key, x, y, k);
+                    const Col c = (a.integrator == INTEGRATOR_FORWARD) ? tr.forward(ray) : tr.recursive(ray, 0);
+                    sum = sum + c;
+                }
+                const Col mean = sum / (float)o.samples;
+                float *px = out + ((size_t)r * o.width + x) * 3;
+                px[0] = mean.r;
+                px[1] = mean.g;
+                px[2] = mean.b;
+                my_segs += tr.segments;
+                my_tests += tr.prim_tests;
+            }
+        }
+        segs += my_segs;
+        tests += my_tests;
+    };
+    int nt = a.threads > 0 ? a.threads : (int)std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+    if (prim_tests_out) *prim_tests_out = tests.load();
+    return segs.load();
+}
+
+}  // namespace twin

@@ -1,0 +1,68 @@
+"""ctypes mirrors of the C structs declared in include/rayna_gpu.h (kept field-for-field identical)."""
+import ctypes as C
+
+RAYNA_OK = 0
+RAYNA_INVALID_ARGUMENT = 1
+RAYNA_UNSUPPORTED = 2
+RAYNA_DEVICE_ERROR = 3
+RAYNA_OUT_OF_MEMORY = 4
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "UNSUPPORTED", 3: "DEVICE_ERROR", 4: "OUT_OF_MEMORY"}
+
+MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_LIGHT = 0, 1, 2, 3, 4
+MESH_SPHERE, MESH_TRIANGLE, MESH_TRIANGLE_LIST, MESH_PARALLELOGRAM, MESH_AXIS_BOX, MESH_SDF = 0, 1, 2, 3, 4, 5
+SDF_SPHERE, SDF_SMIN_SPHERES, SDF_MANDELBULB = 0, 1, 2
+OBJ_SIMPLE, OBJ_VOLUMETRIC = 0, 1
+SKY_NONE, SKY_SIMPLE, SKY_WHITE = 0, 1, 2
+MODE_PBR = 0
+LAYOUT_XMAJOR, LAYOUT_ROWMAJOR = 0, 1
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("colour", C.c_float * 3), ("fuzz", C.c_double),
+                ("refractive_index", C.c_double), ("density", C.c_double)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_int32), ("count", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Sdf(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("first_sphere", C.c_int32), ("n_spheres", C.c_int32),
+                ("max_iterations", C.c_int32), ("bulb_iterations", C.c_int32), ("reserved", C.c_int32),
+                ("epsilon", C.c_double), ("k", C.c_double), ("power", C.c_double), ("bailout", C.c_double),
+                ("centre", C.c_double * 3)]
+
+
+class Object(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("mesh", C.c_int32), ("material", C.c_int32), ("reserved", C.c_int32),
+                ("density", C.c_double)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("objects", C.POINTER(Object)), ("n_objects", C.c_int32),
+                ("meshes", C.POINTER(Mesh)), ("n_meshes", C.c_int32),
+                ("materials", C.POINTER(Material)), ("n_materials", C.c_int32),
+                ("spheres", C.POINTER(C.c_double)), ("n_spheres", C.c_int32),
+                ("tri_vertices", C.POINTER(C.c_double)), ("tri_normals", C.POINTER(C.c_double)),
+                ("n_triangles", C.c_int32),
+                ("parallelograms", C.POINTER(C.c_double)), ("n_parallelograms", C.c_int32),
+                ("boxes", C.POINTER(C.c_double)), ("n_boxes", C.c_int32),
+                ("sdfs", C.POINTER(Sdf)), ("n_sdfs", C.c_int32),
+                ("skybox", C.c_int32)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("pos", C.c_double * 3), ("fwd", C.c_double * 3), ("v_fov", C.c_double),
+                ("focus_dist", C.c_double), ("defocus_angle", C.c_double)]
+
+
+class RenderOptsDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("samples", C.c_uint32),
+                ("ray_depth", C.c_uint32), ("ray_branching", C.c_uint32), ("mode", C.c_int32),
+                ("seed", C.c_uint64)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("duration_s", C.c_double), ("kernel_ms", C.c_double), ("extend_ms", C.c_double),
+                ("shade_ms", C.c_double), ("accum_frames", C.c_uint64), ("samples", C.c_uint64),
+                ("segments", C.c_uint64), ("extend_launches", C.c_uint64)]
