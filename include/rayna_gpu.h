@@ -242,6 +242,13 @@ rayna_status rayna_dfs_order(rayna_renderer *r, int32_t *object, int32_t *local,
 /* FNV-1a hash of the engine BVH topology the renderer built (TLAS and every BLAS). */
 rayna_status rayna_tree_hash(rayna_renderer *r, uint64_t *hash);
 
+/* Host-only scene preparation, as rayna_create performs it, without touching a GPU (no engine counterpart;
+ * it exposes the BVH construction of shared/generic_bvh.rs:45-290 for inspection): the topology hash, the
+ * depth-first order of bounded primitives (object, local) up to `capacity` entries, their count, and the
+ * deepest traversal stack the flattened tree can need. */
+rayna_status rayna_scene_prepare(const rayna_scene_desc *scene, uint64_t *tree_hash, int32_t *object,
+                                 int32_t *local, int32_t capacity, int32_t *n_ranks, int32_t *stack_need);
+
 /* Renderer drop. */
 void rayna_destroy(rayna_renderer *r);
 

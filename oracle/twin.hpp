@@ -932,6 +932,8 @@ inline bool object_intersect(const Ctx<R> &c, int o, const Ray<R> &ray, const In
         const R hit_dist = neg_inv_density * Math<R>::log(u[0]);
         const R dist = entry + hit_dist;
         if (hit_dist > inside) return false;
+        // the engine clamps the exit to the interval end instead, which rejects the same far candidates
+        if (iv.has_end && dist > iv.end) return false;
         R a[4];
         draw4<R>(c.key, c.pixel, c.sample, c.bounce, SLOT_MEDIUM_AUX, a);
         h.t = dist;

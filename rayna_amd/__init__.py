@@ -13,3 +13,6 @@ __all__ = ["AxisBoxMesh", "Camera", "DielectricMaterial", "IsotropicMaterial", "
            "MetalMaterial", "NoSkybox", "ParallelogramMesh", "RaymarchedIsosurfaceMesh", "RenderOpts", "Scene",
            "SdfMandelbulb", "SdfSminSpheres", "SdfSphere", "SimpleObject", "SimpleSkybox", "SphereMesh",
            "TriangleListMesh", "TriangleMesh", "VolumetricObject", "WhiteSkybox", "presets"]
+from .renderer import Render, Renderer, RenderStats, RayError, prepare_scene
+
+__all__ += ["Render", "Renderer", "RenderStats", "RayError", "prepare_scene"]

@@ -1,0 +1,854 @@
+// Wavefront path-tracing kernels for gfx950 (CDNA4, wave64): generate -> (extend -> shade)* -> resolve ->
+// finalize.  Each kernel restates one stage of rayna_engine's per-pixel recursion
+// (rayna_engine/src/render/renderer.rs:350-572) for a whole chunk of paths held as SoA queues in HBM.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "../../include/rayna_gpu.h"
+#include "device_math.hpp"
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace rayna {
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int EXT_BLOCK = 128;
+constexpr float T_MIN = 1e-3f;                // render interval start (renderer.rs:239)
+constexpr float BOX_SLACK = 1.0f + 0x1.0p-16f;  // conservative slab test: culling never drops the closest hit
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 neg(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 mul(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 normalize(f3 a) { return a * (1.0f / length(a)); }
+__device__ __forceinline__ bool try_normalize(f3 a, f3 *out) {  // glam try_normalize
+    const float rcp = 1.0f / length(a);
+    if (isfinite(rcp) && rcp > 0.0f) {
+        *out = a * rcp;
+        return true;
+    }
+    return false;
+}
+__device__ __forceinline__ f3 reflect(f3 d, f3 n) { return d - n * (2.0f * dot(d, n)); }  // shared/math.rs:26
+__device__ __forceinline__ f3 refract(f3 v, f3 n, float ratio) {                        // shared/math.rs:28-33
+    const float cos_theta = fminf(dot(neg(v), n), 1.0f);
+    const f3 perp = (v + n * cos_theta) * ratio;
+    const f3 par = n * -sqrtf(fabsf(1.0f - dot(perp, perp)));
+    return perp + par;
+}
+
+struct Col {
+    float r, g, b;
+};
+__device__ __forceinline__ Col operator+(Col a, Col b) { return {a.r + b.r, a.g + b.g, a.b + b.b}; }
+__device__ __forceinline__ Col operator*(Col a, Col b) { return {a.r * b.r, a.g * b.g, a.b * b.b}; }
+__device__ __forceinline__ Col operator*(Col a, float s) { return {a.r * s, a.g * s, a.b * s}; }
+
+struct DRay {  // shared/ray.rs:15-23
+    f3 o, d, inv;
+};
+__device__ __forceinline__ DRay make_ray(f3 o, f3 d) {
+    DRay r;
+    r.o = o;
+    r.d = normalize(d);
+    r.inv = f3{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
+    return r;
+}
+__device__ __forceinline__ DRay ray_from_dir(f3 o, f3 unit_d) {
+    DRay r;
+    r.o = o;
+    r.d = unit_d;
+    r.inv = f3{1.0f / unit_d.x, 1.0f / unit_d.y, 1.0f / unit_d.z};
+    return r;
+}
+__device__ __forceinline__ f3 at(const DRay &r, float t) { return r.o + r.d * t; }
+
+// ------------------------------------------------------------------------------------------ primitives
+
+// SphereMesh::intersect (sphere.rs:140-195)
+__device__ __forceinline__ bool sphere_t(const GSphere &s, const DRay &r, float t0, float t1, float *t) {
+    const f3 c{s.cx, s.cy, s.cz};
+    const float r2 = s.radius * s.radius;
+    const f3 oc = r.o - c;
+    const float half_b = dot(oc, r.d);
+    const float cc = dot(oc, oc) - r2;
+    const float disc = (half_b * half_b) - cc;
+    if (disc < 0.0f) return false;
+    const float sq = sqrtf(disc);
+    float root = -half_b - sq;
+    if (!(t0 <= root && root <= t1)) {
+        root = -half_b + sq;
+        if (!(t0 <= root && root <= t1)) return false;
+    }
+    *t = root;
+    return true;
+}
+
+// Triangle::intersect (triangle.rs:272-328), Moller-Trumbore; e1, e2 precomputed in float.
+__device__ __forceinline__ bool tri_t(const GTri &g, const DRay &r, float t0, float t1, float *t, float *uu, float *vv,
+                                      float *det_out) {
+    const f3 v0{g.v0x, g.v0y, g.v0z}, e1{g.e1x, g.e1y, g.e1z}, e2{g.e2x, g.e2y, g.e2z};
+    const f3 p = cross(r.d, e2);
+    const float det = dot(e1, p);
+    if (det == 0.0f) return false;
+    const float inv_det = 1.0f / det;
+    const f3 tv = r.o - v0;
+    const float u = dot(tv, p) * inv_det;
+    if (u < 0.0f || u > 1.0f) return false;
+    const f3 q = cross(tv, e1);
+    const float v = dot(r.d, q) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    const float tt = dot(e2, q) * inv_det;
+    if (!(t0 <= tt && tt <= t1)) return false;
+    *t = tt;
+    *uu = u;
+    *vv = v;
+    *det_out = det;
+    return true;
+}
+
+// Triangle::interpolate_normals (triangle.rs:333-338): fold from zero, then try_normalize.
+__device__ __forceinline__ bool tri_normal(const GTriNormals &n, float u, float v, f3 *out) {
+    const float b0 = (1.0f - u) - v;
+    const f3 s{((0.0f + n.n0x * b0) + n.n1x * u) + n.n2x * v, ((0.0f + n.n0y * b0) + n.n1y * u) + n.n2y * v,
+               ((0.0f + n.n0z * b0) + n.n1z * u) + n.n2z * v};
+    return try_normalize(s, out);
+}
+
+// Planar::intersect_bounded + ParallelogramMesh::intersect (planar/mod.rs:385-418, parallelogram.rs:469-477)
+__device__ __forceinline__ bool para_t(const GPara &g, const DRay &r, float t0, float t1, float *t) {
+    const f3 n{g.nx, g.ny, g.nz};
+    const float denom = dot(n, r.d);
+    if (denom == 0.0f) return false;
+    const float tt = -(dot(n, r.o) + g.d) / denom;
+    if (!(t0 <= tt && tt <= t1)) return false;
+    const f3 pos = at(r, tt);
+    const f3 pl = pos - f3{g.px, g.py, g.pz};
+    const f3 w{g.wx, g.wy, g.wz};
+    const float alpha = dot(w, cross(pl, f3{g.vx, g.vy, g.vz}));
+    const float beta = dot(w, cross(f3{g.ux, g.uy, g.uz}, pl));
+    if (!(alpha <= 1.0f && beta <= 1.0f && alpha >= 0.0f && beta >= 0.0f)) return false;
+    *t = tt;
+    return true;
+}
+
+// AxisBoxMesh::intersect (axis_box.rs:64-145).  axis_out: the face axis hit; winding/sgn for the normals.
+__device__ __forceinline__ bool box_t(const GBox &g, const DRay &r, float t0, float t1, float *t, int *axis_out,
+                                      float *winding_out, f3 *sgn_out) {
+    const f3 ro = r.o - f3{g.cx, g.cy, g.cz};
+    const f3 rd = r.d;
+    const f3 radius{g.rx, g.ry, g.rz};
+    const float ax = fabsf(ro.x) * g.ix, ay = fabsf(ro.y) * g.iy, az = fabsf(ro.z) * g.iz;
+    const float winding = dm::signum(fmaxf(fmaxf(ax, ay), az) - 1.0f);
+    const f3 sgn{-dm::signum(rd.x), -dm::signum(rd.y), -dm::signum(rd.z)};
+    f3 pd = mul(radius * winding, sgn) - ro;
+    pd = mul(pd, r.inv);
+    const float pdv[3] = {pd.x, pd.y, pd.z};
+    const float rov[3] = {ro.x, ro.y, ro.z};
+    const float rdv[3] = {rd.x, rd.y, rd.z};
+    const float rad[3] = {radius.x, radius.y, radius.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+        const float dist = pdv[a];
+        if (!(t0 <= dist && dist <= t1)) continue;
+        const float ub = rov[b] + rdv[b] * dist, uc = rov[c] + rdv[c] * dist;
+        if (fabsf(ub) < rad[b] && fabsf(uc) < rad[c]) {
+            *t = dist;
+            *axis_out = a;
+            *winding_out = winding;
+            *sgn_out = sgn;
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool boundary_t(const SceneView &sv, const GVolume &v, const DRay &r, float t0, float t1,
+                                           float *t) {
+    if (v.boundary_kind == ITEM_SPHERE) return sphere_t(sv.spheres[v.boundary_index], r, t0, t1, t);
+    int axis;
+    float w;
+    f3 sg;
+    return box_t(sv.boxes[v.boundary_index], r, t0, t1, t, &axis, &w, &sg);
+}
+
+// VolumetricObject::full_intersect (object/volumetric.rs:86-160): entry with an unbounded interval, exit after
+// entry + 0.001, free flight -ln(U)/density.
+__device__ __forceinline__ bool volume_t(const SceneView &sv, const GVolume &v, const DRay &r, float t_end,
+                                         const dm::Key &key, uint32_t pixel, uint32_t sample, uint32_t bounce,
+                                         float *t) {
+    float entry;
+    if (!boundary_t(sv, v, r, -__builtin_inff(), __builtin_inff(), &entry)) return false;
+    entry = fmaxf(entry, T_MIN);
+    float exit_d;
+    if (!boundary_t(sv, v, r, entry + 0.001f, __builtin_inff(), &exit_d)) return false;
+    const float inside = exit_d - entry;
+    float u[4];
+    dm::draw4(key, pixel, sample, bounce, dm::SLOT_MEDIUM_BASE + (uint32_t)v.object, u);
+    const float hit_dist = v.neg_inv_density * dm::log_f(u[0]);
+    const float dist = entry + hit_dist;
+    if (hit_dist > inside) return false;
+    if (dist > t_end) return false;
+    *t = dist;
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------ SDFs
+
+__device__ __forceinline__ float ipow(float x, int n) {  // binary powering in compiler-rt's __powisf2 order
+    float r = 1.0f, a = x;
+    while (n > 0) {
+        if (n & 1) r = r * a;
+        n >>= 1;
+        if (n) a = a * a;
+    }
+    return r;
+}
+
+__device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
+    if (f.kind == RAYNA_SDF_SPHERE) {
+        const GSphere s = sv.spheres[f.first_sphere];
+        return length(p - f3{s.cx, s.cy, s.cz}) - s.radius;
+    }
+    if (f.kind == RAYNA_SDF_SMIN_SPHERES) {
+        const GSphere s0 = sv.spheres[f.first_sphere];
+        float d = length(p - f3{s0.cx, s0.cy, s0.cz}) - s0.radius;
+        for (int i = 1; i < f.n_spheres; ++i) {
+            const GSphere s = sv.spheres[f.first_sphere + i];
+            const float di = length(p - f3{s.cx, s.cy, s.cz}) - s.radius;
+            const float h = fmaxf(f.k - fabsf(d - di), 0.0f) / f.k;
+            d = fminf(d, di) - h * h * f.k * 0.25f;
+        }
+        return d;
+    }
+    // Mandelbulb distance estimate
+    const f3 p0 = p - f3{f.cx, f.cy, f.cz};
+    f3 z = p0;
+    float dr = 1.0f, r = 0.0f;
+    for (int i = 0; i < f.bulb_iterations; ++i) {
+        r = length(z);
+        if (r > f.bailout) break;
+        const float theta = dm::acos_f(z.z / r);
+        const float phi = dm::atan2_f(z.y, z.x);
+        dr = ipow(r, f.power - 1) * (float)f.power * dr + 1.0f;
+        const float zr = ipow(r, f.power);
+        float st, ct, sp, cp;
+        dm::sincos_rad(theta * (float)f.power, &st, &ct);
+        dm::sincos_rad(phi * (float)f.power, &sp, &cp);
+        z = f3{st * cp, sp * st, ct} * zr + p0;
+    }
+    return 0.5f * dm::log_f(r) * r / dr;
+}
+
+// RaymarchedIsosurfaceMesh::intersect (raymarched.rs:228-277), with the render interval [T_MIN, FLT_MAX].
+__device__ bool sdf_march(const SceneView &sv, const GSdf &f, const DRay &r, float *t, bool *front) {
+    float total = T_MIN;
+    f3 point = at(r, total);
+    int i = 0;
+    for (;;) {
+        const float dist = sdf_eval(sv, f, point);
+        total = total + fabsf(dist);
+        point = at(r, total);
+        if (fabsf(dist) < f.epsilon && T_MIN <= total && total <= FLT_MAX) {
+            *t = total;
+            *front = !signbit(dist);
+            return true;
+        }
+        if (i > f.max_iterations) return false;
+        ++i;
+    }
+}
+
+__device__ f3 sdf_normal(const SceneView &sv, const GSdf &f, f3 p) {
+    const float e = f.epsilon;
+    const f3 g{sdf_eval(sv, f, f3{p.x + e, p.y, p.z}) - sdf_eval(sv, f, f3{p.x - e, p.y, p.z}),
+               sdf_eval(sv, f, f3{p.x, p.y + e, p.z}) - sdf_eval(sv, f, f3{p.x, p.y - e, p.z}),
+               sdf_eval(sv, f, f3{p.x, p.y, p.z + e}) - sdf_eval(sv, f, f3{p.x, p.y, p.z - e})};
+    return normalize(g);
+}
+
+// ------------------------------------------------------------------------------------------ closest hit
+
+struct Best {
+    float t;
+    int32_t rank;
+    uint32_t ref;
+    uint32_t aux;
+};
+
+__device__ __forceinline__ uint32_t make_ref(int kind, int32_t index) {
+    return ((uint32_t)kind << HIT_KIND_SHIFT) | ((uint32_t)index & HIT_INDEX_MASK);
+}
+
+// Closest hit over the BVH with ties to the higher depth-first rank (the engine's shrinking inclusive
+// interval, mesh/advanced/bvh.rs:77-91), then the unbounded objects with the first of equal minima winning
+// (object/list.rs:358-363).  The traversal stack lives in LDS, one column per lane.
+template <int STACK>
+__device__ Best closest_hit(const SceneView &sv, const DRay &ray, const dm::Key &key, uint32_t pixel, uint32_t sample,
+                            uint32_t bounce, int32_t *s_node, float *s_tent) {
+    Best b{FLT_MAX, -1, 0u, 0u};
+    const int lane = threadIdx.x;
+    int sp = 0;
+    if (sv.root >= 0) {
+        s_node[lane] = sv.root;
+        s_tent[lane] = -__builtin_inff();
+        sp = 1;
+    }
+    while (sp > 0) {
+        --sp;
+        const int32_t node = s_node[sp * EXT_BLOCK + lane];
+        const float tent = s_tent[sp * EXT_BLOCK + lane];
+        if (tent > b.t * BOX_SLACK) continue;
+        const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
+        const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+        const int4 ch = reinterpret_cast<const int4 *>(np)[6];
+        const int4 inf = reinterpret_cast<const int4 *>(np)[7];
+        const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, lo_y[4] = {ly.x, ly.y, ly.z, ly.w},
+                    lo_z[4] = {lz.x, lz.y, lz.z, lz.w};
+        const float hi_x[4] = {hx.x, hx.y, hx.z, hx.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w},
+                    hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+        const int32_t child[4] = {ch.x, ch.y, ch.z, ch.w};
+        const int32_t info[4] = {inf.x, inf.y, inf.z, inf.w};
+        float tmin[4];
+        bool hit[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float tx1 = (lo_x[k] - ray.o.x) * ray.inv.x, tx2 = (hi_x[k] - ray.o.x) * ray.inv.x;
+            float t0 = fminf(tx1, tx2), t1 = fmaxf(tx1, tx2);
+            const float ty1 = (lo_y[k] - ray.o.y) * ray.inv.y, ty2 = (hi_y[k] - ray.o.y) * ray.inv.y;
+            t0 = fmaxf(t0, fminf(ty1, ty2));
+            t1 = fminf(t1, fmaxf(ty1, ty2));
+            const float tz1 = (lo_z[k] - ray.o.z) * ray.inv.z, tz2 = (hi_z[k] - ray.o.z) * ray.inv.z;
+            t0 = fmaxf(t0, fminf(tz1, tz2));
+            t1 = fminf(t1, fmaxf(tz1, tz2));
+            const float lo = fmaxf(t0, T_MIN), hi = fminf(t1, b.t);
+            tmin[k] = t0;
+            hit[k] = info[k] != INFO_EMPTY && lo <= hi * BOX_SLACK;
+        }
+        // leaf ranges first, so the distance bound tightens before inner children are ordered
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!hit[k] || info[k] <= 0) continue;
+            const int cnt = info[k] & 0xf;
+            const int start = child[k];
+            if ((info[k] >> 4) == LEAF_TRIS) {
+                for (int i = 0; i < cnt; ++i) {
+                    const GTri g = sv.tris[start + i];
+                    float t, u, v, det;
+                    if (!tri_t(g, ray, T_MIN, b.t, &t, &u, &v, &det)) continue;
+                    if (!(t < b.t || g.rank > b.rank)) continue;
+                    if (g.flags & 1) {
+                        f3 nn;
+                        if (!tri_normal(sv.tri_normals[start + i], u, v, &nn)) continue;
+                    }
+                    b = Best{t, g.rank, make_ref(ITEM_TRIANGLE, start + i), 0u};
+                }
+            } else {
+                for (int i = 0; i < cnt; ++i) {
+                    const GItem it = sv.items[start + i];
+                    float t;
+                    bool ok = false;
+                    uint32_t ref = 0;
+                    switch (it.kind) {
+                        case ITEM_SPHERE:
+                            ok = sphere_t(sv.spheres[it.index], ray, T_MIN, b.t, &t);
+                            ref = make_ref(ITEM_SPHERE, start + i);
+                            break;
+                        case ITEM_TRIANGLE: {
+                            const GTri g = sv.tris[it.index];
+                            float u, v, det;
+                            ok = tri_t(g, ray, T_MIN, b.t, &t, &u, &v, &det);
+                            if (ok && (g.flags & 1)) {
+                                f3 nn;
+                                ok = tri_normal(sv.tri_normals[it.index], u, v, &nn);
+                            }
+                            ref = make_ref(ITEM_TRIANGLE, it.index);
+                            break;
+                        }
+                        case ITEM_PARALLELOGRAM:
+                            ok = para_t(sv.paras[it.index], ray, T_MIN, b.t, &t);
+                            ref = make_ref(ITEM_PARALLELOGRAM, start + i);
+                            break;
+                        case ITEM_AXIS_BOX: {
+                            int axis;
+                            float w;
+                            f3 sg;
+                            ok = box_t(sv.boxes[it.index], ray, T_MIN, b.t, &t, &axis, &w, &sg);
+                            ref = make_ref(ITEM_AXIS_BOX, start + i);
+                            break;
+                        }
+                        case ITEM_VOLUME:
+                            ok = volume_t(sv, sv.volumes[it.index], ray, b.t, key, pixel, sample, bounce, &t);
+                            ref = make_ref(ITEM_VOLUME, start + i);
+                            break;
+                        case ITEM_LINK:
+                            if (sp < STACK) {
+                                s_node[sp * EXT_BLOCK + lane] = it.index;
+                                s_tent[sp * EXT_BLOCK + lane] = tmin[k];
+                                ++sp;
+                            }
+                            break;
+                        default: break;
+                    }
+                    if (ok && (t < b.t || it.rank > b.rank)) b = Best{t, it.rank, ref, 0u};
+                }
+            }
+        }
+        // inner children: push far to near so the nearest is popped first (4-input sorting network,
+        // descending distance; missed or non-inner slots sort last with key -inf)
+        int32_t cn[4];
+        float ct[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool inner = hit[k] && info[k] == INFO_INNER;
+            cn[k] = inner ? child[k] : -1;
+            ct[k] = inner ? tmin[k] : -__builtin_inff();
+        }
+#define RAYNA_CSWAP(a, b)                                      \
+    if (ct[a] < ct[b]) {                                       \
+        const float tt_ = ct[a];                               \
+        ct[a] = ct[b];                                         \
+        ct[b] = tt_;                                           \
+        const int32_t nn_ = cn[a];                             \
+        cn[a] = cn[b];                                         \
+        cn[b] = nn_;                                           \
+    }
+        RAYNA_CSWAP(0, 1)
+        RAYNA_CSWAP(2, 3)
+        RAYNA_CSWAP(0, 2)
+        RAYNA_CSWAP(1, 3)
+        RAYNA_CSWAP(1, 2)
+#undef RAYNA_CSWAP
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (cn[k] >= 0 && sp < STACK) {
+                s_node[sp * EXT_BLOCK + lane] = cn[k];
+                s_tent[sp * EXT_BLOCK + lane] = ct[k];
+                ++sp;
+            }
+        }
+    }
+    for (int i = 0; i < sv.n_sdfs; ++i) {
+        const GSdf f = sv.sdfs[i];
+        float t;
+        bool front;
+        if (sdf_march(sv, f, ray, &t, &front) && t < b.t)
+            b = Best{t, -2 - f.list_pos, make_ref(ITEM_SDF, i), front ? 0u : HIT_AUX_BACK};
+    }
+    return b;
+}
+
+// ------------------------------------------------------------------------------------------ shading
+
+struct SHit {
+    f3 pos, normal, ray_normal;
+    bool front;
+    int32_t material;
+};
+
+__device__ __forceinline__ f3 unit_sphere(float r1, float r2) {  // rng::normal_on_unit_sphere (rng.rs:75-87)
+    float sn, cs;
+    dm::sincos_turns(r1, &sn, &cs);
+    const float q = sqrtf(r2 * (1.0f - r2));
+    return f3{(cs * 2.0f) * q, (sn * 2.0f) * q, 1.0f - 2.0f * r2};
+}
+
+// Re-derives the full intersection record of a hit (the extend kernel keeps only distance and reference).
+__device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint32_t ref, uint32_t aux,
+                             const dm::Key &key, uint32_t pixel, uint32_t sample, uint32_t bounce, SHit *h) {
+    const int kind = (int)(ref >> HIT_KIND_SHIFT);
+    const int32_t idx = (int32_t)(ref & HIT_INDEX_MASK);
+    h->pos = at(ray, t);
+    switch (kind) {
+        case ITEM_TRIANGLE: {
+            const GTri g = sv.tris[idx];
+            float tt, u, v, det;
+            tri_t(g, ray, t, t, &tt, &u, &v, &det);
+            f3 n{0.0f, 0.0f, 0.0f};
+            tri_normal(sv.tri_normals[idx], u, v, &n);
+            h->normal = n;
+            h->front = signbit(det);
+            h->ray_normal = n * -dm::signum(det);
+            h->material = g.material;
+            break;
+        }
+        case ITEM_SPHERE: {
+            const GItem it = sv.items[idx];
+            const GSphere s = sv.spheres[it.index];
+            const f3 local = (h->pos - f3{s.cx, s.cy, s.cz}) / s.radius;
+            const bool inside = dot(ray.d, local) > 0.0f;
+            h->normal = local;
+            h->ray_normal = inside ? neg(local) : local;
+            h->front = !inside;
+            h->material = it.material;
+            break;
+        }
+        case ITEM_PARALLELOGRAM: {
+            const GItem it = sv.items[idx];
+            const GPara g = sv.paras[it.index];
+            const f3 n{g.nx, g.ny, g.nz};
+            const float denom = dot(n, ray.d);
+            h->normal = n;
+            h->front = signbit(denom);
+            h->ray_normal = neg(n) * dm::signum(denom);
+            h->material = it.material;
+            break;
+        }
+        case ITEM_AXIS_BOX: {
+            const GItem it = sv.items[idx];
+            float tt, w;
+            int axis = 0;
+            f3 sg{0.0f, 0.0f, 0.0f};
+            box_t(sv.boxes[it.index], ray, t, t, &tt, &axis, &w, &sg);
+            f3 rn{0.0f, 0.0f, 0.0f};
+            if (axis == 0) rn.x = sg.x;
+            if (axis == 1) rn.y = sg.y;
+            if (axis == 2) rn.z = sg.z;
+            h->ray_normal = rn;
+            h->normal = rn * w;
+            h->front = !signbit(w);
+            h->material = it.material;
+            break;
+        }
+        case ITEM_VOLUME: {
+            const GItem it = sv.items[idx];
+            float a[4];
+            dm::draw4(key, pixel, sample, bounce, dm::SLOT_MEDIUM_AUX, a);
+            h->normal = unit_sphere(a[0], a[1]);
+            h->ray_normal = unit_sphere(a[2], a[3]);
+            h->front = true;
+            h->material = it.material;
+            break;
+        }
+        default: {  // ITEM_SDF
+            const GSdf f = sv.sdfs[idx];
+            const f3 n = sdf_normal(sv, f, h->pos);
+            h->normal = n;
+            h->ray_normal = n;
+            h->front = !(aux & HIT_AUX_BACK);
+            h->material = f.material;
+            break;
+        }
+    }
+}
+
+// Material::scatter (material/*.rs); false for "no scatter".
+__device__ __forceinline__ bool scatter(const GMaterial &m, const DRay &ray, const SHit &h, const float u[4], f3 *dir) {
+    switch (m.kind) {
+        case RAYNA_MAT_LAMBERTIAN: {  // lambertian.rs:29-36
+            const f3 rnd = unit_sphere(u[0], u[1]) * dm::cbrt01(u[2]);
+            const f3 v = h.ray_normal + rnd;
+            if (!try_normalize(v, dir)) *dir = h.ray_normal;
+            return true;
+        }
+        case RAYNA_MAT_METAL: {  // metal.rs:66-82
+            const f3 refl = reflect(ray.d, h.ray_normal);
+            const f3 v = refl + unit_sphere(u[0], u[1]) * m.fuzz;
+            if (dot(v, h.ray_normal) > 0.0f) {
+                *dir = normalize(v);
+                return true;
+            }
+            return false;
+        }
+        case RAYNA_MAT_DIELECTRIC: {  // dielectric.rs:114-134, 169-174
+            const float ratio = h.front ? 1.0f / m.ior : m.ior;
+            const float cos_t = fminf(dot(neg(ray.d), h.ray_normal), 1.0f);
+            const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+            const bool tir = ratio * sin_t > 1.0f;
+            const float r0 = (1.0f - ratio) / (1.0f + ratio);
+            const float r0s = r0 * r0;
+            const float refl = r0s + (1.0f - r0s) * ipow(1.0f - cos_t, 5);
+            const bool schlick = refl > u[0];
+            *dir = (tir || schlick) ? reflect(ray.d, h.ray_normal) : refract(ray.d, h.ray_normal, ratio);
+            return true;
+        }
+        case RAYNA_MAT_ISOTROPIC:  // isotropic.rs:204-206
+            *dir = unit_sphere(u[0], u[1]);
+            return true;
+        default:
+            return false;
+    }
+}
+
+// Material::reflected_light applied to the path throughput (future colour) T.
+__device__ __forceinline__ Col reflected(const GMaterial &m, const DRay &ray, const SHit &h, Col T) {
+    const Col albedo{m.r, m.g, m.b};
+    switch (m.kind) {
+        case RAYNA_MAT_LAMBERTIAN:
+        case RAYNA_MAT_METAL: return T * albedo;
+        case RAYNA_MAT_DIELECTRIC: {
+            if (h.front) return T;
+            const float dist = length(h.pos - ray.o);
+            const float tr = -m.density * dist;
+            return (T * albedo) * dm::exp_f(tr);
+        }
+        case RAYNA_MAT_ISOTROPIC: {
+            const float dist = length(h.pos - ray.o);
+            const float tr = -m.density * dist;
+            return (T * albedo) * dm::exp_f(tr);
+        }
+        default: return Col{0.0f, 0.0f, 0.0f};
+    }
+}
+
+__device__ __forceinline__ Col sky(int kind, f3 d) {  // skybox/simple.rs, skybox/none.rs
+    if (kind == RAYNA_SKY_WHITE) return Col{1.0f, 1.0f, 1.0f};
+    if (kind == RAYNA_SKY_SIMPLE) {
+        const float a = 0.5f * d.y + 0.5f;
+        return Col{1.0f + (0.5f - 1.0f) * a, 1.0f + (0.7f - 1.0f) * a, 1.0f + (1.0f - 1.0f) * a};
+    }
+    return Col{0.0f, 0.0f, 0.0f};
+}
+
+// ------------------------------------------------------------------------------------------ kernels
+
+// Camera ray of every slot of the chunk: render_px_msaa's stratified jitter (renderer.rs:370-389) and
+// Viewport::calc_ray (camera.rs:362-384); throughput 1, radiance 0.
+__global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers pb) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= fp.n_slots) return;
+    const uint32_t pixel = i % fp.n_pixels;
+    const uint32_t k = fp.s0 + i / fp.n_pixels;
+    const uint32_t x = pixel % fp.width, y = pixel / fp.width;
+    float u[4];
+    dm::draw4(dm::Key{fp.key0, fp.key1}, pixel, k, 0, dm::SLOT_CAMERA, u);
+    const uint32_t s = fp.sdim;
+    const float jx = u[0] - 0.5f, jy = u[1] - 0.5f;
+    float cx, cy;
+    if (k < s * s) {
+        const float inv = 1.0f / (float)s;
+        const uint32_t ii = k % s, jj = k / s;
+        cx = ((float)x + jx * inv) + (float)ii * inv;
+        cy = ((float)y + jy * inv) + (float)jj * inv;
+    } else {
+        cx = (float)x + jx;
+        cy = (float)y + jy;
+    }
+    const float W = (float)fp.width, H = (float)fp.height;
+    const float uu = (cx - W / 2.0f) / H;
+    const float vv = (cy - H / 2.0f) / H;
+    const GViewport &vp = fp.vp;
+    const f3 pc{vp.pixel_center[0], vp.pixel_center[1], vp.pixel_center[2]};
+    const f3 vu{vp.vu[0], vp.vu[1], vp.vu[2]}, vvv{vp.vv[0], vp.vv[1], vp.vv[2]};
+    const f3 pix = (pc + vu * uu) + vvv * vv;
+    float sn, cs;
+    dm::sincos_turns(u[3], &sn, &cs);
+    const float rr = sqrtf(u[2]);
+    const float dx = rr * cs, dy = rr * sn;
+    const f3 pos{vp.pos[0], vp.pos[1], vp.pos[2]};
+    const f3 du{vp.du[0], vp.du[1], vp.du[2]}, dv{vp.dv[0], vp.dv[1], vp.dv[2]};
+    const f3 org = (pos + du * dx) + dv * dy;
+    const DRay r = make_ray(org, pix - org);
+    pb.ox[i] = r.o.x;
+    pb.oy[i] = r.o.y;
+    pb.oz[i] = r.o.z;
+    pb.dx[i] = r.d.x;
+    pb.dy[i] = r.d.y;
+    pb.dz[i] = r.d.z;
+    pb.tr[i] = 1.0f;
+    pb.tg[i] = 1.0f;
+    pb.tb[i] = 1.0f;
+    pb.lr[i] = 0.0f;
+    pb.lg[i] = 0.0f;
+    pb.lb[i] = 0.0f;
+}
+
+// Closest hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501).
+template <int STACK>
+__global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams fp, PathBuffers pb,
+                                                      const int32_t *queue, const uint32_t *count_ptr,
+                                                      uint32_t depth) {
+    __shared__ int32_t s_node[STACK * EXT_BLOCK];
+    __shared__ float s_tent[STACK * EXT_BLOCK];
+    const uint32_t count = queue ? *count_ptr : fp.n_slots;
+    const dm::Key key{fp.key0, fp.key1};
+    for (uint32_t q = blockIdx.x * EXT_BLOCK + threadIdx.x; q < count; q += gridDim.x * EXT_BLOCK) {
+        const uint32_t slot = queue ? (uint32_t)queue[q] : q;
+        const DRay ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+        const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
+        const Best b = closest_hit<STACK>(sv, ray, key, pixel, sample, depth, s_node, s_tent);
+        pb.hit_t[slot] = b.t;
+        pb.hit_ref[slot] = b.ref;
+        pb.hit_aux[slot] = b.aux;
+        if (pb.hit_rank) pb.hit_rank[slot] = b.rank;
+    }
+}
+
+// One bounce of ray_colour_recursive in forward form: L += T * Le; scatter; T = reflected(T); paths that
+// scatter below the depth limit are compacted into the next queue (one atomic per wave).
+__global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, PathBuffers pb,
+                                                 const int32_t *queue_in, const uint32_t *count_in,
+                                                 int32_t *queue_out, uint32_t *count_out, uint32_t depth) {
+    const uint32_t count = queue_in ? *count_in : fp.n_slots;
+    const dm::Key key{fp.key0, fp.key1};
+    const uint32_t lane = __lane_id();
+    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
+        const uint32_t q = base + threadIdx.x;
+        bool keep = false;
+        uint32_t slot = 0;
+        if (q < count) {
+            slot = queue_in ? (uint32_t)queue_in[q] : q;
+            const DRay ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
+                                          f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+            Col T{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
+            Col L{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
+            const uint32_t ref = pb.hit_ref[slot];
+            if (ref == 0u) {
+                L = L + T * sky(sv.skybox, ray.d);
+            } else {
+                const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
+                SHit h;
+                hit_geometry(sv, ray, pb.hit_t[slot], ref, pb.hit_aux[slot], key, pixel, sample, depth, &h);
+                const GMaterial m = sv.materials[h.material];
+                const Col le = (m.kind == RAYNA_MAT_LIGHT) ? Col{m.r, m.g, m.b} : Col{0.0f, 0.0f, 0.0f};
+                L = L + T * le;
+                float u[4];
+                dm::draw4(key, pixel, sample, depth, dm::SLOT_SCATTER, u);
+                f3 dir;
+                if (scatter(m, ray, h, u, &dir)) {
+                    T = reflected(m, ray, h, T);
+                    if (depth < fp.ray_depth) {
+                        const DRay nr = make_ray(h.pos, dir);
+                        pb.ox[slot] = nr.o.x;
+                        pb.oy[slot] = nr.o.y;
+                        pb.oz[slot] = nr.o.z;
+                        pb.dx[slot] = nr.d.x;
+                        pb.dy[slot] = nr.d.y;
+                        pb.dz[slot] = nr.d.z;
+                        pb.tr[slot] = T.r;
+                        pb.tg[slot] = T.g;
+                        pb.tb[slot] = T.b;
+                        keep = true;
+                    }
+                }
+            }
+            pb.lr[slot] = L.r;
+            pb.lg[slot] = L.g;
+            pb.lb[slot] = L.b;
+        }
+        const unsigned long long mask = __ballot(keep);
+        if (mask != 0ull) {
+            const uint32_t n = (uint32_t)__popcll(mask);
+            const int leader = __ffsll((long long)mask) - 1;
+            uint32_t wbase = 0;
+            if ((int)lane == leader) wbase = atomicAdd(count_out, n);
+            wbase = __shfl(wbase, leader);
+            if (keep) {
+                const uint32_t off = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+                queue_out[wbase + off] = (int32_t)slot;
+            }
+        }
+    }
+}
+
+// Per-pixel running sum of the chunk's samples in sample order (render_px_msaa's sum, renderer.rs:398-402).
+__global__ __launch_bounds__(BLOCK) void k_resolve(FrameParams fp, PathBuffers pb, float *sum, uint32_t chunk_samples,
+                                                   int first) {
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= fp.n_pixels) return;
+    const uint32_t P = fp.n_pixels;
+    float r = first ? 0.0f : sum[p], g = first ? 0.0f : sum[P + p], b = first ? 0.0f : sum[2 * P + p];
+    for (uint32_t s = 0; s < chunk_samples; ++s) {
+        const uint32_t slot = s * P + p;
+        r = r + pb.lr[slot];
+        g = g + pb.lg[slot];
+        b = b + pb.lb[slot];
+    }
+    sum[p] = r;
+    sum[P + p] = g;
+    sum[2 * P + p] = b;
+}
+
+// Sample mean, accumulation buffer update and output image (accum_buffer.rs:31-45).
+__global__ __launch_bounds__(BLOCK) void k_finalize(FrameParams fp, const float *sum, float *acc, double frames,
+                                                    float *out, int layout) {
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= fp.n_pixels) return;
+    const uint32_t P = fp.n_pixels;
+    const float n = (float)fp.spp;
+    const float mean[3] = {sum[p] / n, sum[P + p] / n, sum[2 * P + p] / n};
+    const uint32_t x = p % fp.width, y = p / fp.width;
+    const size_t o = layout == RAYNA_LAYOUT_XMAJOR ? ((size_t)x * fp.height + y) * 3 : (size_t)p * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float a = acc[c * P + p] + mean[c];
+        acc[c * P + p] = a;
+        out[o + c] = (float)((double)a / frames);
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t value, uint32_t n) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ void k_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
+                                 unsigned long long *segments) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned long long s = first_count;
+        for (uint32_t i = 1; i < n; ++i) s += queue_count[i];
+        *segments += s;
+    }
+}
+
+inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace
+
+int extend_stack_capacity() { return 64; }
+
+void launch_generate(const FrameParams &fp, const PathBuffers &pb, hipStream_t s) {
+    hipLaunchKernelGGL(k_generate, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb);
+}
+
+void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
+                   const uint32_t *count, uint32_t depth, int grid, hipStream_t s) {
+    if (sv.stack_need <= 32)
+        hipLaunchKernelGGL(k_extend<32>, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth);
+    else
+        hipLaunchKernelGGL(k_extend<64>, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth);
+}
+
+void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
+                  const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in, queue_out, count_out,
+                       depth);
+}
+
+void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, uint32_t chunk_samples, int first,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_resolve, dim3(grid_for(fp.n_pixels, BLOCK)), dim3(BLOCK), 0, s, fp, pb, sum, chunk_samples,
+                       first);
+}
+
+void launch_finalize(const FrameParams &fp, const float *sum, float *acc, double frames, float *out, int layout,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize, dim3(grid_for(fp.n_pixels, BLOCK)), dim3(BLOCK), 0, s, fp, sum, acc, frames, out,
+                       layout);
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, BLOCK)), dim3(BLOCK), 0, s, p, value, n);
+}
+
+void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
+                           unsigned long long *segments, hipStream_t s) {
+    hipLaunchKernelGGL(k_count_segments, dim3(1), dim3(64), 0, s, queue_count, n, first_count, segments);
+}
+
+}  // namespace rayna

@@ -1,0 +1,58 @@
+// Interface between the host renderer and the wavefront kernels of kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "layout.hpp"
+
+namespace rayna {
+
+// Path state of one chunk of samples, structure-of-arrays in HBM.  Slot i of a chunk is sample
+// s0 + i / n_pixels of pixel i % n_pixels, so consecutive slots are neighbouring pixels.
+struct PathBuffers {
+    float *ox, *oy, *oz;     // ray origin
+    float *dx, *dy, *dz;     // ray direction (unit)
+    float *tr, *tg, *tb;     // path throughput
+    float *lr, *lg, *lb;     // radiance gathered so far; the sample's value once the path ends
+    float *hit_t;            // closest-hit distance
+    uint32_t *hit_ref;       // kind << 28 | index, 0 on a miss
+    uint32_t *hit_aux;       // HIT_AUX_* bits
+    int32_t *hit_rank;       // optional (parity queries): rank of the closest primitive
+};
+
+struct FrameParams {
+    GViewport vp;
+    uint32_t width, height;
+    uint32_t spp;            // samples per pixel of the frame
+    uint32_t sdim;           // isqrt(spp): the stratified grid is sdim x sdim
+    uint32_t key0, key1;     // Philox key
+    uint32_t ray_depth;
+    uint32_t n_pixels;
+    uint32_t s0;             // first sample of the chunk
+    uint32_t n_slots;        // slots in the chunk
+};
+
+struct Counters {
+    uint32_t *queue_count;   // [ray_depth + 2]: live paths entering each depth of the current chunk
+    unsigned long long *segments;  // running total of closest-hit queries
+};
+
+void launch_generate(const FrameParams &fp, const PathBuffers &pb, hipStream_t s);
+// queue == nullptr: identity queue of fp.n_slots entries; else queue[0 .. *count)
+void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
+                   const uint32_t *count, uint32_t depth, int grid, hipStream_t s);
+void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
+                  const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
+                  hipStream_t s);
+void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, uint32_t chunk_samples, int first,
+                    hipStream_t s);
+void launch_finalize(const FrameParams &fp, const float *sum, float *acc, double frames, float *out, int layout,
+                     hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s);
+void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
+                           unsigned long long *segments, hipStream_t s);
+
+int extend_stack_capacity();
+
+}  // namespace rayna

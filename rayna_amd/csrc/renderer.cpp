@@ -1,0 +1,509 @@
+// The C ABI of include/rayna_gpu.h: a drop-in for rayna_engine's Renderer (render/renderer.rs:36-264) that runs
+// the per-pixel path tracing as wavefront HIP kernels on one gfx950 GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rayna_gpu.h"
+#include "kernels.hpp"
+#include "scene_build.hpp"
+
+using namespace rayna;
+
+namespace {
+
+std::string g_create_error;
+
+constexpr uint64_t TARGET_SLOTS = 1ull << 23;  // paths in flight per chunk (~8.4 M; ~0.6 GB of path state)
+
+struct DevScene {
+    std::vector<void *> allocs;
+    SceneView view{};
+};
+
+}  // namespace
+
+struct rayna_renderer {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    PackedScene packed;
+    DevScene dev;
+    rayna_camera cam{};
+    rayna_render_opts opts{};
+    bool cam_valid = false;
+    GViewport vp{};
+
+    // path state
+    uint64_t slot_capacity = 0;
+    uint32_t chunk_samples = 1;
+    std::vector<void *> path_allocs;
+    PathBuffers pb{};
+    int32_t *queue[2] = {nullptr, nullptr};
+    uint32_t *queue_count = nullptr;
+    unsigned long long *segments = nullptr;
+    float *sum = nullptr, *acc = nullptr, *out_dev = nullptr;
+    uint32_t alloc_pixels = 0;
+
+    uint64_t frames = 0;  // AccumulationBuffer::frame_count (accum_buffer.rs:77)
+    bool acc_dirty = true;
+    int grid_extend = 1024, grid_shade = 1024;
+    std::vector<hipEvent_t> events;
+    std::string error;
+};
+
+namespace {
+
+rayna_status fail(rayna_renderer *r, rayna_status st, const std::string &msg) {
+    if (r) r->error = msg;
+    else g_create_error = msg;
+    return st;
+}
+
+#define HIP_TRY(r, expr)                                                                                      \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return fail((r), e_ == hipErrorOutOfMemory ? RAYNA_OUT_OF_MEMORY : RAYNA_DEVICE_ERROR,           \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                                   \
+    } while (0)
+
+// Camera::calculate_viewport (scene/camera.rs:285-338) in double precision, rounded to float.
+bool viewport(const rayna_camera &c, GViewport &vp) {
+    const double focal = c.focus_dist;
+    if (c.v_fov == 0.0 || focal == 0.0) return false;
+    const double fl = std::sqrt((c.fwd[0] * c.fwd[0] + c.fwd[1] * c.fwd[1]) + c.fwd[2] * c.fwd[2]);
+    const double rf = 1.0 / fl;
+    if (!std::isfinite(rf) || !(rf > 0)) return false;
+    const double w[3] = {-(c.fwd[0] * rf), -(c.fwd[1] * rf), -(c.fwd[2] * rf)};
+    // u = cross(Y, w)
+    const double cu[3] = {1.0 * w[2] - 0.0 * w[1], 0.0 * w[0] - 0.0 * w[2], 0.0 * w[1] - 1.0 * w[0]};
+    const double ul = std::sqrt((cu[0] * cu[0] + cu[1] * cu[1]) + cu[2] * cu[2]);
+    const double ru = 1.0 / ul;
+    if (!std::isfinite(ru) || !(ru > 0)) return false;
+    const double u[3] = {cu[0] * ru, cu[1] * ru, cu[2] * ru};
+    const double v[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]};
+    const double defocus_radius = focal * std::tan(c.defocus_angle / 2.0);
+    const double hh = std::tan(c.v_fov / 2.0);
+    const double size = 2.0 * hh * focal;
+    for (int i = 0; i < 3; ++i) {
+        vp.pos[i] = (float)c.pos[i];
+        vp.pixel_center[i] = (float)(c.pos[i] - w[i] * focal);
+        vp.du[i] = (float)(u[i] * defocus_radius);
+        vp.dv[i] = (float)(v[i] * defocus_radius);
+        vp.vu[i] = (float)(u[i] * size);
+        vp.vv[i] = (float)(-v[i] * size);
+    }
+    return true;
+}
+
+uint32_t isqrt(uint32_t n) {
+    uint32_t s = (uint32_t)std::sqrt((double)n);
+    while (s * s > n) --s;
+    while ((s + 1) * (s + 1) <= n) ++s;
+    return s;
+}
+
+template <class T>
+rayna_status upload(rayna_renderer *r, DevScene &ds, const std::vector<T> &v, const T **out) {
+    *out = nullptr;
+    if (v.empty()) return RAYNA_OK;
+    void *p = nullptr;
+    HIP_TRY(r, hipMalloc(&p, v.size() * sizeof(T)));
+    ds.allocs.push_back(p);
+    HIP_TRY(r, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<const T *>(p);
+    return RAYNA_OK;
+}
+
+void free_scene(DevScene &ds) {
+    for (void *p : ds.allocs) (void)hipFree(p);
+    ds.allocs.clear();
+    ds.view = SceneView{};
+}
+
+rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
+    PackedScene ps;
+    std::string err;
+    const rayna_status st = pack_scene(*d, ps, err);
+    if (st != RAYNA_OK) return fail(r, st, err);
+    if (ps.stack_need > extend_stack_capacity())
+        return fail(r, RAYNA_UNSUPPORTED, "scene BVH is deeper than the traversal stack supports");
+    DevScene ds;
+    SceneView &v = ds.view;
+    rayna_status s2;
+#define UP(vec, field) \
+    if ((s2 = upload(r, ds, ps.vec, &v.field)) != RAYNA_OK) { free_scene(ds); return s2; }
+    UP(nodes, nodes)
+    UP(tris, tris)
+    UP(tri_normals, tri_normals)
+    UP(items, items)
+    UP(spheres, spheres)
+    UP(paras, paras)
+    UP(boxes, boxes)
+    UP(volumes, volumes)
+    UP(sdfs, sdfs)
+    UP(materials, materials)
+#undef UP
+    v.root = ps.root;
+    v.n_sdfs = (int32_t)ps.sdfs.size();
+    v.skybox = ps.skybox;
+    v.stack_need = ps.stack_need;
+    free_scene(r->dev);
+    r->dev = ds;
+    r->packed = std::move(ps);
+    return RAYNA_OK;
+}
+
+void free_paths(rayna_renderer *r) {
+    for (void *p : r->path_allocs) (void)hipFree(p);
+    r->path_allocs.clear();
+    r->pb = PathBuffers{};
+    r->queue[0] = r->queue[1] = nullptr;
+    r->queue_count = nullptr;
+    r->segments = nullptr;
+    r->sum = r->acc = r->out_dev = nullptr;
+    r->slot_capacity = 0;
+    r->alloc_pixels = 0;
+}
+
+rayna_status alloc_paths(rayna_renderer *r) {
+    const rayna_render_opts &o = r->opts;
+    const uint64_t P = (uint64_t)o.width * o.height;
+    uint64_t chunk = std::max<uint64_t>(1, TARGET_SLOTS / std::max<uint64_t>(1, P));
+    chunk = std::min<uint64_t>(chunk, o.samples);
+    const uint64_t slots = P * chunk;
+    if (slots > 0x7fffffffull) return fail(r, RAYNA_UNSUPPORTED, "image too large for one chunk of paths");
+    free_paths(r);
+    auto alloc = [&](size_t bytes, void **out) -> rayna_status {
+        HIP_TRY(r, hipMalloc(out, std::max<size_t>(bytes, 4)));
+        r->path_allocs.push_back(*out);
+        return RAYNA_OK;
+    };
+    rayna_status st;
+    float **fields[] = {&r->pb.ox, &r->pb.oy, &r->pb.oz, &r->pb.dx, &r->pb.dy, &r->pb.dz, &r->pb.tr,
+                        &r->pb.tg, &r->pb.tb, &r->pb.lr, &r->pb.lg, &r->pb.lb, &r->pb.hit_t};
+    for (float **f : fields)
+        if ((st = alloc(slots * sizeof(float), (void **)f)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&r->pb.hit_ref)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&r->pb.hit_aux)) != RAYNA_OK) return st;
+    for (int q = 0; q < 2; ++q)
+        if ((st = alloc(slots * sizeof(int32_t), (void **)&r->queue[q])) != RAYNA_OK) return st;
+    if ((st = alloc((o.ray_depth + 2) * sizeof(uint32_t), (void **)&r->queue_count)) != RAYNA_OK) return st;
+    if ((st = alloc(sizeof(unsigned long long), (void **)&r->segments)) != RAYNA_OK) return st;
+    if ((st = alloc(3 * P * sizeof(float), (void **)&r->sum)) != RAYNA_OK) return st;
+    if ((st = alloc(3 * P * sizeof(float), (void **)&r->acc)) != RAYNA_OK) return st;
+    if ((st = alloc(3 * P * sizeof(float), (void **)&r->out_dev)) != RAYNA_OK) return st;
+    r->pb.hit_rank = nullptr;
+    r->slot_capacity = slots;
+    r->chunk_samples = (uint32_t)chunk;
+    r->alloc_pixels = (uint32_t)P;
+    r->acc_dirty = true;
+    return RAYNA_OK;
+}
+
+rayna_status check_opts(rayna_renderer *r, const rayna_render_opts *o) {
+    if (!o || o->width == 0 || o->height == 0 || o->samples == 0 || o->ray_branching == 0)
+        return fail(r, RAYNA_INVALID_ARGUMENT, "render options must have non-zero width, height, samples, branching");
+    if (o->ray_branching != 1) return fail(r, RAYNA_UNSUPPORTED, "ray_branching > 1 is not supported on the GPU");
+    if (o->mode != RAYNA_MODE_PBR) return fail(r, RAYNA_UNSUPPORTED, "only RenderMode::PBR is supported");
+    if (o->ray_depth > 1000) return fail(r, RAYNA_UNSUPPORTED, "ray_depth above 1000");
+    return RAYNA_OK;
+}
+
+rayna_status set_grid(rayna_renderer *r) {
+    hipDeviceProp_t prop;
+    HIP_TRY(r, hipGetDeviceProperties(&prop, r->device));
+    const int cus = std::max(1, prop.multiProcessorCount);
+    r->grid_extend = cus * 8;
+    r->grid_shade = cus * 8;
+    return RAYNA_OK;
+}
+
+FrameParams frame_params(const rayna_renderer *r, uint64_t frame_index) {
+    FrameParams fp;
+    fp.vp = r->vp;
+    fp.width = r->opts.width;
+    fp.height = r->opts.height;
+    fp.spp = r->opts.samples;
+    fp.sdim = isqrt(r->opts.samples);
+    fp.key0 = (uint32_t)r->opts.seed;
+    fp.key1 = (uint32_t)(r->opts.seed >> 32) ^ (uint32_t)frame_index;
+    fp.ray_depth = r->opts.ray_depth;
+    fp.n_pixels = r->opts.width * r->opts.height;
+    fp.s0 = 0;
+    fp.n_slots = 0;
+    return fp;
+}
+
+// Trace the samples [s0, s0 + n) of every pixel: generate, then extend/shade per depth.
+void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, std::vector<hipEvent_t> *ev, size_t *evi,
+                 uint64_t *extend_launches) {
+    fp.s0 = s0;
+    fp.n_slots = fp.n_pixels * n;
+    launch_fill_u32(r->queue_count, 0u, r->opts.ray_depth + 2, r->stream);
+    launch_generate(fp, r->pb, r->stream);
+    for (uint32_t depth = 0; depth <= r->opts.ray_depth; ++depth) {
+        const int32_t *qin = depth == 0 ? nullptr : r->queue[depth & 1];
+        const uint32_t *cin = depth == 0 ? nullptr : r->queue_count + depth;
+        int32_t *qout = r->queue[(depth + 1) & 1];
+        uint32_t *cout = r->queue_count + depth + 1;
+        if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
+        launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->stream);
+        if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
+        launch_shade(r->dev.view, fp, r->pb, qin, cin, qout, cout, depth, r->grid_shade, r->stream);
+        if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
+        ++*extend_launches;
+    }
+    launch_count_segments(r->queue_count, r->opts.ray_depth + 1, fp.n_slots, r->segments, r->stream);
+}
+
+void write_checkerboard(uint32_t w, uint32_t h, float *out, int layout) {  // renderer.rs:269-292
+    for (uint32_t x = 0; x < w; ++x)
+        for (uint32_t y = 0; y < h; ++y) {
+            const size_t o = layout == RAYNA_LAYOUT_XMAJOR ? ((size_t)x * h + y) * 3 : ((size_t)y * w + x) * 3;
+            const bool black = (x + y) % 2 == 0;
+            out[o] = black ? 0.0f : 1.0f;
+            out[o + 1] = 0.0f;
+            out[o + 2] = black ? 0.0f : 1.0f;
+        }
+}
+
+uint32_t ilog2(uint32_t v) {
+    uint32_t r = 0;
+    while (v >>= 1) ++r;
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+rayna_status rayna_create(const rayna_scene_desc *scene, const rayna_camera *camera, const rayna_render_opts *opts,
+                          rayna_renderer **out) {
+    g_create_error.clear();
+    if (!scene || !camera || !opts || !out) return fail(nullptr, RAYNA_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    rayna_renderer *r = new rayna_renderer();
+    auto bail = [&](rayna_status st) {
+        g_create_error = r->error;
+        rayna_destroy(r);
+        return st;
+    };
+    rayna_status st = check_opts(r, opts);
+    if (st != RAYNA_OK) return bail(st);
+    if (hipGetDevice(&r->device) != hipSuccess) {
+        r->error = "no HIP device";
+        return bail(RAYNA_DEVICE_ERROR);
+    }
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+        r->error = "hipStreamCreate failed";
+        return bail(RAYNA_DEVICE_ERROR);
+    }
+    if ((st = set_grid(r)) != RAYNA_OK) return bail(st);
+    r->opts = *opts;
+    r->cam = *camera;
+    r->cam_valid = viewport(r->cam, r->vp);
+    if ((st = load_scene(r, scene)) != RAYNA_OK) return bail(st);
+    if ((st = alloc_paths(r)) != RAYNA_OK) return bail(st);
+    *out = r;
+    return RAYNA_OK;
+}
+
+rayna_status rayna_set_scene(rayna_renderer *r, const rayna_scene_desc *scene) {
+    if (!r || !scene) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    (void)hipSetDevice(r->device);
+    const rayna_status st = load_scene(r, scene);
+    if (st != RAYNA_OK) return st;
+    return rayna_clear_accumulation(r);
+}
+
+rayna_status rayna_set_camera(rayna_renderer *r, const rayna_camera *camera) {
+    if (!r || !camera) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    r->cam = *camera;
+    r->cam_valid = viewport(r->cam, r->vp);
+    return rayna_clear_accumulation(r);
+}
+
+rayna_status rayna_set_options(rayna_renderer *r, const rayna_render_opts *opts) {
+    if (!r) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    rayna_status st = check_opts(r, opts);
+    if (st != RAYNA_OK) return st;
+    (void)hipSetDevice(r->device);
+    r->opts = *opts;
+    if ((st = alloc_paths(r)) != RAYNA_OK) return st;
+    return rayna_clear_accumulation(r);
+}
+
+rayna_status rayna_clear_accumulation(rayna_renderer *r) {
+    if (!r) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    r->frames = 0;
+    r->acc_dirty = true;
+    return RAYNA_OK;
+}
+
+rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uint32_t out_dims[2],
+                          rayna_render_stats *stats) {
+    if (!r) return fail(r, RAYNA_INVALID_ARGUMENT, "null renderer");
+    if (layout != RAYNA_LAYOUT_XMAJOR && layout != RAYNA_LAYOUT_ROWMAJOR)
+        return fail(r, RAYNA_INVALID_ARGUMENT, "unknown layout");
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(r, hipSetDevice(r->device));
+    rayna_render_stats st{};
+    const uint32_t W = r->opts.width, H = r->opts.height;
+    if (!r->cam_valid) {
+        const uint32_t w2 = ilog2(W), h2 = ilog2(H);
+        if (out_rgb) write_checkerboard(w2, h2, out_rgb, layout);
+        if (out_dims) {
+            out_dims[0] = w2;
+            out_dims[1] = h2;
+        }
+        st.accum_frames = r->frames;
+    } else {
+        if (out_dims) {
+            out_dims[0] = W;
+            out_dims[1] = H;
+        }
+        const uint64_t P = (uint64_t)W * H;
+        if (r->acc_dirty) {
+            HIP_TRY(r, hipMemsetAsync(r->acc, 0, 3 * P * sizeof(float), r->stream));
+            r->acc_dirty = false;
+        }
+        FrameParams fp = frame_params(r, r->frames);
+        r->frames += 1;
+        HIP_TRY(r, hipMemsetAsync(r->segments, 0, sizeof(unsigned long long), r->stream));
+        const uint32_t spp = r->opts.samples;
+        const uint32_t n_chunks = (spp + r->chunk_samples - 1) / r->chunk_samples;
+        std::vector<hipEvent_t> *ev = nullptr;
+        const size_t n_events = (size_t)n_chunks * (r->opts.ray_depth + 1) * 3 + 2;
+        if (stats) {
+            while (r->events.size() < n_events) {
+                hipEvent_t e;
+                HIP_TRY(r, hipEventCreate(&e));
+                r->events.push_back(e);
+            }
+            ev = &r->events;
+        }
+        size_t evi = 0;
+        if (ev) HIP_TRY(r, hipEventRecord((*ev)[evi++], r->stream));
+        uint64_t launches = 0;
+        for (uint32_t c = 0; c < n_chunks; ++c) {
+            const uint32_t s0 = c * r->chunk_samples;
+            const uint32_t n = std::min(r->chunk_samples, spp - s0);
+            trace_chunk(r, fp, s0, n, ev, &evi, &launches);
+            fp.n_slots = fp.n_pixels * n;
+            launch_resolve(fp, r->pb, r->sum, n, c == 0 ? 1 : 0, r->stream);
+        }
+        launch_finalize(fp, r->sum, r->acc, (double)r->frames, r->out_dev, layout, r->stream);
+        if (ev) HIP_TRY(r, hipEventRecord((*ev)[evi++], r->stream));
+        HIP_TRY(r, hipGetLastError());
+        if (out_rgb)
+            HIP_TRY(r, hipMemcpyAsync(out_rgb, r->out_dev, 3 * P * sizeof(float), hipMemcpyDeviceToHost, r->stream));
+        unsigned long long segs = 0;
+        HIP_TRY(r, hipMemcpyAsync(&segs, r->segments, sizeof(segs), hipMemcpyDeviceToHost, r->stream));
+        HIP_TRY(r, hipStreamSynchronize(r->stream));
+        st.accum_frames = r->frames;
+        st.samples = P * spp;
+        st.segments = segs;
+        st.extend_launches = launches;
+        if (ev) {
+            float ms = 0;
+            HIP_TRY(r, hipEventElapsedTime(&ms, (*ev)[0], (*ev)[evi - 1]));
+            st.kernel_ms = ms;
+            double ext = 0, sh = 0;
+            // per depth: [before extend, after extend, after shade]
+            for (size_t i = 1; i + 2 < evi; i += 3) {
+                float a = 0, b = 0;
+                HIP_TRY(r, hipEventElapsedTime(&a, (*ev)[i], (*ev)[i + 1]));
+                HIP_TRY(r, hipEventElapsedTime(&b, (*ev)[i + 1], (*ev)[i + 2]));
+                ext += a;
+                sh += b;
+            }
+            st.extend_ms = ext;
+            st.shade_ms = sh;
+        }
+    }
+    st.duration_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return RAYNA_OK;
+}
+
+rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_t *prim_rank, float *t) {
+    if (!r || !prim_rank || !t) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    if (!r->cam_valid) return fail(r, RAYNA_INVALID_ARGUMENT, "camera is invalid");
+    HIP_TRY(r, hipSetDevice(r->device));
+    const uint32_t P = r->opts.width * r->opts.height;
+    int32_t *rank_dev = nullptr;
+    HIP_TRY(r, hipMalloc((void **)&rank_dev, (size_t)P * sizeof(int32_t)));
+    FrameParams fp = frame_params(r, 0);
+    fp.s0 = sample_index;
+    fp.n_slots = P;
+    PathBuffers pb = r->pb;
+    pb.hit_rank = rank_dev;
+    launch_generate(fp, pb, r->stream);
+    launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(prim_rank, rank_dev, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(t, pb.hit_t, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    (void)hipFree(rank_dev);
+    if (e != hipSuccess) return fail(r, RAYNA_DEVICE_ERROR, std::string("primary hits: ") + hipGetErrorString(e));
+    return RAYNA_OK;
+}
+
+rayna_status rayna_dfs_order(rayna_renderer *r, int32_t *object, int32_t *local, int32_t capacity, int32_t *n_out) {
+    if (!r || !n_out) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    const int32_t n = (int32_t)r->packed.rank_object.size();
+    *n_out = n;
+    if (object && local)
+        for (int32_t i = 0; i < n && i < capacity; ++i) {
+            object[i] = r->packed.rank_object[i];
+            local[i] = r->packed.rank_local[i];
+        }
+    return RAYNA_OK;
+}
+
+rayna_status rayna_tree_hash(rayna_renderer *r, uint64_t *hash) {
+    if (!r || !hash) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    *hash = r->packed.tree_hash;
+    return RAYNA_OK;
+}
+
+rayna_status rayna_scene_prepare(const rayna_scene_desc *scene, uint64_t *tree_hash, int32_t *object,
+                                 int32_t *local, int32_t capacity, int32_t *n_ranks, int32_t *stack_need) {
+    g_create_error.clear();
+    if (!scene) return fail(nullptr, RAYNA_INVALID_ARGUMENT, "null scene");
+    PackedScene ps;
+    std::string err;
+    const rayna_status st = pack_scene(*scene, ps, err);
+    if (st != RAYNA_OK) return fail(nullptr, st, err);
+    if (tree_hash) *tree_hash = ps.tree_hash;
+    if (n_ranks) *n_ranks = (int32_t)ps.rank_object.size();
+    if (stack_need) *stack_need = ps.stack_need;
+    if (object && local)
+        for (int32_t i = 0; i < (int32_t)ps.rank_object.size() && i < capacity; ++i) {
+            object[i] = ps.rank_object[i];
+            local[i] = ps.rank_local[i];
+        }
+    return RAYNA_OK;
+}
+
+void rayna_destroy(rayna_renderer *r) {
+    if (!r) return;
+    (void)hipSetDevice(r->device);
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    free_paths(r);
+    free_scene(r->dev);
+    for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+}
+
+const char *rayna_last_error(const rayna_renderer *r) { return r ? r->error.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,796 @@
+// Host-side scene preparation: the engine's object graph (object/list.rs:297-316), its SAH trees
+// (shared/generic_bvh.rs:45-290), depth-first primitive ranks, tight refit and the BVH4 device layout.
+#include "scene_build.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <thread>
+
+namespace rayna {
+
+namespace {
+
+Aabb64 box_new(const double a[3], const double b[3]) {  // Aabb::new (shared/aabb.rs:85-99)
+    Aabb64 r;
+    for (int i = 0; i < 3; ++i) {
+        r.mn[i] = std::fmin(a[i], b[i]);
+        r.mx[i] = std::fmax(a[i], b[i]);
+    }
+    return r;
+}
+
+Aabb64 origin_box() { return Aabb64{{0, 0, 0}, {0, 0, 0}}; }  // Aabb::default()
+
+void grow(Aabb64 &acc, const Aabb64 &b) {  // Aabb::encompass (aabb.rs:109-114)
+    for (int i = 0; i < 3; ++i) {
+        acc.mn[i] = std::fmin(acc.mn[i], b.mn[i]);
+        acc.mx[i] = std::fmax(acc.mx[i], b.mx[i]);
+    }
+}
+
+double area(const Aabb64 &b) {  // aabb.rs:90
+    const double sx = b.mx[0] - b.mn[0], sy = b.mx[1] - b.mn[1], sz = b.mx[2] - b.mn[2];
+    return ((sx * sy) + (sy * sz) + (sz * sx)) * 2.0;
+}
+
+Aabb64 points_box(const double *p, int n) {  // Aabb::encompass_points (aabb.rs:123-132)
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < n; ++k)
+        for (int i = 0; i < 3; ++i) {
+            mn[i] = std::fmin(mn[i], p[3 * k + i]);
+            mx[i] = std::fmax(mx[i], p[3 * k + i]);
+        }
+    return box_new(mn, mx);
+}
+
+Aabb64 min_padded(const Aabb64 &b, double thresh) {  // Aabb::min_padded (aabb.rs:137-142)
+    double dims[3], centre[3], mn[3], mx[3];
+    for (int i = 0; i < 3; ++i) {
+        dims[i] = b.mx[i] - b.mn[i];
+        centre[i] = b.mn[i] + dims[i] / 2.0;
+        dims[i] = std::fmax(dims[i], thresh);
+    }
+    for (int i = 0; i < 3; ++i) {
+        mn[i] = centre[i] - dims[i] / 2.0;
+        mx[i] = centre[i] + dims[i] / 2.0;
+    }
+    return box_new(mn, mx);
+}
+
+float round_down(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+float round_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// The engine's SAH builder.  Each recursion level works on a range [lo, hi) of three index arrays that hold
+// the range's items sorted by (aabb.min[axis], item index) for axis x, y, z; a split is a stable partition of
+// all three, so no level sorts again.  The candidate splits, costs (unions folded from the origin, as
+// Aabb::encompass_iter does) and tie rules are those of calculate_optimal_split::<1, 2>.
+class Builder {
+   public:
+    explicit Builder(const std::vector<Aabb64> &boxes) : box_(boxes) {}
+
+    RefTree build() {
+        const int n = (int)box_.size();
+        if (n == 0) return t_;
+        for (int a = 0; a < 3; ++a) {
+            ord_[a].resize(n);
+            for (int i = 0; i < n; ++i) ord_[a][i] = i;
+            std::sort(ord_[a].begin(), ord_[a].end(), [&](int x, int y) {
+                const double kx = box_[x].mn[a], ky = box_[y].mn[a];
+                if (kx < ky) return true;
+                if (ky < kx) return false;
+                return x < y;
+            });
+        }
+        mark_.assign(n, 0);
+        tmp_.resize(n);
+        suf_.resize(n + 1);
+        t_.root = generate(0, n, -1);
+        return t_;
+    }
+
+   private:
+    const std::vector<Aabb64> &box_;
+    RefTree t_;
+    std::vector<int32_t> ord_[3];
+    std::vector<uint8_t> mark_;
+    std::vector<int32_t> tmp_;
+    std::vector<double> suf_;
+
+    struct Split {
+        int axis, left;
+    };
+
+    // calculate_optimal_split::<1, 2> on a range of more than two items (generic_bvh.rs:173-290)
+    Split best(int lo, int hi) {
+        const int n = hi - lo;
+        Split s{0, 1};
+        bool have = false;
+        double best_cost = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const int32_t *o = &ord_[a][lo];
+            Aabb64 acc = origin_box();
+            for (int i = n - 1; i >= 1; --i) {
+                grow(acc, box_[o[i]]);
+                suf_[i] = area(acc);
+            }
+            acc = origin_box();
+            for (int p = 1; p <= n - 2; ++p) {
+                grow(acc, box_[o[p - 1]]);
+                const double cost = (double)p * area(acc) + (double)(n - p) * suf_[p];
+                if (!have || cost < best_cost) {  // std::cmp::min_by keeps the earlier of equal costs
+                    have = true;
+                    best_cost = cost;
+                    s = Split{a, p};
+                }
+            }
+        }
+        return s;
+    }
+
+    // split_objects (generic_bvh.rs:156-168): the first `left` items along `axis` go left.
+    void partition(int lo, int hi, int axis, int left) {
+        for (int i = lo; i < lo + left; ++i) mark_[ord_[axis][i]] = 1;
+        for (int a = 0; a < 3; ++a) {
+            if (a == axis) continue;
+            int w = lo, r = 0;
+            for (int i = lo; i < hi; ++i) {
+                const int32_t id = ord_[a][i];
+                if (mark_[id]) ord_[a][w++] = id;
+                else tmp_[r++] = id;
+            }
+            std::memcpy(&ord_[a][w], tmp_.data(), sizeof(int32_t) * r);
+        }
+        for (int i = lo; i < lo + left; ++i) mark_[ord_[axis][i]] = 0;
+    }
+
+    int add_node(int item) {
+        t_.nodes.push_back(RefTree::Node{item, 0, 0});
+        return (int)t_.nodes.size() - 1;
+    }
+
+    void set_children(int node, const std::vector<int32_t> &ch) {
+        t_.nodes[node].first_child = (int32_t)t_.children.size();
+        t_.nodes[node].n_children = (int32_t)ch.size();
+        t_.children.insert(t_.children.end(), ch.begin(), ch.end());
+    }
+
+    // generate_nodes_sah (generic_bvh.rs:97-154).  `order_axis` is the axis of the sort that produced this
+    // range's order in the engine (-1: the input order, only at the root).
+    int generate(int lo, int hi, int order_axis) {
+        const int n = hi - lo;
+        if (n == 1) return add_node(ord_[0][lo]);
+        if (n <= 8) {
+            const int node = add_node(-1);
+            std::vector<int32_t> ids;
+            if (order_axis >= 0) ids.assign(ord_[order_axis].begin() + lo, ord_[order_axis].begin() + hi);
+            else {
+                ids.assign(ord_[0].begin() + lo, ord_[0].begin() + hi);
+                std::sort(ids.begin(), ids.end());
+            }
+            std::vector<int32_t> ch;
+            for (int32_t id : ids) ch.push_back(add_node(id));
+            set_children(node, ch);
+            return node;
+        }
+        const int node = add_node(-1);
+        const Split s = best(lo, hi);
+        partition(lo, hi, s.axis, s.left);
+        std::vector<int32_t> ch;
+        const int halves[2][2] = {{lo, lo + s.left}, {lo + s.left, hi}};
+        for (int h = 0; h < 2; ++h) {
+            const int hl = halves[h][0], hh = halves[h][1];
+            if (hh - hl > 2) {
+                const Split s2 = best(hl, hh);
+                partition(hl, hh, s2.axis, s2.left);
+                ch.push_back(generate(hl, hl + s2.left, s2.axis));
+                ch.push_back(generate(hl + s2.left, hh, s2.axis));
+            } else {
+                ch.push_back(generate(hl, hh, s.axis));
+            }
+        }
+        set_children(node, ch);
+        return node;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------
+
+struct Packer {
+    const rayna_scene_desc &d;
+    PackedScene &P;
+    std::vector<RefTree> blas;              // per mesh (empty tree if not a triangle list)
+    std::vector<std::vector<Aabb64>> blas_tight;  // per mesh, per node
+    RefTree tlas;
+    std::vector<Aabb64> tlas_tight;
+    std::vector<int32_t> bounded;           // tlas item -> object
+    std::vector<Aabb64> obj_tight;
+    int32_t rank = 0;
+
+    Packer(const rayna_scene_desc &dd, PackedScene &pp) : d(dd), P(pp) {}
+
+    Aabb64 tri_box(int tri) const { return points_box(&d.tri_vertices[9 * (size_t)tri], 3); }
+
+    static bool is_nested(const RefTree &t, int n) { return t.nodes[n].item < 0; }
+
+    static bool all_items(const RefTree &t, int n) {
+        const RefTree::Node &nd = t.nodes[n];
+        for (int i = 0; i < nd.n_children; ++i)
+            if (t.nodes[t.children[nd.first_child + i]].item < 0) return false;
+        return true;
+    }
+
+    static std::vector<Aabb64> refit(const RefTree &t, const std::function<Aabb64(int)> &item_box) {
+        std::vector<Aabb64> tight(t.nodes.size());
+        // children always have larger indices than their parent in RefTree, so a reverse sweep is post-order
+        for (int n = (int)t.nodes.size() - 1; n >= 0; --n) {
+            const RefTree::Node &nd = t.nodes[n];
+            if (nd.item >= 0) tight[n] = item_box(nd.item);
+            else {
+                Aabb64 b = tight[t.children[nd.first_child]];
+                for (int i = 1; i < nd.n_children; ++i) grow(b, tight[t.children[nd.first_child + i]]);
+                tight[n] = b;
+            }
+        }
+        return tight;
+    }
+
+    static void set_slot_box(GNode &g, int k, const Aabb64 &b) {
+        g.lo_x[k] = round_down(b.mn[0]);
+        g.lo_y[k] = round_down(b.mn[1]);
+        g.lo_z[k] = round_down(b.mn[2]);
+        g.hi_x[k] = round_up(b.mx[0]);
+        g.hi_y[k] = round_up(b.mx[1]);
+        g.hi_z[k] = round_up(b.mx[2]);
+    }
+
+    static GNode empty_node() {
+        GNode g;
+        for (int k = 0; k < 4; ++k) {
+            g.lo_x[k] = g.lo_y[k] = g.lo_z[k] = INFINITY;
+            g.hi_x[k] = g.hi_y[k] = g.hi_z[k] = -INFINITY;
+            g.child[k] = 0;
+            g.info[k] = INFO_EMPTY;
+        }
+        return g;
+    }
+
+    // One triangle of the scene into the device arrays; returns its index.
+    int32_t add_triangle(int tri, int32_t material, int32_t object, int32_t local) {
+        const double *v = &d.tri_vertices[9 * (size_t)tri];
+        const double *nn = &d.tri_normals[9 * (size_t)tri];
+        GTri g;
+        const float v0[3] = {(float)v[0], (float)v[1], (float)v[2]};
+        const float v1[3] = {(float)v[3], (float)v[4], (float)v[5]};
+        const float v2[3] = {(float)v[6], (float)v[7], (float)v[8]};
+        g.v0x = v0[0];
+        g.v0y = v0[1];
+        g.v0z = v0[2];
+        g.e1x = v1[0] - v0[0];
+        g.e1y = v1[1] - v0[1];
+        g.e1z = v1[2] - v0[2];
+        g.e2x = v2[0] - v0[0];
+        g.e2y = v2[1] - v0[1];
+        g.e2z = v2[2] - v0[2];
+        g.rank = rank++;
+        g.material = material;
+        // Can sum_i b_i n_i vanish for barycentrics b?  Not if every n_i leans towards the mean direction.
+        double c[3] = {nn[0] + nn[3] + nn[6], nn[1] + nn[4] + nn[7], nn[2] + nn[5] + nn[8]};
+        const double cl = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        int flags = 1;
+        if (cl > 0) {
+            flags = 0;
+            for (int i = 0; i < 3; ++i) {
+                const double dd = (c[0] * nn[3 * i] + c[1] * nn[3 * i + 1] + c[2] * nn[3 * i + 2]) / cl;
+                if (!(dd > 1e-3)) flags = 1;
+            }
+        }
+        g.flags = flags;
+        P.tris.push_back(g);
+        GTriNormals tn;
+        tn.n0x = (float)nn[0];
+        tn.n0y = (float)nn[1];
+        tn.n0z = (float)nn[2];
+        tn.n1x = (float)nn[3];
+        tn.n1y = (float)nn[4];
+        tn.n1z = (float)nn[5];
+        tn.n2x = (float)nn[6];
+        tn.n2y = (float)nn[7];
+        tn.n2z = (float)nn[8];
+        tn.pad0 = tn.pad1 = tn.pad2 = 0;
+        P.tri_normals.push_back(tn);
+        P.rank_object.push_back(object);
+        P.rank_local.push_back(local);
+        return (int32_t)P.tris.size() - 1;
+    }
+
+    int32_t add_para(int idx) {  // Planar::new (planar/mod.rs:303-314) in double, then rounded
+        const double *p = &d.parallelograms[9 * (size_t)idx];
+        const double u[3] = {p[3], p[4], p[5]}, v[3] = {p[6], p[7], p[8]};
+        const double nr[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const double l2 = (nr[0] * nr[0] + nr[1] * nr[1]) + nr[2] * nr[2];
+        const double inv = 1.0 / std::sqrt(l2);
+        const double n[3] = {nr[0] * inv, nr[1] * inv, nr[2] * inv};
+        const double dd = -((n[0] * p[0] + n[1] * p[1]) + n[2] * p[2]);
+        GPara g;
+        g.nx = (float)n[0];
+        g.ny = (float)n[1];
+        g.nz = (float)n[2];
+        g.d = (float)dd;
+        g.wx = (float)(nr[0] / l2);
+        g.wy = (float)(nr[1] / l2);
+        g.wz = (float)(nr[2] / l2);
+        g.px = (float)p[0];
+        g.py = (float)p[1];
+        g.pz = (float)p[2];
+        g.ux = (float)u[0];
+        g.uy = (float)u[1];
+        g.uz = (float)u[2];
+        g.vx = (float)v[0];
+        g.vy = (float)v[1];
+        g.vz = (float)v[2];
+        g.pad0 = g.pad1 = g.pad2 = g.pad3 = 0;
+        P.paras.push_back(g);
+        return (int32_t)P.paras.size() - 1;
+    }
+
+    int32_t add_box(int idx) {  // AxisBoxMesh::new (axis_box.rs:29-37) in double, then rounded
+        const double *b = &d.boxes[6 * (size_t)idx];
+        GBox g;
+        double c[3], r[3], iv[3];
+        for (int i = 0; i < 3; ++i) {
+            const double mn = std::fmin(b[i], b[3 + i]), mx = std::fmax(b[i], b[3 + i]);
+            c[i] = (mn + mx) / 2.0;
+            r[i] = (mx - mn) / 2.0;
+            iv[i] = 1.0 / r[i];
+        }
+        g.cx = (float)c[0];
+        g.cy = (float)c[1];
+        g.cz = (float)c[2];
+        g.rx = (float)r[0];
+        g.ry = (float)r[1];
+        g.rz = (float)r[2];
+        g.ix = (float)iv[0];
+        g.iy = (float)iv[1];
+        g.iz = (float)iv[2];
+        g.pad0 = g.pad1 = g.pad2 = 0;
+        P.boxes.push_back(g);
+        return (int32_t)P.boxes.size() - 1;
+    }
+
+    // ---- flattening ----
+
+    // Leaf cluster of a mesh BVH: triangles appended in child order.
+    void blas_leaf(GNode &g, int k, const rayna_mesh &m, int32_t material, int32_t object,
+                   const std::vector<int32_t> &items) {
+        const int32_t start = (int32_t)P.tris.size();
+        for (int32_t it : items) add_triangle(m.index + it, material, object, it);
+        g.child[k] = start;
+        g.info[k] = (int32_t)items.size() | (LEAF_TRIS << 4);
+    }
+
+    int32_t emit_blas(int mesh, int32_t object, int32_t material) {
+        const RefTree &t = blas[mesh];
+        const std::vector<Aabb64> &tight = blas_tight[mesh];
+        const rayna_mesh &m = d.meshes[mesh];
+        std::function<int32_t(int, bool)> emit = [&](int node, bool wrap) -> int32_t {
+            const int32_t gi = (int32_t)P.nodes.size();
+            P.nodes.push_back(empty_node());
+            std::vector<int32_t> slots;
+            if (wrap) slots.push_back(node);
+            else {
+                const RefTree::Node &nd = t.nodes[node];
+                for (int i = 0; i < nd.n_children; ++i) slots.push_back(t.children[nd.first_child + i]);
+            }
+            for (int k = 0; k < (int)slots.size(); ++k) {
+                const int ch = slots[k];
+                set_slot_box(P.nodes[gi], k, tight[ch]);
+                if (!is_nested(t, ch)) {
+                    GNode g = P.nodes[gi];
+                    blas_leaf(g, k, m, material, object, {t.nodes[ch].item});
+                    P.nodes[gi] = g;
+                } else if (all_items(t, ch)) {
+                    std::vector<int32_t> items;
+                    const RefTree::Node &nd = t.nodes[ch];
+                    for (int i = 0; i < nd.n_children; ++i) items.push_back(t.nodes[t.children[nd.first_child + i]].item);
+                    GNode g = P.nodes[gi];
+                    blas_leaf(g, k, m, material, object, items);
+                    P.nodes[gi] = g;
+                } else {
+                    const int32_t sub = emit(ch, false);
+                    P.nodes[gi].child[k] = sub;
+                    P.nodes[gi].info[k] = INFO_INNER;
+                }
+            }
+            return gi;
+        };
+        const bool wrap = !is_nested(t, t.root) || all_items(t, t.root);
+        return emit(t.root, wrap);
+    }
+
+    // One object of the object BVH as a leaf item.
+    GItem object_item(int32_t o, std::string &error, rayna_status &st) {
+        const rayna_object &ob = d.objects[o];
+        const rayna_mesh &m = d.meshes[ob.mesh];
+        GItem it{ITEM_NONE, 0, 0, ob.material};
+        if (ob.kind == RAYNA_OBJ_SIMPLE) {
+            switch (m.kind) {
+                case RAYNA_MESH_SPHERE:
+                    it.kind = ITEM_SPHERE;
+                    it.index = m.index;
+                    break;
+                case RAYNA_MESH_TRIANGLE:
+                    it.kind = ITEM_TRIANGLE;
+                    it.rank = rank;
+                    it.index = add_triangle(m.index, ob.material, o, 0);
+                    return it;
+                case RAYNA_MESH_PARALLELOGRAM:
+                    it.kind = ITEM_PARALLELOGRAM;
+                    it.index = add_para(m.index);
+                    break;
+                case RAYNA_MESH_AXIS_BOX:
+                    it.kind = ITEM_AXIS_BOX;
+                    it.index = add_box(m.index);
+                    break;
+                case RAYNA_MESH_TRIANGLE_LIST:
+                    it.kind = ITEM_LINK;
+                    it.rank = rank;
+                    it.index = emit_blas(ob.mesh, o, ob.material);
+                    return it;
+                default:
+                    st = RAYNA_UNSUPPORTED;
+                    error = "unsupported bounded mesh kind";
+                    return it;
+            }
+        } else {
+            GVolume v;
+            v.object = o;
+            v.neg_inv_density = -1.0f / (float)ob.density;
+            if (m.kind == RAYNA_MESH_SPHERE) {
+                v.boundary_kind = ITEM_SPHERE;
+                v.boundary_index = m.index;
+            } else if (m.kind == RAYNA_MESH_AXIS_BOX) {
+                v.boundary_kind = ITEM_AXIS_BOX;
+                v.boundary_index = add_box(m.index);
+            } else {
+                st = RAYNA_UNSUPPORTED;
+                error = "volumes are supported with sphere and axis-box boundaries";
+                return it;
+            }
+            P.volumes.push_back(v);
+            it.kind = ITEM_VOLUME;
+            it.index = (int32_t)P.volumes.size() - 1;
+        }
+        it.rank = rank++;
+        P.rank_object.push_back(o);
+        P.rank_local.push_back(0);
+        return it;
+    }
+
+    rayna_status emit_tlas(std::string &error) {
+        rayna_status st = RAYNA_OK;
+        const RefTree &t = tlas;
+        std::function<int32_t(int, bool)> emit = [&](int node, bool wrap) -> int32_t {
+            const int32_t gi = (int32_t)P.nodes.size();
+            P.nodes.push_back(empty_node());
+            std::vector<int32_t> slots;
+            if (wrap) slots.push_back(node);
+            else {
+                const RefTree::Node &nd = t.nodes[node];
+                for (int i = 0; i < nd.n_children; ++i) slots.push_back(t.children[nd.first_child + i]);
+            }
+            for (int k = 0; k < (int)slots.size() && st == RAYNA_OK; ++k) {
+                const int ch = slots[k];
+                set_slot_box(P.nodes[gi], k, tlas_tight[ch]);
+                std::vector<int32_t> objs;
+                if (!is_nested(t, ch)) objs.push_back(bounded[t.nodes[ch].item]);
+                else if (all_items(t, ch)) {
+                    const RefTree::Node &nd = t.nodes[ch];
+                    for (int i = 0; i < nd.n_children; ++i)
+                        objs.push_back(bounded[t.nodes[t.children[nd.first_child + i]].item]);
+                }
+                if (!objs.empty()) {
+                    const int32_t start = (int32_t)P.items.size();
+                    P.items.resize(P.items.size() + objs.size());
+                    for (size_t i = 0; i < objs.size(); ++i) {
+                        const GItem it = object_item(objs[i], error, st);
+                        P.items[start + i] = it;
+                    }
+                    P.nodes[gi].child[k] = start;
+                    P.nodes[gi].info[k] = (int32_t)objs.size() | (LEAF_ITEMS << 4);
+                } else {
+                    const int32_t sub = emit(ch, false);
+                    P.nodes[gi].child[k] = sub;
+                    P.nodes[gi].info[k] = INFO_INNER;
+                }
+            }
+            return gi;
+        };
+        if (t.root >= 0) {
+            const bool wrap = !is_nested(t, t.root) || all_items(t, t.root);
+            P.root = emit(t.root, wrap);
+        }
+        return st;
+    }
+
+    int32_t stack_need() const {
+        // S(g) = P - 1 + max(1, max S(child)) for a node that pushes P entries, 0 if it pushes none.
+        std::vector<int32_t> memo(P.nodes.size(), -1);
+        std::function<int32_t(int32_t)> S = [&](int32_t g) -> int32_t {
+            if (memo[g] >= 0) return memo[g];
+            int32_t pushes = 0, deepest = 1;
+            const GNode &n = P.nodes[g];
+            for (int k = 0; k < 4; ++k) {
+                if (n.info[k] == INFO_INNER) {
+                    ++pushes;
+                    deepest = std::max(deepest, S(n.child[k]));
+                } else if (n.info[k] > 0 && (n.info[k] >> 4) == LEAF_ITEMS) {
+                    const int cnt = n.info[k] & 0xf;
+                    for (int i = 0; i < cnt; ++i) {
+                        const GItem &it = P.items[n.child[k] + i];
+                        if (it.kind == ITEM_LINK) {
+                            ++pushes;
+                            deepest = std::max(deepest, S(it.index));
+                        }
+                    }
+                }
+            }
+            memo[g] = pushes == 0 ? 0 : pushes - 1 + deepest;
+            return memo[g];
+        };
+        if (P.root < 0) return 0;
+        return std::max<int32_t>(1, S(P.root));
+    }
+};
+
+}  // namespace
+
+RefTree build_engine_bvh(const std::vector<Aabb64> &boxes) { return Builder(boxes).build(); }
+
+void hash_tree(const RefTree &t, uint64_t &h) {
+    if (t.root < 0) return;
+    auto mix = [&](int64_t v) {
+        for (int i = 0; i < 8; ++i) {
+            h ^= (uint64_t)((v >> (8 * i)) & 0xff);
+            h *= 1099511628211ull;
+        }
+    };
+    std::vector<int32_t> stack{t.root};
+    while (!stack.empty()) {
+        const int32_t n = stack.back();
+        stack.pop_back();
+        const RefTree::Node &nd = t.nodes[n];
+        mix(nd.item);
+        mix(nd.n_children);
+        for (int i = nd.n_children - 1; i >= 0; --i) stack.push_back(t.children[nd.first_child + i]);
+    }
+}
+
+static bool finite_all(const double *p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &error) {
+    P = PackedScene();
+    // ---- validation ----
+    if (d.n_objects < 0 || d.n_meshes < 0 || d.n_materials < 0 || d.n_spheres < 0 || d.n_triangles < 0 ||
+        d.n_parallelograms < 0 || d.n_boxes < 0 || d.n_sdfs < 0) {
+        error = "negative array size in scene";
+        return RAYNA_INVALID_ARGUMENT;
+    }
+    if ((d.n_objects && !d.objects) || (d.n_meshes && !d.meshes) || (d.n_materials && !d.materials) ||
+        (d.n_spheres && !d.spheres) || (d.n_triangles && (!d.tri_vertices || !d.tri_normals)) ||
+        (d.n_parallelograms && !d.parallelograms) || (d.n_boxes && !d.boxes) || (d.n_sdfs && !d.sdfs)) {
+        error = "null array in scene";
+        return RAYNA_INVALID_ARGUMENT;
+    }
+    if (d.skybox < RAYNA_SKY_NONE || d.skybox > RAYNA_SKY_WHITE) {
+        error = "unknown skybox";
+        return RAYNA_INVALID_ARGUMENT;
+    }
+    if (!finite_all(d.spheres, 4 * (size_t)d.n_spheres) || !finite_all(d.tri_vertices, 9 * (size_t)d.n_triangles) ||
+        !finite_all(d.tri_normals, 9 * (size_t)d.n_triangles) ||
+        !finite_all(d.parallelograms, 9 * (size_t)d.n_parallelograms) || !finite_all(d.boxes, 6 * (size_t)d.n_boxes)) {
+        error = "non-finite geometry in scene";
+        return RAYNA_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < d.n_materials; ++i) {
+        const rayna_material &m = d.materials[i];
+        if (m.kind < RAYNA_MAT_LAMBERTIAN || m.kind > RAYNA_MAT_LIGHT) {
+            error = "unknown material kind";
+            return RAYNA_INVALID_ARGUMENT;
+        }
+        GMaterial g{m.kind, m.colour[0], m.colour[1], m.colour[2], (float)m.fuzz, (float)m.refractive_index,
+                    (float)m.density, 0.0f};
+        P.materials.push_back(g);
+    }
+    for (int i = 0; i < d.n_meshes; ++i) {
+        const rayna_mesh &m = d.meshes[i];
+        int64_t limit = 0;
+        switch (m.kind) {
+            case RAYNA_MESH_SPHERE: limit = d.n_spheres; break;
+            case RAYNA_MESH_TRIANGLE: limit = d.n_triangles; break;
+            case RAYNA_MESH_TRIANGLE_LIST: limit = d.n_triangles; break;
+            case RAYNA_MESH_PARALLELOGRAM: limit = d.n_parallelograms; break;
+            case RAYNA_MESH_AXIS_BOX: limit = d.n_boxes; break;
+            case RAYNA_MESH_SDF: limit = d.n_sdfs; break;
+            default: error = "unknown mesh kind"; return RAYNA_INVALID_ARGUMENT;
+        }
+        const int64_t count = m.kind == RAYNA_MESH_TRIANGLE_LIST ? m.count : 1;
+        if (m.index < 0 || count < 0 || (int64_t)m.index + count > limit ||
+            (m.kind != RAYNA_MESH_TRIANGLE_LIST && m.index >= limit)) {
+            error = "mesh index out of range";
+            return RAYNA_INVALID_ARGUMENT;
+        }
+    }
+    for (int i = 0; i < d.n_objects; ++i) {
+        const rayna_object &o = d.objects[i];
+        if (o.kind != RAYNA_OBJ_SIMPLE && o.kind != RAYNA_OBJ_VOLUMETRIC) {
+            error = "unknown object kind";
+            return RAYNA_INVALID_ARGUMENT;
+        }
+        if (o.mesh < 0 || o.mesh >= d.n_meshes || o.material < 0 || o.material >= d.n_materials) {
+            error = "object mesh or material index out of range";
+            return RAYNA_INVALID_ARGUMENT;
+        }
+    }
+    for (int i = 0; i < d.n_sdfs; ++i) {
+        const rayna_sdf &f = d.sdfs[i];
+        if (f.kind < RAYNA_SDF_SPHERE || f.kind > RAYNA_SDF_MANDELBULB || f.max_iterations < 0) {
+            error = "invalid SDF";
+            return RAYNA_INVALID_ARGUMENT;
+        }
+        if (f.kind != RAYNA_SDF_MANDELBULB &&
+            (f.first_sphere < 0 || f.n_spheres < 1 || (int64_t)f.first_sphere + f.n_spheres > d.n_spheres)) {
+            error = "SDF sphere range out of range";
+            return RAYNA_INVALID_ARGUMENT;
+        }
+        if (f.kind == RAYNA_SDF_MANDELBULB &&
+            (f.power != std::floor(f.power) || f.power < 2 || f.power > 16 || f.bulb_iterations < 0)) {
+            error = "Mandelbulb power must be an integer in [2, 16]";
+            return RAYNA_UNSUPPORTED;
+        }
+    }
+    for (int i = 0; i < d.n_spheres; ++i) {
+        const double *s = &d.spheres[4 * (size_t)i];
+        P.spheres.push_back(GSphere{(float)s[0], (float)s[1], (float)s[2], (float)s[3]});
+    }
+    P.skybox = d.skybox;
+
+    Packer K(d, P);
+    // ---- object bounds (reference boxes carry the origin for lists: list.rs:176-181) ----
+    std::vector<Aabb64> obj_ref(d.n_objects);
+    K.obj_tight.resize(d.n_objects);
+    std::vector<int32_t> unbounded;
+    for (int o = 0; o < d.n_objects; ++o) {
+        const rayna_mesh &m = d.meshes[d.objects[o].mesh];
+        bool bnd = true;
+        switch (m.kind) {
+            case RAYNA_MESH_SPHERE: {
+                const double *s = &d.spheres[4 * (size_t)m.index];
+                const double a[3] = {s[0] - s[3], s[1] - s[3], s[2] - s[3]}, b[3] = {s[0] + s[3], s[1] + s[3], s[2] + s[3]};
+                obj_ref[o] = K.obj_tight[o] = box_new(a, b);
+                break;
+            }
+            case RAYNA_MESH_TRIANGLE: obj_ref[o] = K.obj_tight[o] = K.tri_box(m.index); break;
+            case RAYNA_MESH_TRIANGLE_LIST: {
+                if (m.count <= 0) {
+                    bnd = false;
+                    break;
+                }
+                Aabb64 r = origin_box(), t = K.tri_box(m.index);
+                for (int i = 0; i < m.count; ++i) {
+                    const Aabb64 tb = K.tri_box(m.index + i);
+                    grow(r, tb);
+                    grow(t, tb);
+                }
+                obj_ref[o] = r;
+                K.obj_tight[o] = t;
+                break;
+            }
+            case RAYNA_MESH_PARALLELOGRAM: {
+                const double *p = &d.parallelograms[9 * (size_t)m.index];
+                double pts[12];
+                for (int i = 0; i < 3; ++i) {
+                    pts[i] = p[i];
+                    pts[3 + i] = p[i] + p[3 + i];
+                    pts[6 + i] = p[i] + p[6 + i];
+                    pts[9 + i] = p[i] + p[3 + i] + p[6 + i];
+                }
+                obj_ref[o] = K.obj_tight[o] = min_padded(points_box(pts, 4), 1e-6);
+                break;
+            }
+            case RAYNA_MESH_AXIS_BOX: {
+                const double *b = &d.boxes[6 * (size_t)m.index];
+                obj_ref[o] = K.obj_tight[o] = box_new(b, b + 3);
+                break;
+            }
+            default: bnd = false; break;
+        }
+        if (bnd) K.bounded.push_back(o);
+        else unbounded.push_back(o);
+    }
+    // ---- unbounded objects: SDFs (empty triangle lists never hit) ----
+    for (size_t i = 0; i < unbounded.size(); ++i) {
+        const rayna_object &ob = d.objects[unbounded[i]];
+        const rayna_mesh &m = d.meshes[ob.mesh];
+        if (m.kind == RAYNA_MESH_TRIANGLE_LIST) continue;
+        if (ob.kind != RAYNA_OBJ_SIMPLE) {
+            error = "volumes with unbounded boundaries are not supported";
+            return RAYNA_UNSUPPORTED;
+        }
+        const rayna_sdf &f = d.sdfs[m.index];
+        GSdf g;
+        std::memset(&g, 0, sizeof(g));
+        g.kind = f.kind;
+        g.first_sphere = f.first_sphere;
+        g.n_spheres = f.n_spheres;
+        g.max_iterations = f.max_iterations;
+        g.bulb_iterations = f.bulb_iterations;
+        g.power = (int32_t)f.power;
+        g.material = ob.material;
+        g.list_pos = (int32_t)i;
+        g.epsilon = (float)f.epsilon;
+        g.k = (float)f.k;
+        g.bailout = (float)f.bailout;
+        g.cx = (float)f.centre[0];
+        g.cy = (float)f.centre[1];
+        g.cz = (float)f.centre[2];
+        P.sdfs.push_back(g);
+    }
+    // ---- mesh BVHs, built in parallel ----
+    K.blas.resize(d.n_meshes);
+    K.blas_tight.resize(d.n_meshes);
+    {
+        std::vector<int> lists;
+        for (int m = 0; m < d.n_meshes; ++m)
+            if (d.meshes[m].kind == RAYNA_MESH_TRIANGLE_LIST && d.meshes[m].count > 0) lists.push_back(m);
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (;;) {
+                const size_t j = next.fetch_add(1);
+                if (j >= lists.size()) break;
+                const rayna_mesh &me = d.meshes[lists[j]];
+                std::vector<Aabb64> tb(me.count);
+                for (int i = 0; i < me.count; ++i) tb[i] = K.tri_box(me.index + i);
+                K.blas[lists[j]] = build_engine_bvh(tb);
+                K.blas_tight[lists[j]] = Packer::refit(K.blas[lists[j]], [&](int it) { return tb[it]; });
+            }
+        };
+        unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 16u));
+        nt = std::min<unsigned>(nt, (unsigned)std::max<size_t>(1, lists.size()));
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+        work();
+        for (auto &t : pool) t.join();
+    }
+    // ---- object BVH ----
+    {
+        std::vector<Aabb64> ob(K.bounded.size());
+        for (size_t i = 0; i < K.bounded.size(); ++i) ob[i] = obj_ref[K.bounded[i]];
+        K.tlas = build_engine_bvh(ob);
+        K.tlas_tight = Packer::refit(K.tlas, [&](int it) { return K.obj_tight[K.bounded[it]]; });
+    }
+    P.tree_hash = 1469598103934665603ull;
+    hash_tree(K.tlas, P.tree_hash);
+    for (int m = 0; m < d.n_meshes; ++m) hash_tree(K.blas[m], P.tree_hash);
+    const rayna_status st = K.emit_tlas(error);
+    if (st != RAYNA_OK) return st;
+    P.stack_need = K.stack_need();
+    return RAYNA_OK;
+}
+
+}  // namespace rayna

@@ -1,0 +1,59 @@
+// Host-side preparation of a scene for the GPU: the engine's object graph and SAH trees, flattened into the
+// device layout of layout.hpp.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rayna_gpu.h"
+#include "layout.hpp"
+
+namespace rayna {
+
+// One tree of shared/generic_bvh.rs: nodes are either an item (GenericBvhNode::Object) or nested, with children
+// in insertion order.
+struct RefTree {
+    struct Node {
+        int32_t item;          // >= 0: item index; -1: nested
+        int32_t first_child;   // into `children`
+        int32_t n_children;
+    };
+    std::vector<Node> nodes;
+    std::vector<int32_t> children;
+    int32_t root = -1;
+};
+
+struct Aabb64 {
+    double mn[3], mx[3];
+};
+
+// The engine's SAH builder (generic_bvh.rs:45-290) over items with the given boxes, ties in the axis sorts
+// broken by item index.
+RefTree build_engine_bvh(const std::vector<Aabb64> &boxes);
+
+// FNV-1a hash of a tree's depth-first walk (item or -1, child count per node), chained onto `h`.
+void hash_tree(const RefTree &t, uint64_t &h);
+
+struct PackedScene {
+    std::vector<GNode> nodes;
+    std::vector<GTri> tris;
+    std::vector<GTriNormals> tri_normals;
+    std::vector<GItem> items;
+    std::vector<GSphere> spheres;
+    std::vector<GPara> paras;
+    std::vector<GBox> boxes;
+    std::vector<GVolume> volumes;
+    std::vector<GSdf> sdfs;
+    std::vector<GMaterial> materials;
+    int32_t root = -1;
+    int32_t stack_need = 0;
+    int32_t skybox = RAYNA_SKY_NONE;
+    uint64_t tree_hash = 0;
+    std::vector<int32_t> rank_object, rank_local;  // depth-first rank -> (object, index in its mesh)
+};
+
+// Validates and packs a scene.  Returns RAYNA_OK, or an error status with a message in `error`.
+rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &out, std::string &error);
+
+}  // namespace rayna

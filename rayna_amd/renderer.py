@@ -1,0 +1,222 @@
+"""The Renderer: rayna_engine's `Renderer` API (rayna_engine/src/render/renderer.rs:36-264) over librayna_amd.so.
+
+    r = Renderer.new_from(scene, camera, options)      # Renderer::new_from (renderer.rs:86-107)
+    render = r.render()                                 # Renderer::render (renderer.rs:224-264)
+    render.img          # float32 (width, height, 3): the engine's Image shape, indexed [x][y]
+    render.stats        # RenderStats: duration, accum_frames, samples, segments, kernel timings
+    r.set_scene / set_camera / set_options / clear_accumulation
+
+All per-ray work runs in the HIP kernels of the library; there is no CPU fallback.  A missing library or GPU
+raises RayError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _abi as A
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librayna_amd.so")
+_lib: Optional[C.CDLL] = None
+
+# every symbol declared in include/rayna_gpu.h
+EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_options", "rayna_clear_accumulation",
+           "rayna_render", "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare",
+           "rayna_destroy", "rayna_last_error")
+
+
+class RayError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{A.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def lib() -> C.CDLL:
+    """The renderer library (loaded once).  Fails loudly if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rayna_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.rayna_create.restype = C.c_int
+        L.rayna_create.argtypes = [C.POINTER(A.SceneDesc), C.POINTER(A.CameraDesc), C.POINTER(A.RenderOptsDesc),
+                                   C.POINTER(vp)]
+        L.rayna_set_scene.restype = C.c_int
+        L.rayna_set_scene.argtypes = [vp, C.POINTER(A.SceneDesc)]
+        L.rayna_set_camera.restype = C.c_int
+        L.rayna_set_camera.argtypes = [vp, C.POINTER(A.CameraDesc)]
+        L.rayna_set_options.restype = C.c_int
+        L.rayna_set_options.argtypes = [vp, C.POINTER(A.RenderOptsDesc)]
+        L.rayna_clear_accumulation.restype = C.c_int
+        L.rayna_clear_accumulation.argtypes = [vp]
+        L.rayna_render.restype = C.c_int
+        L.rayna_render.argtypes = [vp, C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_uint32),
+                                   C.POINTER(A.RenderStats)]
+        L.rayna_primary_hits.restype = C.c_int
+        L.rayna_primary_hits.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
+        L.rayna_dfs_order.restype = C.c_int
+        L.rayna_dfs_order.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
+                                      C.POINTER(C.c_int32)]
+        L.rayna_tree_hash.restype = C.c_int
+        L.rayna_tree_hash.argtypes = [vp, C.POINTER(C.c_uint64)]
+        L.rayna_scene_prepare.restype = C.c_int
+        L.rayna_scene_prepare.argtypes = [C.POINTER(A.SceneDesc), C.POINTER(C.c_uint64), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32)]
+        L.rayna_destroy.restype = None
+        L.rayna_destroy.argtypes = [vp]
+        L.rayna_last_error.restype = C.c_char_p
+        L.rayna_last_error.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _check(status: int, handle=None):
+    if status != A.RAYNA_OK:
+        msg = lib().rayna_last_error(handle)
+        raise RayError(status, msg.decode() if msg else "")
+
+
+@dataclass
+class RenderStats:
+    """render/render.rs:4-15, plus the GPU's own counters."""
+    duration: float
+    accum_frames: int
+    samples: int
+    segments: int
+    kernel_ms: float
+    extend_ms: float
+    shade_ms: float
+    extend_launches: int
+
+
+@dataclass
+class Render:
+    img: np.ndarray
+    stats: RenderStats
+
+
+def prepare_scene(scene):
+    """Host-only BVH construction (no GPU): (tree hash, depth-first (object, local) arrays, stack need)."""
+    d = scene.desc()
+    h = C.c_uint64(0)
+    n = C.c_int32(0)
+    need = C.c_int32(0)
+    _check(lib().rayna_scene_prepare(C.byref(d), C.byref(h), None, None, 0, C.byref(n), C.byref(need)))
+    obj = np.zeros(n.value, np.int32)
+    loc = np.zeros(n.value, np.int32)
+    _check(lib().rayna_scene_prepare(C.byref(d), C.byref(h), obj.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     loc.ctypes.data_as(C.POINTER(C.c_int32)), n.value, C.byref(n), C.byref(need)))
+    return int(h.value), obj, loc, int(need.value)
+
+
+class Renderer:
+    def __init__(self, scene, camera, options, num_threads: int = 0):
+        self._scene, self._camera, self._options = scene, camera, options
+        self._h = C.c_void_p()
+        self._keep = [scene.desc(), camera.desc(), options.desc()]
+        _check(lib().rayna_create(C.byref(self._keep[0]), C.byref(self._keep[1]), C.byref(self._keep[2]),
+                                  C.byref(self._h)))
+        self.num_threads = num_threads
+
+    @classmethod
+    def new_from(cls, scene, camera, options, num_threads: int = 0):
+        return cls(scene, camera, options, num_threads)
+
+    def close(self):
+        if self._h:
+            lib().rayna_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # getters (renderer.rs:46-51)
+    @property
+    def scene(self):
+        return self._scene
+
+    @property
+    def camera(self):
+        return self._camera
+
+    @property
+    def options(self):
+        return self._options
+
+    def set_scene(self, scene):
+        d = scene.desc()
+        _check(lib().rayna_set_scene(self._h, C.byref(d)), self._h)
+        self._scene = scene
+        self._keep[0] = d
+
+    def set_camera(self, camera):
+        d = camera.desc()
+        _check(lib().rayna_set_camera(self._h, C.byref(d)), self._h)
+        self._camera = camera
+
+    def set_options(self, options):
+        d = options.desc()
+        _check(lib().rayna_set_options(self._h, C.byref(d)), self._h)
+        self._options = options
+
+    def clear_accumulation(self):
+        _check(lib().rayna_clear_accumulation(self._h), self._h)
+
+    def set_thread_count(self, num_threads: int):
+        """The GPU renderer has no host worker pool; the count is kept for API compatibility."""
+        self.num_threads = num_threads
+
+    def render(self, layout: str = "xmajor", to_host: bool = True) -> Render:
+        o = self._options
+        img = np.zeros((o.width * o.height * 3,), np.float32) if to_host else None
+        dims = (C.c_uint32 * 2)()
+        st = A.RenderStats()
+        lay = A.LAYOUT_XMAJOR if layout == "xmajor" else A.LAYOUT_ROWMAJOR
+        ptr = img.ctypes.data_as(C.POINTER(C.c_float)) if to_host else None
+        _check(lib().rayna_render(self._h, ptr, lay, dims, C.byref(st)), self._h)
+        w, h = int(dims[0]), int(dims[1])
+        if img is not None:
+            img = img[: w * h * 3]
+            img = img.reshape(w, h, 3) if lay == A.LAYOUT_XMAJOR else img.reshape(h, w, 3)
+        stats = RenderStats(st.duration_s, int(st.accum_frames), int(st.samples), int(st.segments), st.kernel_ms,
+                            st.extend_ms, st.shade_ms, int(st.extend_launches))
+        return Render(img, stats)
+
+    # ---- parity hooks ----
+    def primary_hits(self, sample: int = 0):
+        o = self._options
+        rank = np.zeros(o.width * o.height, np.int32)
+        t = np.zeros(o.width * o.height, np.float32)
+        _check(lib().rayna_primary_hits(self._h, sample, rank.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        t.ctypes.data_as(C.POINTER(C.c_float))), self._h)
+        return rank.reshape(o.height, o.width), t.reshape(o.height, o.width)
+
+    def dfs_order(self):
+        n = C.c_int32(0)
+        _check(lib().rayna_dfs_order(self._h, None, None, 0, C.byref(n)), self._h)
+        obj = np.zeros(n.value, np.int32)
+        loc = np.zeros(n.value, np.int32)
+        _check(lib().rayna_dfs_order(self._h, obj.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     loc.ctypes.data_as(C.POINTER(C.c_int32)), n.value, C.byref(n)), self._h)
+        return obj, loc
+
+    def tree_hash(self) -> int:
+        h = C.c_uint64(0)
+        _check(lib().rayna_tree_hash(self._h, C.byref(h)), self._h)
+        return int(h.value)
