@@ -1,0 +1,157 @@
+"""Benchmark of the render hot path on one MI355X: frames of the 1080p triangle-mesh workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload triangle_mesh|random_spheres|sdf_volume]
+
+A step is one `Renderer::render()` frame: every sample of every pixel traced through the wavefront kernels, with
+the scene, BVH and path buffers already resident in HBM (the image stays on the device).  Prints one JSON line:
+Msamples/s (whole frame), Mrays/s (closest-hit queries), the roofline of the dominant kernel (extend, the
+closest-hit traversal) and the CPU baseline (the oracle's double-precision restatement of the engine's
+renderer, timed on this host on a bounded pixel sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBPS = 8000.0
+# Algorithmic bytes of one closest-hit query in the extend kernel: queue entry 4 B + ray origin and direction
+# 24 B read, hit distance, reference and flags 12 B written.
+EXTEND_BYTES_PER_SEGMENT = 40
+# Algorithmic bytes of the whole frame per segment (extend + shade) and per sample (generate + resolve):
+# extend 40 B; shade reads queue 4, ray 24, hit 12, throughput 12, radiance 12 and writes ray 24,
+# throughput 12, radiance 12, queue entry 4 -> 116 B; generate writes 48 B per sample, resolve reads 12 B.
+FRAME_BYTES_PER_SEGMENT = 40 + 116
+FRAME_BYTES_PER_SAMPLE = 48 + 12
+
+
+def workload(name: str):
+    from rayna_amd import presets
+    if name == "triangle_mesh":
+        return presets.triangle_mesh()
+    if name == "random_spheres":
+        return presets.random_spheres()
+    if name == "sdf_volume":
+        return presets.sdf_volume()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(scene, cam, opts, target_seconds: float, threads: int):
+    """The oracle's fp64 restatement of the engine renderer (engine BVH boxes, recursive integrator) on a
+    strided pixel sample of the same frame, sized to about `target_seconds` of work."""
+    import oracle
+    tw = oracle.TwinScene(scene)
+    # calibration pass: a tiny sample
+    row_step, col_step = max(1, opts.height // 4), max(1, opts.width // 8)
+    _, _, secs, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_REFERENCE, row_step=row_step,
+                              col_step=col_step, threads=threads)
+    n_px = ((opts.height + row_step - 1) // row_step) * ((opts.width + col_step - 1) // col_step)
+    per_px = max(secs / n_px, 1e-7)
+    want_px = max(1.0, target_seconds / per_px)
+    scale = max(1.0, (opts.width * opts.height / want_px) ** 0.5)
+    row_step = max(1, int(round(scale)))
+    col_step = max(1, int(round(scale)))
+    out, segs, secs, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_REFERENCE, row_step=row_step,
+                                   col_step=col_step, threads=threads)
+    n_px = out.shape[0] * out.shape[1]
+    samples = n_px * opts.samples
+    # the same sample with refitted tight boxes (identical image, the engine's culling removed)
+    out2, _, secs2, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_TIGHT, row_step=row_step,
+                                  col_step=col_step, threads=threads)
+    return {
+        "value": samples / secs / 1e6,
+        "unit": "Msamples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"fp64 oracle, engine BVH boxes, every {row_step}th row x every {col_step}th column at "
+                  f"{opts.samples} spp ({n_px} pixels, {samples} samples, {segs} closest-hit queries) in {secs:.1f} s",
+        "tight_boxes_value": samples / secs2 / 1e6,
+        "tight_boxes_identical_image": bool((out == out2).all()),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="triangle_mesh")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (the renderer is single-GPU; --gpus {a.gpus} is not supported)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from rayna_amd import Renderer
+
+    scene, cam, opts = workload(a.workload)
+    t_build = time.time()
+    r = Renderer(scene, cam, opts)
+    t_build = time.time() - t_build
+    for _ in range(a.warmup):
+        r.render(to_host=False)
+    torch.cuda.synchronize() if torch.cuda.is_available() else None
+    ext_ms = sh_ms = kern_ms = 0.0
+    segs = 0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        st = r.render(to_host=False).stats
+        ext_ms += st.extend_ms
+        sh_ms += st.shade_ms
+        kern_ms += st.kernel_ms
+        segs += st.segments
+        launches += st.extend_launches
+    torch.cuda.synchronize() if torch.cuda.is_available() else None
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    samples_per_frame = opts.width * opts.height * opts.samples
+    value = samples_per_frame * a.steps / secs / 1e6
+    ext_bytes = EXTEND_BYTES_PER_SEGMENT * segs
+    achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    frame_bytes = FRAME_BYTES_PER_SEGMENT * segs + FRAME_BYTES_PER_SAMPLE * samples_per_frame * a.steps
+    line = {
+        "metric": "Msamples/s at 1080p (triangle-mesh workload, equal-seed parity with the fp32 CPU restatement)",
+        "value": value,
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": secs * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "width": opts.width, "height": opts.height, "spp": opts.samples,
+                   "ray_depth": opts.ray_depth, "triangles": scene.n_triangles(), "seed": opts.seed},
+        "mrays_per_s": segs / secs / 1e6,
+        "segments_per_frame": segs // max(1, a.steps),
+        "extend_ms_per_frame": ext_ms / a.steps,
+        "shade_ms_per_frame": sh_ms / a.steps,
+        "kernel_ms_per_frame": kern_ms / a.steps,
+        "extend_launches_per_frame": launches // max(1, a.steps),
+        "frame_alg_gbps": frame_bytes / secs / 1e9,
+        "scene_build_s": t_build,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "k_extend", "bytes_per_segment": EXTEND_BYTES_PER_SEGMENT},
+    }
+    r.close()
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(scene, cam, opts, a.cpu_seconds, a.cpu_threads)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

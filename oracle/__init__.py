@@ -42,7 +42,7 @@ def lib() -> C.CDLL:
         L.twin_camera_invalid.argtypes = [C.POINTER(A.CameraDesc)]
         L.twin_render.restype = C.c_uint64
         L.twin_render.argtypes = [vp, C.POINTER(A.CameraDesc), C.POINTER(A.RenderOptsDesc), C.c_int32, C.c_int32,
-                                  C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_float),
+                                  C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_float),
                                   C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.twin_primary_hits.argtypes = [vp, C.POINTER(A.CameraDesc), C.POINTER(A.RenderOptsDesc), C.c_int32,
                                         C.c_int32, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
@@ -100,17 +100,18 @@ class TwinScene:
         return items, nch
 
     def render(self, cam, opts, precision: int = 64, box_mode: int = BOX_REFERENCE, integrator: Optional[int] = None,
-               frame: int = 0, row_start: int = 0, row_step: int = 1, threads: int = 0):
-        """Per-pixel sample means (rows x width x 3, float32), closest-hit query count, seconds, prim tests."""
+               frame: int = 0, row_start: int = 0, row_step: int = 1, threads: int = 0, col_step: int = 1):
+        """Per-pixel sample means (rows x cols x 3, float32), closest-hit query count, seconds, prim tests."""
         if integrator is None:
             integrator = INTEGRATOR_FORWARD if precision == 32 else INTEGRATOR_RECURSIVE
         n_rows = (opts.height - row_start + row_step - 1) // row_step
-        out = np.zeros((n_rows, opts.width, 3), np.float32)
+        n_cols = (opts.width + col_step - 1) // col_step
+        out = np.zeros((n_rows, n_cols, 3), np.float32)
         secs = C.c_double(0)
         tests = C.c_uint64(0)
         cd, od = cam.desc(), opts.desc()
         segs = lib().twin_render(self.h, C.byref(cd), C.byref(od), precision, box_mode, integrator, frame, row_start,
-                                 row_step, threads, _p(out, C.c_float), C.byref(secs), C.byref(tests))
+                                 row_step, col_step, threads, _p(out, C.c_float), C.byref(secs), C.byref(tests))
         return out, int(segs), secs.value, int(tests.value)
 
     def primary_hits(self, cam, opts, precision: int = 32, box_mode: int = BOX_TIGHT, sample: int = 0):

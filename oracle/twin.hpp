@@ -1253,16 +1253,19 @@ struct RenderArgs {
     int integrator;
     uint32_t row_start, row_step;  // rows row_start, row_start + row_step, ...
     int threads;
+    uint32_t col_step;             // columns 0, col_step, 2 col_step, ... of each selected row
 };
 
 // One frame of per-pixel sample means (render_px_msaa's mean, renderer.rs:398-402), row-major, for the
-// selected rows (out has n_rows * width * 3 floats).  Returns the number of closest-hit queries.
+// selected rows and columns (out has n_rows * n_cols * 3 floats).  Returns the number of closest-hit queries.
 template <class R>
 inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_tests_out) {
     const rayna_render_opts &o = *a.opts;
     const Viewport vp = calculate_viewport(*a.cam);
     const RngKey key = make_key(o.seed, a.frame);
     const uint32_t n_rows = (o.height - a.row_start + a.row_step - 1) / a.row_step;
+    const uint32_t cstep = a.col_step == 0 ? 1 : a.col_step;
+    const uint32_t n_cols = (o.width + cstep - 1) / cstep;
     std::atomic<uint32_t> next{0};
     std::atomic<uint64_t> segs{0}, tests{0};
     auto worker = [&]() {
@@ -1271,7 +1274,8 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
             const uint32_t r = next.fetch_add(1);
             if (r >= n_rows) break;
             const uint32_t y = a.row_start + r * a.row_step;
-            for (uint32_t x = 0; x < o.width; ++x) {
+            for (uint32_t cx = 0; cx < n_cols; ++cx) {
+                const uint32_t x = cx * cstep;
                 Tracer<R> tr{a.scene, key, y * o.width + x, 0, o.ray_depth, a.box_mode};
                 Col sum{0, 0, 0};
                 for (uint32_t k = 0; k < o.samples; ++k) {
@@ -1281,7 +1285,7 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
                     sum = sum + c;
                 }
                 const Col mean = sum / (float)o.samples;
-                float *px = out + ((size_t)r * o.width + x) * 3;
+                float *px = out + ((size_t)r * n_cols + cx) * 3;
                 px[0] = mean.r;
                 px[1] = mean.g;
                 px[2] = mean.b;

@@ -64,13 +64,14 @@ int32_t twin_scene_tree_walk(const twin_scene *t, int32_t mesh, int32_t *items, 
 // 0 if the camera yields a viewport, 1 if calculate_viewport fails (render_failed path).
 int32_t twin_camera_invalid(const rayna_camera *cam) { return calculate_viewport(*cam).valid ? 0 : 1; }
 
-// One frame: per-pixel sample means of rows row_start, row_start + row_step, ... (row-major, 3 floats per
-// pixel).  precision 32 or 64; box_mode 0 = engine boxes, 1 = tight conservative boxes; integrator
+// One frame: per-pixel sample means of rows row_start, row_start + row_step, ... and of every col_step-th
+// column (row-major, 3 floats per pixel).  precision 32 or 64; box_mode 0 = engine boxes, 1 = tight conservative boxes; integrator
 // 0 = recursive, 1 = forward.  Returns the number of closest-hit queries; *seconds gets the wall time.
 uint64_t twin_render(const twin_scene *t, const rayna_camera *cam, const rayna_render_opts *opts, int32_t precision,
                      int32_t box_mode, int32_t integrator, uint32_t frame, uint32_t row_start, uint32_t row_step,
-                     int32_t threads, float *out, double *seconds, uint64_t *prim_tests) {
-    RenderArgs a{&t->s, cam, opts, frame, box_mode, integrator, row_start, row_step == 0 ? 1u : row_step, threads};
+                     uint32_t col_step, int32_t threads, float *out, double *seconds, uint64_t *prim_tests) {
+    RenderArgs a{&t->s, cam, opts, frame, box_mode, integrator, row_start, row_step == 0 ? 1u : row_step, threads,
+                 col_step == 0 ? 1u : col_step};
     const auto t0 = std::chrono::steady_clock::now();
     uint64_t segs;
     if (precision == 32) segs = render_rows<float>(a, out, prim_tests);

@@ -16,6 +16,7 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int EXT_BLOCK = 128;
+constexpr int EXT_LDS_STACK = 16;              // traversal stack entries per lane held in LDS
 constexpr float T_MIN = 1e-3f;                // render interval start (renderer.rs:239)
 constexpr float BOX_SLACK = 1.0f + 0x1.0p-16f;  // conservative slab test: culling never drops the closest hit
 
@@ -297,21 +298,46 @@ __device__ __forceinline__ uint32_t make_ref(int kind, int32_t index) {
 // Closest hit over the BVH with ties to the higher depth-first rank (the engine's shrinking inclusive
 // interval, mesh/advanced/bvh.rs:77-91), then the unbounded objects with the first of equal minima winning
 // (object/list.rs:358-363).  The traversal stack lives in LDS, one column per lane.
-template <int STACK>
-__device__ Best closest_hit(const SceneView &sv, const DRay &ray, const dm::Key &key, uint32_t pixel, uint32_t sample,
-                            uint32_t bounce, int32_t *s_node, float *s_tent) {
-    Best b{FLT_MAX, -1, 0u, 0u};
-    const int lane = threadIdx.x;
-    int sp = 0;
-    if (sv.root >= 0) {
-        s_node[lane] = sv.root;
-        s_tent[lane] = -__builtin_inff();
-        sp = 1;
+// Traversal stack of one lane: the top SL entries in LDS (one column per lane, conflict-free), deeper entries
+// in this thread's column of a global overflow array (`stride` = threads in the grid).  The host sizes the
+// overflow for the deepest stack the tree can need, so no push is ever dropped.
+struct LaneStack {
+    int32_t *s_node;
+    float *s_tent;
+    int2 *g;
+    uint32_t stride;
+    int sp;
+    __device__ __forceinline__ void push(int32_t n, float t) {
+        if (sp < EXT_LDS_STACK) {
+            s_node[sp * EXT_BLOCK + threadIdx.x] = n;
+            s_tent[sp * EXT_BLOCK + threadIdx.x] = t;
+        } else {
+            g[(size_t)(sp - EXT_LDS_STACK) * stride] = make_int2(n, __float_as_int(t));
+        }
+        ++sp;
     }
-    while (sp > 0) {
+    __device__ __forceinline__ void pop(int32_t *n, float *t) {
         --sp;
-        const int32_t node = s_node[sp * EXT_BLOCK + lane];
-        const float tent = s_tent[sp * EXT_BLOCK + lane];
+        if (sp < EXT_LDS_STACK) {
+            *n = s_node[sp * EXT_BLOCK + threadIdx.x];
+            *t = s_tent[sp * EXT_BLOCK + threadIdx.x];
+        } else {
+            const int2 e = g[(size_t)(sp - EXT_LDS_STACK) * stride];
+            *n = e.x;
+            *t = __int_as_float(e.y);
+        }
+    }
+};
+
+__device__ Best closest_hit(const SceneView &sv, const DRay &ray, const dm::Key &key, uint32_t pixel, uint32_t sample,
+                            uint32_t bounce, LaneStack &st) {
+    Best b{FLT_MAX, -1, 0u, 0u};
+    st.sp = 0;
+    if (sv.root >= 0) st.push(sv.root, -__builtin_inff());
+    while (st.sp > 0) {
+        int32_t node;
+        float tent;
+        st.pop(&node, &tent);
         if (tent > b.t * BOX_SLACK) continue;
         const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
         const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
@@ -396,11 +422,7 @@ __device__ Best closest_hit(const SceneView &sv, const DRay &ray, const dm::Key 
                             ref = make_ref(ITEM_VOLUME, start + i);
                             break;
                         case ITEM_LINK:
-                            if (sp < STACK) {
-                                s_node[sp * EXT_BLOCK + lane] = it.index;
-                                s_tent[sp * EXT_BLOCK + lane] = tmin[k];
-                                ++sp;
-                            }
+                            st.push(it.index, tmin[k]);
                             break;
                         default: break;
                     }
@@ -434,13 +456,8 @@ __device__ Best closest_hit(const SceneView &sv, const DRay &ray, const dm::Key 
         RAYNA_CSWAP(1, 2)
 #undef RAYNA_CSWAP
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (cn[k] >= 0 && sp < STACK) {
-                s_node[sp * EXT_BLOCK + lane] = cn[k];
-                s_tent[sp * EXT_BLOCK + lane] = ct[k];
-                ++sp;
-            }
-        }
+        for (int k = 0; k < 4; ++k)
+            if (cn[k] >= 0) st.push(cn[k], ct[k]);
     }
     for (int i = 0; i < sv.n_sdfs; ++i) {
         const GSdf f = sv.sdfs[i];
@@ -668,19 +685,20 @@ __global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers 
 }
 
 // Closest hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501).
-template <int STACK>
 __global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams fp, PathBuffers pb,
                                                       const int32_t *queue, const uint32_t *count_ptr,
-                                                      uint32_t depth) {
-    __shared__ int32_t s_node[STACK * EXT_BLOCK];
-    __shared__ float s_tent[STACK * EXT_BLOCK];
+                                                      uint32_t depth, int2 *overflow) {
+    __shared__ int32_t s_node[EXT_LDS_STACK * EXT_BLOCK];
+    __shared__ float s_tent[EXT_LDS_STACK * EXT_BLOCK];
     const uint32_t count = queue ? *count_ptr : fp.n_slots;
     const dm::Key key{fp.key0, fp.key1};
+    const uint32_t gid = blockIdx.x * EXT_BLOCK + threadIdx.x;
+    LaneStack st{s_node, s_tent, overflow + gid, gridDim.x * EXT_BLOCK, 0};
     for (uint32_t q = blockIdx.x * EXT_BLOCK + threadIdx.x; q < count; q += gridDim.x * EXT_BLOCK) {
         const uint32_t slot = queue ? (uint32_t)queue[q] : q;
         const DRay ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
         const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
-        const Best b = closest_hit<STACK>(sv, ray, key, pixel, sample, depth, s_node, s_tent);
+        const Best b = closest_hit(sv, ray, key, pixel, sample, depth, st);
         pb.hit_t[slot] = b.t;
         pb.hit_ref[slot] = b.ref;
         pb.hit_aux[slot] = b.aux;
@@ -809,18 +827,16 @@ inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block -
 
 }  // namespace
 
-int extend_stack_capacity() { return 64; }
+int extend_lds_stack() { return EXT_LDS_STACK; }
+int extend_block() { return EXT_BLOCK; }
 
 void launch_generate(const FrameParams &fp, const PathBuffers &pb, hipStream_t s) {
     hipLaunchKernelGGL(k_generate, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb);
 }
 
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
-                   const uint32_t *count, uint32_t depth, int grid, hipStream_t s) {
-    if (sv.stack_need <= 32)
-        hipLaunchKernelGGL(k_extend<32>, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth);
-    else
-        hipLaunchKernelGGL(k_extend<64>, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth);
+                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, hipStream_t s) {
+    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth, overflow);
 }
 
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,

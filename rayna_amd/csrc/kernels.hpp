@@ -33,15 +33,11 @@ struct FrameParams {
     uint32_t n_slots;        // slots in the chunk
 };
 
-struct Counters {
-    uint32_t *queue_count;   // [ray_depth + 2]: live paths entering each depth of the current chunk
-    unsigned long long *segments;  // running total of closest-hit queries
-};
-
 void launch_generate(const FrameParams &fp, const PathBuffers &pb, hipStream_t s);
 // queue == nullptr: identity queue of fp.n_slots entries; else queue[0 .. *count)
+// overflow: traversal stack entries beyond the LDS part, extend_block() * grid * (stack_need - extend_lds_stack())
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
-                   const uint32_t *count, uint32_t depth, int grid, hipStream_t s);
+                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, hipStream_t s);
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
                   hipStream_t s);
@@ -53,6 +49,7 @@ void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s);
 void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
                            unsigned long long *segments, hipStream_t s);
 
-int extend_stack_capacity();
+int extend_lds_stack();
+int extend_block();
 
 }  // namespace rayna

@@ -25,6 +25,7 @@ constexpr uint64_t TARGET_SLOTS = 1ull << 23;  // paths in flight per chunk (~8.
 struct DevScene {
     std::vector<void *> allocs;
     SceneView view{};
+    int2 *overflow = nullptr;
 };
 
 }  // namespace
@@ -125,6 +126,7 @@ void free_scene(DevScene &ds) {
     for (void *p : ds.allocs) (void)hipFree(p);
     ds.allocs.clear();
     ds.view = SceneView{};
+    ds.overflow = nullptr;
 }
 
 rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
@@ -132,8 +134,7 @@ rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
     std::string err;
     const rayna_status st = pack_scene(*d, ps, err);
     if (st != RAYNA_OK) return fail(r, st, err);
-    if (ps.stack_need > extend_stack_capacity())
-        return fail(r, RAYNA_UNSUPPORTED, "scene BVH is deeper than the traversal stack supports");
+    if (ps.stack_need > 4096) return fail(r, RAYNA_UNSUPPORTED, "scene BVH is deeper than 4096 stack entries");
     DevScene ds;
     SceneView &v = ds.view;
     rayna_status s2;
@@ -154,6 +155,20 @@ rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
     v.n_sdfs = (int32_t)ps.sdfs.size();
     v.skybox = ps.skybox;
     v.stack_need = ps.stack_need;
+    // traversal stack overflow beyond the LDS entries, one column per extend thread
+    const size_t deep = (size_t)std::max(0, ps.stack_need - extend_lds_stack());
+    if (deep > 0) {
+        void *p = nullptr;
+        const size_t bytes = deep * (size_t)r->grid_extend * extend_block() * sizeof(int2);
+        const hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            free_scene(ds);
+            return fail(r, e == hipErrorOutOfMemory ? RAYNA_OUT_OF_MEMORY : RAYNA_DEVICE_ERROR,
+                        std::string("overflow stack: ") + hipGetErrorString(e));
+        }
+        ds.allocs.push_back(p);
+        ds.overflow = static_cast<int2 *>(p);
+    }
     free_scene(r->dev);
     r->dev = ds;
     r->packed = std::move(ps);
@@ -254,7 +269,7 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
         int32_t *qout = r->queue[(depth + 1) & 1];
         uint32_t *cout = r->queue_count + depth + 1;
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
-        launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->stream);
+        launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, r->stream);
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
         launch_shade(r->dev.view, fp, r->pb, qin, cin, qout, cout, depth, r->grid_shade, r->stream);
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
@@ -446,7 +461,7 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     PathBuffers pb = r->pb;
     pb.hit_rank = rank_dev;
     launch_generate(fp, pb, r->stream);
-    launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->stream);
+    launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, r->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(prim_rank, rank_dev, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(t, pb.hit_t, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);

@@ -151,6 +151,10 @@ def icosphere(subdiv: int, seed: int, amplitude: float = 0.1, frequency: float =
     for m in range(3):
         np.add.at(vn, tris[:, m], fn)
     vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+    # Wind clockwise as seen from outside: the engine's Triangle::intersect treats the side that
+    # (v1 - v0) x (v2 - v0) faces as the back face (front_face = det < 0, triangle.rs:321), so outward-facing
+    # hits need the opposite winding; the vertex normals stay outward.
+    tris = tris[:, [0, 2, 1]]
     return verts[tris], vn[tris], np.concatenate(face_of)
 
 
@@ -166,10 +170,10 @@ def triangle_mesh(seed: int = 1, subdiv: int = 8, width: int = 1920, height: int
         else:
             mat = MetalMaterial(tuple(float(x) for x in 0.5 + 0.5 * rng.random(3)), float(0.3 * rng.random()))
         objects.append(SimpleObject(TriangleListMesh(tv[sel], tn[sel]), mat))
-    # ground: two triangles at y = -1.25 spanning [-8, 8]^2, facing up
+    # ground: two triangles at y = -1.25 spanning [-8, 8]^2, facing up (wound clockwise seen from above)
     g = -1.25
     quad = np.array([[-8, g, -8], [-8, g, 8], [8, g, 8], [8, g, -8]], float)
-    gv = np.array([[quad[0], quad[1], quad[2]], [quad[0], quad[2], quad[3]]])
+    gv = np.array([[quad[0], quad[2], quad[1]], [quad[0], quad[3], quad[2]]])
     gn = np.tile(np.array([0.0, 1.0, 0.0]), (2, 3, 1))
     objects.append(SimpleObject(TriangleListMesh(gv, gn), LambertianMaterial((0.5, 0.5, 0.5))))
     # 8 area lights on a ring of radius 3 at height 2.5, 1x1, facing down (light is two-sided)
