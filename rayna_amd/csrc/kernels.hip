@@ -15,8 +15,15 @@ namespace rayna {
 namespace {
 
 constexpr int BLOCK = 256;
-constexpr int EXT_BLOCK = 128;
-constexpr int EXT_LDS_STACK = 16;              // traversal stack entries per lane held in LDS
+// Extend-kernel shape (overridable at build time for tuning sweeps).
+#ifndef RAYNA_EXT_BLOCK
+#define RAYNA_EXT_BLOCK 128
+#endif
+#ifndef RAYNA_EXT_LDS_STACK
+#define RAYNA_EXT_LDS_STACK 16
+#endif
+constexpr int EXT_BLOCK = RAYNA_EXT_BLOCK;
+constexpr int EXT_LDS_STACK = RAYNA_EXT_LDS_STACK;  // traversal stack entries per lane held in LDS
 constexpr float T_MIN = 1e-3f;                // render interval start (renderer.rs:239)
 constexpr float BOX_SLACK = 1.0f + 0x1.0p-16f;  // conservative slab test: culling never drops the closest hit
 
@@ -295,179 +302,7 @@ __device__ __forceinline__ uint32_t make_ref(int kind, int32_t index) {
     return ((uint32_t)kind << HIT_KIND_SHIFT) | ((uint32_t)index & HIT_INDEX_MASK);
 }
 
-// Closest hit over the BVH with ties to the higher depth-first rank (the engine's shrinking inclusive
-// interval, mesh/advanced/bvh.rs:77-91), then the unbounded objects with the first of equal minima winning
-// (object/list.rs:358-363).  The traversal stack lives in LDS, one column per lane.
-// Traversal stack of one lane: the top SL entries in LDS (one column per lane, conflict-free), deeper entries
-// in this thread's column of a global overflow array (`stride` = threads in the grid).  The host sizes the
-// overflow for the deepest stack the tree can need, so no push is ever dropped.
-struct LaneStack {
-    int32_t *s_node;
-    float *s_tent;
-    int2 *g;
-    uint32_t stride;
-    int sp;
-    __device__ __forceinline__ void push(int32_t n, float t) {
-        if (sp < EXT_LDS_STACK) {
-            s_node[sp * EXT_BLOCK + threadIdx.x] = n;
-            s_tent[sp * EXT_BLOCK + threadIdx.x] = t;
-        } else {
-            g[(size_t)(sp - EXT_LDS_STACK) * stride] = make_int2(n, __float_as_int(t));
-        }
-        ++sp;
-    }
-    __device__ __forceinline__ void pop(int32_t *n, float *t) {
-        --sp;
-        if (sp < EXT_LDS_STACK) {
-            *n = s_node[sp * EXT_BLOCK + threadIdx.x];
-            *t = s_tent[sp * EXT_BLOCK + threadIdx.x];
-        } else {
-            const int2 e = g[(size_t)(sp - EXT_LDS_STACK) * stride];
-            *n = e.x;
-            *t = __int_as_float(e.y);
-        }
-    }
-};
-
-__device__ Best closest_hit(const SceneView &sv, const DRay &ray, const dm::Key &key, uint32_t pixel, uint32_t sample,
-                            uint32_t bounce, LaneStack &st) {
-    Best b{FLT_MAX, -1, 0u, 0u};
-    st.sp = 0;
-    if (sv.root >= 0) st.push(sv.root, -__builtin_inff());
-    while (st.sp > 0) {
-        int32_t node;
-        float tent;
-        st.pop(&node, &tent);
-        if (tent > b.t * BOX_SLACK) continue;
-        const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
-        const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
-        const int4 ch = reinterpret_cast<const int4 *>(np)[6];
-        const int4 inf = reinterpret_cast<const int4 *>(np)[7];
-        const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, lo_y[4] = {ly.x, ly.y, ly.z, ly.w},
-                    lo_z[4] = {lz.x, lz.y, lz.z, lz.w};
-        const float hi_x[4] = {hx.x, hx.y, hx.z, hx.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w},
-                    hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
-        const int32_t child[4] = {ch.x, ch.y, ch.z, ch.w};
-        const int32_t info[4] = {inf.x, inf.y, inf.z, inf.w};
-        float tmin[4];
-        bool hit[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float tx1 = (lo_x[k] - ray.o.x) * ray.inv.x, tx2 = (hi_x[k] - ray.o.x) * ray.inv.x;
-            float t0 = fminf(tx1, tx2), t1 = fmaxf(tx1, tx2);
-            const float ty1 = (lo_y[k] - ray.o.y) * ray.inv.y, ty2 = (hi_y[k] - ray.o.y) * ray.inv.y;
-            t0 = fmaxf(t0, fminf(ty1, ty2));
-            t1 = fminf(t1, fmaxf(ty1, ty2));
-            const float tz1 = (lo_z[k] - ray.o.z) * ray.inv.z, tz2 = (hi_z[k] - ray.o.z) * ray.inv.z;
-            t0 = fmaxf(t0, fminf(tz1, tz2));
-            t1 = fminf(t1, fmaxf(tz1, tz2));
-            const float lo = fmaxf(t0, T_MIN), hi = fminf(t1, b.t);
-            tmin[k] = t0;
-            hit[k] = info[k] != INFO_EMPTY && lo <= hi * BOX_SLACK;
-        }
-        // leaf ranges first, so the distance bound tightens before inner children are ordered
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (!hit[k] || info[k] <= 0) continue;
-            const int cnt = info[k] & 0xf;
-            const int start = child[k];
-            if ((info[k] >> 4) == LEAF_TRIS) {
-                for (int i = 0; i < cnt; ++i) {
-                    const GTri g = sv.tris[start + i];
-                    float t, u, v, det;
-                    if (!tri_t(g, ray, T_MIN, b.t, &t, &u, &v, &det)) continue;
-                    if (!(t < b.t || g.rank > b.rank)) continue;
-                    if (g.flags & 1) {
-                        f3 nn;
-                        if (!tri_normal(sv.tri_normals[start + i], u, v, &nn)) continue;
-                    }
-                    b = Best{t, g.rank, make_ref(ITEM_TRIANGLE, start + i), 0u};
-                }
-            } else {
-                for (int i = 0; i < cnt; ++i) {
-                    const GItem it = sv.items[start + i];
-                    float t;
-                    bool ok = false;
-                    uint32_t ref = 0;
-                    switch (it.kind) {
-                        case ITEM_SPHERE:
-                            ok = sphere_t(sv.spheres[it.index], ray, T_MIN, b.t, &t);
-                            ref = make_ref(ITEM_SPHERE, start + i);
-                            break;
-                        case ITEM_TRIANGLE: {
-                            const GTri g = sv.tris[it.index];
-                            float u, v, det;
-                            ok = tri_t(g, ray, T_MIN, b.t, &t, &u, &v, &det);
-                            if (ok && (g.flags & 1)) {
-                                f3 nn;
-                                ok = tri_normal(sv.tri_normals[it.index], u, v, &nn);
-                            }
-                            ref = make_ref(ITEM_TRIANGLE, it.index);
-                            break;
-                        }
-                        case ITEM_PARALLELOGRAM:
-                            ok = para_t(sv.paras[it.index], ray, T_MIN, b.t, &t);
-                            ref = make_ref(ITEM_PARALLELOGRAM, start + i);
-                            break;
-                        case ITEM_AXIS_BOX: {
-                            int axis;
-                            float w;
-                            f3 sg;
-                            ok = box_t(sv.boxes[it.index], ray, T_MIN, b.t, &t, &axis, &w, &sg);
-                            ref = make_ref(ITEM_AXIS_BOX, start + i);
-                            break;
-                        }
-                        case ITEM_VOLUME:
-                            ok = volume_t(sv, sv.volumes[it.index], ray, b.t, key, pixel, sample, bounce, &t);
-                            ref = make_ref(ITEM_VOLUME, start + i);
-                            break;
-                        case ITEM_LINK:
-                            st.push(it.index, tmin[k]);
-                            break;
-                        default: break;
-                    }
-                    if (ok && (t < b.t || it.rank > b.rank)) b = Best{t, it.rank, ref, 0u};
-                }
-            }
-        }
-        // inner children: push far to near so the nearest is popped first (4-input sorting network,
-        // descending distance; missed or non-inner slots sort last with key -inf)
-        int32_t cn[4];
-        float ct[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const bool inner = hit[k] && info[k] == INFO_INNER;
-            cn[k] = inner ? child[k] : -1;
-            ct[k] = inner ? tmin[k] : -__builtin_inff();
-        }
-#define RAYNA_CSWAP(a, b)                                      \
-    if (ct[a] < ct[b]) {                                       \
-        const float tt_ = ct[a];                               \
-        ct[a] = ct[b];                                         \
-        ct[b] = tt_;                                           \
-        const int32_t nn_ = cn[a];                             \
-        cn[a] = cn[b];                                         \
-        cn[b] = nn_;                                           \
-    }
-        RAYNA_CSWAP(0, 1)
-        RAYNA_CSWAP(2, 3)
-        RAYNA_CSWAP(0, 2)
-        RAYNA_CSWAP(1, 3)
-        RAYNA_CSWAP(1, 2)
-#undef RAYNA_CSWAP
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (cn[k] >= 0) st.push(cn[k], ct[k]);
-    }
-    for (int i = 0; i < sv.n_sdfs; ++i) {
-        const GSdf f = sv.sdfs[i];
-        float t;
-        bool front;
-        if (sdf_march(sv, f, ray, &t, &front) && t < b.t)
-            b = Best{t, -2 - f.list_pos, make_ref(ITEM_SDF, i), front ? 0u : HIT_AUX_BACK};
-    }
-    return b;
-}
+#include "traverse.hpp"
 
 // ------------------------------------------------------------------------------------------ shading
 
@@ -683,29 +518,6 @@ __global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers 
     pb.lg[i] = 0.0f;
     pb.lb[i] = 0.0f;
 }
-
-// Closest hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501).
-__global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams fp, PathBuffers pb,
-                                                      const int32_t *queue, const uint32_t *count_ptr,
-                                                      uint32_t depth, int2 *overflow) {
-    __shared__ int32_t s_node[EXT_LDS_STACK * EXT_BLOCK];
-    __shared__ float s_tent[EXT_LDS_STACK * EXT_BLOCK];
-    const uint32_t count = queue ? *count_ptr : fp.n_slots;
-    const dm::Key key{fp.key0, fp.key1};
-    const uint32_t gid = blockIdx.x * EXT_BLOCK + threadIdx.x;
-    LaneStack st{s_node, s_tent, overflow + gid, gridDim.x * EXT_BLOCK, 0};
-    for (uint32_t q = blockIdx.x * EXT_BLOCK + threadIdx.x; q < count; q += gridDim.x * EXT_BLOCK) {
-        const uint32_t slot = queue ? (uint32_t)queue[q] : q;
-        const DRay ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
-        const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
-        const Best b = closest_hit(sv, ray, key, pixel, sample, depth, st);
-        pb.hit_t[slot] = b.t;
-        pb.hit_ref[slot] = b.ref;
-        pb.hit_aux[slot] = b.aux;
-        if (pb.hit_rank) pb.hit_rank[slot] = b.rank;
-    }
-}
-
 // One bounce of ray_colour_recursive in forward form: L += T * Le; scatter; T = reflected(T); paths that
 // scatter below the depth limit are compacted into the next queue (one atomic per wave).
 __global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, PathBuffers pb,
@@ -828,6 +640,12 @@ inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block -
 }  // namespace
 
 int extend_lds_stack() { return EXT_LDS_STACK; }
+
+int extend_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend, EXT_BLOCK, 0) != hipSuccess || n < 1) n = 4;
+    return n;
+}
 int extend_block() { return EXT_BLOCK; }
 
 void launch_generate(const FrameParams &fp, const PathBuffers &pb, hipStream_t s) {
@@ -835,8 +653,9 @@ void launch_generate(const FrameParams &fp, const PathBuffers &pb, hipStream_t s
 }
 
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
-                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, hipStream_t s) {
-    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth, overflow);
+                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work, hipStream_t s) {
+    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth, overflow,
+                       work);
 }
 
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
@@ -868,3 +687,16 @@ void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t fir
 }
 
 }  // namespace rayna
+
+#ifdef RAYNA_TRAVERSAL_STATS
+// Measurement builds only: the traversal counters summed since the last call (node pops, node expansions,
+// primitive tests, wave-loop iterations per lane), then reset.
+extern "C" int rayna_traversal_stats(unsigned long long out[4]) {
+    if (hipDeviceSynchronize() != hipSuccess) return -1;
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(rayna::g_trav_stats), sizeof(unsigned long long) * 4) != hipSuccess)
+        return -1;
+    const unsigned long long zero[4] = {0, 0, 0, 0};
+    if (hipMemcpyToSymbol(HIP_SYMBOL(rayna::g_trav_stats), zero, sizeof(zero)) != hipSuccess) return -1;
+    return 0;
+}
+#endif

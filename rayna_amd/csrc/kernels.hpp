@@ -35,9 +35,10 @@ struct FrameParams {
 
 void launch_generate(const FrameParams &fp, const PathBuffers &pb, hipStream_t s);
 // queue == nullptr: identity queue of fp.n_slots entries; else queue[0 .. *count)
-// overflow: traversal stack entries beyond the LDS part, extend_block() * grid * (stack_need - extend_lds_stack())
+// overflow: traversal stack entries beyond the LDS part, extend_block() * grid * (stack_need - extend_lds_stack());
+// work: a zeroed counter the launch's waves take rays from
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
-                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, hipStream_t s);
+                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work, hipStream_t s);
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
                   hipStream_t s);
@@ -50,6 +51,7 @@ void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t fir
                            unsigned long long *segments, hipStream_t s);
 
 int extend_lds_stack();
+int extend_blocks_per_cu();  // resident extend blocks per CU (the persistent grid is this times the CU count)
 int extend_block();
 
 }  // namespace rayna

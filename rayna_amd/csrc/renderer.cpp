@@ -209,7 +209,8 @@ rayna_status alloc_paths(rayna_renderer *r) {
     if ((st = alloc(slots * sizeof(uint32_t), (void **)&r->pb.hit_aux)) != RAYNA_OK) return st;
     for (int q = 0; q < 2; ++q)
         if ((st = alloc(slots * sizeof(int32_t), (void **)&r->queue[q])) != RAYNA_OK) return st;
-    if ((st = alloc((o.ray_depth + 2) * sizeof(uint32_t), (void **)&r->queue_count)) != RAYNA_OK) return st;
+    // [0, D+2): paths entering each depth; [D+2, 2D+4): the extend launches' ray counters; [2D+4]: spare
+    if ((st = alloc((2 * (o.ray_depth + 2) + 1) * sizeof(uint32_t), (void **)&r->queue_count)) != RAYNA_OK) return st;
     if ((st = alloc(sizeof(unsigned long long), (void **)&r->segments)) != RAYNA_OK) return st;
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->sum)) != RAYNA_OK) return st;
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->acc)) != RAYNA_OK) return st;
@@ -235,7 +236,7 @@ rayna_status set_grid(rayna_renderer *r) {
     hipDeviceProp_t prop;
     HIP_TRY(r, hipGetDeviceProperties(&prop, r->device));
     const int cus = std::max(1, prop.multiProcessorCount);
-    r->grid_extend = cus * 8;
+    r->grid_extend = cus * extend_blocks_per_cu();
     r->grid_shade = cus * 8;
     return RAYNA_OK;
 }
@@ -261,7 +262,7 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
                  uint64_t *extend_launches) {
     fp.s0 = s0;
     fp.n_slots = fp.n_pixels * n;
-    launch_fill_u32(r->queue_count, 0u, r->opts.ray_depth + 2, r->stream);
+    launch_fill_u32(r->queue_count, 0u, 2 * (r->opts.ray_depth + 2), r->stream);
     launch_generate(fp, r->pb, r->stream);
     for (uint32_t depth = 0; depth <= r->opts.ray_depth; ++depth) {
         const int32_t *qin = depth == 0 ? nullptr : r->queue[depth & 1];
@@ -269,7 +270,8 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
         int32_t *qout = r->queue[(depth + 1) & 1];
         uint32_t *cout = r->queue_count + depth + 1;
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
-        launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, r->stream);
+        uint32_t *work = r->queue_count + (r->opts.ray_depth + 2) + depth;
+        launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, work, r->stream);
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
         launch_shade(r->dev.view, fp, r->pb, qin, cin, qout, cout, depth, r->grid_shade, r->stream);
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
@@ -461,7 +463,9 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     PathBuffers pb = r->pb;
     pb.hit_rank = rank_dev;
     launch_generate(fp, pb, r->stream);
-    launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, r->stream);
+    uint32_t *work = r->queue_count + 2 * (r->opts.ray_depth + 2);
+    launch_fill_u32(work, 0u, 1, r->stream);
+    launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, work, r->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(prim_rank, rank_dev, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(t, pb.hit_t, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);

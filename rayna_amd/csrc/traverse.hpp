@@ -1,0 +1,313 @@
+// Closest-hit traversal of the extend kernel (included by kernels.hip inside its anonymous namespace, after the
+// primitive tests).
+//
+// Result: the closest hit over the BVH with ties to the higher depth-first rank -- what the engine's
+// shrinking inclusive interval computes (mesh/advanced/bvh.rs:77-91, object/bvh.rs:443-481) -- then the
+// unbounded objects, where the first of equal minima wins (object/list.rs:358-363).  Culling is conservative,
+// so the result does not depend on the order nodes are visited in.
+//
+// Organisation (Aila & Laine, "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009):
+//  * while-while: leaf ranges are pushed on the stack like inner nodes; a lane first pops inner nodes until
+//    it reaches a leaf, then tests that leaf's primitives one per iteration, so the lanes of a wave run the
+//    box code together and the primitive code together;
+//  * persistent waves with dynamic fetch: each wave takes rays from a per-launch counter whenever enough of
+//    its lanes are idle, so lanes whose rays finish early pick up new rays instead of waiting.
+
+// Stack entry codes: >= 0 an inner node; < 0 a leaf range (bit 30: item range, bits 27-29: count - 1,
+// bits 0-26: first triangle or item).
+constexpr uint32_t LEAF_CODE_FLAG = 0x80000000u;
+constexpr uint32_t LEAF_CODE_ITEMS = 0x40000000u;
+constexpr uint32_t LEAF_CODE_START = 0x07ffffffu;
+#ifndef RAYNA_REFILL_LANES
+#define RAYNA_REFILL_LANES 16
+#endif
+constexpr int REFILL_LANES = RAYNA_REFILL_LANES;  // refill a wave once this many of its lanes are idle
+#ifndef RAYNA_WAVE_BATCH
+#define RAYNA_WAVE_BATCH 64
+#endif
+constexpr int WAVE_BATCH = RAYNA_WAVE_BATCH;  // queue positions a wave claims per atomic
+
+// Traversal counters, compiled in only for measurement builds (-DRAYNA_TRAVERSAL_STATS): per lane, node pops,
+// node expansions, primitive tests and loop iterations, summed into a device array read by
+// rayna_traversal_stats().
+#ifdef RAYNA_TRAVERSAL_STATS
+__device__ unsigned long long g_trav_stats[4];
+#define RAYNA_STAT(name, v) (stat_##name += (v))
+#define RAYNA_STAT_DECL() unsigned long long stat_pops = 0, stat_expands = 0, stat_prims = 0, stat_iterations = 0
+#define RAYNA_STAT_FLUSH()                            \
+    do {                                              \
+        atomicAdd(&g_trav_stats[0], stat_pops);       \
+        atomicAdd(&g_trav_stats[1], stat_expands);    \
+        atomicAdd(&g_trav_stats[2], stat_prims);      \
+        atomicAdd(&g_trav_stats[3], stat_iterations); \
+    } while (0)
+#else
+#define RAYNA_STAT(name, v) ((void)0)
+#define RAYNA_STAT_DECL() ((void)0)
+#define RAYNA_STAT_FLUSH() ((void)0)
+#endif
+
+__device__ __forceinline__ int32_t leaf_code(int32_t start, int count, bool items) {
+    return (int32_t)(LEAF_CODE_FLAG | (items ? LEAF_CODE_ITEMS : 0u) | ((uint32_t)(count - 1) << 27) |
+                     ((uint32_t)start & LEAF_CODE_START));
+}
+
+// Traversal stack of one lane: the top EXT_LDS_STACK entries in LDS (one column per lane, conflict-free),
+// deeper entries in this thread's column of a global overflow array (`stride` = threads in the grid).  The
+// host sizes the overflow for the deepest stack the tree can need, so no push is ever dropped.
+struct LaneStack {
+    int32_t *s_node;
+    float *s_tent;
+    int2 *g;
+    uint32_t stride;
+    int sp;
+    __device__ __forceinline__ void push(int32_t n, float t) {
+        if (sp < EXT_LDS_STACK) {
+            s_node[sp * EXT_BLOCK + threadIdx.x] = n;
+            s_tent[sp * EXT_BLOCK + threadIdx.x] = t;
+        } else {
+            g[(size_t)(sp - EXT_LDS_STACK) * stride] = make_int2(n, __float_as_int(t));
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(int32_t *n, float *t) {
+        --sp;
+        if (sp < EXT_LDS_STACK) {
+            *n = s_node[sp * EXT_BLOCK + threadIdx.x];
+            *t = s_tent[sp * EXT_BLOCK + threadIdx.x];
+        } else {
+            const int2 e = g[(size_t)(sp - EXT_LDS_STACK) * stride];
+            *n = e.x;
+            *t = __int_as_float(e.y);
+        }
+    }
+};
+
+// Tests the four child boxes of inner node `node` against [T_MIN, best] (conservatively) and pushes the hit
+// children -- inner nodes and leaf ranges alike -- far to near, so the nearest is popped first.
+__device__ __forceinline__ void expand(const SceneView &sv, const DRay &ray, float best, int32_t node, LaneStack &st) {
+    const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
+    const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+    const int4 ch = reinterpret_cast<const int4 *>(np)[6];
+    const int4 inf = reinterpret_cast<const int4 *>(np)[7];
+    const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, lo_y[4] = {ly.x, ly.y, ly.z, ly.w},
+                lo_z[4] = {lz.x, lz.y, lz.z, lz.w};
+    const float hi_x[4] = {hx.x, hx.y, hx.z, hx.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w},
+                hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+    const int32_t child[4] = {ch.x, ch.y, ch.z, ch.w};
+    const int32_t info[4] = {inf.x, inf.y, inf.z, inf.w};
+    int32_t code[4];
+    float key[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float tx1 = (lo_x[k] - ray.o.x) * ray.inv.x, tx2 = (hi_x[k] - ray.o.x) * ray.inv.x;
+        float t0 = fminf(tx1, tx2), t1 = fmaxf(tx1, tx2);
+        const float ty1 = (lo_y[k] - ray.o.y) * ray.inv.y, ty2 = (hi_y[k] - ray.o.y) * ray.inv.y;
+        t0 = fmaxf(t0, fminf(ty1, ty2));
+        t1 = fminf(t1, fmaxf(ty1, ty2));
+        const float tz1 = (lo_z[k] - ray.o.z) * ray.inv.z, tz2 = (hi_z[k] - ray.o.z) * ray.inv.z;
+        t0 = fmaxf(t0, fminf(tz1, tz2));
+        t1 = fminf(t1, fmaxf(tz1, tz2));
+        const float lo = fmaxf(t0, T_MIN), hi = fminf(t1, best);
+        const bool hit = info[k] != INFO_EMPTY && lo <= hi * BOX_SLACK;
+        code[k] = info[k] == INFO_INNER ? child[k]
+                                        : leaf_code(child[k], info[k] & 0xf, (info[k] >> 4) == LEAF_ITEMS);
+        key[k] = hit ? t0 : -__builtin_inff();
+        if (!hit) code[k] = 0x7fffffff;  // marks "not pushed" (no node has this index)
+    }
+#define RAYNA_CSWAP(a, b)              \
+    if (key[a] < key[b]) {             \
+        const float tt_ = key[a];      \
+        key[a] = key[b];               \
+        key[b] = tt_;                  \
+        const int32_t cc_ = code[a];   \
+        code[a] = code[b];             \
+        code[b] = cc_;                 \
+    }
+    RAYNA_CSWAP(0, 1)
+    RAYNA_CSWAP(2, 3)
+    RAYNA_CSWAP(0, 2)
+    RAYNA_CSWAP(1, 3)
+    RAYNA_CSWAP(1, 2)
+#undef RAYNA_CSWAP
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (code[k] != 0x7fffffff) st.push(code[k], key[k]);
+}
+
+// Tests entry `i` of leaf range `code` and keeps it if closer, or equally close with a higher rank.
+__device__ __forceinline__ void leaf_test(const SceneView &sv, const DRay &ray, int32_t code, int i, float leaf_t,
+                                          Best &b, LaneStack &st, const dm::Key &key, uint32_t pixel,
+                                          uint32_t sample, uint32_t bounce) {
+    const int32_t start = (int32_t)((uint32_t)code & LEAF_CODE_START);
+    if (!((uint32_t)code & LEAF_CODE_ITEMS)) {
+        const GTri g = sv.tris[start + i];
+        float t, u, v, det;
+        if (!tri_t(g, ray, T_MIN, b.t, &t, &u, &v, &det)) return;
+        if (!(t < b.t || g.rank > b.rank)) return;
+        if (g.flags & 1) {
+            f3 nn;
+            if (!tri_normal(sv.tri_normals[start + i], u, v, &nn)) return;
+        }
+        b = Best{t, g.rank, make_ref(ITEM_TRIANGLE, start + i), 0u};
+        return;
+    }
+    const GItem it = sv.items[start + i];
+    float t;
+    bool ok = false;
+    uint32_t ref = 0;
+    switch (it.kind) {
+        case ITEM_SPHERE:
+            ok = sphere_t(sv.spheres[it.index], ray, T_MIN, b.t, &t);
+            ref = make_ref(ITEM_SPHERE, start + i);
+            break;
+        case ITEM_TRIANGLE: {
+            const GTri g = sv.tris[it.index];
+            float u, v, det;
+            ok = tri_t(g, ray, T_MIN, b.t, &t, &u, &v, &det);
+            if (ok && (g.flags & 1)) {
+                f3 nn;
+                ok = tri_normal(sv.tri_normals[it.index], u, v, &nn);
+            }
+            ref = make_ref(ITEM_TRIANGLE, it.index);
+            break;
+        }
+        case ITEM_PARALLELOGRAM:
+            ok = para_t(sv.paras[it.index], ray, T_MIN, b.t, &t);
+            ref = make_ref(ITEM_PARALLELOGRAM, start + i);
+            break;
+        case ITEM_AXIS_BOX: {
+            int axis;
+            float w;
+            f3 sg;
+            ok = box_t(sv.boxes[it.index], ray, T_MIN, b.t, &t, &axis, &w, &sg);
+            ref = make_ref(ITEM_AXIS_BOX, start + i);
+            break;
+        }
+        case ITEM_VOLUME:
+            ok = volume_t(sv, sv.volumes[it.index], ray, b.t, key, pixel, sample, bounce, &t);
+            ref = make_ref(ITEM_VOLUME, start + i);
+            break;
+        case ITEM_LINK:
+            st.push(it.index, leaf_t);
+            break;
+        default: break;
+    }
+    if (ok && (t < b.t || it.rank > b.rank)) b = Best{t, it.rank, ref, 0u};
+}
+
+// The unbounded objects, marched with the caller's full interval; strict < keeps the earlier of equal minima.
+__device__ __forceinline__ void unbounded_hits(const SceneView &sv, const DRay &ray, Best &b) {
+    for (int i = 0; i < sv.n_sdfs; ++i) {
+        const GSdf f = sv.sdfs[i];
+        float t;
+        bool front;
+        if (sdf_march(sv, f, ray, &t, &front) && t < b.t)
+            b = Best{t, -2 - f.list_pos, make_ref(ITEM_SDF, i), front ? 0u : HIT_AUX_BACK};
+    }
+}
+
+// Closest hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501).
+__global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams fp, PathBuffers pb,
+                                                      const int32_t *queue, const uint32_t *count_ptr,
+                                                      uint32_t depth, int2 *overflow, uint32_t *work) {
+    __shared__ int32_t s_node[EXT_LDS_STACK * EXT_BLOCK];
+    __shared__ float s_tent[EXT_LDS_STACK * EXT_BLOCK];
+    const uint32_t count = queue ? *count_ptr : fp.n_slots;
+    const dm::Key key{fp.key0, fp.key1};
+    const uint32_t gid = blockIdx.x * EXT_BLOCK + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    LaneStack st{s_node, s_tent, overflow + gid, gridDim.x * EXT_BLOCK, 0};
+    bool active = false, exhausted = false;
+    uint32_t slot = 0, pixel = 0, sample = 0;
+    DRay ray{};
+    Best b{FLT_MAX, -1, 0u, 0u};
+    int32_t leaf = 0;
+    int leaf_i = 0, leaf_n = 0;
+    float leaf_t = 0.0f;
+    // The wave's unhanded queue positions [batch_next, batch_end): claimed WAVE_BATCH at a time with one atomic,
+    // handed to idle lanes without further atomics.  (All of these are wave-uniform.)
+    uint32_t batch_next = 0, batch_end = 0;
+    RAYNA_STAT_DECL();
+    bool last_claim = false;  // the counter has passed `count`: no further claims will yield work
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        if (!exhausted && (idle == ~0ull || __popcll(idle) >= REFILL_LANES)) {
+            const uint32_t n = (uint32_t)__popcll(idle);
+            const uint32_t r = batch_end - batch_next;  // old positions, handed out first
+            uint32_t nb = 0, nb_end = 0;                // new batch [nb, nb_end)
+            if (r < n && !last_claim) {
+                const int leader = __ffsll((long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(work, (uint32_t)WAVE_BATCH);
+                base = __shfl(base, leader);
+                nb = base < count ? base : count;
+                nb_end = (count - nb > (uint32_t)WAVE_BATCH) ? nb + (uint32_t)WAVE_BATCH : count;
+                if (nb_end >= count) last_claim = true;
+            }
+            const uint32_t idx = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));  // rank among idle lanes
+            uint32_t q = 0xffffffffu;
+            if (idx < r) q = batch_next + idx;
+            else if (idx - r < nb_end - nb) q = nb + (idx - r);
+            if (r >= n) {
+                batch_next += n;
+            } else {
+                const uint32_t from_new = (n - r < nb_end - nb) ? n - r : nb_end - nb;
+                batch_next = nb + from_new;
+                batch_end = nb_end;
+            }
+            if (last_claim && batch_next == batch_end) exhausted = true;
+            if (!active) {
+                if (q != 0xffffffffu) {
+                    slot = queue ? (uint32_t)queue[q] : q;
+                    ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
+                                       f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+                    pixel = slot % fp.n_pixels;
+                    sample = fp.s0 + slot / fp.n_pixels;
+                    b = Best{FLT_MAX, -1, 0u, 0u};
+                    st.sp = 0;
+                    if (sv.root >= 0) st.push(sv.root, -__builtin_inff());
+                    leaf = 0;
+                    active = true;
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (exhausted) break;
+            continue;
+        }
+        RAYNA_STAT(iterations, 1);
+        if (active) {
+            while (leaf == 0 && st.sp > 0) {  // inner nodes until a leaf range comes up
+                int32_t code;
+                float tent;
+                st.pop(&code, &tent);
+                RAYNA_STAT(pops, 1);
+                if (tent > b.t * BOX_SLACK) continue;
+                if (code < 0) {
+                    leaf = code;
+                    leaf_i = 0;
+                    leaf_n = (int)(((uint32_t)code >> 27) & 7u) + 1;
+                    leaf_t = tent;
+                    break;
+                }
+                RAYNA_STAT(expands, 1);
+                expand(sv, ray, b.t, code, st);
+            }
+            while (leaf != 0) {  // the leaf's primitives, one per iteration
+                RAYNA_STAT(prims, 1);
+                leaf_test(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth);
+                if (++leaf_i == leaf_n) leaf = 0;
+            }
+            if (st.sp == 0) {
+                unbounded_hits(sv, ray, b);
+                pb.hit_t[slot] = b.t;
+                pb.hit_ref[slot] = b.ref;
+                pb.hit_aux[slot] = b.aux;
+                if (pb.hit_rank) pb.hit_rank[slot] = b.rank;
+                active = false;
+            }
+        }
+    }
+    RAYNA_STAT_FLUSH();
+}

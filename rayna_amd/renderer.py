@@ -21,7 +21,8 @@ import numpy as np
 from . import _abi as A
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "librayna_amd.so")
+# RAYNA_AMD_LIB selects another build of the library (e.g. a tuning variant); the default is the in-tree build.
+LIB_PATH = os.environ.get("RAYNA_AMD_LIB", os.path.join(_HERE, "librayna_amd.so"))
 _lib: Optional[C.CDLL] = None
 
 # every symbol declared in include/rayna_gpu.h
