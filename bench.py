@@ -28,6 +28,11 @@ EXTEND_BYTES_PER_SEGMENT = 40
 # throughput 12, radiance 12, queue entry 4 -> 116 B; generate writes 48 B per sample, resolve reads 12 B.
 FRAME_BYTES_PER_SEGMENT = 40 + 116
 FRAME_BYTES_PER_SAMPLE = 48 + 12
+# HBM traffic of the extend kernel per closest-hit query on the 1080p triangle-mesh frame, from rocprofv3
+# counters over one frame (144 launches, 263,044,862 queries): FETCH_SIZE 3.884e7 KiB read as 64-byte units
+# (x2.0: a 256 MiB copy reads 131,088 KiB) plus WRITE_SIZE 8.325e6 KiB (x1.0: the same copy writes 262,144 KiB).
+# Node and triangle fetches that miss L2 make up the difference to the 40 algorithmic bytes.
+EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (3.884e7 * 2.0 + 8.325e6) * 1024 / 263044862}
 
 
 def workload(name: str):
@@ -52,13 +57,17 @@ def cpu_baseline(scene, cam, opts, target_seconds: float, threads: int):
                               col_step=col_step, threads=threads)
     n_px = ((opts.height + row_step - 1) // row_step) * ((opts.width + col_step - 1) // col_step)
     per_px = max(secs / n_px, 1e-7)
-    want_px = max(1.0, target_seconds / per_px)
-    scale = max(1.0, (opts.width * opts.height / want_px) ** 0.5)
-    row_step = max(1, int(round(scale)))
-    col_step = max(1, int(round(scale)))
-    out, segs, secs, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_REFERENCE, row_step=row_step,
-                                   col_step=col_step, threads=threads)
-    n_px = out.shape[0] * out.shape[1]
+    for _ in range(2):  # a tiny sample overstates the cost per pixel (idle threads), so refine once
+        want_px = max(1.0, target_seconds / per_px)
+        scale = max(1.0, (opts.width * opts.height / want_px) ** 0.5)
+        row_step = max(1, int(round(scale)))
+        col_step = max(1, int(round(scale)))
+        out, segs, secs, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_REFERENCE,
+                                       row_step=row_step, col_step=col_step, threads=threads)
+        n_px = out.shape[0] * out.shape[1]
+        per_px = max(secs / n_px, 1e-7)
+        if secs > 0.4 * target_seconds or scale == 1.0:
+            break
     samples = n_px * opts.samples
     # the same sample with refitted tight boxes (identical image, the engine's culling removed)
     out2, _, secs2, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_TIGHT, row_step=row_step,
@@ -75,6 +84,34 @@ def cpu_baseline(scene, cam, opts, target_seconds: float, threads: int):
     }
 
 
+def adjoint_leg(steps: int):
+    """The inverse-rendering configuration (256x256, 16 spp, the 521 hidden colours of inverse_spheres): device
+    time of a forward frame and of its path-replay adjoint, reusing the frame's radiance as the primal pass,
+    and of the adjoint when it recomputes the primal pass too.  Medians over `steps` frames."""
+    import numpy as np
+    from dataclasses import replace
+    from rayna_amd import Renderer, presets
+
+    scene, cam, opts, _ = presets.inverse_spheres()
+    d = np.random.default_rng(0).standard_normal((opts.height, opts.width, 3)).astype(np.float32)
+    fwd, adj, adj_full = [], [], []
+    with Renderer(scene, cam, opts) as r:
+        for i in range(steps + 1):
+            r.set_options(replace(opts, seed=100 + i))
+            f = r.render(to_host=False).stats
+            _, a = r.adjoint(d, frame=0, layout="rowmajor")
+            _, a2 = r.adjoint(d, frame=0, layout="rowmajor")
+            if i:  # the first round is warm-up
+                fwd.append(f.kernel_ms)
+                adj.append(a.kernel_ms)
+                adj_full.append(a2.kernel_ms)
+    f_ms, a_ms, a2_ms = (float(np.median(x)) for x in (fwd, adj, adj_full))
+    return {"workload": "inverse_spheres", "width": opts.width, "height": opts.height, "spp": opts.samples,
+            "colours": len(scene.materials_list()), "forward_ms": f_ms, "adjoint_ms": a_ms,
+            "adjoint_forward_ratio": a_ms / f_ms, "adjoint_with_primal_ms": a2_ms,
+            "adjoint_with_primal_forward_ratio": a2_ms / f_ms}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +121,9 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio leg")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only copy a buffer of this many MiB three times (the counter calibration kernel)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (the renderer is single-GPU; --gpus {a.gpus} is not supported)",
@@ -91,6 +131,15 @@ def main(argv=None):
         return 2
 
     import torch
+    if a.copy_mib:
+        src = torch.ones(a.copy_mib * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copied {a.copy_mib} MiB x3", flush=True)
+        return 0
     from rayna_amd import Renderer
 
     scene, cam, opts = workload(a.workload)
@@ -118,6 +167,8 @@ def main(argv=None):
     value = samples_per_frame * a.steps / secs / 1e6
     ext_bytes = EXTEND_BYTES_PER_SEGMENT * segs
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    per_seg = EXTEND_TRAFFIC_BYTES_PER_SEGMENT.get(a.workload)
+    traffic = per_seg * segs / launches if per_seg and launches else None
     frame_bytes = FRAME_BYTES_PER_SEGMENT * segs + FRAME_BYTES_PER_SAMPLE * samples_per_frame * a.steps
     line = {
         "metric": "Msamples/s at 1080p (triangle-mesh workload, equal-seed parity with the fp32 CPU restatement)",
@@ -143,10 +194,12 @@ def main(argv=None):
         "frame_alg_gbps": frame_bytes / secs / 1e9,
         "scene_build_s": t_build,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": "k_extend", "bytes_per_segment": EXTEND_BYTES_PER_SEGMENT},
     }
     r.close()
+    if not a.no_adjoint:
+        line["adjoint"] = adjoint_leg(max(3, a.steps))
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, cam, opts, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(line), flush=True)

@@ -228,6 +228,24 @@ rayna_status rayna_clear_accumulation(rayna_renderer *r);
 rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uint32_t out_dims[2],
                           rayna_render_stats *stats);
 
+/* Replaces the colour of materials [0, n) of the current scene: the albedo of lambertian, metal,
+ * dielectric and isotropic materials, the emission of lights (3 floats each, in rayna_scene_desc.materials
+ * order).  Clears accumulation.  No engine counterpart: the engine fixes a material's texture when the scene
+ * is built (material/lambertian.rs:17-20 etc.), so a caller there rebuilds the Scene and calls set_scene
+ * (render/renderer.rs:158-162); this keeps the BVH and the device scene and rewrites only the colours. */
+rayna_status rayna_set_material_colours(rayna_renderer *r, const float *rgb, uint32_t n);
+
+/* Path replay backpropagation (Vicini, Speierer and Jakob 2021) of frame `frame` since the last clear
+ * (0 = the first; the frame's counter-RNG key, so the replay retraces exactly the paths that frame traced).
+ * d_image is dloss/d(the frame's per-pixel sample mean), width*height*3 floats in `layout`; grad_rgb
+ * receives dloss/d(material colour), n_materials*3 doubles (n_materials must be the scene's count).  Paths
+ * are recomputed from the RNG, never stored; when the last rayna_render traced this same frame in one chunk
+ * of paths and no setter ran since, its per-sample radiance is reused as the primal pass.  A channel whose
+ * colour is exactly 0 receives no gradient through scattering.  No engine counterpart (the engine is not
+ * differentiable); `stats` reports the call's device time and closest-hit queries. */
+rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_image, int32_t layout,
+                           double *grad_rgb, uint32_t n_materials, rayna_render_stats *stats);
+
 /* Parity hook (no engine counterpart; the engine's RenderMode debug views are the closest,
  * renderer.rs:428-490): closest hit of the camera ray of sample `sample_index` of every pixel, row-major.
  * prim_rank receives the primitive's position in the depth-first order of the engine's BVH

@@ -38,6 +38,11 @@ def lib() -> C.CDLL:
         L.twin_scene_dfs_order.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32]
         L.twin_scene_tree_walk.restype = C.c_int32
         L.twin_scene_tree_walk.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32]
+        L.twin_scene_set_colours.argtypes = [vp, C.POINTER(C.c_float), C.c_int32]
+        L.twin_adjoint.restype = C.c_uint64
+        L.twin_adjoint.argtypes = [vp, C.POINTER(A.CameraDesc), C.POINTER(A.RenderOptsDesc), C.c_int32, C.c_int32,
+                                   C.c_uint32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
         L.twin_camera_invalid.restype = C.c_int32
         L.twin_camera_invalid.argtypes = [C.POINTER(A.CameraDesc)]
         L.twin_render.restype = C.c_uint64
@@ -113,6 +118,23 @@ class TwinScene:
         segs = lib().twin_render(self.h, C.byref(cd), C.byref(od), precision, box_mode, integrator, frame, row_start,
                                  row_step, col_step, threads, _p(out, C.c_float), C.byref(secs), C.byref(tests))
         return out, int(segs), secs.value, int(tests.value)
+
+    def set_colours(self, rgb):
+        """Replace the material colours (albedo, or emission for lights): an (n_materials, 3) array."""
+        c = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        lib().twin_scene_set_colours(self.h, _p(c, C.c_float), len(c))
+
+    def adjoint(self, cam, opts, d_image, precision: int = 64, box_mode: int = BOX_TIGHT, frame: int = 0,
+                threads: int = 0):
+        """Path-replay gradient of a loss w.r.t. every material colour, (n_materials, 3) float64, given
+        d_image = dloss/d(frame mean), (height, width, 3) row-major.  Returns (grad, replay queries, seconds)."""
+        d = np.ascontiguousarray(d_image, np.float32).reshape(opts.height, opts.width, 3)
+        grad = np.zeros((len(self.scene.materials_list()), 3), np.float64)
+        secs = C.c_double(0)
+        cd, od = cam.desc(), opts.desc()
+        segs = lib().twin_adjoint(self.h, C.byref(cd), C.byref(od), precision, box_mode, frame, threads,
+                                  _p(d, C.c_float), _p(grad, C.c_double), C.byref(secs))
+        return grad, int(segs), secs.value
 
     def primary_hits(self, cam, opts, precision: int = 32, box_mode: int = BOX_TIGHT, sample: int = 0):
         n = opts.width * opts.height

@@ -1240,6 +1240,43 @@ struct Tracer {
         }
         return L;
     }
+
+    // Path replay backpropagation (Vicini, Speierer and Jakob 2021, algorithm 2) of one sample whose radiance
+    // L came from `forward`, with d the sample's share of the loss gradient of its pixel.  The replay follows
+    // the same path (the material colours never change a sampling decision: lambertian.rs:29-36,
+    // metal.rs:17-33, dielectric.rs:19-39, isotropic.rs:29-31).  R = L minus the radiance gathered so far is
+    // what the rest of the path contributes; a light adds d*T to its emission gradient, and a scatter whose
+    // attenuation is albedo times a colour-independent factor adds d*R/albedo to its albedo gradient (a
+    // channel whose albedo is exactly 0 gets no gradient from that vertex).  grad has 3 doubles per material.
+    void adjoint(Ray<R> ray, const Col &L, const double d[3], double *grad) {
+        double rem[3] = {L.r, L.g, L.b};
+        Col T{1, 1, 1};
+        for (uint32_t depth = 0; depth <= depth_max; ++depth) {
+            Hit<R> h;
+            if (!intersect(ray, depth, h)) break;
+            const rayna_material &m = s->materials[h.material];
+            double *g = grad + 3 * (size_t)h.material;
+            if (m.kind == RAYNA_MAT_LIGHT) {
+                const float t[3] = {T.r, T.g, T.b};
+                for (int c = 0; c < 3; ++c) {
+                    g[c] += d[c] * (double)t[c];
+                    rem[c] -= (double)t[c] * (double)m.colour[c];
+                }
+            }
+            R u[4];
+            draw4<R>(key, pixel, sample, depth, SLOT_SCATTER, u);
+            V3<R> dir;
+            if (!scatter(m, ray, h, u, &dir)) break;
+            const bool albedo_scales = m.kind == RAYNA_MAT_LAMBERTIAN || m.kind == RAYNA_MAT_METAL ||
+                                       m.kind == RAYNA_MAT_ISOTROPIC ||
+                                       (m.kind == RAYNA_MAT_DIELECTRIC && !h.front_face);
+            if (albedo_scales && depth < depth_max)
+                for (int c = 0; c < 3; ++c)
+                    if (m.colour[c] != 0.0f) g[c] += d[c] * rem[c] / (double)m.colour[c];
+            T = reflected(m, ray, h, T);
+            ray = Ray<R>::make(h.pos, dir);
+        }
+    }
 };
 
 enum Integrator { INTEGRATOR_RECURSIVE = 0, INTEGRATOR_FORWARD = 1 };
@@ -1303,6 +1340,52 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
     worker();
     for (auto &t : pool) t.join();
     if (prim_tests_out) *prim_tests_out = tests.load();
+    return segs.load();
+}
+
+// Gradient of a loss with respect to every material colour (3 doubles per material, added to grad) for one
+// frame, given d_image = dloss/d(frame mean) as height x width x 3 floats, row-major.  Each sample is traced
+// once for its radiance (the primal pass) and then replayed by Tracer::adjoint.  Returns the number of
+// closest-hit queries of the replay.
+template <class R>
+inline uint64_t adjoint_frame(const RenderArgs &a, const float *d_image, double *grad) {
+    const rayna_render_opts &o = *a.opts;
+    const Viewport vp = calculate_viewport(*a.cam);
+    const RngKey key = make_key(o.seed, a.frame);
+    const size_t n_grad = 3 * a.scene->materials.size();
+    std::atomic<uint32_t> next{0};
+    std::atomic<uint64_t> segs{0};
+    int nt = a.threads > 0 ? a.threads : (int)std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    std::vector<std::vector<double>> part(nt, std::vector<double>(n_grad, 0.0));
+    auto worker = [&](int id) {
+        double *g = part[id].data();
+        uint64_t my_segs = 0;
+        for (;;) {
+            const uint32_t y = next.fetch_add(1);
+            if (y >= o.height) break;
+            for (uint32_t x = 0; x < o.width; ++x) {
+                const float *dp = d_image + ((size_t)y * o.width + x) * 3;
+                const double d[3] = {(double)dp[0] / o.samples, (double)dp[1] / o.samples, (double)dp[2] / o.samples};
+                Tracer<R> tr{a.scene, key, y * o.width + x, 0, o.ray_depth, a.box_mode};
+                for (uint32_t k = 0; k < o.samples; ++k) {
+                    tr.sample = k;
+                    const Ray<R> ray = camera_ray<R>(vp, o, key, x, y, k);
+                    const Col L = tr.forward(ray);
+                    const uint64_t before = tr.segments;
+                    tr.adjoint(ray, L, d, g);
+                    my_segs += tr.segments - before;
+                }
+            }
+        }
+        segs += my_segs;
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto &t : pool) t.join();
+    for (int t = 0; t < nt; ++t)
+        for (size_t i = 0; i < n_grad; ++i) grad[i] += part[t][i];
     return segs.load();
 }
 

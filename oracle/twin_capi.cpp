@@ -81,6 +81,26 @@ uint64_t twin_render(const twin_scene *t, const rayna_camera *cam, const rayna_r
     return segs;
 }
 
+// Replaces the colour (albedo, or emission for a light) of the first n materials: 3 floats each.
+void twin_scene_set_colours(twin_scene *t, const float *rgb, int32_t n) {
+    for (int32_t i = 0; i < n && i < (int32_t)t->s.materials.size(); ++i)
+        for (int c = 0; c < 3; ++c) t->s.materials[i].colour[c] = rgb[3 * i + c];
+}
+
+// Path-replay adjoint of one frame (twin::adjoint_frame): d_image is height x width x 3 floats, row-major;
+// grad (3 doubles per material) is overwritten.  Returns the replay's closest-hit queries.
+uint64_t twin_adjoint(const twin_scene *t, const rayna_camera *cam, const rayna_render_opts *opts, int32_t precision,
+                      int32_t box_mode, uint32_t frame, int32_t threads, const float *d_image, double *grad,
+                      double *seconds) {
+    RenderArgs a{&t->s, cam, opts, frame, box_mode, INTEGRATOR_FORWARD, 0, 1, threads, 1};
+    std::fill(grad, grad + 3 * t->s.materials.size(), 0.0);
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t segs = precision == 32 ? adjoint_frame<float>(a, d_image, grad)
+                                          : adjoint_frame<double>(a, d_image, grad);
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return segs;
+}
+
 // Closest hit of the camera ray of sample `k` of every pixel (row-major): depth-first rank (-1 miss,
 // -2 - i for unbounded object i) and distance.
 void twin_primary_hits(const twin_scene *t, const rayna_camera *cam, const rayna_render_opts *opts, int32_t precision,

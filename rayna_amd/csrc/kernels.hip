@@ -470,7 +470,7 @@ __device__ __forceinline__ Col sky(int kind, f3 d) {  // skybox/simple.rs, skybo
 
 // Camera ray of every slot of the chunk: render_px_msaa's stratified jitter (renderer.rs:370-389) and
 // Viewport::calc_ray (camera.rs:362-384); throughput 1, radiance 0.
-__global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers pb) {
+__global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers pb, int keep_radiance) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= fp.n_slots) return;
     const uint32_t pixel = i % fp.n_pixels;
@@ -514,18 +514,40 @@ __global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers 
     pb.tr[i] = 1.0f;
     pb.tg[i] = 1.0f;
     pb.tb[i] = 1.0f;
-    pb.lr[i] = 0.0f;
-    pb.lg[i] = 0.0f;
-    pb.lb[i] = 0.0f;
+    if (!keep_radiance) {
+        pb.lr[i] = 0.0f;
+        pb.lg[i] = 0.0f;
+        pb.lb[i] = 0.0f;
+    }
 }
+// Adds g to the gradient of material `mat` in gradient copy `copy` (float64 hardware atomics).
+__device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy, int32_t mat, double g0, double g1,
+                                         double g2) {
+    double *p = ap.grad + ((size_t)copy * ap.n_materials + (uint32_t)mat) * 3;
+    unsafeAtomicAdd(p, g0);
+    unsafeAtomicAdd(p + 1, g1);
+    unsafeAtomicAdd(p + 2, g2);
+}
+
 // One bounce of ray_colour_recursive in forward form: L += T * Le; scatter; T = reflected(T); paths that
 // scatter below the depth limit are compacted into the next queue (one atomic per wave).
+//
+// ADJOINT: the same bounce replayed for path replay backpropagation (Vicini, Speierer and Jakob 2021,
+// algorithm 2).  The radiance field holds R, the radiance the path still gathers from this vertex on (the
+// sample's primal radiance at the camera vertex); d is the sample's share of its pixel's loss gradient.  A
+// light adds d*T to its emission gradient and R -= T*Le; a scatter whose attenuation is albedo times a
+// colour-independent factor (lambertian, metal, isotropic, dielectric leaving the medium) adds d*R/albedo to
+// its albedo gradient (nothing for a channel whose albedo is exactly 0).  The colours never change a sampling
+// decision, so the replay follows the primal path exactly.
+template <bool ADJOINT>
 __global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, PathBuffers pb,
                                                  const int32_t *queue_in, const uint32_t *count_in,
-                                                 int32_t *queue_out, uint32_t *count_out, uint32_t depth) {
+                                                 int32_t *queue_out, uint32_t *count_out, uint32_t depth,
+                                                 AdjointParams ap) {
     const uint32_t count = queue_in ? *count_in : fp.n_slots;
     const dm::Key key{fp.key0, fp.key1};
     const uint32_t lane = __lane_id();
+    const uint32_t copy = ADJOINT ? blockIdx.x % ap.copies : 0u;
     for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
         const uint32_t q = base + threadIdx.x;
         bool keep = false;
@@ -537,19 +559,39 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, P
             Col T{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
             Col L{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
             const uint32_t ref = pb.hit_ref[slot];
+            const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
             if (ref == 0u) {
-                L = L + T * sky(sv.skybox, ray.d);
+                if (!ADJOINT) L = L + T * sky(sv.skybox, ray.d);
             } else {
-                const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
                 SHit h;
                 hit_geometry(sv, ray, pb.hit_t[slot], ref, pb.hit_aux[slot], key, pixel, sample, depth, &h);
                 const GMaterial m = sv.materials[h.material];
-                const Col le = (m.kind == RAYNA_MAT_LIGHT) ? Col{m.r, m.g, m.b} : Col{0.0f, 0.0f, 0.0f};
-                L = L + T * le;
+                double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+                if (ADJOINT) {
+                    d0 = (double)ap.d_image[3 * pixel] * ap.inv_spp;
+                    d1 = (double)ap.d_image[3 * pixel + 1] * ap.inv_spp;
+                    d2 = (double)ap.d_image[3 * pixel + 2] * ap.inv_spp;
+                }
+                if (m.kind == RAYNA_MAT_LIGHT) {
+                    const Col le{m.r, m.g, m.b};
+                    if (ADJOINT) {
+                        grad_add(ap, copy, h.material, d0 * T.r, d1 * T.g, d2 * T.b);
+                        L = Col{L.r - T.r * le.r, L.g - T.g * le.g, L.b - T.b * le.b};
+                    } else {
+                        L = L + T * le;
+                    }
+                }
                 float u[4];
                 dm::draw4(key, pixel, sample, depth, dm::SLOT_SCATTER, u);
                 f3 dir;
                 if (scatter(m, ray, h, u, &dir)) {
+                    if (ADJOINT && depth < fp.ray_depth) {
+                        const bool albedo_scales = m.kind != RAYNA_MAT_DIELECTRIC || !h.front;
+                        if (albedo_scales)
+                            grad_add(ap, copy, h.material, m.r != 0.0f ? d0 * (double)L.r / (double)m.r : 0.0,
+                                     m.g != 0.0f ? d1 * (double)L.g / (double)m.g : 0.0,
+                                     m.b != 0.0f ? d2 * (double)L.b / (double)m.b : 0.0);
+                    }
                     T = reflected(m, ray, h, T);
                     if (depth < fp.ray_depth) {
                         const DRay nr = make_ray(h.pos, dir);
@@ -583,6 +625,15 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, P
             }
         }
     }
+}
+
+// Sum of the gradient copies: out[i] = sum over copies of grad[copy * n + i], n = 3 * materials.
+__global__ __launch_bounds__(BLOCK) void k_sum_grad(const double *grad, uint32_t n, uint32_t copies, double *out) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (uint32_t c = 0; c < copies; ++c) s += grad[(size_t)c * n + i];
+    out[i] = s;
 }
 
 // Per-pixel running sum of the chunk's samples in sample order (render_px_msaa's sum, renderer.rs:398-402).
@@ -648,8 +699,9 @@ int extend_blocks_per_cu() {
 }
 int extend_block() { return EXT_BLOCK; }
 
-void launch_generate(const FrameParams &fp, const PathBuffers &pb, hipStream_t s) {
-    hipLaunchKernelGGL(k_generate, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb);
+void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, hipStream_t s) {
+    hipLaunchKernelGGL(k_generate, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb,
+                       keep_radiance ? 1 : 0);
 }
 
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
@@ -660,9 +712,17 @@ void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers
 
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
-                  hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in, queue_out, count_out,
-                       depth);
+                  const AdjointParams *adjoint, hipStream_t s) {
+    if (adjoint)
+        hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in, queue_out,
+                           count_out, depth, *adjoint);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in,
+                           queue_out, count_out, depth, AdjointParams{});
+}
+
+void launch_sum_grad(const double *grad, uint32_t n, uint32_t copies, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_grad, dim3(grid_for(n, BLOCK)), dim3(BLOCK), 0, s, grad, n, copies, out);
 }
 
 void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, uint32_t chunk_samples, int first,

@@ -33,7 +33,17 @@ struct FrameParams {
     uint32_t n_slots;        // slots in the chunk
 };
 
-void launch_generate(const FrameParams &fp, const PathBuffers &pb, hipStream_t s);
+// Inputs and outputs of the adjoint replay (path replay backpropagation) of a frame.
+struct AdjointParams {
+    const float *d_image;    // dloss/d(frame mean), 3 floats per pixel in row-major pixel order
+    double inv_spp;          // 1 / samples per pixel: the share of each sample
+    double *grad;            // copies x n_materials x 3 gradient accumulators
+    uint32_t n_materials;
+    uint32_t copies;         // gradient copies (a block adds into copy blockIdx % copies)
+};
+
+// keep_radiance: leave the radiance field as it is (the adjoint replay keeps the primal radiance there)
+void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, hipStream_t s);
 // queue == nullptr: identity queue of fp.n_slots entries; else queue[0 .. *count)
 // overflow: traversal stack entries beyond the LDS part, extend_block() * grid * (stack_need - extend_lds_stack());
 // work: a zeroed counter the launch's waves take rays from
@@ -41,7 +51,8 @@ void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers
                    const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work, hipStream_t s);
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
-                  hipStream_t s);
+                  const AdjointParams *adjoint, hipStream_t s);
+void launch_sum_grad(const double *grad, uint32_t n, uint32_t copies, double *out, hipStream_t s);
 void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, uint32_t chunk_samples, int first,
                     hipStream_t s);
 void launch_finalize(const FrameParams &fp, const float *sum, float *acc, double frames, float *out, int layout,

@@ -53,6 +53,14 @@ struct rayna_renderer {
 
     uint64_t frames = 0;  // AccumulationBuffer::frame_count (accum_buffer.rs:77)
     bool acc_dirty = true;
+    // frame whose per-sample radiance the radiance field still holds (one-chunk frames only), else -1
+    int64_t primal_frame = -1;
+
+    // adjoint
+    float *dimg = nullptr;        // dloss/d(frame mean), row-major pixels x 3
+    double *grad = nullptr;       // grad_copies x n_materials x 3
+    double *grad_sum = nullptr;   // n_materials x 3
+    uint32_t grad_materials = 0, grad_copies = 0;
     int grid_extend = 1024, grid_shade = 1024;
     std::vector<hipEvent_t> events;
     std::string error;
@@ -183,6 +191,8 @@ void free_paths(rayna_renderer *r) {
     r->queue_count = nullptr;
     r->segments = nullptr;
     r->sum = r->acc = r->out_dev = nullptr;
+    r->dimg = nullptr;
+    r->primal_frame = -1;
     r->slot_capacity = 0;
     r->alloc_pixels = 0;
 }
@@ -215,6 +225,7 @@ rayna_status alloc_paths(rayna_renderer *r) {
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->sum)) != RAYNA_OK) return st;
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->acc)) != RAYNA_OK) return st;
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->out_dev)) != RAYNA_OK) return st;
+    if ((st = alloc(3 * P * sizeof(float), (void **)&r->dimg)) != RAYNA_OK) return st;
     r->pb.hit_rank = nullptr;
     r->slot_capacity = slots;
     r->chunk_samples = (uint32_t)chunk;
@@ -257,13 +268,14 @@ FrameParams frame_params(const rayna_renderer *r, uint64_t frame_index) {
     return fp;
 }
 
-// Trace the samples [s0, s0 + n) of every pixel: generate, then extend/shade per depth.
+// Trace the samples [s0, s0 + n) of every pixel: generate, then extend/shade per depth.  With `adjoint`, the
+// replay of path replay backpropagation: the radiance field must hold the chunk's primal radiance.
 void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, std::vector<hipEvent_t> *ev, size_t *evi,
-                 uint64_t *extend_launches) {
+                 uint64_t *extend_launches, const AdjointParams *adjoint = nullptr) {
     fp.s0 = s0;
     fp.n_slots = fp.n_pixels * n;
     launch_fill_u32(r->queue_count, 0u, 2 * (r->opts.ray_depth + 2), r->stream);
-    launch_generate(fp, r->pb, r->stream);
+    launch_generate(fp, r->pb, adjoint != nullptr, r->stream);
     for (uint32_t depth = 0; depth <= r->opts.ray_depth; ++depth) {
         const int32_t *qin = depth == 0 ? nullptr : r->queue[depth & 1];
         const uint32_t *cin = depth == 0 ? nullptr : r->queue_count + depth;
@@ -273,11 +285,39 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
         uint32_t *work = r->queue_count + (r->opts.ray_depth + 2) + depth;
         launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, work, r->stream);
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
-        launch_shade(r->dev.view, fp, r->pb, qin, cin, qout, cout, depth, r->grid_shade, r->stream);
+        launch_shade(r->dev.view, fp, r->pb, qin, cin, qout, cout, depth, r->grid_shade, adjoint, r->stream);
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
         ++*extend_launches;
     }
     launch_count_segments(r->queue_count, r->opts.ray_depth + 1, fp.n_slots, r->segments, r->stream);
+}
+
+rayna_status ensure_events(rayna_renderer *r, size_t n) {
+    while (r->events.size() < n) {
+        hipEvent_t e;
+        HIP_TRY(r, hipEventCreate(&e));
+        r->events.push_back(e);
+    }
+    return RAYNA_OK;
+}
+
+// Device times from the events of one call: events[0] before the first kernel, events[evi - 1] after the
+// last, and per trace_chunk depth the triple [before extend, after extend, after shade] in between.
+rayna_status event_times(rayna_renderer *r, size_t evi, rayna_render_stats *st) {
+    float ms = 0;
+    HIP_TRY(r, hipEventElapsedTime(&ms, r->events[0], r->events[evi - 1]));
+    st->kernel_ms = ms;
+    double ext = 0, sh = 0;
+    for (size_t i = 1; i + 2 < evi; i += 3) {
+        float a = 0, b = 0;
+        HIP_TRY(r, hipEventElapsedTime(&a, r->events[i], r->events[i + 1]));
+        HIP_TRY(r, hipEventElapsedTime(&b, r->events[i + 1], r->events[i + 2]));
+        ext += a;
+        sh += b;
+    }
+    st->extend_ms = ext;
+    st->shade_ms = sh;
+    return RAYNA_OK;
 }
 
 void write_checkerboard(uint32_t w, uint32_t h, float *out, int layout) {  // renderer.rs:269-292
@@ -337,6 +377,7 @@ rayna_status rayna_set_scene(rayna_renderer *r, const rayna_scene_desc *scene) {
     (void)hipSetDevice(r->device);
     const rayna_status st = load_scene(r, scene);
     if (st != RAYNA_OK) return st;
+    r->primal_frame = -1;
     return rayna_clear_accumulation(r);
 }
 
@@ -344,6 +385,7 @@ rayna_status rayna_set_camera(rayna_renderer *r, const rayna_camera *camera) {
     if (!r || !camera) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
     r->cam = *camera;
     r->cam_valid = viewport(r->cam, r->vp);
+    r->primal_frame = -1;
     return rayna_clear_accumulation(r);
 }
 
@@ -352,8 +394,11 @@ rayna_status rayna_set_options(rayna_renderer *r, const rayna_render_opts *opts)
     rayna_status st = check_opts(r, opts);
     if (st != RAYNA_OK) return st;
     (void)hipSetDevice(r->device);
+    const bool same_shape = opts->width == r->opts.width && opts->height == r->opts.height &&
+                            opts->samples == r->opts.samples && opts->ray_depth == r->opts.ray_depth;
     r->opts = *opts;
-    if ((st = alloc_paths(r)) != RAYNA_OK) return st;
+    if (!same_shape && (st = alloc_paths(r)) != RAYNA_OK) return st;
+    r->primal_frame = -1;
     return rayna_clear_accumulation(r);
 }
 
@@ -392,19 +437,17 @@ rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uin
             r->acc_dirty = false;
         }
         FrameParams fp = frame_params(r, r->frames);
+        const int64_t frame_index = (int64_t)r->frames;
         r->frames += 1;
         HIP_TRY(r, hipMemsetAsync(r->segments, 0, sizeof(unsigned long long), r->stream));
         const uint32_t spp = r->opts.samples;
         const uint32_t n_chunks = (spp + r->chunk_samples - 1) / r->chunk_samples;
         std::vector<hipEvent_t> *ev = nullptr;
         const size_t n_events = (size_t)n_chunks * (r->opts.ray_depth + 1) * 3 + 2;
-        if (stats) {
-            while (r->events.size() < n_events) {
-                hipEvent_t e;
-                HIP_TRY(r, hipEventCreate(&e));
-                r->events.push_back(e);
-            }
-            ev = &r->events;
+        if (stats) ev = &r->events;
+        if (ev) {
+            const rayna_status es = ensure_events(r, n_events);
+            if (es != RAYNA_OK) return es;
         }
         size_t evi = 0;
         if (ev) HIP_TRY(r, hipEventRecord((*ev)[evi++], r->stream));
@@ -417,6 +460,7 @@ rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uin
             launch_resolve(fp, r->pb, r->sum, n, c == 0 ? 1 : 0, r->stream);
         }
         launch_finalize(fp, r->sum, r->acc, (double)r->frames, r->out_dev, layout, r->stream);
+        r->primal_frame = n_chunks == 1 ? frame_index : -1;
         if (ev) HIP_TRY(r, hipEventRecord((*ev)[evi++], r->stream));
         HIP_TRY(r, hipGetLastError());
         if (out_rgb)
@@ -429,24 +473,117 @@ rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uin
         st.segments = segs;
         st.extend_launches = launches;
         if (ev) {
-            float ms = 0;
-            HIP_TRY(r, hipEventElapsedTime(&ms, (*ev)[0], (*ev)[evi - 1]));
-            st.kernel_ms = ms;
-            double ext = 0, sh = 0;
-            // per depth: [before extend, after extend, after shade]
-            for (size_t i = 1; i + 2 < evi; i += 3) {
-                float a = 0, b = 0;
-                HIP_TRY(r, hipEventElapsedTime(&a, (*ev)[i], (*ev)[i + 1]));
-                HIP_TRY(r, hipEventElapsedTime(&b, (*ev)[i + 1], (*ev)[i + 2]));
-                ext += a;
-                sh += b;
-            }
-            st.extend_ms = ext;
-            st.shade_ms = sh;
+            const rayna_status ts = event_times(r, evi, &st);
+            if (ts != RAYNA_OK) return ts;
         }
     }
     st.duration_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
+    return RAYNA_OK;
+}
+
+rayna_status rayna_set_material_colours(rayna_renderer *r, const float *rgb, uint32_t n) {
+    if (!r || (n && !rgb)) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    if (n > r->packed.materials.size()) return fail(r, RAYNA_INVALID_ARGUMENT, "more colours than materials");
+    for (uint32_t i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(rgb[i])) return fail(r, RAYNA_INVALID_ARGUMENT, "non-finite material colour");
+    HIP_TRY(r, hipSetDevice(r->device));
+    for (uint32_t i = 0; i < n; ++i) {
+        GMaterial &m = r->packed.materials[i];
+        m.r = rgb[3 * i];
+        m.g = rgb[3 * i + 1];
+        m.b = rgb[3 * i + 2];
+    }
+    if (n) {
+        // the stream may still be reading the table
+        HIP_TRY(r, hipStreamSynchronize(r->stream));
+        HIP_TRY(r, hipMemcpy(const_cast<GMaterial *>(r->dev.view.materials), r->packed.materials.data(),
+                             n * sizeof(GMaterial), hipMemcpyHostToDevice));
+    }
+    r->primal_frame = -1;
+    return rayna_clear_accumulation(r);
+}
+
+rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_image, int32_t layout, double *grad_rgb,
+                           uint32_t n_materials, rayna_render_stats *stats) {
+    if (!r || !d_image || !grad_rgb) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    if (layout != RAYNA_LAYOUT_XMAJOR && layout != RAYNA_LAYOUT_ROWMAJOR)
+        return fail(r, RAYNA_INVALID_ARGUMENT, "unknown layout");
+    if (n_materials != r->packed.materials.size())
+        return fail(r, RAYNA_INVALID_ARGUMENT, "n_materials is not the scene's material count");
+    if (!r->cam_valid) return fail(r, RAYNA_INVALID_ARGUMENT, "camera is invalid (the frame is the checkerboard)");
+    if (n_materials == 0) return RAYNA_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(r, hipSetDevice(r->device));
+    const uint32_t W = r->opts.width, H = r->opts.height;
+    const uint64_t P = (uint64_t)W * H;
+    // gradient copies: up to 64, within 64 MiB
+    const uint64_t per_copy = (uint64_t)n_materials * 3 * sizeof(double);
+    const uint32_t copies = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, (64ull << 20) / per_copy));
+    if (r->grad_materials != n_materials || r->grad_copies != copies) {
+        if (r->grad) (void)hipFree(r->grad);
+        if (r->grad_sum) (void)hipFree(r->grad_sum);
+        r->grad = r->grad_sum = nullptr;
+        r->grad_materials = r->grad_copies = 0;
+        HIP_TRY(r, hipMalloc((void **)&r->grad, per_copy * copies));
+        HIP_TRY(r, hipMalloc((void **)&r->grad_sum, per_copy));
+        r->grad_materials = n_materials;
+        r->grad_copies = copies;
+    }
+    // d_image in row-major pixel order
+    std::vector<float> d(3 * P);
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            const size_t src = layout == RAYNA_LAYOUT_XMAJOR ? ((size_t)x * H + y) * 3 : ((size_t)y * W + x) * 3;
+            const size_t dst = ((size_t)y * W + x) * 3;
+            d[dst] = d_image[src];
+            d[dst + 1] = d_image[src + 1];
+            d[dst + 2] = d_image[src + 2];
+        }
+    HIP_TRY(r, hipMemcpyAsync(r->dimg, d.data(), d.size() * sizeof(float), hipMemcpyHostToDevice, r->stream));
+    HIP_TRY(r, hipMemsetAsync(r->grad, 0, per_copy * copies, r->stream));
+    HIP_TRY(r, hipMemsetAsync(r->segments, 0, sizeof(unsigned long long), r->stream));
+
+    const uint32_t spp = r->opts.samples;
+    const uint32_t n_chunks = (spp + r->chunk_samples - 1) / r->chunk_samples;
+    const bool reuse = n_chunks == 1 && r->primal_frame == (int64_t)frame;
+    FrameParams fp = frame_params(r, frame);
+    AdjointParams ap{r->dimg, 1.0 / (double)spp, r->grad, n_materials, copies};
+    const size_t n_events = (size_t)n_chunks * (r->opts.ray_depth + 1) * 3 * 2 + 2;
+    if (stats) {
+        const rayna_status es = ensure_events(r, n_events);
+        if (es != RAYNA_OK) return es;
+    }
+    std::vector<hipEvent_t> *ev = stats ? &r->events : nullptr;
+    size_t evi = 0;
+    if (ev) HIP_TRY(r, hipEventRecord((*ev)[evi++], r->stream));
+    uint64_t launches = 0;
+    for (uint32_t c = 0; c < n_chunks; ++c) {
+        const uint32_t s0 = c * r->chunk_samples;
+        const uint32_t n = std::min(r->chunk_samples, spp - s0);
+        if (!reuse) trace_chunk(r, fp, s0, n, ev, &evi, &launches);  // primal pass: the chunk's radiance
+        trace_chunk(r, fp, s0, n, ev, &evi, &launches, &ap);         // adjoint replay
+    }
+    launch_sum_grad(r->grad, 3 * n_materials, copies, r->grad_sum, r->stream);
+    if (ev) HIP_TRY(r, hipEventRecord((*ev)[evi++], r->stream));
+    HIP_TRY(r, hipGetLastError());
+    HIP_TRY(r, hipMemcpyAsync(grad_rgb, r->grad_sum, per_copy, hipMemcpyDeviceToHost, r->stream));
+    unsigned long long segs = 0;
+    HIP_TRY(r, hipMemcpyAsync(&segs, r->segments, sizeof(segs), hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    // the radiance field now holds what the replay left of it
+    r->primal_frame = -1;
+    if (stats) {
+        rayna_render_stats st{};
+        const rayna_status ts = event_times(r, evi, &st);
+        if (ts != RAYNA_OK) return ts;
+        st.accum_frames = r->frames;
+        st.samples = P * spp;
+        st.segments = segs;
+        st.extend_launches = launches;
+        st.duration_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *stats = st;
+    }
     return RAYNA_OK;
 }
 
@@ -462,7 +599,8 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     fp.n_slots = P;
     PathBuffers pb = r->pb;
     pb.hit_rank = rank_dev;
-    launch_generate(fp, pb, r->stream);
+    r->primal_frame = -1;
+    launch_generate(fp, pb, false, r->stream);
     uint32_t *work = r->queue_count + 2 * (r->opts.ray_depth + 2);
     launch_fill_u32(work, 0u, 1, r->stream);
     launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, work, r->stream);
@@ -518,6 +656,8 @@ void rayna_destroy(rayna_renderer *r) {
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     free_paths(r);
     free_scene(r->dev);
+    if (r->grad) (void)hipFree(r->grad);
+    if (r->grad_sum) (void)hipFree(r->grad_sum);
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;

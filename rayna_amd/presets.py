@@ -259,3 +259,53 @@ def mixed_primitives(seed: int = 5, width: int = 96, height: int = 64, samples: 
                  v_fov=50.0, focus_dist=7.0, defocus_angle=0.5)
     opts = RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
     return Scene(objects, SimpleSkybox), cam, opts
+
+
+# ------------------------------------------------------------------------------------------ inverse rendering
+
+
+def inverse_spheres(seed: int = 4, width: int = 256, height: int = 256, samples: int = 16, ray_depth: int = 10):
+    """The inverse-rendering scene: the random-spheres layout with 512 spheres (509 jittered r=0.2 spheres
+    from a 24x24 grid in the textbook's material mix, plus the three r=1 spheres), a spherical isotropic
+    medium (radius 1.2, density 0.5, albedo 0.8: the SDF workload's medium; a sphere rather than a box because
+    the engine's box media never scatter), 8 parallelogram lights (2x2, a ring of radius 5 at height 6) and
+    the r=1000 ground; SimpleSkybox, 256x256, 16 spp, depth 10.
+
+    Returns (scene, camera, opts, hidden): the colours of materials [0, 521) are the hidden parameters,
+    in this order: the 512 sphere albedos, the medium albedo, the 8 light emissions.  Their true values are
+    the scene's colours; the ground (material 521) is known."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    spheres = []
+    big = np.array([4.0, 0.2, 0.0])
+    for a in range(-12, 12):
+        for b in range(-12, 12):
+            choose = rng.random()
+            centre = np.array([a + 0.9 * rng.random(), 0.2, b + 0.9 * rng.random()])
+            if np.linalg.norm(centre - big) <= 0.9:
+                continue
+            if choose < 0.8:
+                mat = LambertianMaterial(tuple(float(x) for x in rng.random(3) * rng.random(3)))
+            elif choose < 0.95:
+                mat = MetalMaterial(tuple(float(x) for x in 0.5 + 0.5 * rng.random(3)), float(0.5 * rng.random()))
+            else:
+                mat = DielectricMaterial(tuple(float(x) for x in 0.5 + 0.5 * rng.random(3)), 1.5, 0.0)
+            spheres.append(SimpleObject(SphereMesh(tuple(centre), 0.2), mat))
+    spheres = spheres[:509]
+    spheres.append(SimpleObject(SphereMesh((0.0, 1.0, 0.0), 1.0), DielectricMaterial((0.9, 0.95, 1.0), 1.5, 0.0)))
+    spheres.append(SimpleObject(SphereMesh((-4.0, 1.0, 0.0), 1.0), LambertianMaterial((0.4, 0.2, 0.1))))
+    spheres.append(SimpleObject(SphereMesh((4.0, 1.0, 0.0), 1.0), MetalMaterial((0.7, 0.6, 0.5), 0.0)))
+    assert len(spheres) == 512
+    objects = list(spheres)
+    objects.append(VolumetricObject(SphereMesh((-2.0, 1.2, 2.0), 1.2), IsotropicMaterial((0.8, 0.8, 0.8), 0.0), 0.5))
+    for i in range(8):
+        ang = 2.0 * math.pi * i / 8
+        c = np.array([5.0 * math.cos(ang), 6.0, 5.0 * math.sin(ang)])
+        emission = tuple(float(x) for x in 1.0 + 2.0 * rng.random(3))
+        objects.append(SimpleObject(ParallelogramMesh(tuple(c - [1.0, 0.0, 1.0]), (2.0, 0.0, 0.0), (0.0, 0.0, 2.0)),
+                                    LightMaterial(emission)))
+    objects.append(SimpleObject(SphereMesh((0.0, -1000.0, 0.0), 1000.0), LambertianMaterial((0.5, 0.5, 0.5))))
+    fwd = np.array([-13.0, -2.0, -3.0])
+    cam = Camera(pos=(13.0, 2.0, 3.0), fwd=tuple(fwd / np.linalg.norm(fwd)), v_fov=20.0, focus_dist=10.0,
+                 defocus_angle=0.6)
+    opts = RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
+    return Scene(objects, SimpleSkybox), cam, opts, np.arange(521)

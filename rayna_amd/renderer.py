@@ -27,8 +27,8 @@ _lib: Optional[C.CDLL] = None
 
 # every symbol declared in include/rayna_gpu.h
 EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_options", "rayna_clear_accumulation",
-           "rayna_render", "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare",
-           "rayna_destroy", "rayna_last_error")
+           "rayna_render", "rayna_set_material_colours", "rayna_adjoint", "rayna_primary_hits", "rayna_dfs_order",
+           "rayna_tree_hash", "rayna_scene_prepare", "rayna_destroy", "rayna_last_error")
 
 
 class RayError(RuntimeError):
@@ -59,6 +59,11 @@ def lib() -> C.CDLL:
         L.rayna_render.restype = C.c_int
         L.rayna_render.argtypes = [vp, C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_uint32),
                                    C.POINTER(A.RenderStats)]
+        L.rayna_set_material_colours.restype = C.c_int
+        L.rayna_set_material_colours.argtypes = [vp, C.POINTER(C.c_float), C.c_uint32]
+        L.rayna_adjoint.restype = C.c_int
+        L.rayna_adjoint.argtypes = [vp, C.c_uint32, C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_double),
+                                    C.c_uint32, C.POINTER(A.RenderStats)]
         L.rayna_primary_hits.restype = C.c_int
         L.rayna_primary_hits.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.rayna_dfs_order.restype = C.c_int
@@ -198,6 +203,31 @@ class Renderer:
         stats = RenderStats(st.duration_s, int(st.accum_frames), int(st.samples), int(st.segments), st.kernel_ms,
                             st.extend_ms, st.shade_ms, int(st.extend_launches))
         return Render(img, stats)
+
+    # ---- differentiation ----
+    def set_material_colours(self, rgb):
+        """Replace the material colours (albedo, or emission for lights), an (n, 3) array in material-table
+        order (Scene.materials_list()).  Clears accumulation."""
+        c = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        _check(lib().rayna_set_material_colours(self._h, c.ctypes.data_as(C.POINTER(C.c_float)), len(c)), self._h)
+
+    def adjoint(self, d_image, frame: int = 0, layout: str = "xmajor"):
+        """Path-replay gradient of a loss with respect to every material colour, (n_materials, 3) float64,
+        given d_image = dloss/d(frame `frame`'s per-pixel mean) in `layout` ((width, height, 3) for "xmajor",
+        (height, width, 3) for "rowmajor").  Returns (grad, RenderStats of the call)."""
+        o = self._options
+        d = np.ascontiguousarray(d_image, np.float32)
+        if d.size != o.width * o.height * 3:
+            raise ValueError(f"d_image has {d.size} values, expected {o.width * o.height * 3}")
+        n = len(self._scene.materials_list())
+        grad = np.zeros((n, 3), np.float64)
+        st = A.RenderStats()
+        lay = A.LAYOUT_XMAJOR if layout == "xmajor" else A.LAYOUT_ROWMAJOR
+        _check(lib().rayna_adjoint(self._h, frame, d.ctypes.data_as(C.POINTER(C.c_float)), lay,
+                                   grad.ctypes.data_as(C.POINTER(C.c_double)), n, C.byref(st)), self._h)
+        stats = RenderStats(st.duration_s, int(st.accum_frames), int(st.samples), int(st.segments), st.kernel_ms,
+                            st.extend_ms, st.shade_ms, int(st.extend_launches))
+        return grad, stats
 
     # ---- parity hooks ----
     def primary_hits(self, sample: int = 0):

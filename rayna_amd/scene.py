@@ -183,6 +183,17 @@ class Scene:
         self.desc()
         return self._cache[1]["n_triangles"]
 
+    def materials_list(self) -> List[MaterialT]:
+        """The material table in the order the C description holds it (first use by object order)."""
+        self.desc()
+        return self._cache[1]["materials"]
+
+    def colours(self) -> np.ndarray:
+        """The material colours (albedo, or emission for lights) as an (n_materials, 3) float32 array."""
+        self.desc()
+        return np.array([[self._cache[0].materials[i].colour[c] for c in range(3)]
+                         for i in range(self._cache[0].n_materials)], np.float32).reshape(-1, 3)
+
 
 @dataclass
 class Camera:
@@ -221,6 +232,7 @@ def _ptr(arr: np.ndarray, ctype):
 
 def _build_desc(scene: Scene):
     materials: List[A.Material] = []
+    mat_objs: List[MaterialT] = []
     mat_ids = {}
     meshes: List[A.Mesh] = []
     spheres: List[np.ndarray] = []
@@ -238,6 +250,7 @@ def _build_desc(scene: Scene):
         if key not in mat_ids:
             mat_ids[key] = len(materials)
             materials.append(m._desc())
+            mat_objs.append(m)
         return mat_ids[key]
 
     def add_mesh(m) -> int:
@@ -314,7 +327,7 @@ def _build_desc(scene: Scene):
     mat_arr = (A.Material * max(1, len(materials)))(*materials)
     sdf_arr = (A.Sdf * max(1, len(sdfs)))(*sdfs)
     keep.update(sph=sph, tv=tv, tn=tn, pa=pa, bx=bx, obj=obj_arr, mesh=mesh_arr, mat=mat_arr, sdf=sdf_arr,
-                n_triangles=len(tv))
+                n_triangles=len(tv), materials=mat_objs)
     d = A.SceneDesc()
     d.objects, d.n_objects = C.cast(obj_arr, C.POINTER(A.Object)), len(objects)
     d.meshes, d.n_meshes = C.cast(mesh_arr, C.POINTER(A.Mesh)), len(meshes)
