@@ -246,6 +246,57 @@ rayna_status rayna_set_material_colours(rayna_renderer *r, const float *rgb, uin
 rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_image, int32_t layout,
                            double *grad_rgb, uint32_t n_materials, rayna_render_stats *stats);
 
+/* ---------------------------------------------------------------- tuning */
+
+/* Launch shape of the wavefront kernels.  Zero fields keep the default. */
+typedef struct rayna_tuning {
+    uint32_t chunk_slots;          /* paths in flight per chunk (default 2^23) */
+    uint32_t refill_lanes;         /* extend: idle lanes of a wave before it takes new rays, 1..64 (default 16) */
+    uint32_t wave_batch;           /* extend: queue positions a wave claims per atomic, 1..4096 (default 64) */
+    uint32_t shade_blocks_per_cu;  /* shade: blocks of 256 threads per compute unit, 1..64 (default 8) */
+    int32_t use_graph;             /* batches: capture the launch sequence in a hipGraph and replay it (1) */
+    int32_t schedule;              /* batches: 0 = requests in the given order, 1 = fewest samples first */
+} rayna_tuning;
+
+/* Applies a tuning to a renderer (chunk_slots takes effect with the next rayna_set_options). */
+rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t);
+
+/* ---------------------------------------------------------------- batches
+ * Many render requests (each its own scene, camera, resolution, samples and seed) traced together: the
+ * requests' samples are packed into chunks of paths, and every chunk runs one generate / extend / shade
+ * pipeline over the paths of all its requests, so small requests share launches and fill the GPU.  Each
+ * request's image is bit for bit the image of a fresh Renderer's first frame with the same inputs.  No engine
+ * counterpart (the engine renders one scene per Renderer::render call, render/renderer.rs:224-264). */
+typedef struct rayna_batch rayna_batch;
+
+typedef struct rayna_request {
+    int32_t scene;            /* index into the scenes given to rayna_batch_create */
+    int32_t layout;           /* RAYNA_LAYOUT_* of out_rgb */
+    rayna_camera camera;
+    rayna_render_opts opts;   /* mode RAYNA_MODE_PBR, ray_branching 1 */
+    float *out_rgb;           /* host buffer of width*height*3 floats, or NULL to leave the image on the device
+                               * (an invalid camera gives the engine's checkerboard, as rayna_render) */
+} rayna_request;
+
+typedef struct rayna_batch_stats {
+    double duration_s;        /* host wall time of the call */
+    double kernel_ms;         /* device time of the batch */
+    uint64_t samples;         /* camera samples traced */
+    uint64_t segments;        /* closest-hit queries */
+    uint32_t chunks;          /* chunks of paths the batch was cut into */
+    uint32_t launches;        /* kernel launches */
+    double *request_ms;       /* if not NULL (n entries): device time from the batch start until each request's
+                               * image was final; filled when the batch ran without a graph */
+} rayna_batch_stats;
+
+/* Packs and uploads the scenes (each as rayna_create does: the engine's SAH trees, flattened). */
+rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes, rayna_batch **out);
+rayna_status rayna_batch_set_tuning(rayna_batch *b, const rayna_tuning *t);
+/* Renders n requests; returns when every image is final (and copied to its out_rgb). */
+rayna_status rayna_render_batch(rayna_batch *b, const rayna_request *reqs, int32_t n, rayna_batch_stats *stats);
+void rayna_batch_destroy(rayna_batch *b);
+const char *rayna_batch_last_error(const rayna_batch *b);
+
 /* Parity hook (no engine counterpart; the engine's RenderMode debug views are the closest,
  * renderer.rs:428-490): closest hit of the camera ray of sample `sample_index` of every pixel, row-major.
  * prim_rank receives the primitive's position in the depth-first order of the engine's BVH

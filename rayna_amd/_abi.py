@@ -66,3 +66,19 @@ class RenderStats(C.Structure):
     _fields_ = [("duration_s", C.c_double), ("kernel_ms", C.c_double), ("extend_ms", C.c_double),
                 ("shade_ms", C.c_double), ("accum_frames", C.c_uint64), ("samples", C.c_uint64),
                 ("segments", C.c_uint64), ("extend_launches", C.c_uint64)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("chunk_slots", C.c_uint32), ("refill_lanes", C.c_uint32), ("wave_batch", C.c_uint32),
+                ("shade_blocks_per_cu", C.c_uint32), ("use_graph", C.c_int32), ("schedule", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("layout", C.c_int32), ("camera", CameraDesc), ("opts", RenderOptsDesc),
+                ("out_rgb", C.POINTER(C.c_float))]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("duration_s", C.c_double), ("kernel_ms", C.c_double), ("samples", C.c_uint64),
+                ("segments", C.c_uint64), ("chunks", C.c_uint32), ("launches", C.c_uint32),
+                ("request_ms", C.POINTER(C.c_double))]

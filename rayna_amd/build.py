@@ -40,7 +40,7 @@ def build_product():
     k = os.path.join(BUILD, "kernels.o")
     _run([HIPCC] + COMMON + DEVICE_FLAGS + ["-c", os.path.join(CSRC, "kernels.hip"), "-o", k])
     objs.append(k)
-    for src in ("renderer.cpp", "scene_build.cpp"):
+    for src in ("renderer.cpp", "batch.cpp", "scene_build.cpp"):
         o = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++"] + COMMON + ["-pthread", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-c",
                                  os.path.join(CSRC, src), "-o", o])

@@ -1,0 +1,555 @@
+// Batches of render requests (include/rayna_gpu.h: rayna_batch_*): several scenes merged into one device
+// scene set, and the samples of many requests packed into chunks of paths that each run one wavefront
+// pipeline (generate, extend and shade per depth, resolve) over all of their requests.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rayna_gpu.h"
+#include "host_util.hpp"
+#include "kernels.hpp"
+#include "scene_build.hpp"
+
+using namespace rayna;
+
+namespace {
+
+std::string g_batch_create_error;
+constexpr uint64_t DEFAULT_CHUNK_SLOTS = 1ull << 23;
+
+// One request's share of a chunk: samples [s0, s0 + n) of request `req`.
+struct Piece {
+    uint32_t req, s0, n;
+};
+
+// The chunks of one batch call and everything uploaded for them.
+struct Plan {
+    std::vector<std::vector<Piece>> chunks;
+    std::vector<RequestParams> table;          // all chunks' entries, chunk c at [first_entry[c], ...)
+    std::vector<uint32_t> first_entry;
+    std::vector<uint32_t> finals;              // per chunk, entries (chunk-local) whose request ends there
+    std::vector<uint32_t> first_final;         // chunk c's finals at [first_final[c], first_final[c + 1])
+    std::vector<uint32_t> max_pixels;          // per chunk
+    std::vector<uint32_t> depth;               // per chunk: deepest ray_depth of its requests
+    std::vector<uint32_t> slots;               // per chunk
+    std::vector<int32_t> last_chunk;           // per request (-1: not traced, e.g. invalid camera)
+    std::vector<uint64_t> out_offset;          // per request, floats into the output buffer
+    uint64_t out_floats = 0;
+    uint32_t max_depth = 0;
+    uint64_t key = 0;                          // hash of the request list (graph reuse)
+};
+
+}  // namespace
+
+struct rayna_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> scene_allocs;
+    SceneView view{};
+    std::vector<SceneSlot> slots;
+    int2 *overflow = nullptr;
+    int cus = 1, grid_extend = 1024, grid_shade = 1024;
+    uint64_t chunk_slots = DEFAULT_CHUNK_SLOTS;
+    ExtendTuning tune;
+    bool use_graph = false;
+    bool shortest_first = false;
+
+    // path state for chunk_slots slots
+    uint64_t capacity = 0;
+    std::vector<void *> path_allocs;
+    PathBuffers pb{};
+    uint32_t *slot_req = nullptr;
+    int32_t *queue[2] = {nullptr, nullptr};
+    uint32_t *queue_count = nullptr;
+    uint32_t queue_count_len = 0;
+    unsigned long long *segments = nullptr;
+
+    // per call
+    RequestParams *table = nullptr;
+    uint64_t table_cap = 0;
+    uint32_t *finals = nullptr;
+    uint64_t finals_cap = 0;
+    float *sum = nullptr, *out = nullptr;
+    uint64_t sum_cap = 0, out_cap = 0;
+    std::vector<hipEvent_t> events;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_key = 0;
+    std::string error;
+};
+
+namespace {
+
+rayna_status bfail(rayna_batch *b, rayna_status st, const std::string &msg) {
+    if (b) b->error = msg;
+    else g_batch_create_error = msg;
+    return st;
+}
+
+#define BHIP(b, expr)                                                                                        \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            return bfail((b), e_ == hipErrorOutOfMemory ? RAYNA_OUT_OF_MEMORY : RAYNA_DEVICE_ERROR,         \
+                         std::string(#expr) + ": " + hipGetErrorString(e_));                                 \
+    } while (0)
+
+template <class T>
+rayna_status bupload(rayna_batch *b, const std::vector<T> &v, const T **out) {
+    *out = nullptr;
+    if (v.empty()) return RAYNA_OK;
+    void *p = nullptr;
+    BHIP(b, hipMalloc(&p, v.size() * sizeof(T)));
+    b->scene_allocs.push_back(p);
+    BHIP(b, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<const T *>(p);
+    return RAYNA_OK;
+}
+
+void free_graph(rayna_batch *b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    b->graph_key = 0;
+}
+
+void free_paths(rayna_batch *b) {
+    for (void *p : b->path_allocs) (void)hipFree(p);
+    b->path_allocs.clear();
+    b->pb = PathBuffers{};
+    b->slot_req = nullptr;
+    b->queue[0] = b->queue[1] = nullptr;
+    b->queue_count = nullptr;
+    b->queue_count_len = 0;
+    b->segments = nullptr;
+    b->capacity = 0;
+    free_graph(b);
+}
+
+rayna_status alloc_paths(rayna_batch *b, uint64_t slots, uint32_t max_depth) {
+    if (b->capacity == slots && b->queue_count_len >= 2 * (max_depth + 2)) return RAYNA_OK;
+    free_paths(b);
+    auto alloc = [&](size_t bytes, void **out) -> rayna_status {
+        BHIP(b, hipMalloc(out, std::max<size_t>(bytes, 4)));
+        b->path_allocs.push_back(*out);
+        return RAYNA_OK;
+    };
+    rayna_status st;
+    float **fields[] = {&b->pb.ox, &b->pb.oy, &b->pb.oz, &b->pb.dx, &b->pb.dy, &b->pb.dz, &b->pb.tr,
+                        &b->pb.tg, &b->pb.tb, &b->pb.lr, &b->pb.lg, &b->pb.lb, &b->pb.hit_t};
+    for (float **f : fields)
+        if ((st = alloc(slots * sizeof(float), (void **)f)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->pb.hit_ref)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->pb.hit_aux)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->slot_req)) != RAYNA_OK) return st;
+    for (int q = 0; q < 2; ++q)
+        if ((st = alloc(slots * sizeof(int32_t), (void **)&b->queue[q])) != RAYNA_OK) return st;
+    const uint32_t qlen = 2 * (std::max<uint32_t>(max_depth, 16) + 2);
+    if ((st = alloc(qlen * sizeof(uint32_t), (void **)&b->queue_count)) != RAYNA_OK) return st;
+    if ((st = alloc(sizeof(unsigned long long), (void **)&b->segments)) != RAYNA_OK) return st;
+    b->pb.hit_rank = nullptr;
+    b->capacity = slots;
+    b->queue_count_len = qlen;
+    return RAYNA_OK;
+}
+
+template <class T>
+rayna_status ensure(rayna_batch *b, T **p, uint64_t *cap, uint64_t n) {
+    if (*cap >= n && *p) return RAYNA_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const uint64_t want = std::max<uint64_t>(n, 1);
+    BHIP(b, hipMalloc((void **)p, want * sizeof(T)));
+    *cap = want;
+    free_graph(b);  // a captured graph holds the old pointers
+    return RAYNA_OK;
+}
+
+uint64_t mix(uint64_t h, uint64_t v) {
+    h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    return h;
+}
+
+rayna_status validate(rayna_batch *b, const rayna_request &q, int i) {
+    const rayna_render_opts &o = q.opts;
+    const std::string at = "request " + std::to_string(i) + ": ";
+    if (q.scene < 0 || q.scene >= (int32_t)b->slots.size()) return bfail(b, RAYNA_INVALID_ARGUMENT, at + "bad scene");
+    if (q.layout != RAYNA_LAYOUT_XMAJOR && q.layout != RAYNA_LAYOUT_ROWMAJOR)
+        return bfail(b, RAYNA_INVALID_ARGUMENT, at + "unknown layout");
+    if (o.width == 0 || o.height == 0 || o.samples == 0 || o.ray_branching == 0)
+        return bfail(b, RAYNA_INVALID_ARGUMENT, at + "zero width, height, samples or branching");
+    if (o.ray_branching != 1) return bfail(b, RAYNA_UNSUPPORTED, at + "ray_branching > 1");
+    if (o.mode != RAYNA_MODE_PBR) return bfail(b, RAYNA_UNSUPPORTED, at + "only RenderMode::PBR");
+    if (o.ray_depth > 1000) return bfail(b, RAYNA_UNSUPPORTED, at + "ray_depth above 1000");
+    if ((uint64_t)o.width * o.height > b->chunk_slots)
+        return bfail(b, RAYNA_UNSUPPORTED, at + "more pixels than paths per chunk");
+    return RAYNA_OK;
+}
+
+// Cuts the requests' samples into chunks of at most chunk_slots paths, in request order and sample order, in
+// whole sample layers (width * height paths each).
+void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vector<GViewport> &vps,
+               const std::vector<char> &valid, Plan &pl) {
+    const uint64_t C = b->chunk_slots;
+    pl = Plan{};
+    pl.last_chunk.assign(n, -1);
+    pl.out_offset.assign(n, 0);
+    uint64_t fill = C;  // forces a new chunk
+    uint64_t h = 1469598103934665603ull;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    if (b->shortest_first)
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            const auto work = [&](int k) { return (uint64_t)reqs[k].opts.width * reqs[k].opts.height * reqs[k].opts.samples; };
+            return work(x) < work(y);
+        });
+    h = mix(h, b->shortest_first ? 1 : 0);
+    for (int oi = 0; oi < n; ++oi) {
+        const int i = order[oi];
+        const rayna_request &q = reqs[i];
+        const uint64_t P = (uint64_t)q.opts.width * q.opts.height;
+        pl.out_offset[i] = pl.out_floats;
+        pl.out_floats += 3 * P;
+        h = mix(h, ((uint64_t)q.scene << 48) ^ ((uint64_t)q.opts.width << 32) ^ ((uint64_t)q.opts.height << 16) ^
+                       q.opts.samples);
+        h = mix(h, q.opts.seed);
+        h = mix(h, ((uint64_t)q.opts.ray_depth << 8) ^ (uint64_t)q.layout);
+        uint64_t cam_bits[9];
+        std::memcpy(cam_bits, &q.camera, sizeof(cam_bits));
+        for (uint64_t w : cam_bits) h = mix(h, w);
+        if (!valid[i]) continue;
+        uint32_t done = 0;
+        while (done < q.opts.samples) {
+            uint64_t fit = (C - fill) / P;
+            if (fit == 0) {
+                pl.chunks.emplace_back();
+                fill = 0;
+                fit = C / P;
+            }
+            const uint32_t take = (uint32_t)std::min<uint64_t>(fit, q.opts.samples - done);
+            pl.chunks.back().push_back(Piece{(uint32_t)i, done, take});
+            fill += take * P;
+            done += take;
+        }
+        pl.last_chunk[i] = (int32_t)pl.chunks.size() - 1;
+    }
+    pl.key = h;
+    for (size_t c = 0; c < pl.chunks.size(); ++c) {
+        pl.first_entry.push_back((uint32_t)pl.table.size());
+        pl.first_final.push_back((uint32_t)pl.finals.size());
+        uint32_t slot = 0, maxp = 0, depth = 0;
+        for (size_t e = 0; e < pl.chunks[c].size(); ++e) {
+            const Piece &pc = pl.chunks[c][e];
+            const rayna_request &q = reqs[pc.req];
+            const uint32_t P = q.opts.width * q.opts.height;
+            RequestParams R{};
+            R.vp = vps[pc.req];
+            R.width = q.opts.width;
+            R.height = q.opts.height;
+            R.spp = q.opts.samples;
+            R.sdim = isqrt(q.opts.samples);
+            R.key0 = (uint32_t)q.opts.seed;
+            R.key1 = (uint32_t)(q.opts.seed >> 32);  // frame 0
+            R.ray_depth = q.opts.ray_depth;
+            R.n_pixels = P;
+            R.s0 = pc.s0;
+            R.slot_base = slot;
+            R.n_slots = pc.n * P;
+            const SceneSlot &sc = b->slots[q.scene];
+            R.root = sc.root;
+            R.first_sdf = sc.first_sdf;
+            R.n_sdfs = sc.n_sdfs;
+            R.skybox = sc.skybox;
+            R.sum_offset = (uint32_t)pl.out_offset[pc.req];
+            R.layout = q.layout;
+            R.first_piece = pc.s0 == 0 ? 1u : 0u;
+            pl.table.push_back(R);
+            if (pc.s0 + pc.n == q.opts.samples) pl.finals.push_back((uint32_t)e);
+            slot += R.n_slots;
+            maxp = std::max(maxp, P);
+            depth = std::max(depth, q.opts.ray_depth);
+        }
+        pl.max_pixels.push_back(maxp);
+        pl.depth.push_back(depth);
+        pl.slots.push_back(slot);
+        pl.max_depth = std::max(pl.max_depth, depth);
+    }
+    pl.first_final.push_back((uint32_t)pl.finals.size());
+}
+
+// Enqueues every chunk of the plan.  `ev` (eager runs only): an event after the start and after each chunk.
+uint32_t enqueue(rayna_batch *b, const Plan &pl, std::vector<hipEvent_t> *ev) {
+    hipStream_t s = b->stream;
+    uint32_t launches = 0;
+    launch_fill_u32(reinterpret_cast<uint32_t *>(b->segments), 0u, 2, s);
+    ++launches;
+    if (ev) (void)hipEventRecord((*ev)[0], s);
+    for (size_t c = 0; c < pl.chunks.size(); ++c) {
+        BatchView bv{b->table + pl.first_entry[c], (uint32_t)pl.chunks[c].size(), b->slot_req};
+        FrameParams fp{};
+        fp.n_slots = pl.slots[c];
+        fp.ray_depth = pl.depth[c];
+        const uint32_t D = pl.depth[c];
+        launch_fill_u32(b->queue_count, 0u, 2 * (D + 2), s);
+        launch_generate(fp, b->pb, false, bv, s);
+        launches += 2;
+        for (uint32_t depth = 0; depth <= D; ++depth) {
+            const int32_t *qin = depth == 0 ? nullptr : b->queue[depth & 1];
+            const uint32_t *cin = depth == 0 ? nullptr : b->queue_count + depth;
+            int32_t *qout = b->queue[(depth + 1) & 1];
+            uint32_t *cout = b->queue_count + depth + 1;
+            uint32_t *work = b->queue_count + (D + 2) + depth;
+            launch_extend(b->view, fp, b->pb, qin, cin, depth, b->grid_extend, b->overflow, work, b->tune, bv, s);
+            launch_shade(b->view, fp, b->pb, qin, cin, qout, cout, depth, b->grid_shade, nullptr, bv, s);
+            launches += 2;
+        }
+        launch_count_segments(b->queue_count, D + 1, fp.n_slots, b->segments, s);
+        launch_resolve_batch(bv, pl.max_pixels[c], b->pb, b->sum, s);
+        const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
+        launch_finalize_batch(bv, b->finals + pl.first_final[c], nf, pl.max_pixels[c], b->sum, b->out, s);
+        launches += 2 + (nf ? 1 : 0);
+        if (ev) (void)hipEventRecord((*ev)[1 + c], s);
+    }
+    return launches;
+}
+
+void write_checkerboard(uint32_t w, uint32_t h, float *out, int layout) {  // renderer.rs:269-292
+    for (uint32_t x = 0; x < w; ++x)
+        for (uint32_t y = 0; y < h; ++y) {
+            const size_t o = layout == RAYNA_LAYOUT_XMAJOR ? ((size_t)x * h + y) * 3 : ((size_t)y * w + x) * 3;
+            const bool black = (x + y) % 2 == 0;
+            out[o] = black ? 0.0f : 1.0f;
+            out[o + 1] = 0.0f;
+            out[o + 2] = black ? 0.0f : 1.0f;
+        }
+}
+
+uint32_t ilog2(uint32_t v) {
+    uint32_t r = 0;
+    while (v >>= 1) ++r;
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes, rayna_batch **out) {
+    g_batch_create_error.clear();
+    if (!scenes || n_scenes <= 0 || !out) return bfail(nullptr, RAYNA_INVALID_ARGUMENT, "no scenes or null out");
+    *out = nullptr;
+    rayna_batch *b = new rayna_batch();
+    auto bail = [&](rayna_status st) {
+        g_batch_create_error = b->error;
+        rayna_batch_destroy(b);
+        return st;
+    };
+    if (hipGetDevice(&b->device) != hipSuccess) {
+        b->error = "no HIP device";
+        return bail(RAYNA_DEVICE_ERROR);
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        b->error = "hipStreamCreate failed";
+        return bail(RAYNA_DEVICE_ERROR);
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) != hipSuccess) {
+        b->error = "hipGetDeviceProperties failed";
+        return bail(RAYNA_DEVICE_ERROR);
+    }
+    b->cus = std::max(1, prop.multiProcessorCount);
+    b->grid_extend = b->cus * extend_blocks_per_cu();
+    b->grid_shade = b->cus * 8;
+    std::vector<PackedScene> packed(n_scenes);
+    for (int i = 0; i < n_scenes; ++i) {
+        std::string err;
+        const rayna_status st = pack_scene(scenes[i], packed[i], err);
+        if (st != RAYNA_OK) {
+            b->error = "scene " + std::to_string(i) + ": " + err;
+            return bail(st);
+        }
+    }
+    PackedScene merged;
+    if (!merge_scenes(packed, merged, b->slots)) {
+        b->error = "the scenes together exceed the index range of the device layout";
+        return bail(RAYNA_UNSUPPORTED);
+    }
+    if (merged.stack_need > 4096) {
+        b->error = "scene BVH is deeper than 4096 stack entries";
+        return bail(RAYNA_UNSUPPORTED);
+    }
+    rayna_status st;
+    SceneView &v = b->view;
+#define UP(vec, field) \
+    if ((st = bupload(b, merged.vec, &v.field)) != RAYNA_OK) return bail(st);
+    UP(nodes, nodes)
+    UP(tris, tris)
+    UP(tri_normals, tri_normals)
+    UP(items, items)
+    UP(spheres, spheres)
+    UP(paras, paras)
+    UP(boxes, boxes)
+    UP(volumes, volumes)
+    UP(sdfs, sdfs)
+    UP(materials, materials)
+#undef UP
+    v.root = -1;  // per request
+    v.n_sdfs = (int32_t)merged.sdfs.size();
+    v.skybox = RAYNA_SKY_NONE;
+    v.stack_need = merged.stack_need;
+    const size_t deep = (size_t)std::max(0, merged.stack_need - extend_lds_stack());
+    if (deep > 0) {
+        void *p = nullptr;
+        if (hipMalloc(&p, deep * (size_t)b->grid_extend * extend_block() * sizeof(int2)) != hipSuccess) {
+            b->error = "overflow stack allocation failed";
+            return bail(RAYNA_OUT_OF_MEMORY);
+        }
+        b->scene_allocs.push_back(p);
+        b->overflow = static_cast<int2 *>(p);
+    }
+    *out = b;
+    return RAYNA_OK;
+}
+
+rayna_status rayna_batch_set_tuning(rayna_batch *b, const rayna_tuning *t) {
+    if (!b || !t) return bfail(b, RAYNA_INVALID_ARGUMENT, "null argument");
+    if (t->refill_lanes > 64 || t->wave_batch > 4096 || t->shade_blocks_per_cu > 64 ||
+        (t->chunk_slots && t->chunk_slots < (1u << 16)) || t->schedule < 0 || t->schedule > 1)
+        return bfail(b, RAYNA_INVALID_ARGUMENT, "tuning out of range");
+    b->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
+    b->tune.wave_batch = t->wave_batch ? t->wave_batch : ExtendTuning{}.wave_batch;
+    b->grid_shade = b->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : 8);
+    b->chunk_slots = t->chunk_slots ? t->chunk_slots : DEFAULT_CHUNK_SLOTS;
+    b->use_graph = t->use_graph != 0;
+    b->shortest_first = t->schedule == 1;
+    free_graph(b);
+    return RAYNA_OK;
+}
+
+rayna_status rayna_render_batch(rayna_batch *b, const rayna_request *reqs, int32_t n, rayna_batch_stats *stats) {
+    if (!b) return bfail(b, RAYNA_INVALID_ARGUMENT, "null batch");
+    if (n < 0 || (n > 0 && !reqs)) return bfail(b, RAYNA_INVALID_ARGUMENT, "bad request list");
+    if (n > 65535) return bfail(b, RAYNA_UNSUPPORTED, "more than 65535 requests in one batch");
+    const auto t0 = std::chrono::steady_clock::now();
+    BHIP(b, hipSetDevice(b->device));
+    std::vector<GViewport> vps(n);
+    std::vector<char> valid(n, 0);
+    for (int i = 0; i < n; ++i) {
+        const rayna_status st = validate(b, reqs[i], i);
+        if (st != RAYNA_OK) return st;
+        valid[i] = viewport(reqs[i].camera, vps[i]) ? 1 : 0;
+    }
+    Plan pl;
+    make_plan(b, reqs, n, vps, valid, pl);
+    rayna_status st;
+    if ((st = alloc_paths(b, b->chunk_slots, pl.max_depth)) != RAYNA_OK) return st;
+    if ((st = ensure(b, &b->table, &b->table_cap, pl.table.size())) != RAYNA_OK) return st;
+    if ((st = ensure(b, &b->finals, &b->finals_cap, pl.finals.size())) != RAYNA_OK) return st;
+    if (pl.out_floats >= (1ull << 32))
+        return bfail(b, RAYNA_UNSUPPORTED, "the batch's images exceed 2^32 floats; split it");
+    if ((st = ensure(b, &b->sum, &b->sum_cap, pl.out_floats)) != RAYNA_OK) return st;
+    if ((st = ensure(b, &b->out, &b->out_cap, pl.out_floats)) != RAYNA_OK) return st;
+    if (!pl.table.empty())
+        BHIP(b, hipMemcpyAsync(b->table, pl.table.data(), pl.table.size() * sizeof(RequestParams),
+                               hipMemcpyHostToDevice, b->stream));
+    if (!pl.finals.empty())
+        BHIP(b, hipMemcpyAsync(b->finals, pl.finals.data(), pl.finals.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, b->stream));
+    const bool graph = b->use_graph;
+    const size_t n_events = pl.chunks.size() + 2;
+    while (b->events.size() < n_events) {
+        hipEvent_t e;
+        BHIP(b, hipEventCreate(&e));
+        b->events.push_back(e);
+    }
+    uint32_t launches = 0;
+    if (graph) {
+        if (!b->graph_exec || b->graph_key != pl.key) {
+            free_graph(b);
+            BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            launches = enqueue(b, pl, nullptr);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess) return bfail(b, RAYNA_DEVICE_ERROR, std::string("graph capture: ") + hipGetErrorString(e));
+            b->graph = g;
+            BHIP(b, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+            b->graph_key = pl.key;
+        } else {
+            // the launch count of the same plan
+            for (size_t c = 0; c < pl.chunks.size(); ++c) {
+                const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
+                launches += 2 + 2 * (pl.depth[c] + 1) + 2 + (nf ? 1 : 0);
+            }
+            launches += 1;
+        }
+        BHIP(b, hipEventRecord(b->events[0], b->stream));
+        BHIP(b, hipGraphLaunch(b->graph_exec, b->stream));
+        BHIP(b, hipEventRecord(b->events[1], b->stream));
+    } else {
+        launches = enqueue(b, pl, &b->events);
+    }
+    BHIP(b, hipGetLastError());
+    for (int i = 0; i < n; ++i) {
+        if (!reqs[i].out_rgb || !valid[i]) continue;
+        BHIP(b, hipMemcpyAsync(reqs[i].out_rgb, b->out + pl.out_offset[i],
+                               3 * (size_t)reqs[i].opts.width * reqs[i].opts.height * sizeof(float),
+                               hipMemcpyDeviceToHost, b->stream));
+    }
+    unsigned long long segs = 0;
+    BHIP(b, hipMemcpyAsync(&segs, b->segments, sizeof(segs), hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int i = 0; i < n; ++i) {
+        if (valid[i] || !reqs[i].out_rgb) continue;
+        write_checkerboard(ilog2(reqs[i].opts.width), ilog2(reqs[i].opts.height), reqs[i].out_rgb, reqs[i].layout);
+    }
+    if (stats) {
+        rayna_batch_stats s{};
+        s.request_ms = stats->request_ms;
+        const size_t last = graph ? 1 : pl.chunks.size();
+        float ms = 0;
+        if (!pl.chunks.empty() || graph) BHIP(b, hipEventElapsedTime(&ms, b->events[0], b->events[last]));
+        s.kernel_ms = ms;
+        for (int i = 0; i < n; ++i) {
+            if (valid[i]) s.samples += (uint64_t)reqs[i].opts.width * reqs[i].opts.height * reqs[i].opts.samples;
+            if (!s.request_ms) continue;
+            s.request_ms[i] = 0.0;
+            if (!graph && pl.last_chunk[i] >= 0) {
+                float m = 0;
+                BHIP(b, hipEventElapsedTime(&m, b->events[0], b->events[1 + pl.last_chunk[i]]));
+                s.request_ms[i] = m;
+            }
+        }
+        s.segments = segs;
+        s.chunks = (uint32_t)pl.chunks.size();
+        s.launches = launches;
+        s.duration_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *stats = s;
+    }
+    return RAYNA_OK;
+}
+
+void rayna_batch_destroy(rayna_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    free_paths(b);
+    for (void *p : b->scene_allocs) (void)hipFree(p);
+    if (b->table) (void)hipFree(b->table);
+    if (b->finals) (void)hipFree(b->finals);
+    if (b->sum) (void)hipFree(b->sum);
+    if (b->out) (void)hipFree(b->out);
+    for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char *rayna_batch_last_error(const rayna_batch *b) { return b ? b->error.c_str() : g_batch_create_error.c_str(); }
+
+}  // extern "C"

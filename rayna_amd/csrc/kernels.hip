@@ -3,6 +3,8 @@
 // (rayna_engine/src/render/renderer.rs:350-572) for a whole chunk of paths held as SoA queues in HBM.
 #include <hip/hip_runtime.h>
 #include <float.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "../../include/rayna_gpu.h"
@@ -469,12 +471,34 @@ __device__ __forceinline__ Col sky(int kind, f3 d) {  // skybox/simple.rs, skybo
 // ------------------------------------------------------------------------------------------ kernels
 
 // Camera ray of every slot of the chunk: render_px_msaa's stratified jitter (renderer.rs:370-389) and
-// Viewport::calc_ray (camera.rs:362-384); throughput 1, radiance 0.
-__global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers pb, int keep_radiance) {
+// Viewport::calc_ray (camera.rs:362-384); throughput 1, radiance 0.  BATCH: each slot finds its request (the
+// last with slot_base <= slot) and records it for the later kernels.
+template <bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers pb, int keep_radiance, BatchView bv) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= fp.n_slots) return;
-    const uint32_t pixel = i % fp.n_pixels;
-    const uint32_t k = fp.s0 + i / fp.n_pixels;
+    uint32_t l = i;  // slot within the request
+    if (BATCH) {
+        uint32_t lo = 0, hi = bv.n_reqs - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (bv.reqs[mid].slot_base <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        bv.slot_req[i] = lo;
+        const RequestParams &R = bv.reqs[lo];
+        fp.vp = R.vp;
+        fp.width = R.width;
+        fp.height = R.height;
+        fp.sdim = R.sdim;
+        fp.key0 = R.key0;
+        fp.key1 = R.key1;
+        fp.n_pixels = R.n_pixels;
+        fp.s0 = R.s0;
+        l = i - R.slot_base;
+    }
+    const uint32_t pixel = l % fp.n_pixels;
+    const uint32_t k = fp.s0 + l / fp.n_pixels;
     const uint32_t x = pixel % fp.width, y = pixel / fp.width;
     float u[4];
     dm::draw4(dm::Key{fp.key0, fp.key1}, pixel, k, 0, dm::SLOT_CAMERA, u);
@@ -539,13 +563,12 @@ __device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy,
 // colour-independent factor (lambertian, metal, isotropic, dielectric leaving the medium) adds d*R/albedo to
 // its albedo gradient (nothing for a channel whose albedo is exactly 0).  The colours never change a sampling
 // decision, so the replay follows the primal path exactly.
-template <bool ADJOINT>
+template <bool ADJOINT, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, PathBuffers pb,
                                                  const int32_t *queue_in, const uint32_t *count_in,
                                                  int32_t *queue_out, uint32_t *count_out, uint32_t depth,
-                                                 AdjointParams ap) {
+                                                 AdjointParams ap, BatchView bv) {
     const uint32_t count = queue_in ? *count_in : fp.n_slots;
-    const dm::Key key{fp.key0, fp.key1};
     const uint32_t lane = __lane_id();
     const uint32_t copy = ADJOINT ? blockIdx.x % ap.copies : 0u;
     for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
@@ -559,9 +582,23 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, P
             Col T{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
             Col L{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
             const uint32_t ref = pb.hit_ref[slot];
-            const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
+            dm::Key key{fp.key0, fp.key1};
+            uint32_t pixel, sample, ray_depth = fp.ray_depth;
+            int32_t skybox = sv.skybox;
+            if (BATCH) {
+                const RequestParams &R = bv.reqs[bv.slot_req[slot]];
+                const uint32_t l = slot - R.slot_base;
+                pixel = l % R.n_pixels;
+                sample = R.s0 + l / R.n_pixels;
+                key = dm::Key{R.key0, R.key1};
+                ray_depth = R.ray_depth;
+                skybox = R.skybox;
+            } else {
+                pixel = slot % fp.n_pixels;
+                sample = fp.s0 + slot / fp.n_pixels;
+            }
             if (ref == 0u) {
-                if (!ADJOINT) L = L + T * sky(sv.skybox, ray.d);
+                if (!ADJOINT) L = L + T * sky(skybox, ray.d);
             } else {
                 SHit h;
                 hit_geometry(sv, ray, pb.hit_t[slot], ref, pb.hit_aux[slot], key, pixel, sample, depth, &h);
@@ -585,7 +622,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, P
                 dm::draw4(key, pixel, sample, depth, dm::SLOT_SCATTER, u);
                 f3 dir;
                 if (scatter(m, ray, h, u, &dir)) {
-                    if (ADJOINT && depth < fp.ray_depth) {
+                    if (ADJOINT && depth < ray_depth) {
                         const bool albedo_scales = m.kind != RAYNA_MAT_DIELECTRIC || !h.front;
                         if (albedo_scales)
                             grad_add(ap, copy, h.material, m.r != 0.0f ? d0 * (double)L.r / (double)m.r : 0.0,
@@ -593,7 +630,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, P
                                      m.b != 0.0f ? d2 * (double)L.b / (double)m.b : 0.0);
                     }
                     T = reflected(m, ray, h, T);
-                    if (depth < fp.ray_depth) {
+                    if (depth < ray_depth) {
                         const DRay nr = make_ray(h.pos, dir);
                         pb.ox[slot] = nr.o.x;
                         pb.oy[slot] = nr.o.y;
@@ -672,6 +709,46 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FrameParams fp, const float 
     }
 }
 
+// k_resolve for the requests of a batch chunk: blockIdx.y selects the request.
+__global__ __launch_bounds__(BLOCK) void k_resolve_batch(BatchView bv, PathBuffers pb, float *sum) {
+    const RequestParams &R = bv.reqs[blockIdx.y];
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t P = R.n_pixels;
+    if (p >= P) return;
+    float *su = sum + R.sum_offset;
+    const bool first = R.first_piece != 0;
+    float r = first ? 0.0f : su[p], g = first ? 0.0f : su[P + p], b = first ? 0.0f : su[2 * P + p];
+    const uint32_t n = R.n_slots / P;
+    for (uint32_t s = 0; s < n; ++s) {
+        const uint32_t slot = R.slot_base + s * P + p;
+        r = r + pb.lr[slot];
+        g = g + pb.lg[slot];
+        b = b + pb.lb[slot];
+    }
+    su[p] = r;
+    su[P + p] = g;
+    su[2 * P + p] = b;
+}
+
+// The sample mean of each finished request of a batch (a first frame: the accumulation is the mean itself),
+// in the request's layout.
+__global__ __launch_bounds__(BLOCK) void k_finalize_batch(BatchView bv, const uint32_t *final_reqs, const float *sum,
+                                                          float *out) {
+    const RequestParams &R = bv.reqs[final_reqs[blockIdx.y]];
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t P = R.n_pixels;
+    if (p >= P) return;
+    const float n = (float)R.spp;
+    const float *su = sum + R.sum_offset;
+    const uint32_t x = p % R.width, y = p / R.width;
+    const size_t o = R.sum_offset + (R.layout == RAYNA_LAYOUT_XMAJOR ? ((size_t)x * R.height + y) * 3 : (size_t)p * 3);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float a = 0.0f + su[c * P + p] / n;
+        out[o + c] = (float)((double)a / 1.0);
+    }
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t value, uint32_t n) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) p[i] = value;
@@ -693,32 +770,60 @@ inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block -
 int extend_lds_stack() { return EXT_LDS_STACK; }
 
 int extend_blocks_per_cu() {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend, EXT_BLOCK, 0) != hipSuccess || n < 1) n = 4;
+    int n = 0, nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend<false>, EXT_BLOCK, 0) != hipSuccess || n < 1) n = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<true>, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
+        n = std::min(n, nb);
     return n;
 }
 int extend_block() { return EXT_BLOCK; }
 
-void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, hipStream_t s) {
-    hipLaunchKernelGGL(k_generate, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb,
-                       keep_radiance ? 1 : 0);
+void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, const BatchView &bv,
+                     hipStream_t s) {
+    if (bv.reqs)
+        hipLaunchKernelGGL(k_generate<true>, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb,
+                           keep_radiance ? 1 : 0, bv);
+    else
+        hipLaunchKernelGGL(k_generate<false>, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb,
+                           keep_radiance ? 1 : 0, bv);
 }
 
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
-                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work, hipStream_t s) {
-    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth, overflow,
-                       work);
+                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work,
+                   const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
+    if (bv.reqs)
+        hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth,
+                           overflow, work, tune, bv);
+    else
+        hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth,
+                           overflow, work, tune, bv);
 }
 
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
-                  const AdjointParams *adjoint, hipStream_t s) {
-    if (adjoint)
-        hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in, queue_out,
-                           count_out, depth, *adjoint);
+                  const AdjointParams *adjoint, const BatchView &bv, hipStream_t s) {
+    if (bv.reqs)  // batches are forward renders
+        hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in,
+                           queue_out, count_out, depth, AdjointParams{}, bv);
+    else if (adjoint)
+        hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in,
+                           queue_out, count_out, depth, *adjoint, bv);
     else
-        hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in,
-                           queue_out, count_out, depth, AdjointParams{});
+        hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in,
+                           queue_out, count_out, depth, AdjointParams{}, bv);
+}
+
+void launch_resolve_batch(const BatchView &bv, uint32_t max_pixels, const PathBuffers &pb, float *sum,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(max_pixels, BLOCK), bv.n_reqs), dim3(BLOCK), 0, s, bv, pb,
+                       sum);
+}
+
+void launch_finalize_batch(const BatchView &bv, const uint32_t *final_reqs, uint32_t n_final, uint32_t max_pixels,
+                           const float *sum, float *out, hipStream_t s) {
+    if (n_final == 0) return;
+    hipLaunchKernelGGL(k_finalize_batch, dim3(grid_for(max_pixels, BLOCK), n_final), dim3(BLOCK), 0, s, bv,
+                       final_reqs, sum, out);
 }
 
 void launch_sum_grad(const double *grad, uint32_t n, uint32_t copies, double *out, hipStream_t s) {

@@ -33,6 +33,26 @@ struct FrameParams {
     uint32_t n_slots;        // slots in the chunk
 };
 
+// One request of a batch, as seen by one chunk of paths: its slots are [slot_base, slot_base + n_slots), sample
+// s0 + l / n_pixels of pixel l % n_pixels for local slot l (sample-major, like a single frame).
+struct RequestParams {
+    GViewport vp;
+    uint32_t width, height, spp, sdim;
+    uint32_t key0, key1, ray_depth, n_pixels;
+    uint32_t s0, slot_base, n_slots;
+    int32_t root, first_sdf, n_sdfs, skybox;  // the request's scene within the merged scene set
+    uint32_t sum_offset;                      // its 3 * n_pixels floats in the batch's sum and output buffers
+    int32_t layout;                           // output layout (RAYNA_LAYOUT_*)
+    uint32_t first_piece;                     // 1 if the chunk holds the request's first samples
+};
+
+// The requests of a batch chunk.  reqs == nullptr: a single frame described by FrameParams and SceneView.
+struct BatchView {
+    const RequestParams *reqs;
+    uint32_t n_reqs;
+    uint32_t *slot_req;  // request (index into reqs) of every slot, written by generate
+};
+
 // Inputs and outputs of the adjoint replay (path replay backpropagation) of a frame.
 struct AdjointParams {
     const float *d_image;    // dloss/d(frame mean), 3 floats per pixel in row-major pixel order
@@ -42,16 +62,32 @@ struct AdjointParams {
     uint32_t copies;         // gradient copies (a block adds into copy blockIdx % copies)
 };
 
-// keep_radiance: leave the radiance field as it is (the adjoint replay keeps the primal radiance there)
-void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, hipStream_t s);
+// Traversal tuning of the extend kernel: a wave takes new rays once `refill_lanes` of its lanes are idle,
+// claiming `wave_batch` queue positions per atomic.
+struct ExtendTuning {
+    uint32_t refill_lanes = 16;
+    uint32_t wave_batch = 64;
+};
+
+// keep_radiance: leave the radiance field as it is (the adjoint replay keeps the primal radiance there).
+// With bv.reqs, fp.n_slots is the chunk's slot count and the requests come from bv.
+void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, const BatchView &bv,
+                     hipStream_t s);
 // queue == nullptr: identity queue of fp.n_slots entries; else queue[0 .. *count)
 // overflow: traversal stack entries beyond the LDS part, extend_block() * grid * (stack_need - extend_lds_stack());
 // work: a zeroed counter the launch's waves take rays from
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
-                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work, hipStream_t s);
+                   const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work,
+                   const ExtendTuning &tune, const BatchView &bv, hipStream_t s);
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
-                  const AdjointParams *adjoint, hipStream_t s);
+                  const AdjointParams *adjoint, const BatchView &bv, hipStream_t s);
+// Batch chunks: per request, the running per-pixel sum of the chunk's samples (in sample order) and, at the
+// request's last chunk, its sample mean in its layout at out + sum_offset.
+void launch_resolve_batch(const BatchView &bv, uint32_t max_pixels, const PathBuffers &pb, float *sum,
+                          hipStream_t s);
+void launch_finalize_batch(const BatchView &bv, const uint32_t *final_reqs, uint32_t n_final, uint32_t max_pixels,
+                           const float *sum, float *out, hipStream_t s);
 void launch_sum_grad(const double *grad, uint32_t n, uint32_t copies, double *out, hipStream_t s);
 void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, uint32_t chunk_samples, int first,
                     hipStream_t s);

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/rayna_gpu.h"
+#include "host_util.hpp"
 #include "kernels.hpp"
 #include "scene_build.hpp"
 
@@ -50,6 +51,7 @@ struct rayna_renderer {
     unsigned long long *segments = nullptr;
     float *sum = nullptr, *acc = nullptr, *out_dev = nullptr;
     uint32_t alloc_pixels = 0;
+    uint32_t queue_count_len = 0;
 
     uint64_t frames = 0;  // AccumulationBuffer::frame_count (accum_buffer.rs:77)
     bool acc_dirty = true;
@@ -62,6 +64,9 @@ struct rayna_renderer {
     double *grad_sum = nullptr;   // n_materials x 3
     uint32_t grad_materials = 0, grad_copies = 0;
     int grid_extend = 1024, grid_shade = 1024;
+    int cus = 1;
+    uint64_t chunk_target = TARGET_SLOTS;
+    ExtendTuning tune;
     std::vector<hipEvent_t> events;
     std::string error;
 };
@@ -81,42 +86,6 @@ rayna_status fail(rayna_renderer *r, rayna_status st, const std::string &msg) {
             return fail((r), e_ == hipErrorOutOfMemory ? RAYNA_OUT_OF_MEMORY : RAYNA_DEVICE_ERROR,           \
                         std::string(#expr) + ": " + hipGetErrorString(e_));                                   \
     } while (0)
-
-// Camera::calculate_viewport (scene/camera.rs:285-338) in double precision, rounded to float.
-bool viewport(const rayna_camera &c, GViewport &vp) {
-    const double focal = c.focus_dist;
-    if (c.v_fov == 0.0 || focal == 0.0) return false;
-    const double fl = std::sqrt((c.fwd[0] * c.fwd[0] + c.fwd[1] * c.fwd[1]) + c.fwd[2] * c.fwd[2]);
-    const double rf = 1.0 / fl;
-    if (!std::isfinite(rf) || !(rf > 0)) return false;
-    const double w[3] = {-(c.fwd[0] * rf), -(c.fwd[1] * rf), -(c.fwd[2] * rf)};
-    // u = cross(Y, w)
-    const double cu[3] = {1.0 * w[2] - 0.0 * w[1], 0.0 * w[0] - 0.0 * w[2], 0.0 * w[1] - 1.0 * w[0]};
-    const double ul = std::sqrt((cu[0] * cu[0] + cu[1] * cu[1]) + cu[2] * cu[2]);
-    const double ru = 1.0 / ul;
-    if (!std::isfinite(ru) || !(ru > 0)) return false;
-    const double u[3] = {cu[0] * ru, cu[1] * ru, cu[2] * ru};
-    const double v[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]};
-    const double defocus_radius = focal * std::tan(c.defocus_angle / 2.0);
-    const double hh = std::tan(c.v_fov / 2.0);
-    const double size = 2.0 * hh * focal;
-    for (int i = 0; i < 3; ++i) {
-        vp.pos[i] = (float)c.pos[i];
-        vp.pixel_center[i] = (float)(c.pos[i] - w[i] * focal);
-        vp.du[i] = (float)(u[i] * defocus_radius);
-        vp.dv[i] = (float)(v[i] * defocus_radius);
-        vp.vu[i] = (float)(u[i] * size);
-        vp.vv[i] = (float)(-v[i] * size);
-    }
-    return true;
-}
-
-uint32_t isqrt(uint32_t n) {
-    uint32_t s = (uint32_t)std::sqrt((double)n);
-    while (s * s > n) --s;
-    while ((s + 1) * (s + 1) <= n) ++s;
-    return s;
-}
 
 template <class T>
 rayna_status upload(rayna_renderer *r, DevScene &ds, const std::vector<T> &v, const T **out) {
@@ -195,15 +164,24 @@ void free_paths(rayna_renderer *r) {
     r->primal_frame = -1;
     r->slot_capacity = 0;
     r->alloc_pixels = 0;
+    r->queue_count_len = 0;
 }
 
 rayna_status alloc_paths(rayna_renderer *r) {
     const rayna_render_opts &o = r->opts;
     const uint64_t P = (uint64_t)o.width * o.height;
-    uint64_t chunk = std::max<uint64_t>(1, TARGET_SLOTS / std::max<uint64_t>(1, P));
+    uint64_t chunk = std::max<uint64_t>(1, r->chunk_target / std::max<uint64_t>(1, P));
     chunk = std::min<uint64_t>(chunk, o.samples);
     const uint64_t slots = P * chunk;
     if (slots > 0x7fffffffull) return fail(r, RAYNA_UNSUPPORTED, "image too large for one chunk of paths");
+    const uint32_t qlen = 2 * (o.ray_depth + 2) + 1;
+    if (slots <= r->slot_capacity && P <= r->alloc_pixels && qlen <= r->queue_count_len) {
+        // the buffers are large enough: keep them (grow-only)
+        r->chunk_samples = (uint32_t)chunk;
+        r->acc_dirty = true;
+        r->primal_frame = -1;
+        return RAYNA_OK;
+    }
     free_paths(r);
     auto alloc = [&](size_t bytes, void **out) -> rayna_status {
         HIP_TRY(r, hipMalloc(out, std::max<size_t>(bytes, 4)));
@@ -220,7 +198,7 @@ rayna_status alloc_paths(rayna_renderer *r) {
     for (int q = 0; q < 2; ++q)
         if ((st = alloc(slots * sizeof(int32_t), (void **)&r->queue[q])) != RAYNA_OK) return st;
     // [0, D+2): paths entering each depth; [D+2, 2D+4): the extend launches' ray counters; [2D+4]: spare
-    if ((st = alloc((2 * (o.ray_depth + 2) + 1) * sizeof(uint32_t), (void **)&r->queue_count)) != RAYNA_OK) return st;
+    if ((st = alloc(qlen * sizeof(uint32_t), (void **)&r->queue_count)) != RAYNA_OK) return st;
     if ((st = alloc(sizeof(unsigned long long), (void **)&r->segments)) != RAYNA_OK) return st;
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->sum)) != RAYNA_OK) return st;
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->acc)) != RAYNA_OK) return st;
@@ -230,6 +208,7 @@ rayna_status alloc_paths(rayna_renderer *r) {
     r->slot_capacity = slots;
     r->chunk_samples = (uint32_t)chunk;
     r->alloc_pixels = (uint32_t)P;
+    r->queue_count_len = qlen;
     r->acc_dirty = true;
     return RAYNA_OK;
 }
@@ -247,6 +226,7 @@ rayna_status set_grid(rayna_renderer *r) {
     hipDeviceProp_t prop;
     HIP_TRY(r, hipGetDeviceProperties(&prop, r->device));
     const int cus = std::max(1, prop.multiProcessorCount);
+    r->cus = cus;
     r->grid_extend = cus * extend_blocks_per_cu();
     r->grid_shade = cus * 8;
     return RAYNA_OK;
@@ -275,7 +255,7 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
     fp.s0 = s0;
     fp.n_slots = fp.n_pixels * n;
     launch_fill_u32(r->queue_count, 0u, 2 * (r->opts.ray_depth + 2), r->stream);
-    launch_generate(fp, r->pb, adjoint != nullptr, r->stream);
+    launch_generate(fp, r->pb, adjoint != nullptr, BatchView{}, r->stream);
     for (uint32_t depth = 0; depth <= r->opts.ray_depth; ++depth) {
         const int32_t *qin = depth == 0 ? nullptr : r->queue[depth & 1];
         const uint32_t *cin = depth == 0 ? nullptr : r->queue_count + depth;
@@ -283,9 +263,11 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
         uint32_t *cout = r->queue_count + depth + 1;
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
         uint32_t *work = r->queue_count + (r->opts.ray_depth + 2) + depth;
-        launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, work, r->stream);
+        launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, work, r->tune,
+                      BatchView{}, r->stream);
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
-        launch_shade(r->dev.view, fp, r->pb, qin, cin, qout, cout, depth, r->grid_shade, adjoint, r->stream);
+        launch_shade(r->dev.view, fp, r->pb, qin, cin, qout, cout, depth, r->grid_shade, adjoint, BatchView{},
+                     r->stream);
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
         ++*extend_launches;
     }
@@ -482,6 +464,17 @@ rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uin
     return RAYNA_OK;
 }
 
+rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t) {
+    if (!r || !t) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    if (t->refill_lanes > 64 || t->wave_batch > 4096 || t->shade_blocks_per_cu > 64)
+        return fail(r, RAYNA_INVALID_ARGUMENT, "tuning out of range");
+    r->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
+    r->tune.wave_batch = t->wave_batch ? t->wave_batch : ExtendTuning{}.wave_batch;
+    r->grid_shade = r->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : 8);
+    r->chunk_target = t->chunk_slots ? t->chunk_slots : TARGET_SLOTS;
+    return RAYNA_OK;
+}
+
 rayna_status rayna_set_material_colours(rayna_renderer *r, const float *rgb, uint32_t n) {
     if (!r || (n && !rgb)) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
     if (n > r->packed.materials.size()) return fail(r, RAYNA_INVALID_ARGUMENT, "more colours than materials");
@@ -600,10 +593,11 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     PathBuffers pb = r->pb;
     pb.hit_rank = rank_dev;
     r->primal_frame = -1;
-    launch_generate(fp, pb, false, r->stream);
+    launch_generate(fp, pb, false, BatchView{}, r->stream);
     uint32_t *work = r->queue_count + 2 * (r->opts.ray_depth + 2);
     launch_fill_u32(work, 0u, 1, r->stream);
-    launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, work, r->stream);
+    launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, work, r->tune,
+                  BatchView{}, r->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(prim_rank, rank_dev, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(t, pb.hit_t, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);

@@ -793,4 +793,59 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
     return RAYNA_OK;
 }
 
+bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vector<SceneSlot> &slots) {
+    out = PackedScene{};
+    slots.clear();
+    for (const PackedScene &p : in) {
+        const int32_t node_off = (int32_t)out.nodes.size(), tri_off = (int32_t)out.tris.size();
+        const int32_t item_off = (int32_t)out.items.size(), sphere_off = (int32_t)out.spheres.size();
+        const int32_t para_off = (int32_t)out.paras.size(), box_off = (int32_t)out.boxes.size();
+        const int32_t vol_off = (int32_t)out.volumes.size(), sdf_off = (int32_t)out.sdfs.size();
+        const int32_t mat_off = (int32_t)out.materials.size();
+        for (GNode g : p.nodes) {
+            for (int k = 0; k < 4; ++k) {
+                if (g.info[k] == INFO_INNER) g.child[k] += node_off;
+                else if (g.info[k] > 0) g.child[k] += (g.info[k] >> 4) == LEAF_TRIS ? tri_off : item_off;
+            }
+            out.nodes.push_back(g);
+        }
+        for (GTri t : p.tris) {
+            t.material += mat_off;
+            out.tris.push_back(t);
+        }
+        out.tri_normals.insert(out.tri_normals.end(), p.tri_normals.begin(), p.tri_normals.end());
+        for (GItem it : p.items) {
+            switch (it.kind) {
+                case ITEM_SPHERE: it.index += sphere_off; break;
+                case ITEM_TRIANGLE: it.index += tri_off; break;
+                case ITEM_PARALLELOGRAM: it.index += para_off; break;
+                case ITEM_AXIS_BOX: it.index += box_off; break;
+                case ITEM_VOLUME: it.index += vol_off; break;
+                case ITEM_LINK: it.index += node_off; break;
+                default: break;
+            }
+            it.material += mat_off;
+            out.items.push_back(it);
+        }
+        out.spheres.insert(out.spheres.end(), p.spheres.begin(), p.spheres.end());
+        out.paras.insert(out.paras.end(), p.paras.begin(), p.paras.end());
+        out.boxes.insert(out.boxes.end(), p.boxes.begin(), p.boxes.end());
+        for (GVolume v : p.volumes) {
+            v.boundary_index += v.boundary_kind == ITEM_SPHERE ? sphere_off : box_off;
+            out.volumes.push_back(v);
+        }
+        for (GSdf f : p.sdfs) {
+            f.first_sphere += sphere_off;
+            f.material += mat_off;
+            out.sdfs.push_back(f);
+        }
+        out.materials.insert(out.materials.end(), p.materials.begin(), p.materials.end());
+        slots.push_back(SceneSlot{p.root >= 0 ? p.root + node_off : -1, sdf_off, (int32_t)p.sdfs.size(), p.skybox,
+                                  p.stack_need});
+        out.stack_need = std::max(out.stack_need, p.stack_need);
+    }
+    // leaf starts have 27 bits in a traversal stack code; hit references keep 28 bits of index
+    return out.tris.size() < (1u << 27) && out.items.size() < (1u << 27) && out.sdfs.size() < (1u << 28);
+}
+
 }  // namespace rayna

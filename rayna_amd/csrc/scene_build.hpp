@@ -56,4 +56,18 @@ struct PackedScene {
 // Validates and packs a scene.  Returns RAYNA_OK, or an error status with a message in `error`.
 rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &out, std::string &error);
 
+// Where one scene of a merged scene set lives.
+struct SceneSlot {
+    int32_t root;       // its tree's root node, -1 if it has no bounded object
+    int32_t first_sdf;  // its unbounded objects [first_sdf, first_sdf + n_sdfs)
+    int32_t n_sdfs;
+    int32_t skybox;
+    int32_t stack_need;
+};
+
+// Concatenates packed scenes into one device layout: every index (child nodes, leaf ranges, item targets,
+// volume boundaries, SDF spheres, materials) is offset into the merged arrays.  Ranks stay per scene (they
+// only order hits within one traversal).  Returns false if the merged arrays overflow a leaf or hit index.
+bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vector<SceneSlot> &slots);
+
 }  // namespace rayna

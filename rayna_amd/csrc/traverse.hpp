@@ -18,14 +18,6 @@
 constexpr uint32_t LEAF_CODE_FLAG = 0x80000000u;
 constexpr uint32_t LEAF_CODE_ITEMS = 0x40000000u;
 constexpr uint32_t LEAF_CODE_START = 0x07ffffffu;
-#ifndef RAYNA_REFILL_LANES
-#define RAYNA_REFILL_LANES 16
-#endif
-constexpr int REFILL_LANES = RAYNA_REFILL_LANES;  // refill a wave once this many of its lanes are idle
-#ifndef RAYNA_WAVE_BATCH
-#define RAYNA_WAVE_BATCH 64
-#endif
-constexpr int WAVE_BATCH = RAYNA_WAVE_BATCH;  // queue positions a wave claims per atomic
 
 // Traversal counters, compiled in only for measurement builds (-DRAYNA_TRAVERSAL_STATS): per lane, node pops,
 // node expansions, primitive tests and loop iterations, summed into a device array read by
@@ -196,9 +188,10 @@ __device__ __forceinline__ void leaf_test(const SceneView &sv, const DRay &ray, 
     if (ok && (t < b.t || it.rank > b.rank)) b = Best{t, it.rank, ref, 0u};
 }
 
-// The unbounded objects, marched with the caller's full interval; strict < keeps the earlier of equal minima.
-__device__ __forceinline__ void unbounded_hits(const SceneView &sv, const DRay &ray, Best &b) {
-    for (int i = 0; i < sv.n_sdfs; ++i) {
+// The unbounded objects [first, first + n), marched with the caller's full interval; strict < keeps the earlier
+// of equal minima.
+__device__ __forceinline__ void unbounded_hits(const SceneView &sv, const DRay &ray, int first, int n, Best &b) {
+    for (int i = first; i < first + n; ++i) {
         const GSdf f = sv.sdfs[i];
         float t;
         bool front;
@@ -207,14 +200,19 @@ __device__ __forceinline__ void unbounded_hits(const SceneView &sv, const DRay &
     }
 }
 
-// Closest hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501).
+// Closest hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501).  BATCH: the slots
+// belong to the requests of bv, each with its own scene root, unbounded objects and RNG key.
+template <bool BATCH>
 __global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams fp, PathBuffers pb,
                                                       const int32_t *queue, const uint32_t *count_ptr,
-                                                      uint32_t depth, int2 *overflow, uint32_t *work) {
+                                                      uint32_t depth, int2 *overflow, uint32_t *work,
+                                                      ExtendTuning tune, BatchView bv) {
     __shared__ int32_t s_node[EXT_LDS_STACK * EXT_BLOCK];
     __shared__ float s_tent[EXT_LDS_STACK * EXT_BLOCK];
     const uint32_t count = queue ? *count_ptr : fp.n_slots;
-    const dm::Key key{fp.key0, fp.key1};
+    const uint32_t refill_lanes = tune.refill_lanes, wave_batch = tune.wave_batch;
+    dm::Key key{fp.key0, fp.key1};
+    int32_t sdf_first = 0, sdf_n = sv.n_sdfs;
     const uint32_t gid = blockIdx.x * EXT_BLOCK + threadIdx.x;
     const uint32_t lane = __lane_id();
     LaneStack st{s_node, s_tent, overflow + gid, gridDim.x * EXT_BLOCK, 0};
@@ -225,24 +223,24 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams 
     int32_t leaf = 0;
     int leaf_i = 0, leaf_n = 0;
     float leaf_t = 0.0f;
-    // The wave's unhanded queue positions [batch_next, batch_end): claimed WAVE_BATCH at a time with one atomic,
+    // The wave's unhanded queue positions [batch_next, batch_end): claimed wave_batch at a time with one atomic,
     // handed to idle lanes without further atomics.  (All of these are wave-uniform.)
     uint32_t batch_next = 0, batch_end = 0;
     RAYNA_STAT_DECL();
     bool last_claim = false;  // the counter has passed `count`: no further claims will yield work
     for (;;) {
         const unsigned long long idle = __ballot(!active);
-        if (!exhausted && (idle == ~0ull || __popcll(idle) >= REFILL_LANES)) {
+        if (!exhausted && (idle == ~0ull || (uint32_t)__popcll(idle) >= refill_lanes)) {
             const uint32_t n = (uint32_t)__popcll(idle);
             const uint32_t r = batch_end - batch_next;  // old positions, handed out first
             uint32_t nb = 0, nb_end = 0;                // new batch [nb, nb_end)
             if (r < n && !last_claim) {
                 const int leader = __ffsll((long long)idle) - 1;
                 uint32_t base = 0;
-                if ((int)lane == leader) base = atomicAdd(work, (uint32_t)WAVE_BATCH);
+                if ((int)lane == leader) base = atomicAdd(work, wave_batch);
                 base = __shfl(base, leader);
                 nb = base < count ? base : count;
-                nb_end = (count - nb > (uint32_t)WAVE_BATCH) ? nb + (uint32_t)WAVE_BATCH : count;
+                nb_end = (count - nb > wave_batch) ? nb + wave_batch : count;
                 if (nb_end >= count) last_claim = true;
             }
             const uint32_t idx = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));  // rank among idle lanes
@@ -262,11 +260,23 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams 
                     slot = queue ? (uint32_t)queue[q] : q;
                     ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
                                        f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
-                    pixel = slot % fp.n_pixels;
-                    sample = fp.s0 + slot / fp.n_pixels;
+                    int32_t root = sv.root;
+                    if (BATCH) {
+                        const RequestParams &R = bv.reqs[bv.slot_req[slot]];
+                        const uint32_t l = slot - R.slot_base;
+                        pixel = l % R.n_pixels;
+                        sample = R.s0 + l / R.n_pixels;
+                        key = dm::Key{R.key0, R.key1};
+                        root = R.root;
+                        sdf_first = R.first_sdf;
+                        sdf_n = R.n_sdfs;
+                    } else {
+                        pixel = slot % fp.n_pixels;
+                        sample = fp.s0 + slot / fp.n_pixels;
+                    }
                     b = Best{FLT_MAX, -1, 0u, 0u};
                     st.sp = 0;
-                    if (sv.root >= 0) st.push(sv.root, -__builtin_inff());
+                    if (root >= 0) st.push(root, -__builtin_inff());
                     leaf = 0;
                     active = true;
                 }
@@ -300,7 +310,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams 
                 if (++leaf_i == leaf_n) leaf = 0;
             }
             if (st.sp == 0) {
-                unbounded_hits(sv, ray, b);
+                unbounded_hits(sv, ray, sdf_first, sdf_n, b);
                 pb.hit_t[slot] = b.t;
                 pb.hit_ref[slot] = b.ref;
                 pb.hit_aux[slot] = b.aux;

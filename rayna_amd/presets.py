@@ -309,3 +309,32 @@ def inverse_spheres(seed: int = 4, width: int = 256, height: int = 256, samples:
                  defocus_angle=0.6)
     opts = RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
     return Scene(objects, SimpleSkybox), cam, opts, np.arange(521)
+
+
+# ------------------------------------------------------------------------------------------ batched requests
+
+
+def batch_requests(seed: int = 3, n: int = 64, mesh_subdiv: int = 8, max_w: int = 1920, max_h: int = 1080,
+                   min_side: int = 128, max_spp: int = 256):
+    """The batched-rendering workload: scenes [random_spheres, triangle_mesh, sdf_volume] and `n` requests.
+    Request i draws t ~ U[0, 1] and renders min_side * (max_w / min_side)^t by min_side * (max_h / min_side)^t
+    pixels (log-uniform along the diagonal from min_side^2 to max_w x max_h), spp ~ U{1..max_spp}, a scene ~
+    U{0, 1, 2} with that scene's camera and ray depth, and seed 100 + i.  Returns (scenes, requests)."""
+    from .batch import Request
+
+    c0 = random_spheres()
+    c1 = triangle_mesh(subdiv=mesh_subdiv)
+    c2 = sdf_volume()
+    presets_ = [c0, c1, c2]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    reqs = []
+    for i in range(n):
+        t = rng.random()
+        w = int(round(min_side * (max_w / min_side) ** t))
+        h = int(round(min_side * (max_h / min_side) ** t))
+        spp = int(rng.integers(1, max_spp + 1))
+        k = int(rng.integers(0, 3))
+        _, cam, opts = presets_[k]
+        reqs.append(Request(k, cam, RenderOpts(width=w, height=h, samples=spp, ray_depth=opts.ray_depth,
+                                               seed=100 + i)))
+    return [c0[0], c1[0], c2[0]], reqs

@@ -27,8 +27,10 @@ _lib: Optional[C.CDLL] = None
 
 # every symbol declared in include/rayna_gpu.h
 EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_options", "rayna_clear_accumulation",
-           "rayna_render", "rayna_set_material_colours", "rayna_adjoint", "rayna_primary_hits", "rayna_dfs_order",
-           "rayna_tree_hash", "rayna_scene_prepare", "rayna_destroy", "rayna_last_error")
+           "rayna_render", "rayna_set_material_colours", "rayna_adjoint", "rayna_set_tuning", "rayna_batch_create",
+           "rayna_batch_set_tuning", "rayna_render_batch", "rayna_batch_destroy", "rayna_batch_last_error",
+           "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare", "rayna_destroy",
+           "rayna_last_error")
 
 
 class RayError(RuntimeError):
@@ -64,6 +66,18 @@ def lib() -> C.CDLL:
         L.rayna_adjoint.restype = C.c_int
         L.rayna_adjoint.argtypes = [vp, C.c_uint32, C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_double),
                                     C.c_uint32, C.POINTER(A.RenderStats)]
+        L.rayna_set_tuning.restype = C.c_int
+        L.rayna_set_tuning.argtypes = [vp, C.POINTER(A.Tuning)]
+        L.rayna_batch_create.restype = C.c_int
+        L.rayna_batch_create.argtypes = [C.POINTER(A.SceneDesc), C.c_int32, C.POINTER(vp)]
+        L.rayna_batch_set_tuning.restype = C.c_int
+        L.rayna_batch_set_tuning.argtypes = [vp, C.POINTER(A.Tuning)]
+        L.rayna_render_batch.restype = C.c_int
+        L.rayna_render_batch.argtypes = [vp, C.POINTER(A.Request), C.c_int32, C.POINTER(A.BatchStats)]
+        L.rayna_batch_destroy.restype = None
+        L.rayna_batch_destroy.argtypes = [vp]
+        L.rayna_batch_last_error.restype = C.c_char_p
+        L.rayna_batch_last_error.argtypes = [vp]
         L.rayna_primary_hits.restype = C.c_int
         L.rayna_primary_hits.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.rayna_dfs_order.restype = C.c_int
@@ -180,6 +194,13 @@ class Renderer:
         d = options.desc()
         _check(lib().rayna_set_options(self._h, C.byref(d)), self._h)
         self._options = options
+
+    def set_tuning(self, chunk_slots: int = 0, refill_lanes: int = 0, wave_batch: int = 0,
+                   shade_blocks_per_cu: int = 0):
+        """Launch shape of the wavefront kernels (0 keeps a default); chunk_slots applies from the next
+        set_options."""
+        t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 0, 0)
+        _check(lib().rayna_set_tuning(self._h, C.byref(t)), self._h)
 
     def clear_accumulation(self):
         _check(lib().rayna_clear_accumulation(self._h), self._h)

@@ -46,7 +46,8 @@ def test_struct_layouts_match_a_compiled_probe(tmp_path):
     probe = tmp_path / "probe.c"
     fields = {"rayna_material": A.Material, "rayna_mesh": A.Mesh, "rayna_object": A.Object, "rayna_sdf": A.Sdf,
               "rayna_camera": A.CameraDesc, "rayna_render_opts": A.RenderOptsDesc,
-              "rayna_render_stats": A.RenderStats, "rayna_scene_desc": A.SceneDesc}
+              "rayna_render_stats": A.RenderStats, "rayna_scene_desc": A.SceneDesc, "rayna_tuning": A.Tuning,
+              "rayna_request": A.Request, "rayna_batch_stats": A.BatchStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in fields.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
