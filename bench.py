@@ -112,6 +112,66 @@ def adjoint_leg(steps: int):
             "adjoint_with_primal_forward_ratio": a2_ms / f_ms}
 
 
+def batch_leg(tune: bool = True):
+    """The batched-rendering configuration: 64 requests (log-uniform 128^2 .. 1920x1080, 1..256 spp, scenes
+    drawn from random_spheres / triangle_mesh / sdf_volume, seeds 100..163).  One render call per request
+    (a Renderer per scene, set_camera + set_options + render, in request order) against one batch call
+    (shortest-first schedule, launch shape from the autotuner run on the same requests at 1/16 of the
+    samples).  Throughput from wall time; latency of request i: its completion time from the start."""
+    import numpy as np
+    from rayna_amd import Renderer, presets
+    from rayna_amd import autotune as AT
+    from rayna_amd.batch import BatchRenderer
+
+    scenes, reqs = presets.batch_requests()
+    total = sum(q.opts.width * q.opts.height * q.opts.samples for q in reqs)
+    warm = AT.scaled_requests(reqs, 1 << 16)  # one sample each: allocates the largest buffers up front
+    firsts = {q.scene: q for q in reversed(reqs)}
+    rends = {k: Renderer(scenes[k], q.camera, q.opts) for k, q in firsts.items()}
+
+    def one_by_one(requests):
+        lat = []
+        t0 = time.perf_counter()
+        for q in requests:
+            r = rends[q.scene]
+            r.set_camera(q.camera)
+            r.set_options(q.opts)
+            r.render(layout=q.layout)
+            lat.append(time.perf_counter() - t0)
+        return time.perf_counter() - t0, np.array(lat) * 1e3
+
+    one_by_one(warm)
+    seq_s, seq_lat = one_by_one(reqs)
+    for r in rends.values():
+        r.close()
+
+    b = BatchRenderer(scenes)
+    b.set_tuning(shortest_first=True)
+    b.render(warm)
+    tuned = None
+    if tune:
+        tr = AT.tune(b, AT.scaled_requests(reqs, 16), repeats=1)
+        tuned = {"best": tr.best, "best_ms": tr.best_ms, "default_ms": tr.default_ms, "trials": len(tr.trials)}
+    cfg = dict(tuned["best"]) if tuned else {}
+    cfg["use_graph"] = False  # eager, so that every request's completion is timed
+    b.set_tuning(shortest_first=True, **cfg)
+    t0 = time.perf_counter()
+    _, st = b.render(reqs)
+    bat_s = time.perf_counter() - t0
+    b.close()
+
+    def pct(a, q):
+        return float(np.percentile(a, q))
+    return {"workload": "batch_requests", "requests": len(reqs), "samples": total,
+            "one_request_per_launch_msamples_per_s": total / seq_s / 1e6,
+            "batched_msamples_per_s": total / bat_s / 1e6, "batched_over_one_per_launch": seq_s / bat_s,
+            "one_request_per_launch_s": seq_s, "batched_s": bat_s, "batched_kernel_ms": st.kernel_ms,
+            "chunks": st.chunks, "launches": st.launches,
+            "latency_ms": {"one_request_per_launch": {"p50": pct(seq_lat, 50), "p99": pct(seq_lat, 99)},
+                           "batched": {"p50": pct(st.request_ms, 50), "p99": pct(st.request_ms, 99)}},
+            "tuning": tuned}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +182,8 @@ def main(argv=None):
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering leg")
+    ap.add_argument("--no-tune", action="store_true", help="batched leg with the default launch shape")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB three times (the counter calibration kernel)")
     a = ap.parse_args(argv)
@@ -200,6 +262,8 @@ def main(argv=None):
     r.close()
     if not a.no_adjoint:
         line["adjoint"] = adjoint_leg(max(3, a.steps))
+    if not a.no_batch:
+        line["batch"] = batch_leg(tune=not a.no_tune)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, cam, opts, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(line), flush=True)

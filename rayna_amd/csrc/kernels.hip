@@ -264,12 +264,20 @@ __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
     return 0.5f * dm::log_f(r) * r / dr;
 }
 
-// RaymarchedIsosurfaceMesh::intersect (raymarched.rs:228-277), with the render interval [T_MIN, FLT_MAX].
-__device__ bool sdf_march(const SceneView &sv, const GSdf &f, const DRay &r, float *t, bool *front) {
+// RaymarchedIsosurfaceMesh::intersect (raymarched.rs:228-277), with the render interval [T_MIN, FLT_MAX].  Two
+// exits give the same result as the full march: the march distance never decreases, so once it reaches t_cut
+// (the closest hit so far) no later hit can be kept; and once the point is outside the escape sphere and the
+// ray moves away from its centre, every later estimate exceeds epsilon, so the march can only end in a miss.
+__device__ bool sdf_march(const SceneView &sv, const GSdf &f, const DRay &r, float t_cut, float *t, bool *front) {
     float total = T_MIN;
     f3 point = at(r, total);
+    const f3 E{f.ex, f.ey, f.ez};
+    const float R2 = f.escape_r * f.escape_r;
     int i = 0;
     for (;;) {
+        if (total >= t_cut) return false;
+        const f3 q = point - E;
+        if (dot(q, q) > R2 && dot(q, r.d) > 0.0f) return false;
         const float dist = sdf_eval(sv, f, point);
         total = total + fabsf(dist);
         point = at(r, total);

@@ -110,6 +110,9 @@ struct alignas(16) GSdf {
     int32_t list_pos;        // position among the scene's unbounded objects (the engine's tie order)
     float epsilon, k, bailout, pad1;
     float cx, cy, cz, pad2;
+    // Escape sphere: a march point farther than escape_r from (ex, ey, ez) whose ray moves away from it can
+    // never again see |distance| < epsilon (infinite: no such bound).
+    float ex, ey, ez, escape_r;
 };
 
 // Material (material/*.rs with SolidTexture), parameters rounded to float.

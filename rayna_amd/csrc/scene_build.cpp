@@ -577,6 +577,60 @@ void hash_tree(const RefTree &t, uint64_t &h) {
     }
 }
 
+// The escape sphere of an SDF as the kernels evaluate it (float centres, radii and parameters): outside it the
+// distance estimate exceeds epsilon, with margin for float rounding.
+//   sphere:      |p - c| - r > eps when |p - c| > r + eps
+//   smooth min:  each fold subtracts at most k/4 from min(d, d_i), so the estimate is at least
+//                min_i (|p - c_i| - r_i) - (n - 1) k / 4, above eps when
+//                |p - E| > max_i (|c_i - E| + r_i) + (n - 1) k / 4 + eps
+//   Mandelbulb:  beyond the bail-out radius the iteration stops at once and the estimate is 0.5 ln(r) r, which
+//                exceeds eps once r > bailout and 0.5 ln(r) r > 2 eps.
+// Points along a ray that moves away from E only get farther from it, so a march past the sphere cannot hit.
+static void sdf_escape_sphere(GSdf &g, const std::vector<GSphere> &spheres) {
+    const double eps = g.epsilon;
+    double E[3] = {g.cx, g.cy, g.cz}, R = INFINITY;
+    if (!(eps > 0.0) || eps > 0.1) {
+        R = INFINITY;
+    } else if (g.kind == RAYNA_SDF_SPHERE) {
+        const GSphere &sp = spheres[g.first_sphere];
+        E[0] = sp.cx;
+        E[1] = sp.cy;
+        E[2] = sp.cz;
+        R = (double)sp.radius + eps;
+    } else if (g.kind == RAYNA_SDF_SMIN_SPHERES) {
+        double m[3] = {0, 0, 0};
+        for (int i = 0; i < g.n_spheres; ++i) {
+            const GSphere &sp = spheres[g.first_sphere + i];
+            m[0] += sp.cx;
+            m[1] += sp.cy;
+            m[2] += sp.cz;
+        }
+        for (int a = 0; a < 3; ++a) E[a] = (double)(float)(m[a] / g.n_spheres);
+        double far = 0.0;
+        for (int i = 0; i < g.n_spheres; ++i) {
+            const GSphere &sp = spheres[g.first_sphere + i];
+            const double dx = sp.cx - E[0], dy = sp.cy - E[1], dz = sp.cz - E[2];
+            far = std::max(far, std::sqrt(dx * dx + dy * dy + dz * dz) + (double)sp.radius);
+        }
+        R = (g.k >= 0.0f) ? far + (g.n_spheres - 1) * (double)g.k / 4.0 + eps : INFINITY;
+    } else {  // Mandelbulb: smallest r >= 1 with 0.5 ln(r) r > 2 eps, by bisection
+        double lo = 1.0, hi = 2.0 + 4.0 * eps;
+        for (int it = 0; it < 80; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (0.5 * std::log(mid) * mid > 2.0 * eps) hi = mid;
+            else lo = mid;
+        }
+        R = std::max((double)g.bailout, hi);
+    }
+    const double Rm = R * (1.0 + 1e-4) + 1e-4;
+    g.ex = (float)E[0];
+    g.ey = (float)E[1];
+    g.ez = (float)E[2];
+    float Rf = (float)Rm;
+    if ((double)Rf < Rm) Rf = std::nextafter(Rf, INFINITY);
+    g.escape_r = std::isfinite(Rm) ? Rf : INFINITY;
+}
+
 static bool finite_all(const double *p, size_t n) {
     for (size_t i = 0; i < n; ++i)
         if (!std::isfinite(p[i])) return false;
@@ -749,6 +803,7 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
         g.cx = (float)f.centre[0];
         g.cy = (float)f.centre[1];
         g.cz = (float)f.centre[2];
+        sdf_escape_sphere(g, P.spheres);
         P.sdfs.push_back(g);
     }
     // ---- mesh BVHs, built in parallel ----

@@ -195,7 +195,7 @@ __device__ __forceinline__ void unbounded_hits(const SceneView &sv, const DRay &
         const GSdf f = sv.sdfs[i];
         float t;
         bool front;
-        if (sdf_march(sv, f, ray, &t, &front) && t < b.t)
+        if (sdf_march(sv, f, ray, b.t, &t, &front) && t < b.t)
             b = Best{t, -2 - f.list_pos, make_ref(ITEM_SDF, i), front ? 0u : HIT_AUX_BACK};
     }
 }
