@@ -46,11 +46,10 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(scene, cam, opts, target_seconds: float, threads: int):
+def cpu_baseline(tw, cam, opts, target_seconds: float, threads: int):
     """The oracle's fp64 restatement of the engine renderer (engine BVH boxes, recursive integrator) on a
     strided pixel sample of the same frame, sized to about `target_seconds` of work."""
     import oracle
-    tw = oracle.TwinScene(scene)
     # calibration pass: a tiny sample
     row_step, col_step = max(1, opts.height // 4), max(1, opts.width // 8)
     _, _, secs, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_REFERENCE, row_step=row_step,
@@ -84,6 +83,27 @@ def cpu_baseline(scene, cam, opts, target_seconds: float, threads: int):
     }
 
 
+def rmse_leg(r, tw, cam, opts, threads: int, ref_spp: int = 16384, grid: int = 16):
+    """Converged-image error of the frame: RMSE over a grid x grid strided pixel sample against an fp64
+    oracle render of the same pixels at ref_spp samples with another seed, for the GPU frame and for the
+    fp64 oracle's own frame at the benchmark's samples (equal RMSE: the GPU estimator is the engine's)."""
+    import numpy as np
+    from dataclasses import replace
+    import oracle
+
+    img = r.render(layout="rowmajor").img  # a fresh first frame (the renderer was cleared by the caller)
+    rs, cs = max(1, opts.height // grid), max(1, opts.width // grid)
+    ref, _, secs, _ = tw.render(cam, replace(opts, samples=ref_spp, seed=opts.seed + 1), precision=64,
+                                box_mode=oracle.BOX_TIGHT, row_step=rs, col_step=cs, threads=threads)
+    f64, _, _, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_TIGHT, row_step=rs, col_step=cs,
+                             threads=threads)
+    gpu = img[::rs, ::cs][: ref.shape[0], : ref.shape[1]]
+    rm = lambda a: float(np.sqrt(((a.astype(np.float64) - ref.astype(np.float64)) ** 2).mean()))
+    return {"reference": f"fp64 oracle, {ref_spp} spp, seed {opts.seed + 1}, every {rs}th row x {cs}th column "
+                         f"({ref.shape[0] * ref.shape[1]} pixels, {secs:.1f} s)",
+            "gpu_rmse": rm(gpu), "fp64_oracle_rmse": rm(f64), "spp": opts.samples}
+
+
 def adjoint_leg(steps: int):
     """The inverse-rendering configuration (256x256, 16 spp, the 521 hidden colours of inverse_spheres): device
     time of a forward frame and of its path-replay adjoint, reusing the frame's radiance as the primal pass,
@@ -114,10 +134,11 @@ def adjoint_leg(steps: int):
 
 def batch_leg(tune: bool = True):
     """The batched-rendering configuration: 64 requests (log-uniform 128^2 .. 1920x1080, 1..256 spp, scenes
-    drawn from random_spheres / triangle_mesh / sdf_volume, seeds 100..163).  One render call per request
-    (a Renderer per scene, set_camera + set_options + render, in request order) against one batch call
-    (shortest-first schedule, launch shape from the autotuner run on the same requests at 1/16 of the
-    samples).  Throughput from wall time; latency of request i: its completion time from the start."""
+    drawn from random_spheres / triangle_mesh / sdf_volume, seeds 100..163).  The launch shape comes from the
+    autotuner, run on the same requests at 1/16 of the samples, and serves both sides: one render call per
+    request (a Renderer per scene, set_camera + set_options + render, in request order) and one batch call
+    (scheduled by scene, and shortest first).  Throughput from wall time; latency of request i: its
+    completion time from the start."""
     import numpy as np
     from rayna_amd import Renderer, presets
     from rayna_amd import autotune as AT
@@ -126,8 +147,22 @@ def batch_leg(tune: bool = True):
     scenes, reqs = presets.batch_requests()
     total = sum(q.opts.width * q.opts.height * q.opts.samples for q in reqs)
     warm = AT.scaled_requests(reqs, 1 << 16)  # one sample each: allocates the largest buffers up front
+
+    b = BatchRenderer(scenes)
+    b.set_tuning(schedule="by_scene")
+    b.render(warm)
+    cfg = dict(AT.DEFAULTS)
+    tuned = None
+    if tune:
+        tr = AT.tune(b, AT.scaled_requests(reqs, 16), repeats=1)
+        cfg = dict(tr.best)
+        tuned = {"best": tr.best, "best_ms": tr.best_ms, "default_ms": tr.default_ms, "trials": len(tr.trials)}
+    shape = {k: cfg[k] for k in ("chunk_slots", "refill_lanes", "wave_batch", "shade_blocks_per_cu")}
+
     firsts = {q.scene: q for q in reversed(reqs)}
     rends = {k: Renderer(scenes[k], q.camera, q.opts) for k, q in firsts.items()}
+    for r in rends.values():
+        r.set_tuning(**shape)
 
     def one_by_one(requests):
         lat = []
@@ -145,19 +180,14 @@ def batch_leg(tune: bool = True):
     for r in rends.values():
         r.close()
 
-    b = BatchRenderer(scenes)
-    b.set_tuning(shortest_first=True)
-    b.render(warm)
-    tuned = None
-    if tune:
-        tr = AT.tune(b, AT.scaled_requests(reqs, 16), repeats=1)
-        tuned = {"best": tr.best, "best_ms": tr.best_ms, "default_ms": tr.default_ms, "trials": len(tr.trials)}
-    cfg = dict(tuned["best"]) if tuned else {}
-    cfg["use_graph"] = False  # eager, so that every request's completion is timed
-    b.set_tuning(shortest_first=True, **cfg)
-    t0 = time.perf_counter()
-    _, st = b.render(reqs)
-    bat_s = time.perf_counter() - t0
+    def batched(schedule):
+        b.set_tuning(schedule=schedule, use_graph=False, **shape)  # eager: every completion is timed
+        t0 = time.perf_counter()
+        _, st = b.render(reqs)
+        return time.perf_counter() - t0, st
+
+    bat_s, st = batched("by_scene")
+    sjf_s, st2 = batched("shortest_first")
     b.close()
 
     def pct(a, q):
@@ -167,9 +197,12 @@ def batch_leg(tune: bool = True):
             "batched_msamples_per_s": total / bat_s / 1e6, "batched_over_one_per_launch": seq_s / bat_s,
             "one_request_per_launch_s": seq_s, "batched_s": bat_s, "batched_kernel_ms": st.kernel_ms,
             "chunks": st.chunks, "launches": st.launches,
+            "batched_shortest_first_msamples_per_s": total / sjf_s / 1e6,
             "latency_ms": {"one_request_per_launch": {"p50": pct(seq_lat, 50), "p99": pct(seq_lat, 99)},
-                           "batched": {"p50": pct(st.request_ms, 50), "p99": pct(st.request_ms, 99)}},
-            "tuning": tuned}
+                           "batched_by_scene": {"p50": pct(st.request_ms, 50), "p99": pct(st.request_ms, 99)},
+                           "batched_shortest_first": {"p50": pct(st2.request_ms, 50),
+                                                      "p99": pct(st2.request_ms, 99)}},
+            "launch_shape": shape, "tuning": tuned}
 
 
 def main(argv=None):
@@ -259,13 +292,19 @@ def main(argv=None):
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": "k_extend", "bytes_per_segment": EXTEND_BYTES_PER_SEGMENT},
     }
+    tw = None
+    if not a.no_cpu_baseline:
+        import oracle
+        tw = oracle.TwinScene(scene)
+        r.clear_accumulation()
+        line["rmse"] = rmse_leg(r, tw, cam, opts, a.cpu_threads)
     r.close()
     if not a.no_adjoint:
         line["adjoint"] = adjoint_leg(max(3, a.steps))
     if not a.no_batch:
         line["batch"] = batch_leg(tune=not a.no_tune)
     if not a.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(scene, cam, opts, a.cpu_seconds, a.cpu_threads)
+        line["cpu_baseline"] = cpu_baseline(tw, cam, opts, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -250,15 +250,17 @@ rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_ima
 
 /* Launch shape of the wavefront kernels.  Zero fields keep the default. */
 typedef struct rayna_tuning {
-    uint32_t chunk_slots;          /* paths in flight per chunk (default 2^23) */
+    uint32_t chunk_slots;          /* paths in flight per chunk (default 2^27) */
     uint32_t refill_lanes;         /* extend: idle lanes of a wave before it takes new rays, 1..64 (default 16) */
     uint32_t wave_batch;           /* extend: queue positions a wave claims per atomic, 1..4096 (default 64) */
     uint32_t shade_blocks_per_cu;  /* shade: blocks of 256 threads per compute unit, 1..64 (default 8) */
     int32_t use_graph;             /* batches: capture the launch sequence in a hipGraph and replay it (1) */
-    int32_t schedule;              /* batches: 0 = requests in the given order, 1 = fewest samples first */
+    int32_t schedule;              /* batches: 0 = requests in the given order, 1 = fewest samples first,
+                                    * 2 = grouped by scene (groups by total samples, ascending), fewest
+                                    * samples first within a group: a wave then rarely mixes scenes */
 } rayna_tuning;
 
-/* Applies a tuning to a renderer (chunk_slots takes effect with the next rayna_set_options). */
+/* Applies a tuning to a renderer.  A new chunk_slots re-plans the path buffers and clears accumulation. */
 rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t);
 
 /* ---------------------------------------------------------------- batches

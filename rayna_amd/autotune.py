@@ -15,13 +15,13 @@ from typing import Dict, List, Sequence
 from .batch import BatchRenderer, Request
 
 SPACE: Dict[str, Sequence] = {
-    "chunk_slots": (1 << 21, 1 << 22, 1 << 23, 1 << 24),
+    "chunk_slots": (1 << 23, 1 << 25, 1 << 26, 1 << 27),
     "refill_lanes": (8, 16, 32, 48),
     "wave_batch": (32, 64, 128),
     "shade_blocks_per_cu": (4, 8, 16),
     "use_graph": (False, True),
 }
-DEFAULTS = {"chunk_slots": 1 << 23, "refill_lanes": 16, "wave_batch": 64, "shade_blocks_per_cu": 8,
+DEFAULTS = {"chunk_slots": 1 << 27, "refill_lanes": 16, "wave_batch": 64, "shade_blocks_per_cu": 8,
             "use_graph": False}
 
 
@@ -38,7 +38,7 @@ def scaled_requests(requests: List[Request], spp_divisor: int) -> List[Request]:
     return [replace(q, opts=replace(q.opts, samples=max(1, q.opts.samples // spp_divisor))) for q in requests]
 
 
-def tune(batch: BatchRenderer, requests: List[Request], repeats: int = 2, shortest_first: bool = True,
+def tune(batch: BatchRenderer, requests: List[Request], repeats: int = 2, schedule: str = "by_scene",
          space: Dict[str, Sequence] = SPACE) -> TuneResult:
     cache: Dict[tuple, float] = {}
 
@@ -46,7 +46,7 @@ def tune(batch: BatchRenderer, requests: List[Request], repeats: int = 2, shorte
         key = tuple(sorted(cfg.items()))
         if key in cache:
             return cache[key]
-        batch.set_tuning(shortest_first=shortest_first, **cfg)
+        batch.set_tuning(schedule=schedule, **cfg)
         times = []
         for _ in range(repeats + (1 if cfg["use_graph"] else 0)):  # a graph's first run includes its capture
             _, st = batch.render(requests, to_host=False)
@@ -68,5 +68,5 @@ def tune(batch: BatchRenderer, requests: List[Request], repeats: int = 2, shorte
             if ms < best_ms:
                 best, best_ms = cfg, ms
     res.best, res.best_ms = best, best_ms
-    batch.set_tuning(shortest_first=shortest_first, **best)
+    batch.set_tuning(schedule=schedule, **best)
     return res

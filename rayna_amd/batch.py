@@ -19,6 +19,9 @@ from . import _abi as A
 from .renderer import RayError, lib
 
 
+SCHEDULES = {"given": 0, "shortest_first": 1, "by_scene": 2}
+
+
 @dataclass
 class Request:
     scene: int          # index into the BatchRenderer's scenes
@@ -54,10 +57,11 @@ class BatchRenderer:
             raise RayError(st, lib().rayna_batch_last_error(self._h).decode())
 
     def set_tuning(self, chunk_slots: int = 0, refill_lanes: int = 0, wave_batch: int = 0,
-                   shade_blocks_per_cu: int = 0, use_graph: bool = False, shortest_first: bool = False):
-        """Launch shape (0 keeps a default), graph capture, and the order requests are traced in."""
+                   shade_blocks_per_cu: int = 0, use_graph: bool = False, schedule: str = "given"):
+        """Launch shape (0 keeps a default), graph capture, and the order requests are traced in: "given",
+        "shortest_first", or "by_scene" (scene groups by total samples, shortest first within a group)."""
         t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 1 if use_graph else 0,
-                     1 if shortest_first else 0)
+                     SCHEDULES[schedule])
         self._check(lib().rayna_batch_set_tuning(self._h, C.byref(t)))
         self.use_graph = use_graph
 

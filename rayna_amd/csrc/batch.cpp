@@ -19,7 +19,9 @@ using namespace rayna;
 namespace {
 
 std::string g_batch_create_error;
-constexpr uint64_t DEFAULT_CHUNK_SLOTS = 1ull << 23;
+// Paths in flight per chunk: 2^27 (a 1080p frame of 64 samples in one chunk; ~10 GB of path state, sized for
+// the 288 GB of HBM of one MI355X).  Whole-frame chunks keep every depth's launch full.
+constexpr uint64_t DEFAULT_CHUNK_SLOTS = 1ull << 27;
 
 // One request's share of a chunk: samples [s0, s0 + n) of request `req`.
 struct Piece {
@@ -35,6 +37,7 @@ struct Plan {
     std::vector<uint32_t> first_final;         // chunk c's finals at [first_final[c], first_final[c + 1])
     std::vector<uint32_t> max_pixels;          // per chunk
     std::vector<uint32_t> depth;               // per chunk: deepest ray_depth of its requests
+    std::vector<uint32_t> n_sdf;               // per chunk: most unbounded objects of its requests' scenes
     std::vector<uint32_t> slots;               // per chunk
     std::vector<int32_t> last_chunk;           // per request (-1: not traced, e.g. invalid camera)
     std::vector<uint64_t> out_offset;          // per request, floats into the output buffer
@@ -52,11 +55,14 @@ struct rayna_batch {
     SceneView view{};
     std::vector<SceneSlot> slots;
     int2 *overflow = nullptr;
-    int cus = 1, grid_extend = 1024, grid_shade = 1024;
+    int cus = 1, grid_extend = 1024, grid_shade = 1024, grid_march = 1024;
+    uint32_t n_sdf_max = 0;  // most unbounded objects of one scene
+    float *mt = nullptr;     // march outcomes, capacity * n_sdf_max
+    uint64_t mt_cap = 0;
     uint64_t chunk_slots = DEFAULT_CHUNK_SLOTS;
     ExtendTuning tune;
     bool use_graph = false;
-    bool shortest_first = false;
+    int schedule = 0;
 
     // path state for chunk_slots slots
     uint64_t capacity = 0;
@@ -124,6 +130,7 @@ void free_paths(rayna_batch *b) {
     b->path_allocs.clear();
     b->pb = PathBuffers{};
     b->slot_req = nullptr;
+    b->mt = nullptr;
     b->queue[0] = b->queue[1] = nullptr;
     b->queue_count = nullptr;
     b->queue_count_len = 0;
@@ -133,7 +140,7 @@ void free_paths(rayna_batch *b) {
 }
 
 rayna_status alloc_paths(rayna_batch *b, uint64_t slots, uint32_t max_depth) {
-    if (b->capacity == slots && b->queue_count_len >= 2 * (max_depth + 2)) return RAYNA_OK;
+    if (b->capacity == slots && b->queue_count_len >= 3 * (max_depth + 2)) return RAYNA_OK;
     free_paths(b);
     auto alloc = [&](size_t bytes, void **out) -> rayna_status {
         BHIP(b, hipMalloc(out, std::max<size_t>(bytes, 4)));
@@ -150,9 +157,11 @@ rayna_status alloc_paths(rayna_batch *b, uint64_t slots, uint32_t max_depth) {
     if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->slot_req)) != RAYNA_OK) return st;
     for (int q = 0; q < 2; ++q)
         if ((st = alloc(slots * sizeof(int32_t), (void **)&b->queue[q])) != RAYNA_OK) return st;
-    const uint32_t qlen = 2 * (std::max<uint32_t>(max_depth, 16) + 2);
+    // [0, D+2) paths entering each depth; [D+2, 2(D+2)) extend counters; [2(D+2), 3(D+2)) march counters
+    const uint32_t qlen = 3 * (std::max<uint32_t>(max_depth, 16) + 2);
     if ((st = alloc(qlen * sizeof(uint32_t), (void **)&b->queue_count)) != RAYNA_OK) return st;
     if ((st = alloc(sizeof(unsigned long long), (void **)&b->segments)) != RAYNA_OK) return st;
+    if (b->n_sdf_max && (st = alloc(slots * b->n_sdf_max * sizeof(float), (void **)&b->mt)) != RAYNA_OK) return st;
     b->pb.hit_rank = nullptr;
     b->capacity = slots;
     b->queue_count_len = qlen;
@@ -170,6 +179,11 @@ rayna_status ensure(rayna_batch *b, T **p, uint64_t *cap, uint64_t n) {
     *cap = want;
     free_graph(b);  // a captured graph holds the old pointers
     return RAYNA_OK;
+}
+
+// March items (slot, unbounded object) are counted in 32 bits.
+void clamp_chunk(rayna_batch *b) {
+    if (b->n_sdf_max && b->chunk_slots * b->n_sdf_max > 0xffffffffull) b->chunk_slots = 0xffffffffull / b->n_sdf_max;
 }
 
 uint64_t mix(uint64_t h, uint64_t v) {
@@ -205,12 +219,19 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
     uint64_t h = 1469598103934665603ull;
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
-    if (b->shortest_first)
+    const auto work = [&](int k) { return (uint64_t)reqs[k].opts.width * reqs[k].opts.height * reqs[k].opts.samples; };
+    if (b->schedule == 1) {
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) < work(y); });
+    } else if (b->schedule == 2) {
+        std::vector<uint64_t> scene_work(b->slots.size(), 0);
+        for (int i = 0; i < n; ++i) scene_work[reqs[i].scene] += work(i);
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-            const auto work = [&](int k) { return (uint64_t)reqs[k].opts.width * reqs[k].opts.height * reqs[k].opts.samples; };
+            const int sx = reqs[x].scene, sy = reqs[y].scene;
+            if (sx != sy) return scene_work[sx] != scene_work[sy] ? scene_work[sx] < scene_work[sy] : sx < sy;
             return work(x) < work(y);
         });
-    h = mix(h, b->shortest_first ? 1 : 0);
+    }
+    h = mix(h, (uint64_t)b->schedule);
     for (int oi = 0; oi < n; ++oi) {
         const int i = order[oi];
         const rayna_request &q = reqs[i];
@@ -244,7 +265,7 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
     for (size_t c = 0; c < pl.chunks.size(); ++c) {
         pl.first_entry.push_back((uint32_t)pl.table.size());
         pl.first_final.push_back((uint32_t)pl.finals.size());
-        uint32_t slot = 0, maxp = 0, depth = 0;
+        uint32_t slot = 0, maxp = 0, depth = 0, nsdf = 0;
         for (size_t e = 0; e < pl.chunks[c].size(); ++e) {
             const Piece &pc = pl.chunks[c][e];
             const rayna_request &q = reqs[pc.req];
@@ -275,9 +296,11 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
             slot += R.n_slots;
             maxp = std::max(maxp, P);
             depth = std::max(depth, q.opts.ray_depth);
+            nsdf = std::max(nsdf, (uint32_t)sc.n_sdfs);
         }
         pl.max_pixels.push_back(maxp);
         pl.depth.push_back(depth);
+        pl.n_sdf.push_back(nsdf);
         pl.slots.push_back(slot);
         pl.max_depth = std::max(pl.max_depth, depth);
     }
@@ -297,7 +320,7 @@ uint32_t enqueue(rayna_batch *b, const Plan &pl, std::vector<hipEvent_t> *ev) {
         fp.n_slots = pl.slots[c];
         fp.ray_depth = pl.depth[c];
         const uint32_t D = pl.depth[c];
-        launch_fill_u32(b->queue_count, 0u, 2 * (D + 2), s);
+        launch_fill_u32(b->queue_count, 0u, 3 * (D + 2), s);
         launch_generate(fp, b->pb, false, bv, s);
         launches += 2;
         for (uint32_t depth = 0; depth <= D; ++depth) {
@@ -307,8 +330,10 @@ uint32_t enqueue(rayna_batch *b, const Plan &pl, std::vector<hipEvent_t> *ev) {
             uint32_t *cout = b->queue_count + depth + 1;
             uint32_t *work = b->queue_count + (D + 2) + depth;
             launch_extend(b->view, fp, b->pb, qin, cin, depth, b->grid_extend, b->overflow, work, b->tune, bv, s);
+            launch_march(b->view, fp, b->pb, qin, cin, pl.n_sdf[c], b->mt, b->queue_count + 2 * (D + 2) + depth,
+                         b->grid_march, b->grid_shade, b->tune, bv, s);
             launch_shade(b->view, fp, b->pb, qin, cin, qout, cout, depth, b->grid_shade, nullptr, bv, s);
-            launches += 2;
+            launches += pl.n_sdf[c] ? 4 : 2;
         }
         launch_count_segments(b->queue_count, D + 1, fp.n_slots, b->segments, s);
         launch_resolve_batch(bv, pl.max_pixels[c], b->pb, b->sum, s);
@@ -366,6 +391,7 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     }
     b->cus = std::max(1, prop.multiProcessorCount);
     b->grid_extend = b->cus * extend_blocks_per_cu();
+    b->grid_march = b->cus * march_blocks_per_cu();
     b->grid_shade = b->cus * 8;
     std::vector<PackedScene> packed(n_scenes);
     for (int i = 0; i < n_scenes; ++i) {
@@ -400,6 +426,8 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     UP(sdfs, sdfs)
     UP(materials, materials)
 #undef UP
+    for (const SceneSlot &sl : b->slots) b->n_sdf_max = std::max<uint32_t>(b->n_sdf_max, (uint32_t)sl.n_sdfs);
+    clamp_chunk(b);
     v.root = -1;  // per request
     v.n_sdfs = (int32_t)merged.sdfs.size();
     v.skybox = RAYNA_SKY_NONE;
@@ -421,14 +449,15 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
 rayna_status rayna_batch_set_tuning(rayna_batch *b, const rayna_tuning *t) {
     if (!b || !t) return bfail(b, RAYNA_INVALID_ARGUMENT, "null argument");
     if (t->refill_lanes > 64 || t->wave_batch > 4096 || t->shade_blocks_per_cu > 64 ||
-        (t->chunk_slots && t->chunk_slots < (1u << 16)) || t->schedule < 0 || t->schedule > 1)
+        (t->chunk_slots && t->chunk_slots < (1u << 16)) || t->schedule < 0 || t->schedule > 2)
         return bfail(b, RAYNA_INVALID_ARGUMENT, "tuning out of range");
     b->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
     b->tune.wave_batch = t->wave_batch ? t->wave_batch : ExtendTuning{}.wave_batch;
     b->grid_shade = b->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : 8);
     b->chunk_slots = t->chunk_slots ? t->chunk_slots : DEFAULT_CHUNK_SLOTS;
+    clamp_chunk(b);
     b->use_graph = t->use_graph != 0;
-    b->shortest_first = t->schedule == 1;
+    b->schedule = t->schedule;
     free_graph(b);
     return RAYNA_OK;
 }
@@ -485,7 +514,7 @@ rayna_status rayna_render_batch(rayna_batch *b, const rayna_request *reqs, int32
             // the launch count of the same plan
             for (size_t c = 0; c < pl.chunks.size(); ++c) {
                 const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
-                launches += 2 + 2 * (pl.depth[c] + 1) + 2 + (nf ? 1 : 0);
+                launches += 2 + (pl.n_sdf[c] ? 4 : 2) * (pl.depth[c] + 1) + 2 + (nf ? 1 : 0);
             }
             launches += 1;
         }

@@ -264,33 +264,6 @@ __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
     return 0.5f * dm::log_f(r) * r / dr;
 }
 
-// RaymarchedIsosurfaceMesh::intersect (raymarched.rs:228-277), with the render interval [T_MIN, FLT_MAX].  Two
-// exits give the same result as the full march: the march distance never decreases, so once it reaches t_cut
-// (the closest hit so far) no later hit can be kept; and once the point is outside the escape sphere and the
-// ray moves away from its centre, every later estimate exceeds epsilon, so the march can only end in a miss.
-__device__ bool sdf_march(const SceneView &sv, const GSdf &f, const DRay &r, float t_cut, float *t, bool *front) {
-    float total = T_MIN;
-    f3 point = at(r, total);
-    const f3 E{f.ex, f.ey, f.ez};
-    const float R2 = f.escape_r * f.escape_r;
-    int i = 0;
-    for (;;) {
-        if (total >= t_cut) return false;
-        const f3 q = point - E;
-        if (dot(q, q) > R2 && dot(q, r.d) > 0.0f) return false;
-        const float dist = sdf_eval(sv, f, point);
-        total = total + fabsf(dist);
-        point = at(r, total);
-        if (fabsf(dist) < f.epsilon && T_MIN <= total && total <= FLT_MAX) {
-            *t = total;
-            *front = !signbit(dist);
-            return true;
-        }
-        if (i > f.max_iterations) return false;
-        ++i;
-    }
-}
-
 __device__ f3 sdf_normal(const SceneView &sv, const GSdf &f, f3 p) {
     const float e = f.epsilon;
     const f3 g{sdf_eval(sv, f, f3{p.x + e, p.y, p.z}) - sdf_eval(sv, f, f3{p.x - e, p.y, p.z}),
@@ -313,6 +286,7 @@ __device__ __forceinline__ uint32_t make_ref(int kind, int32_t index) {
 }
 
 #include "traverse.hpp"
+#include "march.hpp"
 
 // ------------------------------------------------------------------------------------------ shading
 
@@ -786,6 +760,15 @@ int extend_blocks_per_cu() {
 }
 int extend_block() { return EXT_BLOCK; }
 
+int march_blocks_per_cu() {
+    int n = 0, nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_march<false>, MARCH_BLOCK, 0) != hipSuccess || n < 1) n = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_march<true>, MARCH_BLOCK, 0) == hipSuccess && nb >= 1)
+        n = std::min(n, nb);
+    return n;
+}
+int march_block() { return MARCH_BLOCK; }
+
 void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, const BatchView &bv,
                      hipStream_t s) {
     if (bv.reqs)
@@ -819,6 +802,23 @@ void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers 
     else
         hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in,
                            queue_out, count_out, depth, AdjointParams{}, bv);
+}
+
+void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
+                  const uint32_t *count, uint32_t n_sdf, float *mt, uint32_t *work, int grid, int merge_grid,
+                  const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
+    if (n_sdf == 0) return;
+    if (bv.reqs) {
+        hipLaunchKernelGGL(k_march<true>, dim3(grid), dim3(MARCH_BLOCK), 0, s, sv, fp, pb, queue, count, n_sdf, mt,
+                           work, tune, bv);
+        hipLaunchKernelGGL(k_march_merge<true>, dim3(merge_grid), dim3(BLOCK), 0, s, sv, fp, pb, queue, count,
+                           n_sdf, mt, bv);
+    } else {
+        hipLaunchKernelGGL(k_march<false>, dim3(grid), dim3(MARCH_BLOCK), 0, s, sv, fp, pb, queue, count, n_sdf,
+                           mt, work, tune, bv);
+        hipLaunchKernelGGL(k_march_merge<false>, dim3(merge_grid), dim3(BLOCK), 0, s, sv, fp, pb, queue, count,
+                           n_sdf, mt, bv);
+    }
 }
 
 void launch_resolve_batch(const BatchView &bv, uint32_t max_pixels, const PathBuffers &pb, float *sum,

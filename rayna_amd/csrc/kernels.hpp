@@ -82,6 +82,12 @@ void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
                   const AdjointParams *adjoint, const BatchView &bv, hipStream_t s);
+// The unbounded SDF objects of each queued path (n_sdf per path; in a batch, each request's own), marched
+// after launch_extend with its hit as the cut, then merged into the hit record.  mt: count * n_sdf floats.
+// work: a zeroed counter.  No-op for n_sdf == 0.
+void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
+                  const uint32_t *count, uint32_t n_sdf, float *mt, uint32_t *work, int grid, int merge_grid,
+                  const ExtendTuning &tune, const BatchView &bv, hipStream_t s);
 // Batch chunks: per request, the running per-pixel sum of the chunk's samples (in sample order) and, at the
 // request's last chunk, its sample mean in its layout at out + sum_offset.
 void launch_resolve_batch(const BatchView &bv, uint32_t max_pixels, const PathBuffers &pb, float *sum,
@@ -100,5 +106,7 @@ void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t fir
 int extend_lds_stack();
 int extend_blocks_per_cu();  // resident extend blocks per CU (the persistent grid is this times the CU count)
 int extend_block();
+int march_blocks_per_cu();
+int march_block();
 
 }  // namespace rayna

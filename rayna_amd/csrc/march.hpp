@@ -1,0 +1,203 @@
+// The unbounded SDF objects of a closest-hit query, marched apart from the tree (included by kernels.hip inside
+// its anonymous namespace, after sdf_eval).
+//
+// Work item = (path, unbounded object), SDF-major (item = j * count + q), so the items a wave takes mostly
+// belong to one SDF.  A lane runs its march as a state machine: each pass of the persistent loop either starts
+// a march step (cut and escape tests; a sphere or smooth-min estimate is evaluated at once) or performs one
+// iteration of the Mandelbulb estimate, so every lane of a wave executes the same iteration body whatever its
+// march length or iteration count.  A lane whose march ends takes the next item of its wave's claimed batch in
+// the next pass.  The operations of each march are exactly those of RaymarchedIsosurfaceMesh::intersect
+// (raymarched.rs:228-277) and sdf_eval, in the same order, so the outcome is the same bit for bit: the march
+// starts at total = T_MIN; each step evaluates the estimate at the ray point of `total`, advances total by its
+// magnitude and hits when that magnitude is below epsilon, giving up after max_iterations + 2 estimates.  Two
+// exits end a march early with the same outcome as the full march: the march distance never decreases, so once
+// it reaches the cut no later hit can be kept; and once the point is outside the object's escape sphere and the
+// ray moves away from its centre, every later estimate exceeds epsilon.
+//
+// Each item's outcome is written to mt[item]: +t (front side), -t (back side) or +infinity (no hit).  The
+// cut is the path's closest bounded hit; k_march_merge then applies the list order (strict <, object/list.rs:
+// 358-363).  A cut that ignores earlier objects of the list only lets a march run longer: it never changes which
+// hit wins.
+
+constexpr int MARCH_BLOCK = 256;
+
+template <bool BATCH>
+__global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams fp, PathBuffers pb,
+                                                       const int32_t *queue, const uint32_t *count_ptr,
+                                                       uint32_t n_sdf, float *mt, uint32_t *work,
+                                                       ExtendTuning tune, BatchView bv) {
+    const uint32_t count = queue ? *count_ptr : fp.n_slots;
+    const uint32_t n_items = count * n_sdf;
+    const uint32_t refill_lanes = tune.refill_lanes, wave_batch = tune.wave_batch;
+    const uint32_t lane = __lane_id();
+    bool active = false, exhausted = false, last_claim = false;
+    uint32_t batch_next = 0, batch_end = 0;
+    // per-lane march state
+    uint32_t item = 0;
+    int32_t j = 0;  // SDF index
+    DRay ray{};
+    float t_cut = 0.0f, total = 0.0f;
+    int step = 0;
+    bool in_eval = false;  // inside a Mandelbulb estimate
+    int it = 0;            // its iteration
+    f3 z{0.0f, 0.0f, 0.0f}, p0{0.0f, 0.0f, 0.0f};
+    float dr = 1.0f, r = 0.0f;
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        if (!exhausted && idle != 0ull) {
+            const uint32_t n = (uint32_t)__popcll(idle);
+            const uint32_t rem = batch_end - batch_next;
+            uint32_t nb = 0, nb_end = 0;
+            if (rem < n && !last_claim && (idle == ~0ull || n >= refill_lanes)) {
+                const int leader = __ffsll((long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(work, wave_batch);
+                base = __shfl(base, leader);
+                nb = base < n_items ? base : n_items;
+                nb_end = (n_items - nb > wave_batch) ? nb + wave_batch : n_items;
+                if (nb_end >= n_items) last_claim = true;
+            }
+            const uint32_t idx = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+            uint32_t q = 0xffffffffu;
+            if (idx < rem) q = batch_next + idx;
+            else if (idx - rem < nb_end - nb) q = nb + (idx - rem);
+            if (rem >= n) {
+                batch_next += n;
+            } else if (nb_end > nb) {
+                const uint32_t from_new = (n - rem < nb_end - nb) ? n - rem : nb_end - nb;
+                batch_next = nb + from_new;
+                batch_end = nb_end;
+            } else {
+                batch_next = batch_end;
+            }
+            if (last_claim && batch_next == batch_end) exhausted = true;
+            if (!active && q != 0xffffffffu) {
+                item = q;
+                const uint32_t jj = q / count, qq = q % count;
+                const uint32_t slot = queue ? (uint32_t)queue[qq] : qq;
+                int32_t first = 0, n_own = (int32_t)n_sdf;
+                if (BATCH) {
+                    const RequestParams &R = bv.reqs[bv.slot_req[slot]];
+                    first = R.first_sdf;
+                    n_own = R.n_sdfs;
+                }
+                if ((int32_t)jj >= n_own) {
+                    mt[q] = __builtin_inff();  // the request's scene has fewer unbounded objects
+                } else {
+                    j = first + (int32_t)jj;
+                    ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
+                                       f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+                    t_cut = pb.hit_t[slot];
+                    total = T_MIN;
+                    step = 0;
+                    in_eval = false;
+                    active = true;
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (!active) continue;
+        const GSdf &f = sv.sdfs[j];
+        bool have_dist = false;
+        float dist = 0.0f;
+        if (!in_eval) {  // a new step of the march
+            const f3 point = at(ray, total);
+            const f3 qe = point - f3{f.ex, f.ey, f.ez};
+            if (total >= t_cut || (dot(qe, qe) > f.escape_r * f.escape_r && dot(qe, ray.d) > 0.0f)) {
+                mt[item] = __builtin_inff();
+                active = false;
+                continue;
+            }
+            if (f.kind == RAYNA_SDF_MANDELBULB) {
+                p0 = point - f3{f.cx, f.cy, f.cz};
+                z = p0;
+                dr = 1.0f;
+                r = 0.0f;
+                it = 0;
+                in_eval = true;
+            } else {
+                dist = sdf_eval(sv, f, point);
+                have_dist = true;
+            }
+        }
+        if (in_eval) {  // one iteration of the Mandelbulb estimate (sdf_eval's loop)
+            bool done = it >= f.bulb_iterations;
+            if (!done) {
+                r = length(z);
+                if (r > f.bailout) {
+                    done = true;
+                } else {
+                    const float theta = dm::acos_f(z.z / r);
+                    const float phi = dm::atan2_f(z.y, z.x);
+                    dr = ipow(r, f.power - 1) * (float)f.power * dr + 1.0f;
+                    const float zr = ipow(r, f.power);
+                    float st, ct, sp, cp;
+                    dm::sincos_rad(theta * (float)f.power, &st, &ct);
+                    dm::sincos_rad(phi * (float)f.power, &sp, &cp);
+                    z = f3{st * cp, sp * st, ct} * zr + p0;
+                    done = ++it >= f.bulb_iterations;
+                }
+            }
+            if (done) {
+                dist = 0.5f * dm::log_f(r) * r / dr;
+                in_eval = false;
+                have_dist = true;
+            }
+        }
+        if (have_dist) {  // the end of a step
+            total = total + fabsf(dist);
+            if (fabsf(dist) < f.epsilon && T_MIN <= total && total <= FLT_MAX) {
+                mt[item] = signbit(dist) ? -total : total;
+                active = false;
+            } else if (step > f.max_iterations) {
+                mt[item] = __builtin_inff();
+                active = false;
+            } else {
+                ++step;
+            }
+        }
+    }
+}
+
+// The closest hit of each path over its bounded hit and its unbounded objects' marches, in list order with
+// strict < (object/list.rs:358-363).
+template <bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_march_merge(SceneView sv, FrameParams fp, PathBuffers pb,
+                                                       const int32_t *queue, const uint32_t *count_ptr,
+                                                       uint32_t n_sdf, const float *mt, BatchView bv) {
+    const uint32_t count = queue ? *count_ptr : fp.n_slots;
+    for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < count; q += gridDim.x * BLOCK) {
+        const uint32_t slot = queue ? (uint32_t)queue[q] : q;
+        int32_t first = 0, n_own = (int32_t)n_sdf;
+        if (BATCH) {
+            const RequestParams &R = bv.reqs[bv.slot_req[slot]];
+            first = R.first_sdf;
+            n_own = R.n_sdfs;
+        }
+        float bt = pb.hit_t[slot];
+        int32_t rank = pb.hit_rank ? pb.hit_rank[slot] : 0;
+        uint32_t ref = pb.hit_ref[slot], aux = pb.hit_aux[slot];
+        bool changed = false;
+        for (int32_t jj = 0; jj < n_own; ++jj) {
+            const float v = mt[(size_t)jj * count + q];
+            const float t = fabsf(v);
+            if (t < bt) {
+                const int32_t j = first + jj;
+                bt = t;
+                rank = -2 - sv.sdfs[j].list_pos;
+                ref = make_ref(ITEM_SDF, j);
+                aux = signbit(v) ? HIT_AUX_BACK : 0u;
+                changed = true;
+            }
+        }
+        if (changed) {
+            pb.hit_t[slot] = bt;
+            pb.hit_ref[slot] = ref;
+            pb.hit_aux[slot] = aux;
+            if (pb.hit_rank) pb.hit_rank[slot] = rank;
+        }
+    }
+}

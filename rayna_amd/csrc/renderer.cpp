@@ -21,7 +21,10 @@ namespace {
 
 std::string g_create_error;
 
-constexpr uint64_t TARGET_SLOTS = 1ull << 23;  // paths in flight per chunk (~8.4 M; ~0.6 GB of path state)
+// Paths in flight per chunk: 2^27 (a 1080p frame of 64 samples in one chunk; ~10 GB of path state, sized for
+// the 288 GB of HBM of one MI355X).  Whole-frame chunks keep every depth's launch full: at 2^23 paths the
+// per-depth tails of 16 chunks cost the 1080p frame 40% of its time.
+constexpr uint64_t TARGET_SLOTS = 1ull << 27;
 
 struct DevScene {
     std::vector<void *> allocs;
@@ -63,8 +66,10 @@ struct rayna_renderer {
     double *grad = nullptr;       // grad_copies x n_materials x 3
     double *grad_sum = nullptr;   // n_materials x 3
     uint32_t grad_materials = 0, grad_copies = 0;
-    int grid_extend = 1024, grid_shade = 1024;
+    int grid_extend = 1024, grid_shade = 1024, grid_march = 1024;
     int cus = 1;
+    float *mt = nullptr;  // unbounded-object march outcomes, slot_capacity * n_sdfs
+    uint64_t mt_cap = 0;
     uint64_t chunk_target = TARGET_SLOTS;
     ExtendTuning tune;
     std::vector<hipEvent_t> events;
@@ -167,14 +172,32 @@ void free_paths(rayna_renderer *r) {
     r->queue_count_len = 0;
 }
 
+// The march outcome buffer: one float per (slot, unbounded object).
+rayna_status ensure_mt(rayna_renderer *r) {
+    const uint64_t need = r->slot_capacity * (uint64_t)std::max(0, r->dev.view.n_sdfs);
+    if (need <= r->mt_cap) return RAYNA_OK;
+    if (r->mt) (void)hipFree(r->mt);
+    r->mt = nullptr;
+    r->mt_cap = 0;
+    HIP_TRY(r, hipMalloc((void **)&r->mt, need * sizeof(float)));
+    r->mt_cap = need;
+    return RAYNA_OK;
+}
+
 rayna_status alloc_paths(rayna_renderer *r) {
     const rayna_render_opts &o = r->opts;
     const uint64_t P = (uint64_t)o.width * o.height;
-    uint64_t chunk = std::max<uint64_t>(1, r->chunk_target / std::max<uint64_t>(1, P));
+    // march items (slot, unbounded object) are counted in 32 bits
+    const uint64_t n_sdf = (uint64_t)std::max(1, r->dev.view.n_sdfs);
+    const uint64_t target = std::min<uint64_t>(r->chunk_target, 0xffffffffull / n_sdf);
+    uint64_t chunk = std::max<uint64_t>(1, target / std::max<uint64_t>(1, P));
     chunk = std::min<uint64_t>(chunk, o.samples);
     const uint64_t slots = P * chunk;
-    if (slots > 0x7fffffffull) return fail(r, RAYNA_UNSUPPORTED, "image too large for one chunk of paths");
-    const uint32_t qlen = 2 * (o.ray_depth + 2) + 1;
+    if (slots > 0x7fffffffull || slots * n_sdf > 0xffffffffull)
+        return fail(r, RAYNA_UNSUPPORTED, "image too large for one chunk of paths");
+    // [0, D+2) paths entering each depth; [D+2, 2(D+2)) extend ray counters; [2(D+2), 3(D+2)) march item
+    // counters; [3(D+2)], [3(D+2)+1]: the primary-hit query's two counters
+    const uint32_t qlen = 3 * (o.ray_depth + 2) + 2;
     if (slots <= r->slot_capacity && P <= r->alloc_pixels && qlen <= r->queue_count_len) {
         // the buffers are large enough: keep them (grow-only)
         r->chunk_samples = (uint32_t)chunk;
@@ -197,7 +220,6 @@ rayna_status alloc_paths(rayna_renderer *r) {
     if ((st = alloc(slots * sizeof(uint32_t), (void **)&r->pb.hit_aux)) != RAYNA_OK) return st;
     for (int q = 0; q < 2; ++q)
         if ((st = alloc(slots * sizeof(int32_t), (void **)&r->queue[q])) != RAYNA_OK) return st;
-    // [0, D+2): paths entering each depth; [D+2, 2D+4): the extend launches' ray counters; [2D+4]: spare
     if ((st = alloc(qlen * sizeof(uint32_t), (void **)&r->queue_count)) != RAYNA_OK) return st;
     if ((st = alloc(sizeof(unsigned long long), (void **)&r->segments)) != RAYNA_OK) return st;
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->sum)) != RAYNA_OK) return st;
@@ -228,6 +250,7 @@ rayna_status set_grid(rayna_renderer *r) {
     const int cus = std::max(1, prop.multiProcessorCount);
     r->cus = cus;
     r->grid_extend = cus * extend_blocks_per_cu();
+    r->grid_march = cus * march_blocks_per_cu();
     r->grid_shade = cus * 8;
     return RAYNA_OK;
 }
@@ -254,7 +277,8 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
                  uint64_t *extend_launches, const AdjointParams *adjoint = nullptr) {
     fp.s0 = s0;
     fp.n_slots = fp.n_pixels * n;
-    launch_fill_u32(r->queue_count, 0u, 2 * (r->opts.ray_depth + 2), r->stream);
+    const uint32_t D2 = r->opts.ray_depth + 2;
+    launch_fill_u32(r->queue_count, 0u, 3 * D2, r->stream);
     launch_generate(fp, r->pb, adjoint != nullptr, BatchView{}, r->stream);
     for (uint32_t depth = 0; depth <= r->opts.ray_depth; ++depth) {
         const int32_t *qin = depth == 0 ? nullptr : r->queue[depth & 1];
@@ -262,9 +286,11 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
         int32_t *qout = r->queue[(depth + 1) & 1];
         uint32_t *cout = r->queue_count + depth + 1;
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
-        uint32_t *work = r->queue_count + (r->opts.ray_depth + 2) + depth;
+        uint32_t *work = r->queue_count + D2 + depth;
         launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, work, r->tune,
                       BatchView{}, r->stream);
+        launch_march(r->dev.view, fp, r->pb, qin, cin, (uint32_t)r->dev.view.n_sdfs, r->mt, r->queue_count + 2 * D2 + depth,
+                     r->grid_march, r->grid_shade, r->tune, BatchView{}, r->stream);
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
         launch_shade(r->dev.view, fp, r->pb, qin, cin, qout, cout, depth, r->grid_shade, adjoint, BatchView{},
                      r->stream);
@@ -357,8 +383,9 @@ rayna_status rayna_create(const rayna_scene_desc *scene, const rayna_camera *cam
 rayna_status rayna_set_scene(rayna_renderer *r, const rayna_scene_desc *scene) {
     if (!r || !scene) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
     (void)hipSetDevice(r->device);
-    const rayna_status st = load_scene(r, scene);
+    rayna_status st = load_scene(r, scene);
     if (st != RAYNA_OK) return st;
+    if ((st = alloc_paths(r)) != RAYNA_OK) return st;  // the chunk depends on the unbounded-object count
     r->primal_frame = -1;
     return rayna_clear_accumulation(r);
 }
@@ -398,6 +425,10 @@ rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uin
         return fail(r, RAYNA_INVALID_ARGUMENT, "unknown layout");
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(r, hipSetDevice(r->device));
+    {
+        const rayna_status ms = ensure_mt(r);
+        if (ms != RAYNA_OK) return ms;
+    }
     rayna_render_stats st{};
     const uint32_t W = r->opts.width, H = r->opts.height;
     if (!r->cam_valid) {
@@ -471,7 +502,12 @@ rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t) {
     r->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
     r->tune.wave_batch = t->wave_batch ? t->wave_batch : ExtendTuning{}.wave_batch;
     r->grid_shade = r->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : 8);
-    r->chunk_target = t->chunk_slots ? t->chunk_slots : TARGET_SLOTS;
+    const uint64_t target = t->chunk_slots ? t->chunk_slots : TARGET_SLOTS;
+    if (target != r->chunk_target) {
+        r->chunk_target = target;
+        (void)hipSetDevice(r->device);
+        return alloc_paths(r);  // clears accumulation
+    }
     return RAYNA_OK;
 }
 
@@ -508,6 +544,10 @@ rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_ima
     if (n_materials == 0) return RAYNA_OK;
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(r, hipSetDevice(r->device));
+    {
+        const rayna_status ms = ensure_mt(r);
+        if (ms != RAYNA_OK) return ms;
+    }
     const uint32_t W = r->opts.width, H = r->opts.height;
     const uint64_t P = (uint64_t)W * H;
     // gradient copies: up to 64, within 64 MiB
@@ -584,6 +624,10 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     if (!r || !prim_rank || !t) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
     if (!r->cam_valid) return fail(r, RAYNA_INVALID_ARGUMENT, "camera is invalid");
     HIP_TRY(r, hipSetDevice(r->device));
+    {
+        const rayna_status ms = ensure_mt(r);
+        if (ms != RAYNA_OK) return ms;
+    }
     const uint32_t P = r->opts.width * r->opts.height;
     int32_t *rank_dev = nullptr;
     HIP_TRY(r, hipMalloc((void **)&rank_dev, (size_t)P * sizeof(int32_t)));
@@ -594,10 +638,12 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     pb.hit_rank = rank_dev;
     r->primal_frame = -1;
     launch_generate(fp, pb, false, BatchView{}, r->stream);
-    uint32_t *work = r->queue_count + 2 * (r->opts.ray_depth + 2);
-    launch_fill_u32(work, 0u, 1, r->stream);
+    uint32_t *work = r->queue_count + 3 * (r->opts.ray_depth + 2);
+    launch_fill_u32(work, 0u, 2, r->stream);
     launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, work, r->tune,
                   BatchView{}, r->stream);
+    launch_march(r->dev.view, fp, pb, nullptr, nullptr, (uint32_t)r->dev.view.n_sdfs, r->mt, work + 1, r->grid_march,
+                 r->grid_shade, r->tune, BatchView{}, r->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(prim_rank, rank_dev, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(t, pb.hit_t, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
@@ -652,6 +698,7 @@ void rayna_destroy(rayna_renderer *r) {
     free_scene(r->dev);
     if (r->grad) (void)hipFree(r->grad);
     if (r->grad_sum) (void)hipFree(r->grad_sum);
+    if (r->mt) (void)hipFree(r->mt);
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;

@@ -2,9 +2,9 @@
 // primitive tests).
 //
 // Result: the closest hit over the BVH with ties to the higher depth-first rank -- what the engine's
-// shrinking inclusive interval computes (mesh/advanced/bvh.rs:77-91, object/bvh.rs:443-481) -- then the
-// unbounded objects, where the first of equal minima wins (object/list.rs:358-363).  Culling is conservative,
-// so the result does not depend on the order nodes are visited in.
+// shrinking inclusive interval computes (mesh/advanced/bvh.rs:77-91, object/bvh.rs:443-481).  The unbounded
+// objects follow in march.hpp.  Culling is conservative, so the result does not depend on the order nodes are
+// visited in.
 //
 // Organisation (Aila & Laine, "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009):
 //  * while-while: leaf ranges are pushed on the stack like inner nodes; a lane first pops inner nodes until
@@ -188,20 +188,9 @@ __device__ __forceinline__ void leaf_test(const SceneView &sv, const DRay &ray, 
     if (ok && (t < b.t || it.rank > b.rank)) b = Best{t, it.rank, ref, 0u};
 }
 
-// The unbounded objects [first, first + n), marched with the caller's full interval; strict < keeps the earlier
-// of equal minima.
-__device__ __forceinline__ void unbounded_hits(const SceneView &sv, const DRay &ray, int first, int n, Best &b) {
-    for (int i = first; i < first + n; ++i) {
-        const GSdf f = sv.sdfs[i];
-        float t;
-        bool front;
-        if (sdf_march(sv, f, ray, b.t, &t, &front) && t < b.t)
-            b = Best{t, -2 - f.list_pos, make_ref(ITEM_SDF, i), front ? 0u : HIT_AUX_BACK};
-    }
-}
-
-// Closest hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501).  BATCH: the slots
-// belong to the requests of bv, each with its own scene root, unbounded objects and RNG key.
+// Closest bounded hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501); the
+// unbounded objects are marched afterwards by k_march.  BATCH: the slots belong to the requests of bv, each
+// with its own scene root and RNG key.
 template <bool BATCH>
 __global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams fp, PathBuffers pb,
                                                       const int32_t *queue, const uint32_t *count_ptr,
@@ -212,7 +201,6 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams 
     const uint32_t count = queue ? *count_ptr : fp.n_slots;
     const uint32_t refill_lanes = tune.refill_lanes, wave_batch = tune.wave_batch;
     dm::Key key{fp.key0, fp.key1};
-    int32_t sdf_first = 0, sdf_n = sv.n_sdfs;
     const uint32_t gid = blockIdx.x * EXT_BLOCK + threadIdx.x;
     const uint32_t lane = __lane_id();
     LaneStack st{s_node, s_tent, overflow + gid, gridDim.x * EXT_BLOCK, 0};
@@ -268,8 +256,6 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams 
                         sample = R.s0 + l / R.n_pixels;
                         key = dm::Key{R.key0, R.key1};
                         root = R.root;
-                        sdf_first = R.first_sdf;
-                        sdf_n = R.n_sdfs;
                     } else {
                         pixel = slot % fp.n_pixels;
                         sample = fp.s0 + slot / fp.n_pixels;
@@ -310,7 +296,6 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams 
                 if (++leaf_i == leaf_n) leaf = 0;
             }
             if (st.sp == 0) {
-                unbounded_hits(sv, ray, sdf_first, sdf_n, b);
                 pb.hit_t[slot] = b.t;
                 pb.hit_ref[slot] = b.ref;
                 pb.hit_aux[slot] = b.aux;

@@ -197,8 +197,8 @@ class Renderer:
 
     def set_tuning(self, chunk_slots: int = 0, refill_lanes: int = 0, wave_batch: int = 0,
                    shade_blocks_per_cu: int = 0):
-        """Launch shape of the wavefront kernels (0 keeps a default); chunk_slots applies from the next
-        set_options."""
+        """Launch shape of the wavefront kernels (0 keeps a default).  A new chunk_slots re-plans the path
+        buffers and clears accumulation."""
         t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 0, 0)
         _check(lib().rayna_set_tuning(self._h, C.byref(t)), self._h)
 

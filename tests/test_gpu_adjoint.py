@@ -67,6 +67,7 @@ def test_adjoint_frames_and_chunks(gpu):
     print("case adjoint two chunks 512x256x72 frame 1", flush=True)
     d = np.random.default_rng(8).standard_normal((256, 512, 3)).astype(np.float32)
     with Renderer(scene, cam, opts) as r:
+        r.set_tuning(chunk_slots=1 << 23)
         g, st = r.adjoint(d, frame=1, layout="rowmajor")
     tw = oracle.TwinScene(scene)
     g32, segs, _ = tw.adjoint(cam, opts, d, precision=32, frame=1, threads=16)

@@ -29,7 +29,8 @@ def _single(scenes, q):
 
 
 @pytest.mark.parametrize("tuning", [dict(), dict(chunk_slots=1 << 16), dict(chunk_slots=1 << 16, use_graph=True),
-                                    dict(shortest_first=True, refill_lanes=8, wave_batch=32, shade_blocks_per_cu=4)])
+                                    dict(schedule="shortest_first", refill_lanes=8, wave_batch=32, shade_blocks_per_cu=4),
+                                    dict(schedule="by_scene", chunk_slots=1 << 17)])
 def test_batch_matches_single_renders(gpu, tuning):
     scenes, reqs = _requests()
     print("case batch", tuning, flush=True)
@@ -73,7 +74,7 @@ def test_batch_latency_order(gpu):
     """With shortest-first scheduling the smallest request finishes first."""
     scenes, reqs = _requests(n=12, seed=9)
     with BatchRenderer(scenes) as b:
-        b.set_tuning(chunk_slots=1 << 16, shortest_first=True)
+        b.set_tuning(chunk_slots=1 << 16, schedule="shortest_first")
         _, st = b.render(reqs)
     work = np.array([q.opts.width * q.opts.height * q.opts.samples for q in reqs])
     assert st.request_ms[np.argmin(work)] <= st.request_ms.min() + 1e-9

@@ -111,6 +111,24 @@ def test_accumulation_and_setters(gpu):
         assert xm.shape == (40, 24, 3) and np.array_equal(xm.transpose(1, 0, 2), f0)
 
 
+def test_chunking_and_tuning_leave_the_image_unchanged(gpu):
+    """The same frame traced as one chunk of paths or as several (and with other launch shapes) is bit for
+    bit the same image, with the same closest-hit count, as the oracle's."""
+    sc, cam, opts = presets.mixed_primitives(width=96, height=64, samples=12)
+    ref, segs, _, _ = oracle.TwinScene(sc).render(cam, opts, precision=32, box_mode=oracle.BOX_TIGHT)
+    with Renderer(sc, cam, opts) as r:
+        for tune in (dict(), dict(chunk_slots=1 << 14), dict(chunk_slots=1 << 16, refill_lanes=64, wave_batch=1),
+                     dict(refill_lanes=1, wave_batch=4096, shade_blocks_per_cu=1)):
+            print("case chunking", tune, flush=True)
+            r.set_tuning(**tune)
+            r.clear_accumulation()
+            out = r.render(layout="rowmajor")
+            assert np.array_equal(out.img, ref), tune
+            assert out.stats.segments == segs, tune
+            if tune.get("chunk_slots"):
+                assert out.stats.extend_launches > opts.ray_depth + 1
+
+
 def test_invalid_camera_renders_the_checkerboard(gpu):
     sc, cam, opts = presets.random_spheres(width=40, height=24, samples=1)
     bad = Camera(cam.pos, cam.fwd, 0.0, cam.focus_dist)
