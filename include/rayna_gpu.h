@@ -173,9 +173,18 @@ typedef struct rayna_camera {
 
 /* ---------------------------------------------------------------- render options (render/render_opts.rs:8-29)
  * `seed` keys the counter-based RNG that replaces the engine's entropy-seeded generators
- * (render/renderer.rs:207); every (pixel, sample, bounce, draw slot) has its own stream.  Only
- * RAYNA_MODE_PBR is rendered; ray_branching must be 1. */
-enum { RAYNA_MODE_PBR = 0 };
+ * (render/renderer.rs:207); every (pixel, sample, bounce, draw slot) has its own stream.  All eight
+ * RenderModes are rendered; ray_branching must be 1. */
+enum {
+    RAYNA_MODE_PBR = 0,             /* physically based path tracing */
+    RAYNA_MODE_RAY_NORMAL = 1,      /* the normal against the ray, as (n / 2 + 0.5) */
+    RAYNA_MODE_OUTWARD_NORMAL = 2,  /* the outward normal, as (n / 2 + 0.5) */
+    RAYNA_MODE_SCATTER = 3,         /* the material's scatter direction, as (d / 2 + 0.5) */
+    RAYNA_MODE_FRONT_FACE = 4,      /* white for back faces, red for front faces */
+    RAYNA_MODE_DISTANCE = 5,        /* a palette ramp over 2 cbrt(distance) */
+    RAYNA_MODE_UV = 6,              /* (u, v, 0), clamped */
+    RAYNA_MODE_SIDE = 7             /* a palette colour per side index */
+};  /* RenderMode (render_opts.rs:32-52), rendered as render_px_once (renderer.rs:410-490) */
 
 typedef struct rayna_render_opts {
     uint32_t width;
@@ -227,6 +236,12 @@ rayna_status rayna_clear_accumulation(rayna_renderer *r);
  * then `out_dims` (if not NULL) receives that size and out_rgb must hold width*height*3 floats. */
 rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uint32_t out_dims[2],
                           rayna_render_stats *stats);
+
+/* rayna_render followed by the UI's conversion of the image to 8 bits (rayna_ui/src/ext/img_ext.rs:15-45):
+ * every channel raised to 1/2.2, scaled by 255 and cast to u8 with Rust's saturating cast, as RGBA (alpha
+ * 255) in row-major order; out_rgba holds width*height*4 bytes (the checkerboard's size for an invalid
+ * camera, as rayna_render).  The conversion runs on the GPU. */
+rayna_status rayna_render_srgb8(rayna_renderer *r, uint8_t *out_rgba, uint32_t out_dims[2], rayna_render_stats *stats);
 
 /* Replaces the colour of materials [0, n) of the current scene: the albedo of lambertian, metal,
  * dielectric and isotropic materials, the emission of lights (3 floats each, in rayna_scene_desc.materials

@@ -39,6 +39,7 @@ def lib() -> C.CDLL:
         L.twin_scene_tree_walk.restype = C.c_int32
         L.twin_scene_tree_walk.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32]
         L.twin_scene_set_colours.argtypes = [vp, C.POINTER(C.c_float), C.c_int32]
+        L.twin_to_srgb8.argtypes = [C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8)]
         L.twin_adjoint.restype = C.c_uint64
         L.twin_adjoint.argtypes = [vp, C.POINTER(A.CameraDesc), C.POINTER(A.RenderOptsDesc), C.c_int32, C.c_int32,
                                    C.c_uint32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_double),
@@ -72,6 +73,15 @@ def lib() -> C.CDLL:
 
 def _p(a: np.ndarray, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def to_srgb8(img_xmajor: np.ndarray) -> np.ndarray:
+    """The UI's 8-bit conversion of an x-major (width, height, 3) image: (height, width, 4) uint8 RGBA."""
+    img = np.ascontiguousarray(img_xmajor, np.float32)
+    w, h = img.shape[0], img.shape[1]
+    out = np.zeros((h, w, 4), np.uint8)
+    lib().twin_to_srgb8(_p(img, C.c_float), w, h, _p(out, C.c_uint8))
+    return out
 
 
 class TwinScene:

@@ -593,6 +593,8 @@ template <class R>
 struct Hit {
     R t;
     V3<R> pos, normal, ray_normal;
+    R uv[2] = {R(0), R(0)};  // Intersection::uv (shared/intersect.rs:7-43)
+    int side = 0;            // Intersection::side
     bool front_face;
     int object;
     int rank;  // depth-first rank of a bounded primitive; -2 - i for the i-th unbounded object
@@ -661,6 +663,13 @@ inline bool sphere_intersect(const double *sp, const Ray<R> &ray, const Interval
     const bool inside = dot(ray.dir, local) > R(0);
     h.ray_normal = inside ? -local : local;
     h.front_face = !inside;
+    // sphere_uv (sphere.rs:114-121)
+    const R PI = sizeof(R) == 4 ? R(3.14159274101257324f) : R(3.14159265358979323846);
+    const R theta = Math<R>::acos(-local.y);
+    const R phi = Math<R>::atan2(-local.z, local.x) + PI;
+    h.uv[0] = phi / (R(2) * PI);
+    h.uv[1] = theta / PI;
+    h.side = 0;
     return true;
 }
 
@@ -694,6 +703,9 @@ inline bool triangle_intersect(const double *tv, const double *tn, const Ray<R> 
     h.normal = n;
     h.front_face = std::signbit(det);
     h.ray_normal = n * -signum(det);
+    h.uv[0] = u;  // triangle.rs:104
+    h.uv[1] = v;
+    h.side = 0;
     return true;
 }
 
@@ -723,6 +735,9 @@ inline bool parallelogram_intersect(const double *pp, const Ray<R> &ray, const I
     h.normal = n;
     h.front_face = std::signbit(denom);
     h.ray_normal = (-n) * signum(denom);
+    h.uv[0] = alpha;  // planar/mod.rs:191
+    h.uv[1] = beta;
+    h.side = 0;
     return true;
 }
 
@@ -761,6 +776,11 @@ inline bool axis_box_intersect(const double *bp, const Ray<R> &ray, const Interv
             h.ray_normal = rn;
             h.normal = rn * winding;
             h.front_face = !std::signbit(winding);
+            // uvs remapped from [-radius, radius] to [0, 1]; side (1, 5, 9)[axis] + sgn as u32, halved
+            // (axis_box.rs:121-129): `-1.0 as u32` saturates to 0, so side = 2 axis + (sgn > 0)
+            h.uv[0] = (ub / radius[b] + R(1)) / R(2);
+            h.uv[1] = (uc / radius[c] + R(1)) / R(2);
+            h.side = 2 * a + (sgn[a] > R(0) ? 1 : 0);
             return true;
         }
     }
@@ -843,6 +863,9 @@ inline bool sdf_intersect(const Scene &s, const rayna_sdf &f, const Ray<R> &ray,
             const V3<R> n = normalize(g);
             h.t = total;
             h.pos = p;
+            h.uv[0] = R(0);  // raymarched.rs:106,109: uv zero, side = the march iteration
+            h.uv[1] = R(0);
+            h.side = i;
             h.normal = n;
             h.ray_normal = n;
             h.front_face = !std::signbit(dist);
@@ -947,6 +970,11 @@ inline bool object_intersect(const Ctx<R> &c, int o, const Ray<R> &ray, const In
         };
         h.normal = unit(a[0], a[1]);
         h.ray_normal = unit(a[2], a[3]);
+        R b[4];  // rng::vector_in_unit_square_01 (volumetric.rs:153)
+        draw4<R>(c.key, c.pixel, c.sample, c.bounce, SLOT_MEDIUM_UV, b);
+        h.uv[0] = b[0];
+        h.uv[1] = b[1];
+        h.side = 0;
         h.front_face = true;
         h.rank = s.obj_rank[o];
         hit = true;
@@ -1241,6 +1269,51 @@ struct Tracer {
         return L;
     }
 
+    // render_px_once for the non-PBR modes (renderer.rs:428-490): one closest hit, coloured by the mode; the sky
+    // on a miss.  mode: RAYNA_MODE_* (render_opts.rs:32-52).
+    Col mode_colour(const Ray<R> &ray, int mode) {
+        Hit<R> h;
+        if (!intersect(ray, 0, h)) return sky<R>(s->skybox, ray);
+        static const float COLOURS[13][3] = {{1.0f, 1.0f, 1.0f}, {1.0f, 0.0f, 0.0f}, {1.0f, 0.5f, 0.0f},
+                                             {1.0f, 1.0f, 0.0f}, {0.5f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f},
+                                             {0.0f, 1.0f, 0.5f}, {0.0f, 1.0f, 1.0f}, {0.0f, 0.5f, 1.0f},
+                                             {0.0f, 0.0f, 1.0f}, {0.5f, 0.0f, 1.0f}, {1.0f, 0.0f, 1.0f},
+                                             {0.0f, 0.0f, 0.0f}};
+        auto pal = [](int i) { return Col{COLOURS[i][0], COLOURS[i][1], COLOURS[i][2]}; };
+        auto vis = [](const V3<R> &v) {  // (f / 2.) as Channel + 0.5
+            return Col{(float)(v.x / R(2)) + 0.5f, (float)(v.y / R(2)) + 0.5f, (float)(v.z / R(2)) + 0.5f};
+        };
+        switch (mode) {
+            case RAYNA_MODE_RAY_NORMAL: return vis(h.ray_normal);
+            case RAYNA_MODE_OUTWARD_NORMAL: return vis(h.normal);
+            case RAYNA_MODE_SCATTER: {
+                R u[4];
+                draw4<R>(key, pixel, sample, 0, SLOT_SCATTER, u);
+                V3<R> dir{R(0), R(0), R(0)};
+                if (!scatter(s->materials[h.material], ray, h, u, &dir)) dir = V3<R>{R(0), R(0), R(0)};
+                return vis(dir);
+            }
+            case RAYNA_MODE_FRONT_FACE: return pal(h.front_face ? 1 : 0);
+            case RAYNA_MODE_DISTANCE: {
+                const R val = R(2) * Math<R>::cbrt01(h.t);
+                const R fl = std::fmin(std::fmax(std::floor(val), R(0)), R(12));
+                const R ce = std::fmin(std::fmax(std::ceil(val), R(0)), R(12));
+                const R frac = val - fl;
+                const Col a = pal((int)fl), b = pal((int)ce);
+                // Colour::lerp = a + (b - a) * frac, the product of a colour and a Number rounded from the
+                // Number's precision (colour.rs:177-181)
+                auto mulc = [&](float c) { return (float)((R)c * frac); };
+                return Col{a.r + mulc(b.r - a.r), a.g + mulc(b.g - a.g), a.b + mulc(b.b - a.b)};
+            }
+            case RAYNA_MODE_UV: {
+                auto cl = [](float x) { return std::fmin(std::fmax(x, 0.0f), 1.0f); };
+                return Col{cl((float)h.uv[0]), cl((float)h.uv[1]), 0.0f};
+            }
+            case RAYNA_MODE_SIDE: return pal(h.side % 12 + 1);
+            default: return Col{0, 0, 0};
+        }
+    }
+
     // Path replay backpropagation (Vicini, Speierer and Jakob 2021, algorithm 2) of one sample whose radiance
     // L came from `forward`, with d the sample's share of the loss gradient of its pixel.  The replay follows
     // the same path (the material colours never change a sampling decision: lambertian.rs:29-36,
@@ -1318,7 +1391,8 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
                 for (uint32_t k = 0; k < o.samples; ++k) {
                     tr.sample = k;
                     const Ray<R> ray = camera_ray<R>(vp, o, key, x, y, k);
-                    const Col c = (a.integrator == INTEGRATOR_FORWARD) ? tr.forward(ray) : tr.recursive(ray, 0);
+                    const Col c = o.mode != RAYNA_MODE_PBR ? tr.mode_colour(ray, o.mode)
+                                  : (a.integrator == INTEGRATOR_FORWARD) ? tr.forward(ray) : tr.recursive(ray, 0);
                     sum = sum + c;
                 }
                 const Col mean = sum / (float)o.samples;
@@ -1341,6 +1415,29 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
     for (auto &t : pool) t.join();
     if (prim_tests_out) *prim_tests_out = tests.load();
     return segs.load();
+}
+
+// The UI's conversion of a rendered Image to an 8-bit image (rayna_ui/src/ext/img_ext.rs:15-45): every channel
+// raised to 1/2.2 (f32 powf), scaled by 255 and cast to u8 (Rust's saturating cast: NaN and negatives to 0,
+// above 255 to 255); RGBA with alpha 255, row-major.  img: x-major width x height x 3 floats.
+inline uint8_t channel_u8(float c) {
+    const float INV_GAMMA = 1.0f / 2.2f;
+    const float g = pow_f32(c, INV_GAMMA) * 255.0f;
+    if (!(g > 0.0f)) return 0;
+    if (g >= 255.0f) return 255;
+    return (uint8_t)g;
+}
+
+inline void to_srgb8(const float *img, uint32_t w, uint32_t h, uint8_t *rgba) {
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x) {
+            const float *px = img + ((size_t)x * h + y) * 3;
+            uint8_t *o = rgba + ((size_t)y * w + x) * 4;
+            o[0] = channel_u8(px[0]);
+            o[1] = channel_u8(px[1]);
+            o[2] = channel_u8(px[2]);
+            o[3] = 255;
+        }
 }
 
 // Gradient of a loss with respect to every material colour (3 doubles per material, added to grad) for one

@@ -81,6 +81,9 @@ uint64_t twin_render(const twin_scene *t, const rayna_camera *cam, const rayna_r
     return segs;
 }
 
+// The UI's 8-bit conversion (twin::to_srgb8) of an x-major width x height x 3 float image to RGBA, row-major.
+void twin_to_srgb8(const float *img, uint32_t w, uint32_t h, uint8_t *rgba) { to_srgb8(img, w, h, rgba); }
+
 // Replaces the colour (albedo, or emission for a light) of the first n materials: 3 floats each.
 void twin_scene_set_colours(twin_scene *t, const float *rgb, int32_t n) {
     for (int32_t i = 0; i < n && i < (int32_t)t->s.materials.size(); ++i)

@@ -4,13 +4,13 @@ Host API mirroring rayna_engine (Scene / objects / meshes / materials / Camera /
 C ABI of librayna_amd.so (include/rayna_gpu.h), whose HIP kernels do all per-ray work on the GPU.
 """
 from .scene import (AxisBoxMesh, Camera, DielectricMaterial, IsotropicMaterial, LambertianMaterial, LightMaterial,
-                    MetalMaterial, NoSkybox, ParallelogramMesh, RaymarchedIsosurfaceMesh, RenderOpts, Scene,
+                    MetalMaterial, NoSkybox, ParallelogramMesh, RaymarchedIsosurfaceMesh, RenderMode, RenderOpts, Scene,
                     SdfMandelbulb, SdfSminSpheres, SdfSphere, SimpleObject, SimpleSkybox, SphereMesh, TriangleListMesh,
                     TriangleMesh, VolumetricObject, WhiteSkybox)
 from . import presets
 
 __all__ = ["AxisBoxMesh", "Camera", "DielectricMaterial", "IsotropicMaterial", "LambertianMaterial", "LightMaterial",
-           "MetalMaterial", "NoSkybox", "ParallelogramMesh", "RaymarchedIsosurfaceMesh", "RenderOpts", "Scene",
+           "MetalMaterial", "NoSkybox", "ParallelogramMesh", "RaymarchedIsosurfaceMesh", "RenderMode", "RenderOpts", "Scene",
            "SdfMandelbulb", "SdfSminSpheres", "SdfSphere", "SimpleObject", "SimpleSkybox", "SphereMesh",
            "TriangleListMesh", "TriangleMesh", "VolumetricObject", "WhiteSkybox", "presets"]
 from .renderer import Render, Renderer, RenderStats, RayError, prepare_scene

@@ -14,6 +14,7 @@ SDF_SPHERE, SDF_SMIN_SPHERES, SDF_MANDELBULB = 0, 1, 2
 OBJ_SIMPLE, OBJ_VOLUMETRIC = 0, 1
 SKY_NONE, SKY_SIMPLE, SKY_WHITE = 0, 1, 2
 MODE_PBR = 0
+MODE_RAY_NORMAL, MODE_OUTWARD_NORMAL, MODE_SCATTER, MODE_FRONT_FACE, MODE_DISTANCE, MODE_UV, MODE_SIDE = range(1, 8)
 LAYOUT_XMAJOR, LAYOUT_ROWMAJOR = 0, 1
 
 

@@ -2,6 +2,7 @@
 
     python -m rayna_amd.build            # product library
     python -m rayna_amd.build --oracle   # also the oracle (make -C oracle)
+    python -m rayna_amd.build --cpp-tests  # also the C++ API tests (tests/cpp)
 
 Device code is compiled by hipcc for gfx950 only, without floating-point contraction, with correctly rounded
 single-precision division and square root, and with packed-FP32 instructions disabled.  Host code is compiled
@@ -55,13 +56,32 @@ def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle")])
 
 
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+
+
+def build_cpp_tests():
+    """The C++ API's tests (tests/cpp/*.cpp, against include/rayna/engine.hpp), linked to the in-tree library."""
+    out_dir = os.path.join(CPP_TESTS, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    for name in sorted(os.listdir(CPP_TESTS)):
+        if not name.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, name[:-4])
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}",
+              os.path.join(CPP_TESTS, name), f"-L{HERE}", "-lrayna_amd", f"-Wl,-rpath,{HERE}",
+              "-Wl,-rpath,$ORIGIN/../../../rayna_amd", "-o", exe])
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle", action="store_true", help="also build the oracle (CPU restatement)")
+    ap.add_argument("--cpp-tests", action="store_true", help="also build the C++ API tests (tests/cpp)")
     a = ap.parse_args(argv)
     build_product()
     if a.oracle:
         build_oracle()
+    if a.cpp_tests:
+        build_cpp_tests()
 
 
 if __name__ == "__main__":

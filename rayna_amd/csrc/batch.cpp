@@ -161,7 +161,8 @@ rayna_status alloc_paths(rayna_batch *b, uint64_t slots, uint32_t max_depth) {
     const uint32_t qlen = 3 * (std::max<uint32_t>(max_depth, 16) + 2);
     if ((st = alloc(qlen * sizeof(uint32_t), (void **)&b->queue_count)) != RAYNA_OK) return st;
     if ((st = alloc(sizeof(unsigned long long), (void **)&b->segments)) != RAYNA_OK) return st;
-    if (b->n_sdf_max && (st = alloc(slots * b->n_sdf_max * sizeof(float), (void **)&b->mt)) != RAYNA_OK) return st;
+    if (b->n_sdf_max && (st = alloc(2 * slots * b->n_sdf_max * sizeof(float), (void **)&b->mt)) != RAYNA_OK)
+        return st;
     b->pb.hit_rank = nullptr;
     b->capacity = slots;
     b->queue_count_len = qlen;
@@ -330,8 +331,8 @@ uint32_t enqueue(rayna_batch *b, const Plan &pl, std::vector<hipEvent_t> *ev) {
             uint32_t *cout = b->queue_count + depth + 1;
             uint32_t *work = b->queue_count + (D + 2) + depth;
             launch_extend(b->view, fp, b->pb, qin, cin, depth, b->grid_extend, b->overflow, work, b->tune, bv, s);
-            launch_march(b->view, fp, b->pb, qin, cin, pl.n_sdf[c], b->mt, b->queue_count + 2 * (D + 2) + depth,
-                         b->grid_march, b->grid_shade, b->tune, bv, s);
+            launch_march(b->view, fp, b->pb, qin, cin, pl.n_sdf[c], b->mt, b->capacity * b->n_sdf_max,
+                         b->queue_count + 2 * (D + 2) + depth, b->grid_march, b->grid_shade, b->tune, bv, s);
             launch_shade(b->view, fp, b->pb, qin, cin, qout, cout, depth, b->grid_shade, nullptr, bv, s);
             launches += pl.n_sdf[c] ? 4 : 2;
         }

@@ -85,6 +85,14 @@ __device__ __forceinline__ float exp_f(float x) {
     return (p * s1) * s2;
 }
 
+// x^y for the UI's gamma (x >= 0, y > 0): exp(y log x); 0 for x = 0, NaN for x < 0 or NaN.
+__device__ __forceinline__ float pow_f(float x, float y) {
+    if (x != x || x < 0.0f) return __builtin_nanf("");
+    if (x == 0.0f) return 0.0f;
+    return exp_f(y * log_f(x));
+}
+
+// Cube root of x >= 0: exponent-thirds initial guess and four Newton steps (also valid above 1).
 __device__ __forceinline__ float cbrt01(float x) {
     if (x <= 0.0f) return 0.0f;
     float y = __uint_as_float(__float_as_uint(x) / 3u + 0x2a5137a0u);
@@ -184,7 +192,7 @@ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint
 __device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1.0p-24f; }
 
 // Draw-site slots (the fourth counter word); identical to the CPU restatement's.
-enum : uint32_t { SLOT_CAMERA = 0, SLOT_SCATTER = 1, SLOT_MEDIUM_AUX = 2, SLOT_MEDIUM_BASE = 16 };
+enum : uint32_t { SLOT_CAMERA = 0, SLOT_SCATTER = 1, SLOT_MEDIUM_AUX = 2, SLOT_MEDIUM_UV = 3, SLOT_MEDIUM_BASE = 16 };
 
 struct Key {
     uint32_t k0, k1;

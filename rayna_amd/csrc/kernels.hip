@@ -139,7 +139,8 @@ __device__ __forceinline__ bool tri_normal(const GTriNormals &n, float u, float 
 }
 
 // Planar::intersect_bounded + ParallelogramMesh::intersect (planar/mod.rs:385-418, parallelogram.rs:469-477)
-__device__ __forceinline__ bool para_t(const GPara &g, const DRay &r, float t0, float t1, float *t) {
+__device__ __forceinline__ bool para_t(const GPara &g, const DRay &r, float t0, float t1, float *t,
+                                       float *uv_out = nullptr) {
     const f3 n{g.nx, g.ny, g.nz};
     const float denom = dot(n, r.d);
     if (denom == 0.0f) return false;
@@ -152,12 +153,16 @@ __device__ __forceinline__ bool para_t(const GPara &g, const DRay &r, float t0, 
     const float beta = dot(w, cross(f3{g.ux, g.uy, g.uz}, pl));
     if (!(alpha <= 1.0f && beta <= 1.0f && alpha >= 0.0f && beta >= 0.0f)) return false;
     *t = tt;
+    if (uv_out) {  // planar/mod.rs:191
+        uv_out[0] = alpha;
+        uv_out[1] = beta;
+    }
     return true;
 }
 
 // AxisBoxMesh::intersect (axis_box.rs:64-145).  axis_out: the face axis hit; winding/sgn for the normals.
 __device__ __forceinline__ bool box_t(const GBox &g, const DRay &r, float t0, float t1, float *t, int *axis_out,
-                                      float *winding_out, f3 *sgn_out) {
+                                      float *winding_out, f3 *sgn_out, float *uv_out = nullptr) {
     const f3 ro = r.o - f3{g.cx, g.cy, g.cz};
     const f3 rd = r.d;
     const f3 radius{g.rx, g.ry, g.rz};
@@ -181,6 +186,10 @@ __device__ __forceinline__ bool box_t(const GBox &g, const DRay &r, float t0, fl
             *axis_out = a;
             *winding_out = winding;
             *sgn_out = sgn;
+            if (uv_out) {  // remapped from [-radius, radius] to [0, 1] (axis_box.rs:121-122)
+                uv_out[0] = (ub / rad[b] + 1.0f) / 2.0f;
+                uv_out[1] = (uc / rad[c] + 1.0f) / 2.0f;
+            }
             return true;
         }
     }
@@ -379,6 +388,61 @@ __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint
             h->material = f.material;
             break;
         }
+    }
+}
+
+// Intersection::uv and ::side of a hit (shared/intersect.rs:7-43), for the debug render modes: sphere_uv
+// (sphere.rs:114-121), the triangle's barycentrics (triangle.rs:104), the plane coordinates (planar/mod.rs:191),
+// the box face's uv and side (axis_box.rs:121-129: side = 2 axis + (sgn > 0), `-1.0 as u32` saturating to 0),
+// the volume's random uv (volumetric.rs:153) and the SDF's march iteration (raymarched.rs:106-109, carried in the
+// hit flags from bit 8).
+__device__ void hit_uv_side(const SceneView &sv, const DRay &ray, float t, uint32_t ref, uint32_t aux,
+                            const dm::Key &key, uint32_t pixel, uint32_t sample, uint32_t bounce, float uv[2],
+                            int *side) {
+    const int kind = (int)(ref >> HIT_KIND_SHIFT);
+    const int32_t idx = (int32_t)(ref & HIT_INDEX_MASK);
+    uv[0] = uv[1] = 0.0f;
+    *side = 0;
+    switch (kind) {
+        case ITEM_TRIANGLE: {
+            float tt, det;
+            tri_t(sv.tris[idx], ray, t, t, &tt, &uv[0], &uv[1], &det);
+            break;
+        }
+        case ITEM_SPHERE: {
+            const GSphere s = sv.spheres[sv.items[idx].index];
+            const f3 local = (at(ray, t) - f3{s.cx, s.cy, s.cz}) / s.radius;
+            const float PI = 3.14159274101257324f;
+            const float theta = dm::acos_f(-local.y);
+            const float phi = dm::atan2_f(-local.z, local.x) + PI;
+            uv[0] = phi / (2.0f * PI);
+            uv[1] = theta / PI;
+            break;
+        }
+        case ITEM_PARALLELOGRAM: {
+            float tt;
+            para_t(sv.paras[sv.items[idx].index], ray, t, t, &tt, uv);
+            break;
+        }
+        case ITEM_AXIS_BOX: {
+            float tt, w;
+            int axis = 0;
+            f3 sg{0.0f, 0.0f, 0.0f};
+            box_t(sv.boxes[sv.items[idx].index], ray, t, t, &tt, &axis, &w, &sg, uv);
+            const float sa = axis == 0 ? sg.x : axis == 1 ? sg.y : sg.z;
+            *side = 2 * axis + (sa > 0.0f ? 1 : 0);
+            break;
+        }
+        case ITEM_VOLUME: {
+            float b[4];
+            dm::draw4(key, pixel, sample, bounce, dm::SLOT_MEDIUM_UV, b);
+            uv[0] = b[0];
+            uv[1] = b[1];
+            break;
+        }
+        default:  // ITEM_SDF
+            *side = (int)(aux >> HIT_AUX_SIDE_SHIFT);
+            break;
     }
 }
 
@@ -731,6 +795,85 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_batch(BatchView bv, const ui
     }
 }
 
+// render_px_once for the non-PBR modes (renderer.rs:428-490): the camera ray's closest hit coloured by the mode,
+// the sky on a miss.
+__global__ __launch_bounds__(BLOCK) void k_shade_mode(SceneView sv, FrameParams fp, PathBuffers pb) {
+    const dm::Key key{fp.key0, fp.key1};
+    for (uint32_t slot = blockIdx.x * BLOCK + threadIdx.x; slot < fp.n_slots; slot += gridDim.x * BLOCK) {
+        const DRay ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+        const uint32_t ref = pb.hit_ref[slot];
+        Col L;
+        if (ref == 0u) {
+            L = sky(sv.skybox, ray.d);
+        } else {
+            const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
+            const float t = pb.hit_t[slot];
+            const uint32_t aux = pb.hit_aux[slot];
+            SHit h;
+            hit_geometry(sv, ray, t, ref, aux, key, pixel, sample, 0, &h);
+            const float PAL[13][3] = {{1.0f, 1.0f, 1.0f}, {1.0f, 0.0f, 0.0f}, {1.0f, 0.5f, 0.0f}, {1.0f, 1.0f, 0.0f},
+                                      {0.5f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.5f}, {0.0f, 1.0f, 1.0f},
+                                      {0.0f, 0.5f, 1.0f}, {0.0f, 0.0f, 1.0f}, {0.5f, 0.0f, 1.0f}, {1.0f, 0.0f, 1.0f},
+                                      {0.0f, 0.0f, 0.0f}};
+            f3 vis{0.0f, 0.0f, 0.0f};
+            bool use_vis = true;
+            switch (fp.mode) {
+                case RAYNA_MODE_RAY_NORMAL: vis = h.ray_normal; break;
+                case RAYNA_MODE_OUTWARD_NORMAL: vis = h.normal; break;
+                case RAYNA_MODE_SCATTER: {
+                    float u[4];
+                    dm::draw4(key, pixel, sample, 0, dm::SLOT_SCATTER, u);
+                    if (!scatter(sv.materials[h.material], ray, h, u, &vis)) vis = f3{0.0f, 0.0f, 0.0f};
+                    break;
+                }
+                default: use_vis = false; break;
+            }
+            if (use_vis) {  // (f / 2.) as Channel + 0.5
+                L = Col{vis.x / 2.0f + 0.5f, vis.y / 2.0f + 0.5f, vis.z / 2.0f + 0.5f};
+            } else if (fp.mode == RAYNA_MODE_FRONT_FACE) {
+                const int c = h.front ? 1 : 0;
+                L = Col{PAL[c][0], PAL[c][1], PAL[c][2]};
+            } else if (fp.mode == RAYNA_MODE_DISTANCE) {
+                const float val = 2.0f * dm::cbrt01(t);
+                const float fl = fminf(fmaxf(floorf(val), 0.0f), 12.0f);
+                const float ce = fminf(fmaxf(ceilf(val), 0.0f), 12.0f);
+                const float frac = val - fl;
+                const int ia = (int)fl, ib = (int)ce;
+                // Colour::lerp: a + (b - a) * frac
+                L = Col{PAL[ia][0] + (PAL[ib][0] - PAL[ia][0]) * frac, PAL[ia][1] + (PAL[ib][1] - PAL[ia][1]) * frac,
+                        PAL[ia][2] + (PAL[ib][2] - PAL[ia][2]) * frac};
+            } else {
+                float uv[2];
+                int side;
+                hit_uv_side(sv, ray, t, ref, aux, key, pixel, sample, 0, uv, &side);
+                if (fp.mode == RAYNA_MODE_UV) {
+                    L = Col{fminf(fmaxf(uv[0], 0.0f), 1.0f), fminf(fmaxf(uv[1], 0.0f), 1.0f), 0.0f};
+                } else {  // RAYNA_MODE_SIDE
+                    const int c = side % 12 + 1;
+                    L = Col{PAL[c][0], PAL[c][1], PAL[c][2]};
+                }
+            }
+        }
+        pb.lr[slot] = L.r;
+        pb.lg[slot] = L.g;
+        pb.lb[slot] = L.b;
+    }
+}
+
+// The UI's 8-bit conversion (rayna_ui/src/ext/img_ext.rs:15-45): c^(1/2.2) * 255 with Rust's saturating u8 cast.
+__device__ __forceinline__ uint8_t channel_u8(float c) {
+    const float g = dm::pow_f(c, 1.0f / 2.2f) * 255.0f;
+    if (!(g > 0.0f)) return 0;
+    if (g >= 255.0f) return 255;
+    return (uint8_t)g;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_to_srgb8(const float *img, uint32_t n_pixels, uchar4 *rgba) {
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= n_pixels) return;
+    rgba[p] = make_uchar4(channel_u8(img[3 * p]), channel_u8(img[3 * p + 1]), channel_u8(img[3 * p + 2]), 255);
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t value, uint32_t n) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) p[i] = value;
@@ -805,19 +948,19 @@ void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers 
 }
 
 void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
-                  const uint32_t *count, uint32_t n_sdf, float *mt, uint32_t *work, int grid, int merge_grid,
-                  const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
+                  const uint32_t *count, uint32_t n_sdf, float *mt, uint64_t mt_stride, uint32_t *work, int grid,
+                  int merge_grid, const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
     if (n_sdf == 0) return;
     if (bv.reqs) {
         hipLaunchKernelGGL(k_march<true>, dim3(grid), dim3(MARCH_BLOCK), 0, s, sv, fp, pb, queue, count, n_sdf, mt,
-                           work, tune, bv);
+                           mt_stride, work, tune, bv);
         hipLaunchKernelGGL(k_march_merge<true>, dim3(merge_grid), dim3(BLOCK), 0, s, sv, fp, pb, queue, count,
-                           n_sdf, mt, bv);
+                           n_sdf, mt, mt_stride, bv);
     } else {
         hipLaunchKernelGGL(k_march<false>, dim3(grid), dim3(MARCH_BLOCK), 0, s, sv, fp, pb, queue, count, n_sdf,
-                           mt, work, tune, bv);
+                           mt, mt_stride, work, tune, bv);
         hipLaunchKernelGGL(k_march_merge<false>, dim3(merge_grid), dim3(BLOCK), 0, s, sv, fp, pb, queue, count,
-                           n_sdf, mt, bv);
+                           n_sdf, mt, mt_stride, bv);
     }
 }
 
@@ -836,6 +979,15 @@ void launch_finalize_batch(const BatchView &bv, const uint32_t *final_reqs, uint
 
 void launch_sum_grad(const double *grad, uint32_t n, uint32_t copies, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_sum_grad, dim3(grid_for(n, BLOCK)), dim3(BLOCK), 0, s, grad, n, copies, out);
+}
+
+void launch_shade_mode(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_shade_mode, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb);
+}
+
+void launch_to_srgb8(const float *img, uint32_t n_pixels, uint8_t *rgba, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_srgb8, dim3(grid_for(n_pixels, BLOCK)), dim3(BLOCK), 0, s, img, n_pixels,
+                       reinterpret_cast<uchar4 *>(rgba));
 }
 
 void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, uint32_t chunk_samples, int first,

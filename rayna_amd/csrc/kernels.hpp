@@ -31,6 +31,7 @@ struct FrameParams {
     uint32_t n_pixels;
     uint32_t s0;             // first sample of the chunk
     uint32_t n_slots;        // slots in the chunk
+    int32_t mode;            // RAYNA_MODE_*
 };
 
 // One request of a batch, as seen by one chunk of paths: its slots are [slot_base, slot_base + n_slots), sample
@@ -83,11 +84,12 @@ void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers 
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
                   const AdjointParams *adjoint, const BatchView &bv, hipStream_t s);
 // The unbounded SDF objects of each queued path (n_sdf per path; in a batch, each request's own), marched
-// after launch_extend with its hit as the cut, then merged into the hit record.  mt: count * n_sdf floats.
+// after launch_extend with its hit as the cut, then merged into the hit record.  mt: two planes of mt_stride
+// >= count * n_sdf floats (outcome, march iteration).
 // work: a zeroed counter.  No-op for n_sdf == 0.
 void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
-                  const uint32_t *count, uint32_t n_sdf, float *mt, uint32_t *work, int grid, int merge_grid,
-                  const ExtendTuning &tune, const BatchView &bv, hipStream_t s);
+                  const uint32_t *count, uint32_t n_sdf, float *mt, uint64_t mt_stride, uint32_t *work, int grid,
+                  int merge_grid, const ExtendTuning &tune, const BatchView &bv, hipStream_t s);
 // Batch chunks: per request, the running per-pixel sum of the chunk's samples (in sample order) and, at the
 // request's last chunk, its sample mean in its layout at out + sum_offset.
 void launch_resolve_batch(const BatchView &bv, uint32_t max_pixels, const PathBuffers &pb, float *sum,
@@ -95,6 +97,11 @@ void launch_resolve_batch(const BatchView &bv, uint32_t max_pixels, const PathBu
 void launch_finalize_batch(const BatchView &bv, const uint32_t *final_reqs, uint32_t n_final, uint32_t max_pixels,
                            const float *sum, float *out, hipStream_t s);
 void launch_sum_grad(const double *grad, uint32_t n, uint32_t copies, double *out, hipStream_t s);
+// The non-PBR render modes: the mode colour of each slot's camera ray (after launch_extend at depth 0) into
+// its radiance field.
+void launch_shade_mode(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, int grid, hipStream_t s);
+// The UI's 8-bit conversion of a row-major float image (width * height * 3) to RGBA.
+void launch_to_srgb8(const float *img, uint32_t n_pixels, uint8_t *rgba, hipStream_t s);
 void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, uint32_t chunk_samples, int first,
                     hipStream_t s);
 void launch_finalize(const FrameParams &fp, const float *sum, float *acc, double frames, float *out, int layout,

@@ -124,7 +124,9 @@ struct alignas(16) GMaterial {
 
 // Hit record written by the extend kernel: distance, packed reference (kind << 28 | index), rank, aux bits.
 enum : uint32_t { HIT_KIND_SHIFT = 28, HIT_INDEX_MASK = 0x0fffffffu };
-enum : uint32_t { HIT_AUX_BACK = 1u };  // SDF hits: the final distance sample was negative (front_face false)
+// Hit flags of SDF hits: bit 0, the final distance sample was negative (front_face false); from bit 8, the march
+// iteration of the hit (Intersection::side).
+enum : uint32_t { HIT_AUX_BACK = 1u, HIT_AUX_SIDE_SHIFT = 8u };
 
 // Camera constants in float, from Camera::calculate_viewport evaluated in double (camera.rs:285-338).
 struct GViewport {

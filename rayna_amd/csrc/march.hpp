@@ -14,7 +14,8 @@
 // it reaches the cut no later hit can be kept; and once the point is outside the object's escape sphere and the
 // ray moves away from its centre, every later estimate exceeds epsilon.
 //
-// Each item's outcome is written to mt[item]: +t (front side), -t (back side) or +infinity (no hit).  The
+// Each item's outcome is written to mt[item]: +t (front side), -t (back side) or +infinity (no hit), and at a hit
+// the march iteration to mt[mt_stride + item] (as integer bits; the hit's side).  The
 // cut is the path's closest bounded hit; k_march_merge then applies the list order (strict <, object/list.rs:
 // 358-363).  A cut that ignores earlier objects of the list only lets a march run longer: it never changes which
 // hit wins.
@@ -24,7 +25,7 @@ constexpr int MARCH_BLOCK = 256;
 template <bool BATCH>
 __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams fp, PathBuffers pb,
                                                        const int32_t *queue, const uint32_t *count_ptr,
-                                                       uint32_t n_sdf, float *mt, uint32_t *work,
+                                                       uint32_t n_sdf, float *mt, uint64_t mt_stride, uint32_t *work,
                                                        ExtendTuning tune, BatchView bv) {
     const uint32_t count = queue ? *count_ptr : fp.n_slots;
     const uint32_t n_items = count * n_sdf;
@@ -151,6 +152,7 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
             total = total + fabsf(dist);
             if (fabsf(dist) < f.epsilon && T_MIN <= total && total <= FLT_MAX) {
                 mt[item] = signbit(dist) ? -total : total;
+                mt[mt_stride + item] = __int_as_float(step);  // Intersection::side (raymarched.rs:109)
                 active = false;
             } else if (step > f.max_iterations) {
                 mt[item] = __builtin_inff();
@@ -167,7 +169,8 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_march_merge(SceneView sv, FrameParams fp, PathBuffers pb,
                                                        const int32_t *queue, const uint32_t *count_ptr,
-                                                       uint32_t n_sdf, const float *mt, BatchView bv) {
+                                                       uint32_t n_sdf, const float *mt, uint64_t mt_stride,
+                                                       BatchView bv) {
     const uint32_t count = queue ? *count_ptr : fp.n_slots;
     for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < count; q += gridDim.x * BLOCK) {
         const uint32_t slot = queue ? (uint32_t)queue[q] : q;
@@ -189,7 +192,8 @@ __global__ __launch_bounds__(BLOCK) void k_march_merge(SceneView sv, FrameParams
                 bt = t;
                 rank = -2 - sv.sdfs[j].list_pos;
                 ref = make_ref(ITEM_SDF, j);
-                aux = signbit(v) ? HIT_AUX_BACK : 0u;
+                aux = (signbit(v) ? HIT_AUX_BACK : 0u) |
+                      ((uint32_t)__float_as_int(mt[mt_stride + (size_t)jj * count + q]) << HIT_AUX_SIDE_SHIFT);
                 changed = true;
             }
         }

@@ -53,6 +53,8 @@ struct rayna_renderer {
     uint32_t *queue_count = nullptr;
     unsigned long long *segments = nullptr;
     float *sum = nullptr, *acc = nullptr, *out_dev = nullptr;
+    uint8_t *rgba_dev = nullptr;  // the 8-bit output of rayna_render_srgb8
+    uint64_t rgba_cap = 0;
     uint32_t alloc_pixels = 0;
     uint32_t queue_count_len = 0;
 
@@ -172,14 +174,14 @@ void free_paths(rayna_renderer *r) {
     r->queue_count_len = 0;
 }
 
-// The march outcome buffer: one float per (slot, unbounded object).
+// The march outcome buffer: two planes (outcome, march iteration) of one float per (slot, unbounded object).
 rayna_status ensure_mt(rayna_renderer *r) {
     const uint64_t need = r->slot_capacity * (uint64_t)std::max(0, r->dev.view.n_sdfs);
     if (need <= r->mt_cap) return RAYNA_OK;
     if (r->mt) (void)hipFree(r->mt);
     r->mt = nullptr;
     r->mt_cap = 0;
-    HIP_TRY(r, hipMalloc((void **)&r->mt, need * sizeof(float)));
+    HIP_TRY(r, hipMalloc((void **)&r->mt, 2 * need * sizeof(float)));
     r->mt_cap = need;
     return RAYNA_OK;
 }
@@ -239,7 +241,7 @@ rayna_status check_opts(rayna_renderer *r, const rayna_render_opts *o) {
     if (!o || o->width == 0 || o->height == 0 || o->samples == 0 || o->ray_branching == 0)
         return fail(r, RAYNA_INVALID_ARGUMENT, "render options must have non-zero width, height, samples, branching");
     if (o->ray_branching != 1) return fail(r, RAYNA_UNSUPPORTED, "ray_branching > 1 is not supported on the GPU");
-    if (o->mode != RAYNA_MODE_PBR) return fail(r, RAYNA_UNSUPPORTED, "only RenderMode::PBR is supported");
+    if (o->mode < RAYNA_MODE_PBR || o->mode > RAYNA_MODE_SIDE) return fail(r, RAYNA_INVALID_ARGUMENT, "unknown render mode");
     if (o->ray_depth > 1000) return fail(r, RAYNA_UNSUPPORTED, "ray_depth above 1000");
     return RAYNA_OK;
 }
@@ -268,6 +270,7 @@ FrameParams frame_params(const rayna_renderer *r, uint64_t frame_index) {
     fp.n_pixels = r->opts.width * r->opts.height;
     fp.s0 = 0;
     fp.n_slots = 0;
+    fp.mode = r->opts.mode;
     return fp;
 }
 
@@ -289,8 +292,15 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
         uint32_t *work = r->queue_count + D2 + depth;
         launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, work, r->tune,
                       BatchView{}, r->stream);
-        launch_march(r->dev.view, fp, r->pb, qin, cin, (uint32_t)r->dev.view.n_sdfs, r->mt, r->queue_count + 2 * D2 + depth,
-                     r->grid_march, r->grid_shade, r->tune, BatchView{}, r->stream);
+        launch_march(r->dev.view, fp, r->pb, qin, cin, (uint32_t)r->dev.view.n_sdfs, r->mt, r->mt_cap,
+                     r->queue_count + 2 * D2 + depth, r->grid_march, r->grid_shade, r->tune, BatchView{}, r->stream);
+        if (fp.mode != RAYNA_MODE_PBR) {  // render_px_once: the camera ray's hit coloured by the mode
+            if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
+            launch_shade_mode(r->dev.view, fp, r->pb, r->grid_shade, r->stream);
+            if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
+            ++*extend_launches;
+            break;
+        }
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
         launch_shade(r->dev.view, fp, r->pb, qin, cin, qout, cout, depth, r->grid_shade, adjoint, BatchView{},
                      r->stream);
@@ -495,6 +505,39 @@ rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uin
     return RAYNA_OK;
 }
 
+rayna_status rayna_render_srgb8(rayna_renderer *r, uint8_t *out_rgba, uint32_t out_dims[2], rayna_render_stats *stats) {
+    if (!r || !out_rgba) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    uint32_t dims[2] = {0, 0};
+    const rayna_status st = rayna_render(r, nullptr, RAYNA_LAYOUT_ROWMAJOR, dims, stats);
+    if (st != RAYNA_OK) return st;
+    if (out_dims) {
+        out_dims[0] = dims[0];
+        out_dims[1] = dims[1];
+    }
+    const uint64_t P = (uint64_t)dims[0] * dims[1];
+    if (!r->cam_valid) {  // the checkerboard: channels of exactly 0 and 1 map to 0 and 255
+        std::vector<float> chk(3 * P);
+        write_checkerboard(dims[0], dims[1], chk.data(), RAYNA_LAYOUT_ROWMAJOR);
+        for (uint64_t p = 0; p < P; ++p) {
+            for (int c = 0; c < 3; ++c) out_rgba[4 * p + c] = chk[3 * p + c] >= 1.0f ? 255 : 0;
+            out_rgba[4 * p + 3] = 255;
+        }
+        return RAYNA_OK;
+    }
+    if (r->rgba_cap < 4 * P) {
+        if (r->rgba_dev) (void)hipFree(r->rgba_dev);
+        r->rgba_dev = nullptr;
+        r->rgba_cap = 0;
+        HIP_TRY(r, hipMalloc((void **)&r->rgba_dev, 4 * P));
+        r->rgba_cap = 4 * P;
+    }
+    launch_to_srgb8(r->out_dev, (uint32_t)P, r->rgba_dev, r->stream);
+    HIP_TRY(r, hipGetLastError());
+    HIP_TRY(r, hipMemcpyAsync(out_rgba, r->rgba_dev, 4 * P, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    return RAYNA_OK;
+}
+
 rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t) {
     if (!r || !t) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
     if (t->refill_lanes > 64 || t->wave_batch > 4096 || t->shade_blocks_per_cu > 64)
@@ -541,6 +584,7 @@ rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_ima
     if (n_materials != r->packed.materials.size())
         return fail(r, RAYNA_INVALID_ARGUMENT, "n_materials is not the scene's material count");
     if (!r->cam_valid) return fail(r, RAYNA_INVALID_ARGUMENT, "camera is invalid (the frame is the checkerboard)");
+    if (r->opts.mode != RAYNA_MODE_PBR) return fail(r, RAYNA_UNSUPPORTED, "the adjoint is of RenderMode::PBR frames");
     if (n_materials == 0) return RAYNA_OK;
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(r, hipSetDevice(r->device));
@@ -642,8 +686,8 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     launch_fill_u32(work, 0u, 2, r->stream);
     launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, work, r->tune,
                   BatchView{}, r->stream);
-    launch_march(r->dev.view, fp, pb, nullptr, nullptr, (uint32_t)r->dev.view.n_sdfs, r->mt, work + 1, r->grid_march,
-                 r->grid_shade, r->tune, BatchView{}, r->stream);
+    launch_march(r->dev.view, fp, pb, nullptr, nullptr, (uint32_t)r->dev.view.n_sdfs, r->mt, r->mt_cap, work + 1,
+                 r->grid_march, r->grid_shade, r->tune, BatchView{}, r->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(prim_rank, rank_dev, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(t, pb.hit_t, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
@@ -699,6 +743,7 @@ void rayna_destroy(rayna_renderer *r) {
     if (r->grad) (void)hipFree(r->grad);
     if (r->grad_sum) (void)hipFree(r->grad_sum);
     if (r->mt) (void)hipFree(r->mt);
+    if (r->rgba_dev) (void)hipFree(r->rgba_dev);
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;

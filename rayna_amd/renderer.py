@@ -27,7 +27,7 @@ _lib: Optional[C.CDLL] = None
 
 # every symbol declared in include/rayna_gpu.h
 EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_options", "rayna_clear_accumulation",
-           "rayna_render", "rayna_set_material_colours", "rayna_adjoint", "rayna_set_tuning", "rayna_batch_create",
+           "rayna_render", "rayna_render_srgb8", "rayna_set_material_colours", "rayna_adjoint", "rayna_set_tuning", "rayna_batch_create",
            "rayna_batch_set_tuning", "rayna_render_batch", "rayna_batch_destroy", "rayna_batch_last_error",
            "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare", "rayna_destroy",
            "rayna_last_error")
@@ -61,6 +61,8 @@ def lib() -> C.CDLL:
         L.rayna_render.restype = C.c_int
         L.rayna_render.argtypes = [vp, C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_uint32),
                                    C.POINTER(A.RenderStats)]
+        L.rayna_render_srgb8.restype = C.c_int
+        L.rayna_render_srgb8.argtypes = [vp, C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(A.RenderStats)]
         L.rayna_set_material_colours.restype = C.c_int
         L.rayna_set_material_colours.argtypes = [vp, C.POINTER(C.c_float), C.c_uint32]
         L.rayna_adjoint.restype = C.c_int
@@ -224,6 +226,20 @@ class Renderer:
         stats = RenderStats(st.duration_s, int(st.accum_frames), int(st.samples), int(st.segments), st.kernel_ms,
                             st.extend_ms, st.shade_ms, int(st.extend_launches))
         return Render(img, stats)
+
+    def render_srgb8(self) -> Render:
+        """render() followed by the UI's 8-bit conversion (rayna_ui/src/ext/img_ext.rs:15-45) on the GPU:
+        (height, width, 4) uint8 RGBA, gamma 1/2.2."""
+        o = self._options
+        img = np.zeros(o.width * o.height * 4, np.uint8)
+        dims = (C.c_uint32 * 2)()
+        st = A.RenderStats()
+        _check(lib().rayna_render_srgb8(self._h, img.ctypes.data_as(C.POINTER(C.c_uint8)), dims, C.byref(st)),
+               self._h)
+        w, h = int(dims[0]), int(dims[1])
+        stats = RenderStats(st.duration_s, int(st.accum_frames), int(st.samples), int(st.segments), st.kernel_ms,
+                            st.extend_ms, st.shade_ms, int(st.extend_launches))
+        return Render(img[: w * h * 4].reshape(h, w, 4), stats)
 
     # ---- differentiation ----
     def set_material_colours(self, rgb):

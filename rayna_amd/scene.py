@@ -18,6 +18,7 @@ from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Union
 
 import numpy as np
+from enum import IntEnum
 
 from . import _abi as A
 
@@ -206,6 +207,18 @@ class Camera:
     def desc(self) -> A.CameraDesc:
         return A.CameraDesc((C.c_double * 3)(*self.pos), (C.c_double * 3)(*self.fwd), math.radians(self.v_fov),
                             float(self.focus_dist), math.radians(self.defocus_angle))
+
+
+class RenderMode(IntEnum):
+    """RenderMode (render/render_opts.rs:32-52), in the engine's order."""
+    PBR = A.MODE_PBR
+    RayNormal = A.MODE_RAY_NORMAL
+    OutwardNormal = A.MODE_OUTWARD_NORMAL
+    Scatter = A.MODE_SCATTER
+    FrontFace = A.MODE_FRONT_FACE
+    Distance = A.MODE_DISTANCE
+    Uv = A.MODE_UV
+    Side = A.MODE_SIDE
 
 
 @dataclass
