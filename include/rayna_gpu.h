@@ -114,9 +114,16 @@ typedef struct rayna_sdf {
 } rayna_sdf;
 
 /* ---------------------------------------------------------------- objects
- * rayna_engine/src/object/mod.rs:47-52 (ObjectInstance), identity transform only.
- *   SIMPLE      SimpleObject::new(mesh, material, None)                    object/simple.rs:175-200
- *   VOLUMETRIC  VolumetricObject::new(mesh, material, density, None)       object/volumetric.rs:42-72
+ * rayna_engine/src/object/mod.rs:47-52 (ObjectInstance).
+ *   SIMPLE      SimpleObject::new_uncorrected(mesh, material, transform)   object/simple.rs:175-200
+ *   VOLUMETRIC  VolumetricObject::new_uncorrected(mesh, material, density, transform)
+ *                                                                          object/volumetric.rs:42-72
+ * `transform` selects the object's ObjectTransform (object/transform.rs): 0 is the identity, k > 0 the matrix
+ * transforms[k - 1] of the scene, mapping object space to world space.  The engine's corrected constructors
+ * (SimpleObject::new, ObjectTransform::new_corrected) rotate about the mesh's centre; a caller passes the
+ * corrected matrix, translate(-centre) then transform then translate(centre).  Non-identity transforms are
+ * supported on sphere, triangle, parallelogram and axis-box meshes (simple and volumetric objects); on
+ * triangle lists and SDFs they return RAYNA_UNSUPPORTED.
  */
 enum {
     RAYNA_OBJ_SIMPLE = 0,
@@ -127,7 +134,7 @@ typedef struct rayna_object {
     int32_t kind;
     int32_t mesh;
     int32_t material;
-    int32_t reserved;
+    int32_t transform;  /* 0: identity; k: transforms[k - 1] */
     double density;
 } rayna_object;
 
@@ -140,7 +147,8 @@ enum {
 
 /* The scene: the object list in insertion order (the order `ObjectList::new_uncorrected` sees), and
  * flat parameter arrays.  spheres [n][4] = centre xyz, radius; tri_vertices and tri_normals [n][3][3];
- * parallelograms [n][9] = p, u, v; boxes [n][6] = corner a, corner b. */
+ * parallelograms [n][9] = p, u, v; boxes [n][6] = corner a, corner b; transforms [n][12] = the affine matrix
+ * row by row, [m00 m01 m02 t0; m10 m11 m12 t1; m20 m21 m22 t2] (point p maps to M p + t). */
 typedef struct rayna_scene_desc {
     const rayna_object *objects;
     int32_t n_objects;
@@ -160,6 +168,8 @@ typedef struct rayna_scene_desc {
     const rayna_sdf *sdfs;
     int32_t n_sdfs;
     int32_t skybox;
+    const double *transforms;
+    int32_t n_transforms;
 } rayna_scene_desc;
 
 /* ---------------------------------------------------------------- camera (scene/camera.rs:10-38); angles in radians */

@@ -25,7 +25,13 @@
 //      interval computes (mesh/advanced/bvh.rs:77-91) -- no longer depends on the traversal order;
 //  (5) a volume's free-flight candidate is kept when it lies inside the volume, without clamping the
 //      exit to the current (order-dependent) shrunk interval end (object/volumetric.rs:112-123); the
-//      closest-hit selection then discards it exactly when the engine would, up to rounding at a tie.
+//      closest-hit selection then discards it exactly when the engine would, up to rounding at a tie;
+//  (6) an object with a non-identity transform tests its mesh against the object-space ray with the
+//      interval [start, +inf) and keeps the hit when its world distance is within the interval end,
+//      where the engine passes the world interval, end included, to the object-space test
+//      (object/simple.rs:107-117).  The two agree for rigid transforms up to rounding; the engine's form
+//      makes the closest hit depend on the traversal order under scaling.  The inverse map is computed by
+//      the adjugate in double, not by glam's 4x4 cofactor expansion (a difference of rounding only).
 #pragma once
 
 #include <algorithm>
@@ -379,6 +385,9 @@ struct Scene {
     std::vector<double> spheres, tri_v, tri_n, paras, boxes;
     std::vector<rayna_sdf> sdfs;
     int skybox = RAYNA_SKY_NONE;
+    // ObjectTransform per object: index into xf_fwd / xf_inv (12 doubles each, row-major 3x4), -1 identity
+    std::vector<int> obj_xf;
+    std::vector<double> xf_fwd, xf_inv;
 
     // object graph
     std::vector<int> bounded, unbounded;  // object indices, input order
@@ -452,6 +461,65 @@ inline bool mesh_aabb(const Scene &s, int mesh, Aabb *ref, Aabb *tight) {
     }
 }
 
+// ---- ObjectTransform (object/transform.rs) ----
+// Affine maps are row-major 3x4 [m | t], p -> m p + t; points and vectors map in the order of glam's
+// transform_point3 / transform_vector3, ((c0 x + c1 y) + c2 z) + w.
+
+inline bool affine_is_identity(const double *m) {  // transform == Transform3::IDENTITY
+    static const double I[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    for (int i = 0; i < 12; ++i)
+        if (!(m[i] == I[i])) return false;
+    return true;
+}
+
+// Transform3::inverse of an affine map: adjugate of the linear part over its determinant (cofactor expansion
+// along the first row), translation -(inverse linear part) t.  False if singular or not finite.
+inline bool affine_inverse(const double *m, double *inv) {
+    const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+    const double A = e * i - f * h, B = f * g - d * i, Cc = d * h - e * g;
+    const double det = (a * A + b * B) + c * Cc;
+    if (!(det != 0.0) || !std::isfinite(det)) return false;
+    const double l[9] = {A / det, (c * h - b * i) / det, (b * f - c * e) / det,
+                         B / det, (a * i - c * g) / det, (c * d - a * f) / det,
+                         Cc / det, (b * g - a * h) / det, (a * e - b * d) / det};
+    for (int r = 0; r < 3; ++r) {
+        inv[4 * r] = l[3 * r];
+        inv[4 * r + 1] = l[3 * r + 1];
+        inv[4 * r + 2] = l[3 * r + 2];
+        inv[4 * r + 3] = -((l[3 * r] * m[3] + l[3 * r + 1] * m[7]) + l[3 * r + 2] * m[11]);
+    }
+    for (int k = 0; k < 12; ++k)
+        if (!std::isfinite(inv[k])) return false;
+    return true;
+}
+
+// The map in precision R: the double entries rounded to R, then glam's order of operations.
+template <class R>
+inline V3<R> affine_point(const double *m, const V3<R> &p) {
+    auto row = [&](int r) {
+        return (((R)m[4 * r] * p.x + (R)m[4 * r + 1] * p.y) + (R)m[4 * r + 2] * p.z) + (R)m[4 * r + 3];
+    };
+    return V3<R>{row(0), row(1), row(2)};
+}
+template <class R>
+inline V3<R> affine_vector(const double *m, const V3<R> &v) {
+    auto row = [&](int r) { return ((R)m[4 * r] * v.x + (R)m[4 * r + 1] * v.y) + (R)m[4 * r + 2] * v.z; };
+    return V3<R>{row(0), row(1), row(2)};
+}
+
+// ObjectTransform::calculate_aabb: the box around the eight mapped corners (Aabb::corners, encompass_points).
+inline Aabb affine_aabb(const double *m, const Aabb &b) {
+    double pts[8][3];
+    for (int k = 0; k < 8; ++k) {
+        const V3<double> c{(k & 1) ? b.mx[0] : b.mn[0], (k & 2) ? b.mx[1] : b.mn[1], (k & 4) ? b.mx[2] : b.mn[2]};
+        const V3<double> w = affine_point<double>(m, c);
+        pts[k][0] = w.x;
+        pts[k][1] = w.y;
+        pts[k][2] = w.z;
+    }
+    return encompass_points(pts, 8);
+}
+
 // Build the object graph: bounded/unbounded split, TLAS and BLASes, depth-first ranks, node boxes.
 inline void build_scene(Scene &s, const rayna_scene_desc &d, bool literal) {
     s.objects.assign(d.objects, d.objects + d.n_objects);
@@ -466,12 +534,27 @@ inline void build_scene(Scene &s, const rayna_scene_desc &d, bool literal) {
     s.skybox = d.skybox;
 
     const int no = d.n_objects;
+    s.obj_xf.assign(no, -1);
+    for (int o = 0; o < no; ++o) {
+        const int k = s.objects[o].transform;
+        if (k <= 0 || k > d.n_transforms) continue;
+        const double *m = d.transforms + 12 * (size_t)(k - 1);
+        double inv[12];
+        if (affine_is_identity(m) || !affine_inverse(m, inv)) continue;
+        s.obj_xf[o] = (int)(s.xf_fwd.size() / 12);
+        s.xf_fwd.insert(s.xf_fwd.end(), m, m + 12);
+        s.xf_inv.insert(s.xf_inv.end(), inv, inv + 12);
+    }
     s.obj_ref_box.assign(no, aabb_origin());
     s.obj_tight_box.assign(no, aabb_origin());
     s.obj_bounded.assign(no, 0);
     for (int o = 0; o < no; ++o) {
         Aabb r, t;
         if (mesh_aabb(s, s.objects[o].mesh, &r, &t)) {
+            if (s.obj_xf[o] >= 0) {
+                r = affine_aabb(&s.xf_fwd[12 * (size_t)s.obj_xf[o]], r);
+                t = affine_aabb(&s.xf_fwd[12 * (size_t)s.obj_xf[o]], t);
+            }
             s.obj_ref_box[o] = r;
             s.obj_tight_box[o] = t;
             s.obj_bounded[o] = 1;
@@ -918,10 +1001,10 @@ inline bool blas_node(const Ctx<R> &c, int bi, int node, int first_tri, int rank
     return found;
 }
 
-// Object::full_intersect of one object of the list (SimpleObject simple.rs:215-225,
+// Object::full_intersect of one object of the list in its own space (SimpleObject simple.rs:215-225,
 // VolumetricObject volumetric.rs:86-160).
 template <class R>
-inline bool object_intersect(const Ctx<R> &c, int o, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
+inline bool object_intersect_local(const Ctx<R> &c, int o, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
     const Scene &s = *c.s;
     const rayna_object &obj = s.objects[o];
     const rayna_mesh &m = s.meshes[obj.mesh];
@@ -984,6 +1067,26 @@ inline bool object_intersect(const Ctx<R> &c, int o, const Ray<R> &ray, const In
         h.material = obj.material;
     }
     return hit;
+}
+
+// Object::full_intersect with the object's transform (object/simple.rs:107-117, object/volumetric.rs:92,158):
+// incoming_ray maps the ray to object space (Ray::new normalises the direction), outgoing_intersection maps the
+// position out, the normals by the linear part (normalised), and measures the distance from the world ray's
+// origin (object/transform.rs).  Interval handling: deviation (6).
+template <class R>
+inline bool object_intersect(const Ctx<R> &c, int o, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
+    const Scene &s = *c.s;
+    const int xi = s.obj_xf[o];
+    if (xi < 0) return object_intersect_local(c, o, ray, iv, h);
+    const double *fwd = &s.xf_fwd[12 * (size_t)xi], *inv = &s.xf_inv[12 * (size_t)xi];
+    const Ray<R> local = Ray<R>::make(affine_point<R>(inv, ray.pos), affine_vector<R>(inv, ray.dir));
+    const Interval<R> open{iv.has_start, false, iv.start, R(0)};
+    if (!object_intersect_local(c, o, local, open, h)) return false;
+    h.normal = normalize(affine_vector<R>(fwd, h.normal));
+    h.ray_normal = normalize(affine_vector<R>(fwd, h.ray_normal));
+    h.pos = affine_point<R>(fwd, h.pos);
+    h.t = length(h.pos - ray.pos);
+    return !iv.has_end || h.t <= iv.end;
 }
 
 // BvhObject::bvh_node_intersect (object/bvh.rs:443-481)

@@ -35,7 +35,7 @@ class Sdf(C.Structure):
 
 
 class Object(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("mesh", C.c_int32), ("material", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("kind", C.c_int32), ("mesh", C.c_int32), ("material", C.c_int32), ("transform", C.c_int32),
                 ("density", C.c_double)]
 
 
@@ -49,7 +49,8 @@ class SceneDesc(C.Structure):
                 ("parallelograms", C.POINTER(C.c_double)), ("n_parallelograms", C.c_int32),
                 ("boxes", C.POINTER(C.c_double)), ("n_boxes", C.c_int32),
                 ("sdfs", C.POINTER(Sdf)), ("n_sdfs", C.c_int32),
-                ("skybox", C.c_int32)]
+                ("skybox", C.c_int32),
+                ("transforms", C.POINTER(C.c_double)), ("n_transforms", C.c_int32)]
 
 
 class CameraDesc(C.Structure):

@@ -426,6 +426,7 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     UP(volumes, volumes)
     UP(sdfs, sdfs)
     UP(materials, materials)
+    UP(xforms, xforms)
 #undef UP
     for (const SceneSlot &sl : b->slots) b->n_sdf_max = std::max<uint32_t>(b->n_sdf_max, (uint32_t)sl.n_sdfs);
     clamp_chunk(b);

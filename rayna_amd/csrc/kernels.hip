@@ -226,6 +226,24 @@ __device__ __forceinline__ bool volume_t(const SceneView &sv, const GVolume &v, 
     return true;
 }
 
+// ------------------------------------------------------------------------------------------ object transforms
+
+// Affine maps (row-major 3x4, p -> m p + t) in the order of glam's transform_point3 / transform_vector3.
+__device__ __forceinline__ f3 affine_point(const float *m, f3 p) {
+    return f3{((m[0] * p.x + m[1] * p.y) + m[2] * p.z) + m[3], ((m[4] * p.x + m[5] * p.y) + m[6] * p.z) + m[7],
+              ((m[8] * p.x + m[9] * p.y) + m[10] * p.z) + m[11]};
+}
+__device__ __forceinline__ f3 affine_vector(const float *m, f3 v) {
+    return f3{(m[0] * v.x + m[1] * v.y) + m[2] * v.z, (m[4] * v.x + m[5] * v.y) + m[6] * v.z,
+              (m[8] * v.x + m[9] * v.y) + m[10] * v.z};
+}
+
+// ObjectTransform::incoming_ray (object/transform.rs): the world ray mapped to object space, its direction
+// normalised again (Ray::new).
+__device__ __forceinline__ DRay object_ray(const GXform *x, const DRay &r) {
+    return make_ray(affine_point(x->inv, r.o), affine_vector(x->inv, r.d));
+}
+
 // ------------------------------------------------------------------------------------------ SDFs
 
 __device__ __forceinline__ float ipow(float x, int n) {  // binary powering in compiler-rt's __powisf2 order
@@ -312,53 +330,84 @@ __device__ __forceinline__ f3 unit_sphere(float r1, float r2) {  // rng::normal_
     return f3{(cs * 2.0f) * q, (sn * 2.0f) * q, 1.0f - 2.0f * r2};
 }
 
-// Re-derives the full intersection record of a hit (the extend kernel keeps only distance and reference).
+// The primitive a hit reference names: its kind and index into that kind's array (the inner primitive or volume
+// of a transformed object), its material, and the ray and distance to evaluate it with (the object-space ray and
+// distance of a transformed object).  x: the transform, or null.
+struct HitPrim {
+    int kind;
+    int32_t prim;
+    int32_t material;
+    DRay ray;
+    float t;
+    const GXform *x;
+};
+
+__device__ __forceinline__ HitPrim hit_prim(const SceneView &sv, const DRay &ray, float t, uint32_t ref,
+                                            uint32_t aux) {
+    HitPrim p{(int)(ref >> HIT_KIND_SHIFT), (int32_t)(ref & HIT_INDEX_MASK), 0, ray, t, nullptr};
+    if (p.kind == ITEM_TRIANGLE) {
+        p.material = sv.tris[p.prim].material;
+    } else if (p.kind == ITEM_SDF) {
+        p.material = sv.sdfs[p.prim].material;
+    } else {
+        const GItem it = sv.items[p.prim];
+        p.material = it.material;
+        p.prim = it.index;
+        if (p.kind == ITEM_XFORM) {
+            p.x = sv.xforms + it.index;
+            p.kind = p.x->inner_kind;
+            p.prim = p.x->inner_index;
+            p.ray = object_ray(p.x, ray);
+            p.t = __uint_as_float(aux);
+        }
+    }
+    return p;
+}
+
+// Re-derives the full intersection record of a hit (the extend kernel keeps only distance and reference).  A
+// transformed object's record is derived in object space and mapped out as outgoing_intersection does
+// (object/transform.rs): the position by the object-to-world map, both normals by its linear part, normalised.
 __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint32_t ref, uint32_t aux,
                              const dm::Key &key, uint32_t pixel, uint32_t sample, uint32_t bounce, SHit *h) {
-    const int kind = (int)(ref >> HIT_KIND_SHIFT);
-    const int32_t idx = (int32_t)(ref & HIT_INDEX_MASK);
-    h->pos = at(ray, t);
-    switch (kind) {
+    const HitPrim p = hit_prim(sv, ray, t, ref, aux);
+    const DRay &r = p.ray;
+    h->pos = at(r, p.t);
+    h->material = p.material;
+    switch (p.kind) {
         case ITEM_TRIANGLE: {
-            const GTri g = sv.tris[idx];
+            const GTri g = sv.tris[p.prim];
             float tt, u, v, det;
-            tri_t(g, ray, t, t, &tt, &u, &v, &det);
+            tri_t(g, r, p.t, p.t, &tt, &u, &v, &det);
             f3 n{0.0f, 0.0f, 0.0f};
-            tri_normal(sv.tri_normals[idx], u, v, &n);
+            tri_normal(sv.tri_normals[p.prim], u, v, &n);
             h->normal = n;
             h->front = signbit(det);
             h->ray_normal = n * -dm::signum(det);
-            h->material = g.material;
             break;
         }
         case ITEM_SPHERE: {
-            const GItem it = sv.items[idx];
-            const GSphere s = sv.spheres[it.index];
+            const GSphere s = sv.spheres[p.prim];
             const f3 local = (h->pos - f3{s.cx, s.cy, s.cz}) / s.radius;
-            const bool inside = dot(ray.d, local) > 0.0f;
+            const bool inside = dot(r.d, local) > 0.0f;
             h->normal = local;
             h->ray_normal = inside ? neg(local) : local;
             h->front = !inside;
-            h->material = it.material;
             break;
         }
         case ITEM_PARALLELOGRAM: {
-            const GItem it = sv.items[idx];
-            const GPara g = sv.paras[it.index];
+            const GPara g = sv.paras[p.prim];
             const f3 n{g.nx, g.ny, g.nz};
-            const float denom = dot(n, ray.d);
+            const float denom = dot(n, r.d);
             h->normal = n;
             h->front = signbit(denom);
             h->ray_normal = neg(n) * dm::signum(denom);
-            h->material = it.material;
             break;
         }
         case ITEM_AXIS_BOX: {
-            const GItem it = sv.items[idx];
             float tt, w;
             int axis = 0;
             f3 sg{0.0f, 0.0f, 0.0f};
-            box_t(sv.boxes[it.index], ray, t, t, &tt, &axis, &w, &sg);
+            box_t(sv.boxes[p.prim], r, p.t, p.t, &tt, &axis, &w, &sg);
             f3 rn{0.0f, 0.0f, 0.0f};
             if (axis == 0) rn.x = sg.x;
             if (axis == 1) rn.y = sg.y;
@@ -366,28 +415,28 @@ __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint
             h->ray_normal = rn;
             h->normal = rn * w;
             h->front = !signbit(w);
-            h->material = it.material;
             break;
         }
         case ITEM_VOLUME: {
-            const GItem it = sv.items[idx];
             float a[4];
             dm::draw4(key, pixel, sample, bounce, dm::SLOT_MEDIUM_AUX, a);
             h->normal = unit_sphere(a[0], a[1]);
             h->ray_normal = unit_sphere(a[2], a[3]);
             h->front = true;
-            h->material = it.material;
             break;
         }
         default: {  // ITEM_SDF
-            const GSdf f = sv.sdfs[idx];
-            const f3 n = sdf_normal(sv, f, h->pos);
+            const f3 n = sdf_normal(sv, sv.sdfs[p.prim], h->pos);
             h->normal = n;
             h->ray_normal = n;
             h->front = !(aux & HIT_AUX_BACK);
-            h->material = f.material;
             break;
         }
+    }
+    if (p.x) {
+        h->normal = normalize(affine_vector(p.x->fwd, h->normal));
+        h->ray_normal = normalize(affine_vector(p.x->fwd, h->ray_normal));
+        h->pos = affine_point(p.x->fwd, h->pos);
     }
 }
 
@@ -395,22 +444,24 @@ __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint
 // (sphere.rs:114-121), the triangle's barycentrics (triangle.rs:104), the plane coordinates (planar/mod.rs:191),
 // the box face's uv and side (axis_box.rs:121-129: side = 2 axis + (sgn > 0), `-1.0 as u32` saturating to 0),
 // the volume's random uv (volumetric.rs:153) and the SDF's march iteration (raymarched.rs:106-109, carried in the
-// hit flags from bit 8).
-__device__ void hit_uv_side(const SceneView &sv, const DRay &ray, float t, uint32_t ref, uint32_t aux,
+// hit flags from bit 8).  A transform leaves both as the object-space test found them.
+__device__ void hit_uv_side(const SceneView &sv, const DRay &world_ray, float world_t, uint32_t ref, uint32_t aux,
                             const dm::Key &key, uint32_t pixel, uint32_t sample, uint32_t bounce, float uv[2],
                             int *side) {
-    const int kind = (int)(ref >> HIT_KIND_SHIFT);
-    const int32_t idx = (int32_t)(ref & HIT_INDEX_MASK);
+    const HitPrim p = hit_prim(sv, world_ray, world_t, ref, aux);
+    const DRay &ray = p.ray;
+    const float t = p.t;
+    const int32_t idx = p.prim;
     uv[0] = uv[1] = 0.0f;
     *side = 0;
-    switch (kind) {
+    switch (p.kind) {
         case ITEM_TRIANGLE: {
             float tt, det;
             tri_t(sv.tris[idx], ray, t, t, &tt, &uv[0], &uv[1], &det);
             break;
         }
         case ITEM_SPHERE: {
-            const GSphere s = sv.spheres[sv.items[idx].index];
+            const GSphere s = sv.spheres[idx];
             const f3 local = (at(ray, t) - f3{s.cx, s.cy, s.cz}) / s.radius;
             const float PI = 3.14159274101257324f;
             const float theta = dm::acos_f(-local.y);
@@ -421,14 +472,14 @@ __device__ void hit_uv_side(const SceneView &sv, const DRay &ray, float t, uint3
         }
         case ITEM_PARALLELOGRAM: {
             float tt;
-            para_t(sv.paras[sv.items[idx].index], ray, t, t, &tt, uv);
+            para_t(sv.paras[idx], ray, t, t, &tt, uv);
             break;
         }
         case ITEM_AXIS_BOX: {
             float tt, w;
             int axis = 0;
             f3 sg{0.0f, 0.0f, 0.0f};
-            box_t(sv.boxes[sv.items[idx].index], ray, t, t, &tt, &axis, &w, &sg, uv);
+            box_t(sv.boxes[idx], ray, t, t, &tt, &axis, &w, &sg, uv);
             const float sa = axis == 0 ? sg.x : axis == 1 ? sg.y : sg.z;
             *side = 2 * axis + (sa > 0.0f ? 1 : 0);
             break;
@@ -609,11 +660,12 @@ __device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy,
 // colour-independent factor (lambertian, metal, isotropic, dielectric leaving the medium) adds d*R/albedo to
 // its albedo gradient (nothing for a channel whose albedo is exactly 0).  The colours never change a sampling
 // decision, so the replay follows the primal path exactly.
+//
+// Register budget: 96 VGPRs, so that five waves share each SIMD (the kernel is latency-bound).
 template <bool ADJOINT, bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_shade(SceneView sv, FrameParams fp, PathBuffers pb,
-                                                 const int32_t *queue_in, const uint32_t *count_in,
-                                                 int32_t *queue_out, uint32_t *count_out, uint32_t depth,
-                                                 AdjointParams ap, BatchView bv) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_shade(
+    SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue_in, const uint32_t *count_in,
+    int32_t *queue_out, uint32_t *count_out, uint32_t depth, AdjointParams ap, BatchView bv) {
     const uint32_t count = queue_in ? *count_in : fp.n_slots;
     const uint32_t lane = __lane_id();
     const uint32_t copy = ADJOINT ? blockIdx.x % ap.copies : 0u;

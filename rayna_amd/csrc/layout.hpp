@@ -60,7 +60,8 @@ enum : int32_t {
     ITEM_AXIS_BOX = 4,      // index: box array
     ITEM_VOLUME = 5,        // index: volume array
     ITEM_LINK = 6,          // index: GNode (the mesh BVH of a triangle-list object)
-    ITEM_SDF = 7            // hit references only: index = unbounded SDF object
+    ITEM_SDF = 7,           // hit references only: index = unbounded SDF object
+    ITEM_XFORM = 8          // index: transformed-object array (an object with a non-identity ObjectTransform)
 };
 
 struct alignas(16) GItem {
@@ -115,6 +116,17 @@ struct alignas(16) GSdf {
     float ex, ey, ez, escape_r;
 };
 
+// An object with a non-identity ObjectTransform (object/transform.rs): its primitive or volume in object space,
+// and the affine maps both ways, row-major 3x4 ([m | t]: p -> m p + t), computed in double and rounded.
+struct alignas(16) GXform {
+    int32_t inner_kind;   // ITEM_SPHERE, ITEM_TRIANGLE, ITEM_PARALLELOGRAM, ITEM_AXIS_BOX or ITEM_VOLUME
+    int32_t inner_index;  // into that kind's array
+    int32_t pad0, pad1;
+    float fwd[12];        // object -> world
+    float inv[12];        // world -> object
+};
+static_assert(sizeof(GXform) == 112, "GXform must be 112 bytes");
+
 // Material (material/*.rs with SolidTexture), parameters rounded to float.
 struct alignas(16) GMaterial {
     int32_t kind;
@@ -125,7 +137,8 @@ struct alignas(16) GMaterial {
 // Hit record written by the extend kernel: distance, packed reference (kind << 28 | index), rank, aux bits.
 enum : uint32_t { HIT_KIND_SHIFT = 28, HIT_INDEX_MASK = 0x0fffffffu };
 // Hit flags of SDF hits: bit 0, the final distance sample was negative (front_face false); from bit 8, the march
-// iteration of the hit (Intersection::side).
+// iteration of the hit (Intersection::side).  For an ITEM_XFORM hit the aux word holds the bits of the float
+// distance along the object-space ray.
 enum : uint32_t { HIT_AUX_BACK = 1u, HIT_AUX_SIDE_SHIFT = 8u };
 
 // Camera constants in float, from Camera::calculate_viewport evaluated in double (camera.rs:285-338).
@@ -145,6 +158,7 @@ struct SceneView {
     const GVolume *volumes;
     const GSdf *sdfs;
     const GMaterial *materials;
+    const GXform *xforms;
     int32_t root;        // -1: no bounded object
     int32_t n_sdfs;
     int32_t skybox;

@@ -134,6 +134,7 @@ rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
     UP(volumes, volumes)
     UP(sdfs, sdfs)
     UP(materials, materials)
+    UP(xforms, xforms)
 #undef UP
     v.root = ps.root;
     v.n_sdfs = (int32_t)ps.sdfs.size();

@@ -60,6 +60,52 @@ Aabb64 min_padded(const Aabb64 &b, double thresh) {  // Aabb::min_padded (aabb.r
     return box_new(mn, mx);
 }
 
+// ---- ObjectTransform (object/transform.rs) ----
+// An affine map is row-major 3x4: [m00 m01 m02 t0; m10 m11 m12 t1; m20 m21 m22 t2], p -> m p + t.  Points and
+// vectors are mapped in the order of glam's transform_point3 / transform_vector3: ((c0 x + c1 y) + c2 z) + w.
+
+bool affine_is_identity(const double *m) {  // ObjectTransform::is_identity (transform == Transform3::IDENTITY)
+    static const double I[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    for (int i = 0; i < 12; ++i)
+        if (!(m[i] == I[i])) return false;
+    return true;
+}
+
+// The inverse of an affine map: the linear part by its adjugate over the determinant, the translation as
+// -(inverse linear part) t.  False when the map is singular or the inverse is not finite.
+bool affine_inverse(const double *m, double *inv) {
+    const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+    const double A = e * i - f * h, B = f * g - d * i, Cc = d * h - e * g;
+    const double det = (a * A + b * B) + c * Cc;
+    if (!(det != 0.0) || !std::isfinite(det)) return false;
+    const double l[9] = {A / det, (c * h - b * i) / det, (b * f - c * e) / det,
+                         B / det, (a * i - c * g) / det, (c * d - a * f) / det,
+                         Cc / det, (b * g - a * h) / det, (a * e - b * d) / det};
+    for (int r = 0; r < 3; ++r) {
+        inv[4 * r] = l[3 * r];
+        inv[4 * r + 1] = l[3 * r + 1];
+        inv[4 * r + 2] = l[3 * r + 2];
+        inv[4 * r + 3] = -((l[3 * r] * m[3] + l[3 * r + 1] * m[7]) + l[3 * r + 2] * m[11]);
+    }
+    for (int k = 0; k < 12; ++k)
+        if (!std::isfinite(inv[k])) return false;
+    return true;
+}
+
+void affine_point(const double *m, const double *p, double *out) {
+    for (int r = 0; r < 3; ++r) out[r] = ((m[4 * r] * p[0] + m[4 * r + 1] * p[1]) + m[4 * r + 2] * p[2]) + m[4 * r + 3];
+}
+
+// ObjectTransform::calculate_aabb: the box around the eight mapped corners of the object-space box.
+Aabb64 affine_box(const double *m, const Aabb64 &b) {
+    double pts[24];
+    for (int k = 0; k < 8; ++k) {
+        const double c[3] = {(k & 1) ? b.mx[0] : b.mn[0], (k & 2) ? b.mx[1] : b.mn[1], (k & 4) ? b.mx[2] : b.mn[2]};
+        affine_point(m, c, pts + 3 * k);
+    }
+    return points_box(pts, 8);
+}
+
 float round_down(double x) {
     float f = (float)x;
     if ((double)f > x) f = std::nextafter(f, -INFINITY);
@@ -216,6 +262,8 @@ struct Packer {
     std::vector<Aabb64> tlas_tight;
     std::vector<int32_t> bounded;           // tlas item -> object
     std::vector<Aabb64> obj_tight;
+    std::vector<int32_t> obj_xf;            // object -> index into xf_fwd / xf_inv (12 doubles each), -1: identity
+    std::vector<double> xf_fwd, xf_inv;
     int32_t rank = 0;
 
     Packer(const rayna_scene_desc &dd, PackedScene &pp) : d(dd), P(pp) {}
@@ -419,8 +467,31 @@ struct Packer {
         return emit(t.root, wrap);
     }
 
+    // A transformed object's item: the primitive or volume item `it` wrapped with its maps.
+    GItem wrap_transform(int32_t o, GItem it) {
+        const int32_t x = obj_xf[o];
+        if (x < 0) return it;
+        GXform g;
+        g.inner_kind = it.kind;
+        g.inner_index = it.index;
+        g.pad0 = g.pad1 = 0;
+        for (int k = 0; k < 12; ++k) {
+            g.fwd[k] = (float)xf_fwd[12 * (size_t)x + k];
+            g.inv[k] = (float)xf_inv[12 * (size_t)x + k];
+        }
+        P.xforms.push_back(g);
+        it.kind = ITEM_XFORM;
+        it.index = (int32_t)P.xforms.size() - 1;
+        return it;
+    }
+
     // One object of the object BVH as a leaf item.
     GItem object_item(int32_t o, std::string &error, rayna_status &st) {
+        const GItem it = object_item_local(o, error, st);
+        return st == RAYNA_OK ? wrap_transform(o, it) : it;
+    }
+
+    GItem object_item_local(int32_t o, std::string &error, rayna_status &st) {
         const rayna_object &ob = d.objects[o];
         const rayna_mesh &m = d.meshes[ob.mesh];
         GItem it{ITEM_NONE, 0, 0, ob.material};
@@ -641,13 +712,14 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
     P = PackedScene();
     // ---- validation ----
     if (d.n_objects < 0 || d.n_meshes < 0 || d.n_materials < 0 || d.n_spheres < 0 || d.n_triangles < 0 ||
-        d.n_parallelograms < 0 || d.n_boxes < 0 || d.n_sdfs < 0) {
+        d.n_parallelograms < 0 || d.n_boxes < 0 || d.n_sdfs < 0 || d.n_transforms < 0) {
         error = "negative array size in scene";
         return RAYNA_INVALID_ARGUMENT;
     }
     if ((d.n_objects && !d.objects) || (d.n_meshes && !d.meshes) || (d.n_materials && !d.materials) ||
         (d.n_spheres && !d.spheres) || (d.n_triangles && (!d.tri_vertices || !d.tri_normals)) ||
-        (d.n_parallelograms && !d.parallelograms) || (d.n_boxes && !d.boxes) || (d.n_sdfs && !d.sdfs)) {
+        (d.n_parallelograms && !d.parallelograms) || (d.n_boxes && !d.boxes) || (d.n_sdfs && !d.sdfs) ||
+        (d.n_transforms && !d.transforms)) {
         error = "null array in scene";
         return RAYNA_INVALID_ARGUMENT;
     }
@@ -657,7 +729,8 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
     }
     if (!finite_all(d.spheres, 4 * (size_t)d.n_spheres) || !finite_all(d.tri_vertices, 9 * (size_t)d.n_triangles) ||
         !finite_all(d.tri_normals, 9 * (size_t)d.n_triangles) ||
-        !finite_all(d.parallelograms, 9 * (size_t)d.n_parallelograms) || !finite_all(d.boxes, 6 * (size_t)d.n_boxes)) {
+        !finite_all(d.parallelograms, 9 * (size_t)d.n_parallelograms) || !finite_all(d.boxes, 6 * (size_t)d.n_boxes) ||
+        !finite_all(d.transforms, 12 * (size_t)d.n_transforms)) {
         error = "non-finite geometry in scene";
         return RAYNA_INVALID_ARGUMENT;
     }
@@ -700,6 +773,10 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
             error = "object mesh or material index out of range";
             return RAYNA_INVALID_ARGUMENT;
         }
+        if (o.transform < 0 || o.transform > d.n_transforms) {
+            error = "object transform index out of range";
+            return RAYNA_INVALID_ARGUMENT;
+        }
     }
     for (int i = 0; i < d.n_sdfs; ++i) {
         const rayna_sdf &f = d.sdfs[i];
@@ -725,6 +802,28 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
     P.skybox = d.skybox;
 
     Packer K(d, P);
+    // ---- object transforms: identity matrices are the identity transform (transform.rs: is_identity) ----
+    K.obj_xf.assign(d.n_objects, -1);
+    for (int o = 0; o < d.n_objects; ++o) {
+        const rayna_object &ob = d.objects[o];
+        if (ob.transform == 0) continue;
+        const double *m = &d.transforms[12 * (size_t)(ob.transform - 1)];
+        if (affine_is_identity(m)) continue;
+        const int mk = d.meshes[ob.mesh].kind;
+        if (mk != RAYNA_MESH_SPHERE && mk != RAYNA_MESH_TRIANGLE && mk != RAYNA_MESH_PARALLELOGRAM &&
+            mk != RAYNA_MESH_AXIS_BOX) {
+            error = "object transforms are supported on sphere, triangle, parallelogram and axis-box meshes";
+            return RAYNA_UNSUPPORTED;
+        }
+        double inv[12];
+        if (!affine_inverse(m, inv)) {
+            error = "object transform is not invertible";
+            return RAYNA_INVALID_ARGUMENT;
+        }
+        K.obj_xf[o] = (int32_t)(K.xf_fwd.size() / 12);
+        K.xf_fwd.insert(K.xf_fwd.end(), m, m + 12);
+        K.xf_inv.insert(K.xf_inv.end(), inv, inv + 12);
+    }
     // ---- object bounds (reference boxes carry the origin for lists: list.rs:176-181) ----
     std::vector<Aabb64> obj_ref(d.n_objects);
     K.obj_tight.resize(d.n_objects);
@@ -773,6 +872,11 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
                 break;
             }
             default: bnd = false; break;
+        }
+        if (bnd && K.obj_xf[o] >= 0) {  // ObjectTransform::calculate_aabb
+            const double *m = &K.xf_fwd[12 * (size_t)K.obj_xf[o]];
+            obj_ref[o] = affine_box(m, obj_ref[o]);
+            K.obj_tight[o] = affine_box(m, K.obj_tight[o]);
         }
         if (bnd) K.bounded.push_back(o);
         else unbounded.push_back(o);
@@ -856,7 +960,7 @@ bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vec
         const int32_t item_off = (int32_t)out.items.size(), sphere_off = (int32_t)out.spheres.size();
         const int32_t para_off = (int32_t)out.paras.size(), box_off = (int32_t)out.boxes.size();
         const int32_t vol_off = (int32_t)out.volumes.size(), sdf_off = (int32_t)out.sdfs.size();
-        const int32_t mat_off = (int32_t)out.materials.size();
+        const int32_t mat_off = (int32_t)out.materials.size(), xf_off = (int32_t)out.xforms.size();
         for (GNode g : p.nodes) {
             for (int k = 0; k < 4; ++k) {
                 if (g.info[k] == INFO_INNER) g.child[k] += node_off;
@@ -877,10 +981,21 @@ bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vec
                 case ITEM_AXIS_BOX: it.index += box_off; break;
                 case ITEM_VOLUME: it.index += vol_off; break;
                 case ITEM_LINK: it.index += node_off; break;
+                case ITEM_XFORM: it.index += xf_off; break;
                 default: break;
             }
             it.material += mat_off;
             out.items.push_back(it);
+        }
+        for (GXform x : p.xforms) {
+            switch (x.inner_kind) {
+                case ITEM_SPHERE: x.inner_index += sphere_off; break;
+                case ITEM_TRIANGLE: x.inner_index += tri_off; break;
+                case ITEM_PARALLELOGRAM: x.inner_index += para_off; break;
+                case ITEM_AXIS_BOX: x.inner_index += box_off; break;
+                default: x.inner_index += vol_off; break;  // ITEM_VOLUME
+            }
+            out.xforms.push_back(x);
         }
         out.spheres.insert(out.spheres.end(), p.spheres.begin(), p.spheres.end());
         out.paras.insert(out.paras.end(), p.paras.begin(), p.paras.end());

@@ -46,6 +46,7 @@ struct PackedScene {
     std::vector<GVolume> volumes;
     std::vector<GSdf> sdfs;
     std::vector<GMaterial> materials;
+    std::vector<GXform> xforms;
     int32_t root = -1;
     int32_t stack_need = 0;
     int32_t skybox = RAYNA_SKY_NONE;

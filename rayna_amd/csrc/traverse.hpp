@@ -145,57 +145,73 @@ __device__ __forceinline__ void leaf_test(const SceneView &sv, const DRay &ray, 
         return;
     }
     const GItem it = sv.items[start + i];
-    float t;
+    if (it.kind == ITEM_LINK) {
+        st.push(it.index, leaf_t);
+        return;
+    }
+    // An object with a non-identity transform (SimpleObject / VolumetricObject::full_intersect around
+    // ObjectTransform, object/transform.rs): its primitive or volume is tested against the object-space ray with
+    // the interval [T_MIN, +inf), and the hit's distance is that of its mapped position from the world ray's
+    // origin (outgoing_intersection).  Testing the object-space distance against the open interval, instead of
+    // against the current closest world distance, keeps the closest hit independent of the order objects are
+    // tested in.  The hit keeps the object-space distance in its aux word.
+    int kind = it.kind;
+    int32_t j = it.index;
+    DRay r = ray;
+    float t_end = b.t;
+    const GXform *x = nullptr;
+    if (kind == ITEM_XFORM) {
+        x = sv.xforms + j;
+        kind = x->inner_kind;
+        j = x->inner_index;
+        r = object_ray(x, ray);
+        t_end = __builtin_inff();
+    }
+    float t = 0.0f;
     bool ok = false;
-    uint32_t ref = 0;
-    switch (it.kind) {
-        case ITEM_SPHERE:
-            ok = sphere_t(sv.spheres[it.index], ray, T_MIN, b.t, &t);
-            ref = make_ref(ITEM_SPHERE, start + i);
-            break;
+    switch (kind) {
+        case ITEM_SPHERE: ok = sphere_t(sv.spheres[j], r, T_MIN, t_end, &t); break;
         case ITEM_TRIANGLE: {
-            const GTri g = sv.tris[it.index];
+            const GTri g = sv.tris[j];
             float u, v, det;
-            ok = tri_t(g, ray, T_MIN, b.t, &t, &u, &v, &det);
+            ok = tri_t(g, r, T_MIN, t_end, &t, &u, &v, &det);
             if (ok && (g.flags & 1)) {
                 f3 nn;
-                ok = tri_normal(sv.tri_normals[it.index], u, v, &nn);
+                ok = tri_normal(sv.tri_normals[j], u, v, &nn);
             }
-            ref = make_ref(ITEM_TRIANGLE, it.index);
             break;
         }
-        case ITEM_PARALLELOGRAM:
-            ok = para_t(sv.paras[it.index], ray, T_MIN, b.t, &t);
-            ref = make_ref(ITEM_PARALLELOGRAM, start + i);
-            break;
+        case ITEM_PARALLELOGRAM: ok = para_t(sv.paras[j], r, T_MIN, t_end, &t); break;
         case ITEM_AXIS_BOX: {
             int axis;
             float w;
             f3 sg;
-            ok = box_t(sv.boxes[it.index], ray, T_MIN, b.t, &t, &axis, &w, &sg);
-            ref = make_ref(ITEM_AXIS_BOX, start + i);
+            ok = box_t(sv.boxes[j], r, T_MIN, t_end, &t, &axis, &w, &sg);
             break;
         }
-        case ITEM_VOLUME:
-            ok = volume_t(sv, sv.volumes[it.index], ray, b.t, key, pixel, sample, bounce, &t);
-            ref = make_ref(ITEM_VOLUME, start + i);
-            break;
-        case ITEM_LINK:
-            st.push(it.index, leaf_t);
-            break;
+        case ITEM_VOLUME: ok = volume_t(sv, sv.volumes[j], r, t_end, key, pixel, sample, bounce, &t); break;
         default: break;
     }
-    if (ok && (t < b.t || it.rank > b.rank)) b = Best{t, it.rank, ref, 0u};
+    if (!ok) return;
+    uint32_t aux = 0u;
+    if (x) {
+        aux = __float_as_uint(t);
+        t = length(affine_point(x->fwd, at(r, t)) - ray.o);
+        if (!(t <= b.t)) return;
+    }
+    if (t < b.t || it.rank > b.rank)
+        b = Best{t, it.rank,
+                 it.kind == ITEM_TRIANGLE ? make_ref(ITEM_TRIANGLE, it.index) : make_ref(it.kind, start + i), aux};
 }
 
 // Closest bounded hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501); the
 // unbounded objects are marched afterwards by k_march.  BATCH: the slots belong to the requests of bv, each
-// with its own scene root and RNG key.
+// with its own scene root and RNG key.  Register budget: 96 VGPRs, the most that keeps the five waves per SIMD
+// the LDS stack allows.
 template <bool BATCH>
-__global__ __launch_bounds__(EXT_BLOCK) void k_extend(SceneView sv, FrameParams fp, PathBuffers pb,
-                                                      const int32_t *queue, const uint32_t *count_ptr,
-                                                      uint32_t depth, int2 *overflow, uint32_t *work,
-                                                      ExtendTuning tune, BatchView bv) {
+__global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_extend(
+    SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue, const uint32_t *count_ptr, uint32_t depth,
+    int2 *overflow, uint32_t *work, ExtendTuning tune, BatchView bv) {
     __shared__ int32_t s_node[EXT_LDS_STACK * EXT_BLOCK];
     __shared__ float s_tent[EXT_LDS_STACK * EXT_BLOCK];
     const uint32_t count = queue ? *count_ptr : fp.n_slots;

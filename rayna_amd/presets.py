@@ -9,6 +9,9 @@ Each function returns (Scene, Camera, RenderOpts) for one workload:
   sdf_volume       a power-8 Mandelbulb and 16 smooth-min spheres inside a 4x4x4 constant-density isotropic
                    medium; 1280x720, 128 spp, depth 12, SimpleSkybox
   sphere_colours   the scene of the engine's one integration test (rayna_engine/tests/simple_renders.rs:35-53)
+  cornell          the engine's CORNELL preset (preset.rs:606-683): five walls, a light and two boxes rotated
+                   about their centres; NoSkybox
+  transformed_mix  objects of every transformable kind under rotations, scalings and shears (parity workload)
 All randomness comes from numpy's PCG64 seeded with `seed`, so every scene is a pure function of its arguments.
 """
 from __future__ import annotations
@@ -19,8 +22,8 @@ import numpy as np
 
 from .scene import (AxisBoxMesh, Camera, DielectricMaterial, IsotropicMaterial, LambertianMaterial, LightMaterial,
                     MetalMaterial, NoSkybox, ParallelogramMesh, RaymarchedIsosurfaceMesh, RenderOpts, Scene,
-                    SdfMandelbulb, SdfSminSpheres, SimpleObject, SimpleSkybox, SphereMesh, TriangleListMesh,
-                    TriangleMesh, VolumetricObject, WhiteSkybox)
+                    SdfMandelbulb, SdfSminSpheres, SimpleObject, SimpleSkybox, SphereMesh, Transform3,
+                    TriangleListMesh, TriangleMesh, VolumetricObject, WhiteSkybox)
 
 # ------------------------------------------------------------------------------------------ random spheres
 
@@ -224,6 +227,72 @@ def sphere_colours(albedo, cam_z: float, width: int = 320, height: int = 320, sa
     scene = Scene([SimpleObject(SphereMesh((0.0, 0.0, 0.0), 1.0), LambertianMaterial(tuple(albedo)))], WhiteSkybox)
     cam = Camera(pos=(0.0, 0.0, cam_z), fwd=(0.0, 0.0, 1.0), v_fov=45.0, focus_dist=1.0, defocus_angle=0.0)
     return scene, cam, RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
+
+
+def cornell(width: int = 600, height: int = 600, samples: int = 64, ray_depth: int = 8, seed: int = 0):
+    """CORNELL (preset.rs:606-683): the walls as parallelograms (SimpleObject::new with no transform), the light
+    quad, and the two boxes under Transform3::from_axis_angle(Y, 15 deg / -18 deg), corrected about their centres."""
+    red, green, grey = (0.65, 0.05, 0.05), (0.12, 0.45, 0.15), (0.85, 0.74, 0.55)
+    X, Y, Z = (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0)
+    grey_m = LambertianMaterial(grey)
+
+    def quad(p, u, v, mat):
+        return SimpleObject(ParallelogramMesh(p, u, v), mat)
+
+    objects = [
+        quad((0.0, 0.0, 0.0), Y, Z, LambertianMaterial(red)),   # left
+        quad((0.0, 0.0, 0.0), X, Y, LambertianMaterial(grey)),  # back
+        quad((0.0, 0.0, 0.0), Z, X, LambertianMaterial(grey)),  # floor
+        quad((1.0, 0.0, 0.0), Z, Y, LambertianMaterial(green)),  # right
+        quad((0.0, 1.0, 0.0), X, Z, LambertianMaterial(grey)),  # ceiling
+        quad((0.4, 0.9999, 0.4), (0.2, 0.0, 0.0), (0.0, 0.0, 0.2), LightMaterial((15.0, 15.0, 15.0))),
+        SimpleObject(AxisBoxMesh((0.231, 0.0, 0.117), (0.531, 0.595, 0.414)), grey_m,
+                     Transform3.from_axis_angle(Y, 15.0)),
+        SimpleObject(AxisBoxMesh((0.477, 0.0, 0.531), (0.774, 0.297, 0.829)), LambertianMaterial(grey),
+                     Transform3.from_axis_angle(Y, -18.0)),
+    ]
+    cam = Camera(pos=(0.5, 0.5, 2.3), fwd=(0.0, 0.0, -1.0), v_fov=40.0, focus_dist=1.0, defocus_angle=0.0)
+    opts = RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
+    return Scene(objects, NoSkybox), cam, opts
+
+
+def transformed_mix(seed: int = 6, width: int = 96, height: int = 64, samples: int = 4, ray_depth: int = 6):
+    """Every transformable object kind (sphere, triangle, parallelogram, axis box, sphere and box volumes) under
+    seeded rotations about random axes, non-uniform scalings and a shear, corrected and uncorrected, beside
+    untransformed objects and a light, under a SimpleSkybox."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+
+    def rot():
+        a = rng.normal(size=3)
+        return Transform3.from_axis_angle(a / np.linalg.norm(a), float(rng.uniform(-180.0, 180.0)))
+
+    def scale():
+        return Transform3.from_scale(rng.uniform(0.5, 1.8, size=3))
+
+    shear = Transform3(np.array([[1.0, 0.35, 0.0, 0.0], [0.0, 1.0, 0.0, 0.0], [0.2, 0.0, 1.0, 0.0],
+                                 [0.0, 0.0, 0.0, 1.0]]))
+    mats = [LambertianMaterial((0.7, 0.3, 0.2)), MetalMaterial((0.8, 0.8, 0.7), 0.2),
+            DielectricMaterial((0.95, 0.95, 1.0), 1.5, 0.3), LambertianMaterial((0.2, 0.5, 0.7))]
+    objects = [
+        SimpleObject(SphereMesh((-2.0, 1.0, 0.0), 0.7), mats[1], scale().then(rot())),
+        SimpleObject(TriangleMesh(((-1.0, 0.2, -1.0), (1.0, 0.4, -1.2), (0.0, 1.8, -1.5)),
+                                  ((0.0, 0.0, 1.0), (0.2, 0.0, 0.98), (0.0, 0.3, 0.954))), mats[0], rot()),
+        SimpleObject(ParallelogramMesh((0.5, 0.1, 0.5), (1.0, 0.0, 0.2), (0.0, 1.0, 0.3)), mats[3], shear),
+        SimpleObject(AxisBoxMesh((1.5, 0.0, -0.5), (2.5, 1.2, 0.5)), mats[0], scale().then(rot())),
+        SimpleObject(AxisBoxMesh((-0.6, 0.0, 1.0), (0.6, 0.8, 1.8)), mats[2], rot()),
+        SimpleObject(SphereMesh((0.0, 0.5, 0.0), 0.5), mats[3], Transform3.from_translation((0.3, 0.4, -2.0)),
+                     corrected=False),
+        VolumetricObject(AxisBoxMesh.new_centred((-1.5, 0.8, 1.5), (1.0, 1.4, 1.0)),
+                         IsotropicMaterial((0.9, 0.8, 0.7), 0.05), 1.2, rot()),
+        VolumetricObject(SphereMesh((1.2, 0.8, 1.8), 0.6), IsotropicMaterial((0.6, 0.7, 0.9), 0.1), 2.0,
+                         scale().then(shear)),
+        SimpleObject(ParallelogramMesh((-1.0, 3.0, -1.0), (2.0, 0.0, 0.0), (0.0, 0.0, 2.0)),
+                     LightMaterial((4.0, 4.0, 4.0))),
+        SimpleObject(SphereMesh((0.0, -1000.0, 0.0), 1000.0), LambertianMaterial((0.5, 0.5, 0.5))),
+    ]
+    cam = Camera(pos=(0.0, 2.0, 7.0), fwd=(0.0, -0.25, -1.0), v_fov=50.0, focus_dist=6.0, defocus_angle=0.0)
+    opts = RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
+    return Scene(objects, SimpleSkybox), cam, opts
 
 
 def mixed_primitives(seed: int = 5, width: int = 96, height: int = 64, samples: int = 4, ray_depth: int = 6):

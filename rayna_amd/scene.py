@@ -4,7 +4,7 @@ Names and argument meanings follow the engine:
   materials  rayna_engine/src/material/{lambertian,metal,dielectric,isotropic,light}.rs
   meshes     rayna_engine/src/mesh/primitive/{sphere,triangle,axis_box}.rs, mesh/planar/parallelogram.rs,
              mesh/advanced/list.rs (a list of triangles), mesh/isosurface/raymarched.rs
-  objects    rayna_engine/src/object/{simple,volumetric}.rs (identity transform)
+  objects    rayna_engine/src/object/{simple,volumetric}.rs, with object/transform.rs (ObjectTransform)
   scene      rayna_engine/src/scene/mod.rs:11-24 (StandardScene: objects + skybox)
   camera     rayna_engine/src/scene/camera.rs:10-38 (angles given in degrees, as Angle::from_degrees)
   options    rayna_engine/src/render/render_opts.rs:8-72
@@ -149,20 +149,115 @@ class RaymarchedIsosurfaceMesh:
 
 MeshT = Union[SphereMesh, TriangleMesh, TriangleListMesh, ParallelogramMesh, AxisBoxMesh, RaymarchedIsosurfaceMesh]
 
+# ------------------------------------------------------------------------------------------ transforms
+
+
+class Transform3:
+    """An affine map of 3D space held as a 4x4 matrix (`glamour::Transform3<f64>`, core/types.rs:18), mapping
+    object space to world space.  `a.then(b)` applies a first, then b; angles are in degrees (Angle::from_degrees).
+    """
+
+    def __init__(self, matrix=None):
+        self.matrix = np.eye(4) if matrix is None else np.array(matrix, float).reshape(4, 4)
+
+    @staticmethod
+    def from_translation(v) -> "Transform3":
+        m = np.eye(4)
+        m[:3, 3] = np.asarray(v, float)
+        return Transform3(m)
+
+    @staticmethod
+    def from_scale(v) -> "Transform3":
+        m = np.eye(4)
+        m[0, 0], m[1, 1], m[2, 2] = (float(x) for x in np.broadcast_to(np.asarray(v, float), (3,)))
+        return Transform3(m)
+
+    @staticmethod
+    def from_axis_angle(axis, degrees: float) -> "Transform3":
+        """A rotation by `degrees` about the unit vector `axis` (Rodrigues' formula, arranged as glam's
+        Mat4::from_axis_angle)."""
+        x, y, z = (float(a) for a in axis)
+        s, c = math.sin(math.radians(degrees)), math.cos(math.radians(degrees))
+        omc = 1.0 - c
+        xy, xz, yz = x * y * omc, x * z * omc, y * z * omc
+        m = np.eye(4)
+        m[:3, 0] = (x * x * omc + c, xy + z * s, xz - y * s)
+        m[:3, 1] = (xy - z * s, y * y * omc + c, yz + x * s)
+        m[:3, 2] = (xz + y * s, yz - x * s, z * z * omc + c)
+        return Transform3(m)
+
+    @staticmethod
+    def from_scale_rotation_translation(scale, axis, degrees: float, translation) -> "Transform3":
+        return Transform3.from_scale(scale).then(Transform3.from_axis_angle(axis, degrees)).then_translate(translation)
+
+    def then(self, other: "Transform3") -> "Transform3":
+        return Transform3(other.matrix @ self.matrix)
+
+    def then_translate(self, v) -> "Transform3":
+        return self.then(Transform3.from_translation(v))
+
+    def corrected(self, centre) -> "Transform3":
+        """ObjectTransform::new_corrected (object/transform.rs): the map applied about `centre`."""
+        c = np.asarray(centre, float)
+        return Transform3.from_translation(-c).then(self).then_translate(c)
+
+    def map_point(self, p) -> np.ndarray:
+        return self.matrix[:3, :3] @ np.asarray(p, float) + self.matrix[:3, 3]
+
+    def map_vector(self, v) -> np.ndarray:
+        return self.matrix[:3, :3] @ np.asarray(v, float)
+
+    def inverse(self) -> "Transform3":
+        return Transform3(np.linalg.inv(self.matrix))
+
+
+Transform3.IDENTITY = Transform3()
+
+
+def mesh_centre(m) -> np.ndarray:
+    """Mesh::centre (mesh/mod.rs:93): the point a corrected transform turns about."""
+    if isinstance(m, SphereMesh):
+        return np.asarray(m.pos, float)
+    if isinstance(m, TriangleMesh):
+        v = np.asarray(m.vertices, float).reshape(3, 3)
+        return (v[0] + v[1] + v[2]) / 3.0
+    if isinstance(m, ParallelogramMesh):
+        return np.asarray(m.p, float)
+    if isinstance(m, AxisBoxMesh):
+        a, b = np.asarray(m.a, float), np.asarray(m.b, float)
+        return (np.minimum(a, b) + np.maximum(a, b)) / 2.0
+    return np.zeros(3)  # SDFs (raymarched.rs:69); triangle lists keep their own centre, unused here
+
+
 # ------------------------------------------------------------------------------------------ objects
 
 
 @dataclass
 class SimpleObject:
+    """`SimpleObject::new(mesh, material, transform)` (object/simple.rs:67-71): the transform turns about the
+    mesh's centre; `corrected=False` is `new_uncorrected` (the transform as given)."""
     mesh: MeshT
     material: MaterialT
+    transform: Optional[Transform3] = None
+    corrected: bool = True
 
 
 @dataclass
 class VolumetricObject:
+    """`VolumetricObject::new(mesh, material, density, transform)` (object/volumetric.rs:42-72)."""
     mesh: MeshT
     material: MaterialT
     density: float
+    transform: Optional[Transform3] = None
+    corrected: bool = True
+
+
+def object_matrix(o) -> Optional[np.ndarray]:
+    """The object-to-world matrix an object's ObjectTransform holds, or None for no transform."""
+    if o.transform is None:
+        return None
+    t = o.transform.corrected(mesh_centre(o.mesh)) if o.corrected else o.transform
+    return t.matrix
 
 
 NoSkybox, SimpleSkybox, WhiteSkybox = A.SKY_NONE, A.SKY_SIMPLE, A.SKY_WHITE
@@ -257,6 +352,7 @@ def _build_desc(scene: Scene):
     boxes: List[Sequence[float]] = []
     sdfs: List[A.Sdf] = []
     objects: List[A.Object] = []
+    transforms: List[np.ndarray] = []
 
     def mat_index(m):
         key = id(m)
@@ -324,10 +420,15 @@ def _build_desc(scene: Scene):
 
     for o in scene.objects:
         mi = add_mesh(o.mesh)
+        m = object_matrix(o)
+        ti = 0
+        if m is not None:
+            transforms.append(np.ascontiguousarray(m[:3, :4], float))
+            ti = len(transforms)
         if isinstance(o, VolumetricObject):
-            objects.append(A.Object(A.OBJ_VOLUMETRIC, mi, mat_index(o.material), 0, float(o.density)))
+            objects.append(A.Object(A.OBJ_VOLUMETRIC, mi, mat_index(o.material), ti, float(o.density)))
         else:
-            objects.append(A.Object(A.OBJ_SIMPLE, mi, mat_index(o.material), 0, 0.0))
+            objects.append(A.Object(A.OBJ_SIMPLE, mi, mat_index(o.material), ti, 0.0))
 
     keep = {}
     sph = np.ascontiguousarray(np.vstack(spheres) if spheres else np.zeros((0, 4)), float)
@@ -335,11 +436,12 @@ def _build_desc(scene: Scene):
     tn = np.ascontiguousarray(np.concatenate(tri_n) if tri_n else np.zeros((0, 3, 3)), float)
     pa = np.ascontiguousarray(np.array(paras, float).reshape(-1, 9))
     bx = np.ascontiguousarray(np.array(boxes, float).reshape(-1, 6))
+    xf = np.ascontiguousarray(np.array(transforms, float).reshape(-1, 12))
     obj_arr = (A.Object * max(1, len(objects)))(*objects)
     mesh_arr = (A.Mesh * max(1, len(meshes)))(*meshes)
     mat_arr = (A.Material * max(1, len(materials)))(*materials)
     sdf_arr = (A.Sdf * max(1, len(sdfs)))(*sdfs)
-    keep.update(sph=sph, tv=tv, tn=tn, pa=pa, bx=bx, obj=obj_arr, mesh=mesh_arr, mat=mat_arr, sdf=sdf_arr,
+    keep.update(sph=sph, tv=tv, tn=tn, pa=pa, bx=bx, xf=xf, obj=obj_arr, mesh=mesh_arr, mat=mat_arr, sdf=sdf_arr,
                 n_triangles=len(tv), materials=mat_objs)
     d = A.SceneDesc()
     d.objects, d.n_objects = C.cast(obj_arr, C.POINTER(A.Object)), len(objects)
@@ -351,4 +453,5 @@ def _build_desc(scene: Scene):
     d.boxes, d.n_boxes = _ptr(bx, C.c_double), len(bx)
     d.sdfs, d.n_sdfs = C.cast(sdf_arr, C.POINTER(A.Sdf)), len(sdfs)
     d.skybox = int(scene.skybox)
+    d.transforms, d.n_transforms = _ptr(xf, C.c_double), len(xf)
     return d, keep
