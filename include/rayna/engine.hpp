@@ -2,10 +2,15 @@
 //
 // The names, argument meanings and error behaviour follow rayna_engine (Rust):
 //   scene      StandardScene { objects, skybox }                      scene/mod.rs:18-24
-//   objects    SimpleObject::new_uncorrected(mesh, material)          object/simple.rs:175-200
-//              VolumetricObject::new(mesh, material, density)         object/volumetric.rs:42-72
+//   objects    SimpleObject::new_uncorrected(mesh, material, transform) object/simple.rs:67-100
+//              SimpleObject::make(mesh, material, transform) -- the engine's SimpleObject::new (`new` is a C++
+//              keyword): the transform turned about the mesh's centre (ObjectTransform::new_corrected)
+//              VolumetricObject::make / new_uncorrected(mesh, material, density, transform)
+//                                                                     object/volumetric.rs:42-72
+//   transforms Transform3 (glamour::Transform3<f64>): from_translation, from_scale, from_axis_angle,
+//              from_scale_rotation_translation, then, then_translate  core/types.rs:18, object/transform.rs
 //   meshes     SphereMesh, Triangle, MeshList (Vec<Triangle>), ParallelogramMesh, AxisBoxMesh,
-//              RaymarchedIsosurfaceMesh                               mesh/mod.rs:68-85
+//              RaymarchedIsosurfaceMesh, BvhMesh (of primitive meshes)  mesh/mod.rs:68-85
 //   materials  LambertianMaterial, MetalMaterial, DielectricMaterial, IsotropicMaterial, LightMaterial
 //                                                                     material/mod.rs:168-175
 //   skyboxes   NoSkybox, SimpleSkybox, WhiteSkybox                     skybox/mod.rs:28-34
@@ -54,7 +59,65 @@ struct ColourRgb {
 
 struct Angle {
     Number radians = 0;
-    static Angle from_degrees(Number d) { return {d * 3.14159265358979323846 / 180.0}; }
+    static Angle from_degrees(Number d) { return {d * (3.14159265358979323846 / 180.0)}; }
+};
+
+// An affine map, object space to world space: row-major 3x4 [m | t], p -> m p + t.
+struct Transform3 {
+    Number m[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    static Transform3 IDENTITY() { return {}; }
+    static Transform3 from_translation(const Vector3 &v) {
+        Transform3 r;
+        r.m[3] = v.x;
+        r.m[7] = v.y;
+        r.m[11] = v.z;
+        return r;
+    }
+    static Transform3 from_scale(const Vector3 &s) {
+        Transform3 r;
+        r.m[0] = s.x;
+        r.m[5] = s.y;
+        r.m[10] = s.z;
+        return r;
+    }
+    // A rotation about the unit vector `axis` (Rodrigues' formula, arranged as glam's Mat4::from_axis_angle).
+    static Transform3 from_axis_angle(const Vector3 &axis, Angle angle) {
+        const Number s = std::sin(angle.radians), c = std::cos(angle.radians), omc = 1.0 - c;
+        const Number x = axis.x, y = axis.y, z = axis.z;
+        const Number xy = x * y * omc, xz = x * z * omc, yz = y * z * omc;
+        Transform3 r;
+        const Number rows[3][3] = {{x * x * omc + c, xy - z * s, xz + y * s},
+                                   {xy + z * s, y * y * omc + c, yz - x * s},
+                                   {xz - y * s, yz + x * s, z * z * omc + c}};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) r.m[4 * i + j] = rows[i][j];
+        return r;
+    }
+    static Transform3 from_scale_rotation_translation(const Vector3 &scale, const Vector3 &axis, Angle angle,
+                                                      const Vector3 &translation) {
+        return from_scale(scale).then(from_axis_angle(axis, angle)).then_translate(translation);
+    }
+    // self, then `next`: next.m * self.m
+    Transform3 then(const Transform3 &next) const {
+        Transform3 r;
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 4; ++j) {
+                Number v = next.m[4 * i] * m[j] + next.m[4 * i + 1] * m[4 + j] + next.m[4 * i + 2] * m[8 + j];
+                if (j == 3) v += next.m[4 * i + 3];
+                r.m[4 * i + j] = v;
+            }
+        }
+        return r;
+    }
+    Transform3 then_translate(const Vector3 &v) const { return then(from_translation(v)); }
+    Point3 map_point(const Point3 &p) const {
+        return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+                m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+    }
+    // ObjectTransform::new_corrected (object/transform.rs): the map applied about `centre`.
+    Transform3 corrected(const Point3 &centre) const {
+        return from_translation({-centre.x, -centre.y, -centre.z}).then(*this).then_translate(centre);
+    }
 };
 
 // ---------------------------------------------------------------------------------------------- materials
@@ -77,21 +140,82 @@ struct SdfSminSpheres { std::vector<Point3> centres; std::vector<Number> radii; 
 struct SdfMandelbulb { Point3 centre; int power = 8; int iterations = 10; Number bailout = 2; };
 using SdfFunction = std::variant<SdfSphere, SdfSminSpheres, SdfMandelbulb>;
 struct RaymarchedIsosurfaceMesh { SdfFunction sdf; int max_iterations = 150; Number epsilon = 1e-7; };
+using PrimitiveMesh = std::variant<SphereMesh, Triangle, ParallelogramMesh, AxisBoxMesh>;
+struct BvhMesh { std::vector<PrimitiveMesh> meshes; };  // BvhMesh::new(meshes), mesh/advanced/bvh.rs:28-45
 using MeshInstance =
-    std::variant<SphereMesh, Triangle, MeshList, ParallelogramMesh, AxisBoxMesh, RaymarchedIsosurfaceMesh>;
+    std::variant<SphereMesh, Triangle, MeshList, ParallelogramMesh, AxisBoxMesh, RaymarchedIsosurfaceMesh, BvhMesh>;
+
+// Mesh::centre (mesh/mod.rs:93): the point the engine's corrected transforms turn about.
+inline Point3 mesh_centre(const MeshInstance &m) {
+    auto prim = [](const auto &x) -> Point3 {
+        using T = std::decay_t<decltype(x)>;
+        if constexpr (std::is_same_v<T, SphereMesh>) return x.pos;
+        if constexpr (std::is_same_v<T, Triangle>)  // triangle.rs:42-45
+            return {(x.vertices[0].x + x.vertices[1].x + x.vertices[2].x) / 3.0,
+                    (x.vertices[0].y + x.vertices[1].y + x.vertices[2].y) / 3.0,
+                    (x.vertices[0].z + x.vertices[1].z + x.vertices[2].z) / 3.0};
+        if constexpr (std::is_same_v<T, ParallelogramMesh>) return x.p;
+        if constexpr (std::is_same_v<T, AxisBoxMesh>)
+            return {(std::fmin(x.a.x, x.b.x) + std::fmax(x.a.x, x.b.x)) / 2.0,
+                    (std::fmin(x.a.y, x.b.y) + std::fmax(x.a.y, x.b.y)) / 2.0,
+                    (std::fmin(x.a.z, x.b.z) + std::fmax(x.a.z, x.b.z)) / 2.0};
+        return Point3{};
+    };
+    return std::visit([&](const auto &x) -> Point3 {
+        using T = std::decay_t<decltype(x)>;
+        if constexpr (std::is_same_v<T, MeshList>) {  // the sum of the triangles' centres (list.rs:33-36)
+            Point3 c{};
+            for (const Triangle &t : x.triangles) {
+                const Point3 tc = prim(t);
+                c = {c.x + tc.x, c.y + tc.y, c.z + tc.z};
+            }
+            return c;
+        } else if constexpr (std::is_same_v<T, BvhMesh>) {  // the mean of the meshes' centres (bvh.rs:36-44)
+            Point3 c{};
+            for (const PrimitiveMesh &p : x.meshes) {
+                const Point3 pc = std::visit(prim, p);
+                c = {c.x + pc.x, c.y + pc.y, c.z + pc.z};
+            }
+            const Number n = (Number)x.meshes.size();
+            return {c.x / n, c.y / n, c.z / n};
+        } else if constexpr (std::is_same_v<T, RaymarchedIsosurfaceMesh>) {
+            return Point3{};  // raymarched.rs:69
+        } else {
+            return prim(x);
+        }
+    }, m);
+}
 
 // ---------------------------------------------------------------------------------------------- objects
 struct SimpleObject {
     MeshInstance mesh;
     MaterialInstance material;
-    static SimpleObject new_uncorrected(MeshInstance mesh, MaterialInstance material) {
-        return SimpleObject{std::move(mesh), std::move(material)};
+    Transform3 transform;  // object to world, as given to the renderer (already corrected)
+    static SimpleObject new_uncorrected(MeshInstance mesh, MaterialInstance material,
+                                        Transform3 transform = Transform3::IDENTITY()) {
+        return SimpleObject{std::move(mesh), std::move(material), transform};
+    }
+    // SimpleObject::new: the transform turned about the mesh's centre.
+    static SimpleObject make(MeshInstance mesh, MaterialInstance material,
+                             Transform3 transform = Transform3::IDENTITY()) {
+        const Transform3 t = transform.corrected(mesh_centre(mesh));
+        return SimpleObject{std::move(mesh), std::move(material), t};
     }
 };
 struct VolumetricObject {
     MeshInstance mesh;
     MaterialInstance material;
     Number density = 1;
+    Transform3 transform;
+    static VolumetricObject new_uncorrected(MeshInstance mesh, MaterialInstance material, Number density,
+                                            Transform3 transform = Transform3::IDENTITY()) {
+        return VolumetricObject{std::move(mesh), std::move(material), density, transform};
+    }
+    static VolumetricObject make(MeshInstance mesh, MaterialInstance material, Number density,
+                                 Transform3 transform = Transform3::IDENTITY()) {
+        const Transform3 t = transform.corrected(mesh_centre(mesh));
+        return VolumetricObject{std::move(mesh), std::move(material), density, t};
+    }
 };
 using ObjectInstance = std::variant<SimpleObject, VolumetricObject>;
 
@@ -164,7 +288,7 @@ struct SceneArrays {
     std::vector<rayna_object> objects;
     std::vector<rayna_mesh> meshes;
     std::vector<rayna_material> materials;
-    std::vector<double> spheres, tri_v, tri_n, paras, boxes;
+    std::vector<double> spheres, tri_v, tri_n, paras, boxes, transforms;
     std::vector<rayna_sdf> sdfs;
     rayna_scene_desc desc{};
 
@@ -226,6 +350,11 @@ struct SceneArrays {
                 put3(boxes, x.a);
                 put3(boxes, x.b);
             }
+            if constexpr (std::is_same_v<T, BvhMesh>) {  // the members take the mesh slots just before it
+                const int32_t first = (int32_t)meshes.size();
+                for (const PrimitiveMesh &p : x.meshes) std::visit([&](const auto &q) { mesh(MeshInstance{q}); }, p);
+                r = {RAYNA_MESH_BVH, first, (int32_t)x.meshes.size(), 0};
+            }
             if constexpr (std::is_same_v<T, RaymarchedIsosurfaceMesh>) {
                 rayna_sdf s{};
                 s.max_iterations = x.max_iterations;
@@ -274,8 +403,16 @@ struct SceneArrays {
                 const int32_t mi = mesh(x.mesh);
                 materials.push_back(material(x.material));
                 const int32_t mat = (int32_t)materials.size() - 1;
-                if constexpr (std::is_same_v<T, SimpleObject>) objects.push_back({RAYNA_OBJ_SIMPLE, mi, mat, 0, 0.0});
-                else objects.push_back({RAYNA_OBJ_VOLUMETRIC, mi, mat, 0, x.density});
+                int32_t ti = 0;
+                bool identity = true;
+                const Transform3 I = Transform3::IDENTITY();
+                for (int k = 0; k < 12; ++k) identity = identity && x.transform.m[k] == I.m[k];
+                if (!identity) {
+                    transforms.insert(transforms.end(), x.transform.m, x.transform.m + 12);
+                    ti = (int32_t)(transforms.size() / 12);
+                }
+                if constexpr (std::is_same_v<T, SimpleObject>) objects.push_back({RAYNA_OBJ_SIMPLE, mi, mat, ti, 0.0});
+                else objects.push_back({RAYNA_OBJ_VOLUMETRIC, mi, mat, ti, x.density});
             }, o);
         }
         desc.objects = objects.data();
@@ -296,6 +433,8 @@ struct SceneArrays {
         desc.sdfs = sdfs.data();
         desc.n_sdfs = (int32_t)sdfs.size();
         desc.skybox = (int32_t)sc.skybox;
+        desc.transforms = transforms.data();
+        desc.n_transforms = (int32_t)(transforms.size() / 12);
     }
 };
 

@@ -4,9 +4,9 @@
  * The ABI replaces `rayna_engine::render::renderer::Renderer` (rayna_engine/src/render/renderer.rs:36-264)
  * for the scene graph that the engine's `StandardScene` presets build: a flat list of objects
  * (`Vec<SimpleObject|VolumetricObject>.into()` -> `ObjectList`, rayna_engine/src/object/list.rs:43-129),
- * each a mesh plus a material, with a `SkyboxInstance`.  Scenes cross the ABI as plain arrays: there are
- * no closures, no `Dynamic*` plug-ins and no non-identity transforms (the renderer returns
- * RAYNA_UNSUPPORTED for those).
+ * each a mesh plus a material and an optional `ObjectTransform`, with a `SkyboxInstance`.  Scenes cross the
+ * ABI as plain arrays: there are no closures and no `Dynamic*` plug-ins (the renderer returns
+ * RAYNA_UNSUPPORTED for what it cannot express).
  *
  * All geometry is given in double precision, as the engine's `Number = f64`
  * (rayna_engine/src/core/types.rs:7); colours are single precision (`Channel = f32`, types.rs:3).
@@ -66,7 +66,10 @@ typedef struct rayna_material {
  *   PARALLELOGRAM  ParallelogramMesh::new(Planar::new(p,u,v))   planar/parallelogram.rs:444-458
  *   AXIS_BOX       AxisBoxMesh::new(a, b)                       primitive/axis_box.rs:28-37
  *   SDF            RaymarchedIsosurfaceMesh::new_custom         isosurface/raymarched.rs:51-57
- * `index` selects the element of the kind's array; TRIANGLE_LIST uses triangles [index, index+count).
+ *   BVH            BvhMesh::new(meshes) (its own SAH BVH)        advanced/bvh.rs:28-45
+ * `index` selects the element of the kind's array; TRIANGLE_LIST uses triangles [index, index+count); BVH
+ * uses the meshes [index, index+count) of the mesh array itself, each a SPHERE, TRIANGLE, PARALLELOGRAM or
+ * AXIS_BOX mesh.
  */
 enum {
     RAYNA_MESH_SPHERE = 0,
@@ -74,7 +77,8 @@ enum {
     RAYNA_MESH_TRIANGLE_LIST = 2,
     RAYNA_MESH_PARALLELOGRAM = 3,
     RAYNA_MESH_AXIS_BOX = 4,
-    RAYNA_MESH_SDF = 5
+    RAYNA_MESH_SDF = 5,
+    RAYNA_MESH_BVH = 6
 };
 
 typedef struct rayna_mesh {
@@ -122,8 +126,9 @@ typedef struct rayna_sdf {
  * transforms[k - 1] of the scene, mapping object space to world space.  The engine's corrected constructors
  * (SimpleObject::new, ObjectTransform::new_corrected) rotate about the mesh's centre; a caller passes the
  * corrected matrix, translate(-centre) then transform then translate(centre).  Non-identity transforms are
- * supported on sphere, triangle, parallelogram and axis-box meshes (simple and volumetric objects); on
- * triangle lists and SDFs they return RAYNA_UNSUPPORTED.
+ * supported on every bounded mesh: primitives (simple and volumetric objects), and triangle lists and BVH
+ * meshes (simple objects: an instance of the mesh's own tree); on SDFs they return RAYNA_UNSUPPORTED.  Volumes
+ * take sphere and axis-box boundaries.
  */
 enum {
     RAYNA_OBJ_SIMPLE = 0,

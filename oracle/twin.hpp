@@ -456,6 +456,19 @@ inline bool mesh_aabb(const Scene &s, int mesh, Aabb *ref, Aabb *tight) {
             *ref = *tight = aabb_new(bp, bp + 3);
             return true;
         }
+        case RAYNA_MESH_BVH: {  // BvhMesh::aabb (bvh.rs:116-129): the root's box, a Nested root's folded from the origin
+            if (m.count <= 0) return false;
+            Aabb r = aabb_origin(), t, sr, st;
+            mesh_aabb(s, m.index, &sr, &t);
+            for (int i = 0; i < m.count; ++i) {
+                mesh_aabb(s, m.index + i, &sr, &st);
+                r = encompass(r, sr);
+                t = encompass(t, st);
+            }
+            *ref = m.count == 1 ? t : r;
+            *tight = t;
+            return true;
+        }
         default:  // raymarched.rs:65: unbounded
             return false;
     }
@@ -564,14 +577,21 @@ inline void build_scene(Scene &s, const rayna_scene_desc &d, bool literal) {
         }
     }
 
-    // BLAS per triangle-list mesh (MeshList -> BvhMesh -> GenericBvh over the triangles, list.rs:162-190)
+    // BLAS per triangle-list mesh (MeshList -> BvhMesh -> GenericBvh over the triangles, list.rs:162-190) and per
+    // BVH mesh (BvhMesh::new -> GenericBvh over the meshes, bvh.rs:35-45)
     s.mesh_blas.assign(s.meshes.size(), -1);
     for (size_t m = 0; m < s.meshes.size(); ++m) {
         const rayna_mesh &me = s.meshes[m];
-        if (me.kind != RAYNA_MESH_TRIANGLE_LIST || me.count <= 0) continue;
+        if ((me.kind != RAYNA_MESH_TRIANGLE_LIST && me.kind != RAYNA_MESH_BVH) || me.count <= 0) continue;
         std::vector<Aabb> tb(me.count);
-        for (int i = 0; i < me.count; ++i)
-            tb[i] = encompass_points((const double(*)[3]) & s.tri_v[9 * (me.index + i)], 3);
+        for (int i = 0; i < me.count; ++i) {
+            if (me.kind == RAYNA_MESH_BVH) {
+                Aabb unused;
+                mesh_aabb(s, me.index + i, &tb[i], &unused);
+            } else {
+                tb[i] = encompass_points((const double(*)[3]) & s.tri_v[9 * (me.index + i)], 3);
+            }
+        }
         BvhBuilder bb(tb, literal);
         Bvh b = bb.build();
         // node boxes + depth-first positions
@@ -973,17 +993,21 @@ inline bool prim_mesh_intersect(const Scene &s, const rayna_mesh &m, const Ray<R
     }
 }
 
-// BvhMesh::bvh_node_intersect (mesh/advanced/bvh.rs:63-102) over a triangle list.
+// BvhMesh::bvh_node_intersect (mesh/advanced/bvh.rs:63-102) over a triangle list or the meshes of a BVH mesh.
 template <class R>
-inline bool blas_node(const Ctx<R> &c, int bi, int node, int first_tri, int rank_base, const Ray<R> &ray,
+inline bool blas_node(const Ctx<R> &c, int bi, const rayna_mesh &m, int node, int rank_base, const Ray<R> &ray,
                       const Interval<R> &iv, Hit<R> &out) {
     const Scene &s = *c.s;
     const BvhNode &nd = s.blas[bi].nodes[node];
     if (!box_hit(c, s.blas_boxes[bi][node], ray, iv)) return false;
     if (nd.item >= 0) {
-        const int tri = first_tri + nd.item;
         if (c.prim_tests) ++*c.prim_tests;
-        if (!triangle_intersect(&s.tri_v[9 * tri], &s.tri_n[9 * tri], ray, iv, out)) return false;
+        if (m.kind == RAYNA_MESH_BVH) {
+            if (!prim_mesh_intersect(s, s.meshes[m.index + nd.item], ray, iv, out)) return false;
+        } else {
+            const int tri = m.index + nd.item;
+            if (!triangle_intersect(&s.tri_v[9 * tri], &s.tri_n[9 * tri], ray, iv, out)) return false;
+        }
         out.rank = rank_base + s.blas_dfs[bi][nd.item];
         return true;
     }
@@ -991,7 +1015,7 @@ inline bool blas_node(const Ctx<R> &c, int bi, int node, int first_tri, int rank
     bool found = false;
     for (int ch : nd.children) {
         Hit<R> h;
-        if (blas_node(c, bi, ch, first_tri, rank_base, ray, shrunk, h)) {
+        if (blas_node(c, bi, m, ch, rank_base, ray, shrunk, h)) {
             shrunk.has_end = true;
             shrunk.end = h.t;
             out = h;
@@ -1010,10 +1034,10 @@ inline bool object_intersect_local(const Ctx<R> &c, int o, const Ray<R> &ray, co
     const rayna_mesh &m = s.meshes[obj.mesh];
     bool hit = false;
     if (obj.kind == RAYNA_OBJ_SIMPLE) {
-        if (m.kind == RAYNA_MESH_TRIANGLE_LIST) {
+        if (m.kind == RAYNA_MESH_TRIANGLE_LIST || m.kind == RAYNA_MESH_BVH) {
             const int bi = s.mesh_blas[obj.mesh];
             if (bi < 0) return false;
-            hit = blas_node(c, bi, s.blas[bi].root, m.index, s.obj_rank[o], ray, iv, h);
+            hit = blas_node(c, bi, m, s.blas[bi].root, s.obj_rank[o], ray, iv, h);
         } else {
             if (c.prim_tests) ++*c.prim_tests;
             hit = prim_mesh_intersect(s, m, ray, iv, h);
@@ -1023,7 +1047,7 @@ inline bool object_intersect_local(const Ctx<R> &c, int o, const Ray<R> &ray, co
         // volume: entry with the full interval, exit after entry + 0.001, free flight -ln(U)/density
         Interval<R> full{false, false, R(0), R(0)};
         Hit<R> he, hx;
-        if (m.kind == RAYNA_MESH_TRIANGLE_LIST) return false;  // not supported as a volume boundary
+        if (m.kind == RAYNA_MESH_TRIANGLE_LIST || m.kind == RAYNA_MESH_BVH) return false;  // not a volume boundary
         if (c.prim_tests) ++*c.prim_tests;
         if (!prim_mesh_intersect(s, m, ray, full, he)) return false;
         R entry = he.t;

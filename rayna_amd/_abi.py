@@ -9,7 +9,7 @@ RAYNA_OUT_OF_MEMORY = 4
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "UNSUPPORTED", 3: "DEVICE_ERROR", 4: "OUT_OF_MEMORY"}
 
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_LIGHT = 0, 1, 2, 3, 4
-MESH_SPHERE, MESH_TRIANGLE, MESH_TRIANGLE_LIST, MESH_PARALLELOGRAM, MESH_AXIS_BOX, MESH_SDF = 0, 1, 2, 3, 4, 5
+MESH_SPHERE, MESH_TRIANGLE, MESH_TRIANGLE_LIST, MESH_PARALLELOGRAM, MESH_AXIS_BOX, MESH_SDF, MESH_BVH = 0, 1, 2, 3, 4, 5, 6
 SDF_SPHERE, SDF_SMIN_SPHERES, SDF_MANDELBULB = 0, 1, 2
 OBJ_SIMPLE, OBJ_VOLUMETRIC = 0, 1
 SKY_NONE, SKY_SIMPLE, SKY_WHITE = 0, 1, 2

@@ -154,6 +154,7 @@ rayna_status alloc_paths(rayna_batch *b, uint64_t slots, uint32_t max_depth) {
         if ((st = alloc(slots * sizeof(float), (void **)f)) != RAYNA_OK) return st;
     if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->pb.hit_ref)) != RAYNA_OK) return st;
     if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->pb.hit_aux)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->pb.hit_sub)) != RAYNA_OK) return st;
     if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->slot_req)) != RAYNA_OK) return st;
     for (int q = 0; q < 2; ++q)
         if ((st = alloc(slots * sizeof(int32_t), (void **)&b->queue[q])) != RAYNA_OK) return st;
@@ -434,6 +435,7 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     v.n_sdfs = (int32_t)merged.sdfs.size();
     v.skybox = RAYNA_SKY_NONE;
     v.stack_need = merged.stack_need;
+    v.instances = has_instances(merged) ? 1 : 0;
     const size_t deep = (size_t)std::max(0, merged.stack_need - extend_lds_stack());
     if (deep > 0) {
         void *p = nullptr;

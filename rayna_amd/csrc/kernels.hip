@@ -306,6 +306,7 @@ struct Best {
     int32_t rank;
     uint32_t ref;
     uint32_t aux;
+    uint32_t sub;  // an instance's hit: its primitive's reference (hit_sub)
 };
 
 __device__ __forceinline__ uint32_t make_ref(int kind, int32_t index) {
@@ -331,8 +332,9 @@ __device__ __forceinline__ f3 unit_sphere(float r1, float r2) {  // rng::normal_
 }
 
 // The primitive a hit reference names: its kind and index into that kind's array (the inner primitive or volume
-// of a transformed object), its material, and the ray and distance to evaluate it with (the object-space ray and
-// distance of a transformed object).  x: the transform, or null.
+// of a transformed object, or the primitive `sub` names inside an instanced mesh BVH), its material, and the ray
+// and distance to evaluate it with (the object-space ray and distance of a transformed object).  x: the
+// transform, or null.
 struct HitPrim {
     int kind;
     int32_t prim;
@@ -343,7 +345,7 @@ struct HitPrim {
 };
 
 __device__ __forceinline__ HitPrim hit_prim(const SceneView &sv, const DRay &ray, float t, uint32_t ref,
-                                            uint32_t aux) {
+                                            uint32_t aux, const uint32_t *sub) {
     HitPrim p{(int)(ref >> HIT_KIND_SHIFT), (int32_t)(ref & HIT_INDEX_MASK), 0, ray, t, nullptr};
     if (p.kind == ITEM_TRIANGLE) {
         p.material = sv.tris[p.prim].material;
@@ -359,6 +361,12 @@ __device__ __forceinline__ HitPrim hit_prim(const SceneView &sv, const DRay &ray
             p.prim = p.x->inner_index;
             p.ray = object_ray(p.x, ray);
             p.t = __uint_as_float(aux);
+            if (p.kind == ITEM_LINK) {  // an instance: the primitive inside it
+                const uint32_t s = *sub;
+                p.kind = (int)(s >> HIT_KIND_SHIFT);
+                p.prim = (int32_t)(s & HIT_INDEX_MASK);
+                if (p.kind != ITEM_TRIANGLE) p.prim = sv.items[p.prim].index;
+            }
         }
     }
     return p;
@@ -368,8 +376,9 @@ __device__ __forceinline__ HitPrim hit_prim(const SceneView &sv, const DRay &ray
 // transformed object's record is derived in object space and mapped out as outgoing_intersection does
 // (object/transform.rs): the position by the object-to-world map, both normals by its linear part, normalised.
 __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint32_t ref, uint32_t aux,
-                             const dm::Key &key, uint32_t pixel, uint32_t sample, uint32_t bounce, SHit *h) {
-    const HitPrim p = hit_prim(sv, ray, t, ref, aux);
+                             const uint32_t *sub, const dm::Key &key, uint32_t pixel, uint32_t sample,
+                             uint32_t bounce, SHit *h) {
+    const HitPrim p = hit_prim(sv, ray, t, ref, aux, sub);
     const DRay &r = p.ray;
     h->pos = at(r, p.t);
     h->material = p.material;
@@ -446,9 +455,9 @@ __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint
 // the volume's random uv (volumetric.rs:153) and the SDF's march iteration (raymarched.rs:106-109, carried in the
 // hit flags from bit 8).  A transform leaves both as the object-space test found them.
 __device__ void hit_uv_side(const SceneView &sv, const DRay &world_ray, float world_t, uint32_t ref, uint32_t aux,
-                            const dm::Key &key, uint32_t pixel, uint32_t sample, uint32_t bounce, float uv[2],
-                            int *side) {
-    const HitPrim p = hit_prim(sv, world_ray, world_t, ref, aux);
+                            const uint32_t *sub, const dm::Key &key, uint32_t pixel, uint32_t sample,
+                            uint32_t bounce, float uv[2], int *side) {
+    const HitPrim p = hit_prim(sv, world_ray, world_t, ref, aux, sub);
     const DRay &ray = p.ray;
     const float t = p.t;
     const int32_t idx = p.prim;
@@ -699,7 +708,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
                 if (!ADJOINT) L = L + T * sky(skybox, ray.d);
             } else {
                 SHit h;
-                hit_geometry(sv, ray, pb.hit_t[slot], ref, pb.hit_aux[slot], key, pixel, sample, depth, &h);
+                hit_geometry(sv, ray, pb.hit_t[slot], ref, pb.hit_aux[slot], pb.hit_sub + slot, key, pixel, sample,
+                             depth, &h);
                 const GMaterial m = sv.materials[h.material];
                 double d0 = 0.0, d1 = 0.0, d2 = 0.0;
                 if (ADJOINT) {
@@ -862,7 +872,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_mode(SceneView sv, FrameParams 
             const float t = pb.hit_t[slot];
             const uint32_t aux = pb.hit_aux[slot];
             SHit h;
-            hit_geometry(sv, ray, t, ref, aux, key, pixel, sample, 0, &h);
+            hit_geometry(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, 0, &h);
             const float PAL[13][3] = {{1.0f, 1.0f, 1.0f}, {1.0f, 0.0f, 0.0f}, {1.0f, 0.5f, 0.0f}, {1.0f, 1.0f, 0.0f},
                                       {0.5f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.5f}, {0.0f, 1.0f, 1.0f},
                                       {0.0f, 0.5f, 1.0f}, {0.0f, 0.0f, 1.0f}, {0.5f, 0.0f, 1.0f}, {1.0f, 0.0f, 1.0f},
@@ -897,7 +907,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_mode(SceneView sv, FrameParams 
             } else {
                 float uv[2];
                 int side;
-                hit_uv_side(sv, ray, t, ref, aux, key, pixel, sample, 0, uv, &side);
+                hit_uv_side(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, 0, uv, &side);
                 if (fp.mode == RAYNA_MODE_UV) {
                     L = Col{fminf(fmaxf(uv[0], 0.0f), 1.0f), fminf(fmaxf(uv[1], 0.0f), 1.0f), 0.0f};
                 } else {  // RAYNA_MODE_SIDE
@@ -948,8 +958,13 @@ int extend_lds_stack() { return EXT_LDS_STACK; }
 
 int extend_blocks_per_cu() {
     int n = 0, nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend<false>, EXT_BLOCK, 0) != hipSuccess || n < 1) n = 4;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<true>, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend<false, false>, EXT_BLOCK, 0) != hipSuccess || n < 1)
+        n = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<true, false>, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
+        n = std::min(n, nb);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false, true>, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
+        n = std::min(n, nb);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<true, true>, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
         n = std::min(n, nb);
     return n;
 }
@@ -977,12 +992,10 @@ void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_rad
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
                    const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work,
                    const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
-    if (bv.reqs)
-        hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth,
-                           overflow, work, tune, bv);
-    else
-        hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth,
-                           overflow, work, tune, bv);
+    auto kernel = bv.reqs ? (sv.instances ? k_extend<true, true> : k_extend<true, false>)
+                          : (sv.instances ? k_extend<false, true> : k_extend<false, false>);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth, overflow, work,
+                       tune, bv);
 }
 
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,

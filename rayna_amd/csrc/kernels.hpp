@@ -17,7 +17,8 @@ struct PathBuffers {
     float *lr, *lg, *lb;     // radiance gathered so far; the sample's value once the path ends
     float *hit_t;            // closest-hit distance
     uint32_t *hit_ref;       // kind << 28 | index, 0 on a miss
-    uint32_t *hit_aux;       // HIT_AUX_* bits
+    uint32_t *hit_aux;       // HIT_AUX_* bits; the object-space distance of a transformed object's hit
+    uint32_t *hit_sub;       // an instance's hit: the reference of its primitive inside the instanced mesh BVH
     int32_t *hit_rank;       // optional (parity queries): rank of the closest primitive
 };
 

@@ -163,6 +163,7 @@ struct SceneView {
     int32_t n_sdfs;
     int32_t skybox;
     int32_t stack_need;  // deepest traversal stack the tree can require
+    int32_t instances;   // 1 if some object is a transformed mesh BVH (an instance)
 };
 
 }  // namespace rayna

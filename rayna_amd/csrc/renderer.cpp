@@ -140,6 +140,7 @@ rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
     v.n_sdfs = (int32_t)ps.sdfs.size();
     v.skybox = ps.skybox;
     v.stack_need = ps.stack_need;
+    v.instances = has_instances(ps) ? 1 : 0;
     // traversal stack overflow beyond the LDS entries, one column per extend thread
     const size_t deep = (size_t)std::max(0, ps.stack_need - extend_lds_stack());
     if (deep > 0) {
@@ -221,6 +222,7 @@ rayna_status alloc_paths(rayna_renderer *r) {
         if ((st = alloc(slots * sizeof(float), (void **)f)) != RAYNA_OK) return st;
     if ((st = alloc(slots * sizeof(uint32_t), (void **)&r->pb.hit_ref)) != RAYNA_OK) return st;
     if ((st = alloc(slots * sizeof(uint32_t), (void **)&r->pb.hit_aux)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&r->pb.hit_sub)) != RAYNA_OK) return st;
     for (int q = 0; q < 2; ++q)
         if ((st = alloc(slots * sizeof(int32_t), (void **)&r->queue[q])) != RAYNA_OK) return st;
     if ((st = alloc(qlen * sizeof(uint32_t), (void **)&r->queue_count)) != RAYNA_OK) return st;

@@ -270,6 +270,35 @@ struct Packer {
 
     Aabb64 tri_box(int tri) const { return points_box(&d.tri_vertices[9 * (size_t)tri], 3); }
 
+    // The box of a primitive mesh: sphere.rs:123-132, triangle.rs:253, parallelogram.rs:445-457, axis_box.rs:29-37.
+    Aabb64 prim_box(int mesh) const {
+        const rayna_mesh &m = d.meshes[mesh];
+        switch (m.kind) {
+            case RAYNA_MESH_SPHERE: {
+                const double *s = &d.spheres[4 * (size_t)m.index];
+                const double a[3] = {s[0] - s[3], s[1] - s[3], s[2] - s[3]};
+                const double b[3] = {s[0] + s[3], s[1] + s[3], s[2] + s[3]};
+                return box_new(a, b);
+            }
+            case RAYNA_MESH_TRIANGLE: return tri_box(m.index);
+            case RAYNA_MESH_PARALLELOGRAM: {
+                const double *p = &d.parallelograms[9 * (size_t)m.index];
+                double pts[12];
+                for (int i = 0; i < 3; ++i) {
+                    pts[i] = p[i];
+                    pts[3 + i] = p[i] + p[3 + i];
+                    pts[6 + i] = p[i] + p[6 + i];
+                    pts[9 + i] = p[i] + p[3 + i] + p[6 + i];
+                }
+                return min_padded(points_box(pts, 4), 1e-6);
+            }
+            default: {  // RAYNA_MESH_AXIS_BOX
+                const double *b = &d.boxes[6 * (size_t)m.index];
+                return box_new(b, b + 3);
+            }
+        }
+    }
+
     static bool is_nested(const RefTree &t, int n) { return t.nodes[n].item < 0; }
 
     static bool all_items(const RefTree &t, int n) {
@@ -419,9 +448,47 @@ struct Packer {
 
     // ---- flattening ----
 
-    // Leaf cluster of a mesh BVH: triangles appended in child order.
+    // One primitive mesh of a BVH mesh as a leaf item, ranked next in depth-first order.
+    GItem sub_item(int sub, int32_t material, int32_t object, int32_t local) {
+        const rayna_mesh &sm = d.meshes[sub];
+        GItem it{ITEM_NONE, 0, rank, material};
+        switch (sm.kind) {
+            case RAYNA_MESH_SPHERE:
+                it.kind = ITEM_SPHERE;
+                it.index = sm.index;
+                break;
+            case RAYNA_MESH_TRIANGLE:  // add_triangle ranks it
+                it.kind = ITEM_TRIANGLE;
+                it.index = add_triangle(sm.index, material, object, local);
+                return it;
+            case RAYNA_MESH_PARALLELOGRAM:
+                it.kind = ITEM_PARALLELOGRAM;
+                it.index = add_para(sm.index);
+                break;
+            default:  // RAYNA_MESH_AXIS_BOX
+                it.kind = ITEM_AXIS_BOX;
+                it.index = add_box(sm.index);
+                break;
+        }
+        ++rank;
+        P.rank_object.push_back(object);
+        P.rank_local.push_back(local);
+        return it;
+    }
+
+    // Leaf cluster of a mesh BVH in child order: triangles of a triangle list, or items of a BVH mesh.
     void blas_leaf(GNode &g, int k, const rayna_mesh &m, int32_t material, int32_t object,
                    const std::vector<int32_t> &items) {
+        if (m.kind == RAYNA_MESH_BVH) {
+            const int32_t start = (int32_t)P.items.size();
+            for (int32_t it : items) {
+                const GItem gi = sub_item(m.index + it, material, object, it);
+                P.items.push_back(gi);
+            }
+            g.child[k] = start;
+            g.info[k] = (int32_t)items.size() | (LEAF_ITEMS << 4);
+            return;
+        }
         const int32_t start = (int32_t)P.tris.size();
         for (int32_t it : items) add_triangle(m.index + it, material, object, it);
         g.child[k] = start;
@@ -515,6 +582,7 @@ struct Packer {
                     it.index = add_box(m.index);
                     break;
                 case RAYNA_MESH_TRIANGLE_LIST:
+                case RAYNA_MESH_BVH:
                     it.kind = ITEM_LINK;
                     it.rank = rank;
                     it.index = emit_blas(ob.mesh, o, ob.material);
@@ -595,33 +663,48 @@ struct Packer {
         return st;
     }
 
+    // The deepest traversal stack the flattened tree can need, for any ray and any culling.  An inner node pushes
+    // its m non-empty children (inner nodes and leaf ranges alike) and they are popped one by one, so the height
+    // above the point where the node was popped peaks at max(m, m - 1 + the deepest child's need).  A leaf range
+    // pushes the root of each mesh-BVH item it tests; those are processed after the range, in any order.  An
+    // instance (a transformed mesh BVH) ends the range's pass: it pushes the rest of the range, three saved words
+    // of the closest hit, a marker and its root, so its tree runs 4 entries above the rest of the range.
     int32_t stack_need() const {
-        // S(g) = P - 1 + max(1, max S(child)) for a node that pushes P entries, 0 if it pushes none.
         std::vector<int32_t> memo(P.nodes.size(), -1);
-        std::function<int32_t(int32_t)> S = [&](int32_t g) -> int32_t {
-            if (memo[g] >= 0) return memo[g];
-            int32_t pushes = 0, deepest = 1;
-            const GNode &n = P.nodes[g];
-            for (int k = 0; k < 4; ++k) {
-                if (n.info[k] == INFO_INNER) {
-                    ++pushes;
-                    deepest = std::max(deepest, S(n.child[k]));
-                } else if (n.info[k] > 0 && (n.info[k] >> 4) == LEAF_ITEMS) {
-                    const int cnt = n.info[k] & 0xf;
-                    for (int i = 0; i < cnt; ++i) {
-                        const GItem &it = P.items[n.child[k] + i];
-                        if (it.kind == ITEM_LINK) {
-                            ++pushes;
-                            deepest = std::max(deepest, S(it.index));
-                        }
-                    }
+        std::function<int32_t(int32_t)> node_need;
+        std::function<int32_t(int32_t, int32_t)> leaf_need = [&](int32_t start, int32_t count) -> int32_t {
+            int32_t links = 0, deepest_link = 0, peak = 0;
+            for (int32_t i = 0; i < count; ++i) {
+                const GItem &it = P.items[start + i];
+                if (it.kind == ITEM_LINK) {
+                    ++links;
+                    deepest_link = std::max(deepest_link, node_need(it.index));
+                } else if (it.kind == ITEM_XFORM && P.xforms[it.index].inner_kind == ITEM_LINK) {
+                    const int32_t rest = i + 1 < count ? 1 : 0;
+                    peak = std::max(peak, links + rest + 5);
+                    peak = std::max(peak, links + rest + 4 + node_need(P.xforms[it.index].inner_index));
+                    if (rest) peak = std::max(peak, links + leaf_need(start + i + 1, count - i - 1));
+                    break;
                 }
             }
-            memo[g] = pushes == 0 ? 0 : pushes - 1 + deepest;
+            if (links > 0) peak = std::max(peak, std::max(links, links - 1 + deepest_link));
+            return peak;
+        };
+        node_need = [&](int32_t g) -> int32_t {
+            if (memo[g] >= 0) return memo[g];
+            int32_t m = 0, deepest = 0;
+            const GNode &n = P.nodes[g];
+            for (int k = 0; k < 4; ++k) {
+                if (n.info[k] == INFO_EMPTY) continue;
+                ++m;
+                if (n.info[k] == INFO_INNER) deepest = std::max(deepest, node_need(n.child[k]));
+                else if ((n.info[k] >> 4) == LEAF_ITEMS) deepest = std::max(deepest, leaf_need(n.child[k], n.info[k] & 0xf));
+            }
+            memo[g] = m == 0 ? 0 : std::max(m, m - 1 + deepest);
             return memo[g];
         };
         if (P.root < 0) return 0;
-        return std::max<int32_t>(1, S(P.root));
+        return std::max<int32_t>(1, node_need(P.root));
     }
 };
 
@@ -754,13 +837,26 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
             case RAYNA_MESH_PARALLELOGRAM: limit = d.n_parallelograms; break;
             case RAYNA_MESH_AXIS_BOX: limit = d.n_boxes; break;
             case RAYNA_MESH_SDF: limit = d.n_sdfs; break;
+            case RAYNA_MESH_BVH: limit = d.n_meshes; break;
             default: error = "unknown mesh kind"; return RAYNA_INVALID_ARGUMENT;
         }
-        const int64_t count = m.kind == RAYNA_MESH_TRIANGLE_LIST ? m.count : 1;
-        if (m.index < 0 || count < 0 || (int64_t)m.index + count > limit ||
-            (m.kind != RAYNA_MESH_TRIANGLE_LIST && m.index >= limit)) {
+        const bool ranged = m.kind == RAYNA_MESH_TRIANGLE_LIST || m.kind == RAYNA_MESH_BVH;
+        const int64_t count = ranged ? m.count : 1;
+        if (m.index < 0 || count < 0 || (int64_t)m.index + count > limit || (!ranged && m.index >= limit)) {
             error = "mesh index out of range";
             return RAYNA_INVALID_ARGUMENT;
+        }
+    }
+    for (int i = 0; i < d.n_meshes; ++i) {  // a BVH mesh holds primitive meshes only
+        const rayna_mesh &m = d.meshes[i];
+        if (m.kind != RAYNA_MESH_BVH) continue;
+        for (int j = m.index; j < m.index + m.count; ++j) {
+            const int k = d.meshes[j].kind;
+            if (k != RAYNA_MESH_SPHERE && k != RAYNA_MESH_TRIANGLE && k != RAYNA_MESH_PARALLELOGRAM &&
+                k != RAYNA_MESH_AXIS_BOX) {
+                error = "a BVH mesh holds sphere, triangle, parallelogram and axis-box meshes only";
+                return RAYNA_UNSUPPORTED;
+            }
         }
     }
     for (int i = 0; i < d.n_objects; ++i) {
@@ -809,10 +905,8 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
         if (ob.transform == 0) continue;
         const double *m = &d.transforms[12 * (size_t)(ob.transform - 1)];
         if (affine_is_identity(m)) continue;
-        const int mk = d.meshes[ob.mesh].kind;
-        if (mk != RAYNA_MESH_SPHERE && mk != RAYNA_MESH_TRIANGLE && mk != RAYNA_MESH_PARALLELOGRAM &&
-            mk != RAYNA_MESH_AXIS_BOX) {
-            error = "object transforms are supported on sphere, triangle, parallelogram and axis-box meshes";
+        if (d.meshes[ob.mesh].kind == RAYNA_MESH_SDF) {
+            error = "object transforms are not supported on SDF meshes";
             return RAYNA_UNSUPPORTED;
         }
         double inv[12];
@@ -832,46 +926,27 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
         const rayna_mesh &m = d.meshes[d.objects[o].mesh];
         bool bnd = true;
         switch (m.kind) {
-            case RAYNA_MESH_SPHERE: {
-                const double *s = &d.spheres[4 * (size_t)m.index];
-                const double a[3] = {s[0] - s[3], s[1] - s[3], s[2] - s[3]}, b[3] = {s[0] + s[3], s[1] + s[3], s[2] + s[3]};
-                obj_ref[o] = K.obj_tight[o] = box_new(a, b);
-                break;
-            }
-            case RAYNA_MESH_TRIANGLE: obj_ref[o] = K.obj_tight[o] = K.tri_box(m.index); break;
-            case RAYNA_MESH_TRIANGLE_LIST: {
+            case RAYNA_MESH_TRIANGLE_LIST:
+            case RAYNA_MESH_BVH: {
+                // MeshList (list.rs:176-181) folds its triangles' boxes from the origin; BvhMesh takes its root's
+                // box (bvh.rs:116-129), the origin-folded union, or the one mesh's own box
                 if (m.count <= 0) {
                     bnd = false;
                     break;
                 }
-                Aabb64 r = origin_box(), t = K.tri_box(m.index);
+                const bool list = m.kind == RAYNA_MESH_TRIANGLE_LIST;
+                Aabb64 r = origin_box(), t = list ? K.tri_box(m.index) : K.prim_box(m.index);
                 for (int i = 0; i < m.count; ++i) {
-                    const Aabb64 tb = K.tri_box(m.index + i);
+                    const Aabb64 tb = list ? K.tri_box(m.index + i) : K.prim_box(m.index + i);
                     grow(r, tb);
                     grow(t, tb);
                 }
-                obj_ref[o] = r;
+                obj_ref[o] = (!list && m.count == 1) ? t : r;
                 K.obj_tight[o] = t;
                 break;
             }
-            case RAYNA_MESH_PARALLELOGRAM: {
-                const double *p = &d.parallelograms[9 * (size_t)m.index];
-                double pts[12];
-                for (int i = 0; i < 3; ++i) {
-                    pts[i] = p[i];
-                    pts[3 + i] = p[i] + p[3 + i];
-                    pts[6 + i] = p[i] + p[6 + i];
-                    pts[9 + i] = p[i] + p[3 + i] + p[6 + i];
-                }
-                obj_ref[o] = K.obj_tight[o] = min_padded(points_box(pts, 4), 1e-6);
-                break;
-            }
-            case RAYNA_MESH_AXIS_BOX: {
-                const double *b = &d.boxes[6 * (size_t)m.index];
-                obj_ref[o] = K.obj_tight[o] = box_new(b, b + 3);
-                break;
-            }
-            default: bnd = false; break;
+            case RAYNA_MESH_SDF: bnd = false; break;
+            default: obj_ref[o] = K.obj_tight[o] = K.prim_box(d.objects[o].mesh); break;
         }
         if (bnd && K.obj_xf[o] >= 0) {  // ObjectTransform::calculate_aabb
             const double *m = &K.xf_fwd[12 * (size_t)K.obj_xf[o]];
@@ -885,7 +960,7 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
     for (size_t i = 0; i < unbounded.size(); ++i) {
         const rayna_object &ob = d.objects[unbounded[i]];
         const rayna_mesh &m = d.meshes[ob.mesh];
-        if (m.kind == RAYNA_MESH_TRIANGLE_LIST) continue;
+        if (m.kind == RAYNA_MESH_TRIANGLE_LIST || m.kind == RAYNA_MESH_BVH) continue;
         if (ob.kind != RAYNA_OBJ_SIMPLE) {
             error = "volumes with unbounded boundaries are not supported";
             return RAYNA_UNSUPPORTED;
@@ -916,7 +991,9 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
     {
         std::vector<int> lists;
         for (int m = 0; m < d.n_meshes; ++m)
-            if (d.meshes[m].kind == RAYNA_MESH_TRIANGLE_LIST && d.meshes[m].count > 0) lists.push_back(m);
+            if ((d.meshes[m].kind == RAYNA_MESH_TRIANGLE_LIST || d.meshes[m].kind == RAYNA_MESH_BVH) &&
+                d.meshes[m].count > 0)
+                lists.push_back(m);
         std::atomic<size_t> next{0};
         auto work = [&]() {
             for (;;) {
@@ -924,7 +1001,8 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
                 if (j >= lists.size()) break;
                 const rayna_mesh &me = d.meshes[lists[j]];
                 std::vector<Aabb64> tb(me.count);
-                for (int i = 0; i < me.count; ++i) tb[i] = K.tri_box(me.index + i);
+                for (int i = 0; i < me.count; ++i)
+                    tb[i] = me.kind == RAYNA_MESH_BVH ? K.prim_box(me.index + i) : K.tri_box(me.index + i);
                 K.blas[lists[j]] = build_engine_bvh(tb);
                 K.blas_tight[lists[j]] = Packer::refit(K.blas[lists[j]], [&](int it) { return tb[it]; });
             }
@@ -989,6 +1067,7 @@ bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vec
         }
         for (GXform x : p.xforms) {
             switch (x.inner_kind) {
+                case ITEM_LINK: x.inner_index += node_off; break;
                 case ITEM_SPHERE: x.inner_index += sphere_off; break;
                 case ITEM_TRIANGLE: x.inner_index += tri_off; break;
                 case ITEM_PARALLELOGRAM: x.inner_index += para_off; break;

@@ -54,6 +54,13 @@ struct PackedScene {
     std::vector<int32_t> rank_object, rank_local;  // depth-first rank -> (object, index in its mesh)
 };
 
+// True if some object of the packed scene is an instance (a mesh BVH under a non-identity transform).
+inline bool has_instances(const PackedScene &p) {
+    for (const GXform &x : p.xforms)
+        if (x.inner_kind == ITEM_LINK) return true;
+    return false;
+}
+
 // Validates and packs a scene.  Returns RAYNA_OK, or an error status with a message in `error`.
 rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &out, std::string &error);
 

@@ -127,27 +127,71 @@ __device__ __forceinline__ void expand(const SceneView &sv, const DRay &ray, flo
         if (code[k] != 0x7fffffff) st.push(code[k], key[k]);
 }
 
-// Tests entry `i` of leaf range `code` and keeps it if closer, or equally close with a higher rank.
-__device__ __forceinline__ void leaf_test(const SceneView &sv, const DRay &ray, int32_t code, int i, float leaf_t,
+// Instances (an object's mesh BVH under a non-identity transform, SimpleObject::full_intersect around
+// ObjectTransform, object/transform.rs): the lane traverses the instanced tree in object space with a closest hit
+// of its own, found as BvhMesh::intersect finds it (mesh/advanced/bvh.rs:63-102) from an interval [T_MIN, +inf).
+// Entering pushes the rest of the current leaf range, the closest hit so far (three words), a marker -- the
+// instance's item index with a NaN distance -- and the instanced root; the lane's ray becomes the object-space
+// ray.  When the marker is popped every entry of the instance has been, and the instance's closest hit, mapped
+// out (its distance measured from the world ray's origin), competes with the saved one.
+__device__ __forceinline__ void enter_instance(const GXform *x, int32_t item, int32_t code, int i, float leaf_t,
+                                               DRay &ray, Best &b, LaneStack &st) {
+    const int32_t start = (int32_t)((uint32_t)code & LEAF_CODE_START);
+    const int n = (int)(((uint32_t)code >> 27) & 7u) + 1;
+    if (i + 1 < n) st.push(leaf_code(start + i + 1, n - i - 1, true), leaf_t);
+    st.push((int32_t)b.ref, b.t);
+    st.push(b.rank, __uint_as_float(b.aux));
+    st.push((int32_t)b.sub, 0.0f);
+    st.push(item, __builtin_nanf(""));
+    st.push(x->inner_index, -__builtin_inff());
+    ray = object_ray(x, ray);
+    b = Best{FLT_MAX, -1, 0u, 0u, 0u};
+}
+
+__device__ __forceinline__ void finish_instance(const SceneView &sv, const PathBuffers &pb, uint32_t slot,
+                                                int32_t item, DRay &ray, Best &b, LaneStack &st) {
+    int32_t sub, rank, ref;
+    float unused, aux, t;
+    st.pop(&sub, &unused);
+    st.pop(&rank, &aux);
+    st.pop(&ref, &t);
+    Best w{t, rank, (uint32_t)ref, __float_as_uint(aux), (uint32_t)sub};
+    const DRay world = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
+                                    f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+    if (b.ref != 0u) {
+        const GXform *x = sv.xforms + sv.items[item].index;
+        const float tw = length(affine_point(x->fwd, at(ray, b.t)) - world.o);
+        if (tw <= w.t && (tw < w.t || b.rank > w.rank))
+            w = Best{tw, b.rank, make_ref(ITEM_XFORM, item), __float_as_uint(b.t), b.ref};
+    }
+    b = w;
+    ray = world;
+}
+
+// Tests entry `i` of leaf range `code` and keeps it if closer, or equally close with a higher rank.  Returns true
+// when the entry is an instance, which ends the pass over the range.  INST: the scene holds instances (without,
+// the instance code is compiled out).
+template <bool INST>
+__device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_t code, int i, float leaf_t,
                                           Best &b, LaneStack &st, const dm::Key &key, uint32_t pixel,
                                           uint32_t sample, uint32_t bounce) {
     const int32_t start = (int32_t)((uint32_t)code & LEAF_CODE_START);
     if (!((uint32_t)code & LEAF_CODE_ITEMS)) {
         const GTri g = sv.tris[start + i];
         float t, u, v, det;
-        if (!tri_t(g, ray, T_MIN, b.t, &t, &u, &v, &det)) return;
-        if (!(t < b.t || g.rank > b.rank)) return;
+        if (!tri_t(g, ray, T_MIN, b.t, &t, &u, &v, &det)) return false;
+        if (!(t < b.t || g.rank > b.rank)) return false;
         if (g.flags & 1) {
             f3 nn;
-            if (!tri_normal(sv.tri_normals[start + i], u, v, &nn)) return;
+            if (!tri_normal(sv.tri_normals[start + i], u, v, &nn)) return false;
         }
-        b = Best{t, g.rank, make_ref(ITEM_TRIANGLE, start + i), 0u};
-        return;
+        b = Best{t, g.rank, make_ref(ITEM_TRIANGLE, start + i), 0u, 0u};
+        return false;
     }
     const GItem it = sv.items[start + i];
     if (it.kind == ITEM_LINK) {
         st.push(it.index, leaf_t);
-        return;
+        return false;
     }
     // An object with a non-identity transform (SimpleObject / VolumetricObject::full_intersect around
     // ObjectTransform, object/transform.rs): its primitive or volume is tested against the object-space ray with
@@ -162,6 +206,10 @@ __device__ __forceinline__ void leaf_test(const SceneView &sv, const DRay &ray, 
     const GXform *x = nullptr;
     if (kind == ITEM_XFORM) {
         x = sv.xforms + j;
+        if (INST && x->inner_kind == ITEM_LINK) {
+            enter_instance(x, start + i, code, i, leaf_t, ray, b, st);
+            return true;
+        }
         kind = x->inner_kind;
         j = x->inner_index;
         r = object_ray(x, ray);
@@ -192,23 +240,25 @@ __device__ __forceinline__ void leaf_test(const SceneView &sv, const DRay &ray, 
         case ITEM_VOLUME: ok = volume_t(sv, sv.volumes[j], r, t_end, key, pixel, sample, bounce, &t); break;
         default: break;
     }
-    if (!ok) return;
+    if (!ok) return false;
     uint32_t aux = 0u;
     if (x) {
         aux = __float_as_uint(t);
         t = length(affine_point(x->fwd, at(r, t)) - ray.o);
-        if (!(t <= b.t)) return;
+        if (!(t <= b.t)) return false;
     }
     if (t < b.t || it.rank > b.rank)
         b = Best{t, it.rank,
-                 it.kind == ITEM_TRIANGLE ? make_ref(ITEM_TRIANGLE, it.index) : make_ref(it.kind, start + i), aux};
+                 it.kind == ITEM_TRIANGLE ? make_ref(ITEM_TRIANGLE, it.index) : make_ref(it.kind, start + i), aux,
+                 0u};
+    return false;
 }
 
 // Closest bounded hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501); the
 // unbounded objects are marched afterwards by k_march.  BATCH: the slots belong to the requests of bv, each
-// with its own scene root and RNG key.  Register budget: 96 VGPRs, the most that keeps the five waves per SIMD
-// the LDS stack allows.
-template <bool BATCH>
+// with its own scene root and RNG key.  INST: the scene holds instances.  Register budget: 96 VGPRs, the most that
+// keeps the five waves per SIMD the LDS stack allows.
+template <bool BATCH, bool INST>
 __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_extend(
     SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue, const uint32_t *count_ptr, uint32_t depth,
     int2 *overflow, uint32_t *work, ExtendTuning tune, BatchView bv) {
@@ -276,7 +326,7 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) 
                         pixel = slot % fp.n_pixels;
                         sample = fp.s0 + slot / fp.n_pixels;
                     }
-                    b = Best{FLT_MAX, -1, 0u, 0u};
+                    b = Best{FLT_MAX, -1, 0u, 0u, 0u};
                     st.sp = 0;
                     if (root >= 0) st.push(root, -__builtin_inff());
                     leaf = 0;
@@ -295,7 +345,10 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) 
                 float tent;
                 st.pop(&code, &tent);
                 RAYNA_STAT(pops, 1);
-                if (tent > b.t * BOX_SLACK) continue;
+                if (!(tent <= b.t * BOX_SLACK)) {  // culled, or an instance's marker (a NaN distance)
+                    if (INST && tent != tent) finish_instance(sv, pb, slot, code, ray, b, st);
+                    continue;
+                }
                 if (code < 0) {
                     leaf = code;
                     leaf_i = 0;
@@ -308,13 +361,15 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) 
             }
             while (leaf != 0) {  // the leaf's primitives, one per iteration
                 RAYNA_STAT(prims, 1);
-                leaf_test(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth);
-                if (++leaf_i == leaf_n) leaf = 0;
+                if (leaf_test<INST>(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth) ||
+                    ++leaf_i == leaf_n)
+                    leaf = 0;
             }
             if (st.sp == 0) {
                 pb.hit_t[slot] = b.t;
                 pb.hit_ref[slot] = b.ref;
                 pb.hit_aux[slot] = b.aux;
+                if (INST && b.sub) pb.hit_sub[slot] = b.sub;
                 if (pb.hit_rank) pb.hit_rank[slot] = b.rank;
                 active = false;
             }

@@ -12,6 +12,8 @@ Each function returns (Scene, Camera, RenderOpts) for one workload:
   cornell          the engine's CORNELL preset (preset.rs:606-683): five walls, a light and two boxes rotated
                    about their centres; NoSkybox
   transformed_mix  objects of every transformable kind under rotations, scalings and shears (parity workload)
+  rttnw            the engine's RTTNW_DEMO (preset.rs:394-605): a BvhMesh floor of 400 boxes, a BvhMesh cube of
+                   1000 balls rotated as an instance, glass, metal and medium-filled spheres, a light; NoSkybox
 All randomness comes from numpy's PCG64 seeded with `seed`, so every scene is a pure function of its arguments.
 """
 from __future__ import annotations
@@ -20,7 +22,7 @@ import math
 
 import numpy as np
 
-from .scene import (AxisBoxMesh, Camera, DielectricMaterial, IsotropicMaterial, LambertianMaterial, LightMaterial,
+from .scene import (AxisBoxMesh, BvhMesh, Camera, DielectricMaterial, IsotropicMaterial, LambertianMaterial, LightMaterial,
                     MetalMaterial, NoSkybox, ParallelogramMesh, RaymarchedIsosurfaceMesh, RenderOpts, Scene,
                     SdfMandelbulb, SdfSminSpheres, SimpleObject, SimpleSkybox, SphereMesh, Transform3,
                     TriangleListMesh, TriangleMesh, VolumetricObject, WhiteSkybox)
@@ -293,6 +295,42 @@ def transformed_mix(seed: int = 6, width: int = 96, height: int = 64, samples: i
     cam = Camera(pos=(0.0, 2.0, 7.0), fwd=(0.0, -0.25, -1.0), v_fov=50.0, focus_dist=6.0, defocus_angle=0.0)
     opts = RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
     return Scene(objects, SimpleSkybox), cam, opts
+
+
+def rttnw(seed: int = 0, width: int = 600, height: int = 600, samples: int = 32, ray_depth: int = 10,
+          balls: int = 1000):
+    """RTTNW_DEMO (preset.rs:394-605) with numpy's PCG64 for the engine's thread_rng: box heights U[0, 1] and ball
+    centres U[0, 1)^3 * 0.825.  The engine's noise-textured sphere (WorldNoiseTexture over Perlin noise) is given
+    a solid grey albedo of 0.5."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    objects = []
+    floor = []
+    for i in range(20):  # BOXES (FLOOR): COUNT = 20, WIDTH = 1
+        for j in range(20):
+            low = np.array([-10.0 + i, 0.0, -10.0 + j])
+            high = low + np.array([1.0, float(rng.uniform(0.0, 1.0)), 1.0])
+            floor.append(AxisBoxMesh(tuple(low), tuple(high)))
+    objects.append(SimpleObject(BvhMesh(floor), LambertianMaterial((0.48, 0.83, 0.53))))
+    objects.append(SimpleObject(ParallelogramMesh((1.23, 5.54, 1.47), (3.0, 0.0, 0.0), (0.0, 0.0, 2.65)),
+                                LightMaterial((7.0, 7.0, 7.0))))
+    objects.append(SimpleObject(SphereMesh((4.0, 4.0, 2.0), 0.5), LambertianMaterial((0.7, 0.3, 0.1))))
+    objects.append(SimpleObject(SphereMesh((2.6, 1.5, 0.45), 0.5), DielectricMaterial((1.0, 1.0, 1.0), 1.5, 1.0)))
+    objects.append(SimpleObject(SphereMesh((0.0, 1.5, 1.45), 0.5), MetalMaterial((0.8, 0.8, 0.9), 1.0)))
+    objects.append(SimpleObject(SphereMesh((3.6, 1.5, 1.45), 0.7), DielectricMaterial((1.0, 1.0, 1.0), 1.5, 0.0)))
+    objects.append(VolumetricObject(SphereMesh((3.6, 1.5, 1.45), 0.6999), IsotropicMaterial((0.2, 0.4, 0.9), 0.3),
+                                    2.0))
+    objects.append(SimpleObject(SphereMesh((2.2, 2.8, 3.0), 0.8), LambertianMaterial((0.5, 0.5, 0.5))))
+    spread = 0.825  # CUBE OF BALLS
+    cube = [SphereMesh(tuple(float(x) for x in rng.random(3) * spread), 0.1) for _ in range(balls)]
+    objects.append(SimpleObject(BvhMesh(cube), LambertianMaterial((0.85, 0.85, 0.85)),
+                                Transform3.from_scale_rotation_translation(
+                                    (1.0, 1.0, 1.0), (0.0, 1.0, 0.0), 15.0,
+                                    np.array([-1.0, 2.7, 3.95]) + spread)))
+    fwd = np.array([-1.0, 0.0, 3.0])
+    cam = Camera(pos=(4.78, 2.78, -6.0), fwd=tuple(fwd / np.linalg.norm(fwd)), v_fov=40.0, focus_dist=1.0,
+                 defocus_angle=0.0)
+    opts = RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
+    return Scene(objects, NoSkybox), cam, opts
 
 
 def mixed_primitives(seed: int = 5, width: int = 96, height: int = 64, samples: int = 4, ray_depth: int = 6):

@@ -3,7 +3,8 @@
 Names and argument meanings follow the engine:
   materials  rayna_engine/src/material/{lambertian,metal,dielectric,isotropic,light}.rs
   meshes     rayna_engine/src/mesh/primitive/{sphere,triangle,axis_box}.rs, mesh/planar/parallelogram.rs,
-             mesh/advanced/list.rs (a list of triangles), mesh/isosurface/raymarched.rs
+             mesh/advanced/list.rs (a list of triangles), mesh/advanced/bvh.rs (a BVH over primitive meshes),
+             mesh/isosurface/raymarched.rs
   objects    rayna_engine/src/object/{simple,volumetric}.rs, with object/transform.rs (ObjectTransform)
   scene      rayna_engine/src/scene/mod.rs:11-24 (StandardScene: objects + skybox)
   camera     rayna_engine/src/scene/camera.rs:10-38 (angles given in degrees, as Angle::from_degrees)
@@ -147,7 +148,15 @@ class RaymarchedIsosurfaceMesh:
     epsilon: float = 1e-7
 
 
-MeshT = Union[SphereMesh, TriangleMesh, TriangleListMesh, ParallelogramMesh, AxisBoxMesh, RaymarchedIsosurfaceMesh]
+@dataclass
+class BvhMesh:
+    """`BvhMesh::new(meshes)` (mesh/advanced/bvh.rs:28-45): an SAH tree over sphere, triangle, parallelogram and
+    axis-box meshes, intersected as one mesh (one material, one transform)."""
+    meshes: List[Union[SphereMesh, TriangleMesh, ParallelogramMesh, AxisBoxMesh]]
+
+
+MeshT = Union[SphereMesh, TriangleMesh, TriangleListMesh, ParallelogramMesh, AxisBoxMesh, RaymarchedIsosurfaceMesh,
+              BvhMesh]
 
 # ------------------------------------------------------------------------------------------ transforms
 
@@ -226,7 +235,15 @@ def mesh_centre(m) -> np.ndarray:
     if isinstance(m, AxisBoxMesh):
         a, b = np.asarray(m.a, float), np.asarray(m.b, float)
         return (np.minimum(a, b) + np.maximum(a, b)) / 2.0
-    return np.zeros(3)  # SDFs (raymarched.rs:69); triangle lists keep their own centre, unused here
+    if isinstance(m, BvhMesh):  # the mean of the meshes' centres (bvh.rs:36-44)
+        c = np.zeros(3)
+        for sub in m.meshes:
+            c = c + mesh_centre(sub)
+        return c / len(m.meshes)
+    if isinstance(m, TriangleListMesh):  # MeshList::new sums its meshes' centres without dividing (list.rs:33-36)
+        v = np.asarray(m.vertices, float).reshape(-1, 3, 3)
+        return ((v[:, 0] + v[:, 1] + v[:, 2]) / 3.0).sum(axis=0)
+    return np.zeros(3)  # SDFs (raymarched.rs:69)
 
 
 # ------------------------------------------------------------------------------------------ objects
@@ -386,6 +403,12 @@ def _build_desc(scene: Scene):
         elif isinstance(m, AxisBoxMesh):
             meshes.append(A.Mesh(A.MESH_AXIS_BOX, len(boxes), 1, 0))
             boxes.append([*m.a, *m.b])
+        elif isinstance(m, BvhMesh):
+            subs = [add_mesh(sub) for sub in m.meshes]
+            for sub in m.meshes:
+                if not isinstance(sub, (SphereMesh, TriangleMesh, ParallelogramMesh, AxisBoxMesh)):
+                    raise TypeError(f"a BvhMesh holds primitive meshes, not {type(sub).__name__}")
+            meshes.append(A.Mesh(A.MESH_BVH, subs[0] if subs else len(meshes), len(subs), 0))
         elif isinstance(m, RaymarchedIsosurfaceMesh):
             f = m.sdf
             d = A.Sdf()
