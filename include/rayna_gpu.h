@@ -34,8 +34,9 @@ typedef enum rayna_status {
 } rayna_status;
 
 /* ---------------------------------------------------------------- materials
- * rayna_engine/src/material/mod.rs:168-175 (MaterialInstance) with SolidTexture albedo
- * (rayna_engine/src/texture/solid.rs:29).
+ * rayna_engine/src/material/mod.rs:168-175 (MaterialInstance).  The albedo (emission for LIGHT) is the solid
+ * `colour` (SolidTexture, rayna_engine/src/texture/solid.rs:29) when `texture` is 0, else the texture
+ * textures[texture - 1] of the scene.
  *   LAMBERTIAN  lambertian.rs:28-48   colour = albedo
  *   METAL       metal.rs:65-95        colour = albedo, fuzz
  *   DIELECTRIC  dielectric.rs:113-175 colour = albedo, refractive_index, density
@@ -53,6 +54,7 @@ enum {
 typedef struct rayna_material {
     int32_t kind;
     float colour[3];
+    int32_t texture;  /* 0: the solid colour; k: textures[k - 1] */
     double fuzz;
     double refractive_index;
     double density;
@@ -143,11 +145,57 @@ typedef struct rayna_object {
     double density;
 } rayna_object;
 
+/* ---------------------------------------------------------------- textures
+ * rayna_engine/src/texture/mod.rs:26-37 (TextureInstance):
+ *   SOLID          SolidTexture { albedo = colour }                        texture/solid.rs:8-30
+ *   WORLD_CHECKER  WorldCheckerTexture { offset xyz, even, odd, scale }    texture/checker.rs:12-26
+ *   UV_CHECKER     UvCheckerTexture { offset xy, even, odd, scale }        texture/checker.rs:28-42
+ *   IMAGE          ImageTexture { image, scale = image_scale, offset xy }  texture/image.rs:7-40
+ *   UV_NOISE       UvNoiseTexture, the noise at (u, v, 0)                  texture/noise.rs:62-70
+ *   WORLD_NOISE    WorldNoiseTexture, the noise at the hit position        texture/noise.rs:84-92
+ *   LOCAL_NOISE    LocalNoiseTexture, the noise at the hit's local position texture/noise.rs:106-114
+ * `odd` and `even` name textures as a material does (k: textures[k - 1]) and must name earlier textures.
+ * The noise textures take ScalePoint(noise).set_scale(scale) of an improved Perlin noise (K. Perlin, "Improving
+ * Noise", SIGGRAPH 2002) whose permutation is drawn from `noise_seed[i]`; the engine's `noise` crate generates
+ * its own permutation, so the values differ from the engine's for the same seed.  `noise_rgb` = 0 is
+ * ColourSource::Greyscale (seed 0), 1 ColourSource::Rgb (seeds 0, 1, 2); both map [-1, 1] to [0, 1].
+ */
+enum {
+    RAYNA_TEX_SOLID = 0,
+    RAYNA_TEX_WORLD_CHECKER = 1,
+    RAYNA_TEX_UV_CHECKER = 2,
+    RAYNA_TEX_IMAGE = 3,
+    RAYNA_TEX_UV_NOISE = 4,
+    RAYNA_TEX_WORLD_NOISE = 5,
+    RAYNA_TEX_LOCAL_NOISE = 6
+};
+
+typedef struct rayna_texture {
+    int32_t kind;
+    float colour[3];
+    int32_t odd, even;     /* checkers */
+    int32_t image;         /* IMAGE: index into the scene's images */
+    int32_t noise_rgb;     /* noise: 0 greyscale, 1 rgb */
+    uint32_t noise_seed[3];
+    int32_t reserved;
+    double scale;          /* checkers: cell size; noise: point scale */
+    double offset[3];      /* WORLD_CHECKER xyz; UV_CHECKER and IMAGE xy */
+    double image_scale[2]; /* IMAGE: uv scale */
+} rayna_texture;
+
+/* An image (core/image.rs): width x height RGB floats, row-major (pixel (x, y) at rgb[3 * (y * width + x)]),
+ * sampled bilinearly with clamped corners as Image::get_bilinear does (core/image.rs:86-110). */
+typedef struct rayna_image {
+    uint32_t width, height;
+    const float *rgb;
+} rayna_image;
+
 /* ---------------------------------------------------------------- skybox (skybox/mod.rs:196-204) */
 enum {
-    RAYNA_SKY_NONE = 0,   /* NoSkybox     skybox/none.rs:8-13 */
-    RAYNA_SKY_SIMPLE = 1, /* SimpleSkybox skybox/simple.rs:12-22 */
-    RAYNA_SKY_WHITE = 2   /* WhiteSkybox  skybox/simple.rs:26-30 */
+    RAYNA_SKY_NONE = 0,   /* NoSkybox       skybox/none.rs:8-13 */
+    RAYNA_SKY_SIMPLE = 1, /* SimpleSkybox   skybox/simple.rs:12-22 */
+    RAYNA_SKY_WHITE = 2,  /* WhiteSkybox    skybox/simple.rs:26-30 */
+    RAYNA_SKY_HDRI = 3    /* HdrImageSkybox skybox/hdri.rs:8-27: images[sky_image] */
 };
 
 /* The scene: the object list in insertion order (the order `ObjectList::new_uncorrected` sees), and
@@ -175,6 +223,11 @@ typedef struct rayna_scene_desc {
     int32_t skybox;
     const double *transforms;
     int32_t n_transforms;
+    const rayna_texture *textures;
+    int32_t n_textures;
+    const rayna_image *images;
+    int32_t n_images;
+    int32_t sky_image;     /* RAYNA_SKY_HDRI: the sky's image */
 } rayna_scene_desc;
 
 /* ---------------------------------------------------------------- camera (scene/camera.rs:10-38); angles in radians */

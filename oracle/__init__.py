@@ -63,6 +63,10 @@ def lib() -> C.CDLL:
         L.twin_scatter_f64.argtypes = [C.POINTER(A.Material), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.twin_sky.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_float)]
+        L.twin_texture_value.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_float)]
+        L.twin_scene_sky.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_float)]
+        L.twin_noise_permutation.argtypes = [C.c_uint32, C.POINTER(C.c_int32)]
         L.twin_camera_ray.argtypes = [C.POINTER(A.CameraDesc), C.POINTER(A.RenderOptsDesc), C.c_uint32, C.c_uint32,
                                       C.c_uint32, C.POINTER(C.c_double)]
         L.twin_sdf_eval.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.c_int64,
@@ -167,6 +171,20 @@ class TwinScene:
         return dict(rank=int(rank), t=out[0], pos=out[1:4], normal=out[4:7], ray_normal=out[7:10],
                     front_face=bool(out[10]), material=int(mat.value))
 
+    def texture_value(self, tex: int, pos, pos_l=(0.0, 0.0, 0.0), uv=(0.0, 0.0), precision: int = 64):
+        """The value of texture `tex` (0-based, the scene description's order) at a hit."""
+        p, pl, u = (np.ascontiguousarray(v, np.float64) for v in (pos, pos_l, uv))
+        out = np.zeros(3, np.float32)
+        lib().twin_texture_value(self.h, tex, precision, _p(p, C.c_double), _p(pl, C.c_double), _p(u, C.c_double),
+                                 _p(out, C.c_float))
+        return out
+
+    def sky(self, d, precision: int = 64):
+        dd = np.ascontiguousarray(d, np.float64)
+        out = np.zeros(3, np.float32)
+        lib().twin_scene_sky(self.h, precision, _p(dd, C.c_double), _p(out, C.c_float))
+        return out
+
     def sdf_eval(self, sdf: int, points, precision: int = 64):
         p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
         out = np.zeros(len(p))
@@ -210,6 +228,12 @@ def scatter_f64(material, ray, hit, u):
     ok = lib().twin_scatter_f64(C.byref(m), _p(r, C.c_double), _p(h, C.c_double), _p(uu, C.c_double),
                                 _p(out, C.c_double))
     return bool(ok), out[:3], out[3:]
+
+
+def noise_permutation(seed: int) -> np.ndarray:
+    out = np.zeros(512, np.int32)
+    lib().twin_noise_permutation(seed, _p(out, C.c_int32))
+    return out
 
 
 def sky(kind: int, d):

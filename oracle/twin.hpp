@@ -388,6 +388,15 @@ struct Scene {
     // ObjectTransform per object: index into xf_fwd / xf_inv (12 doubles each, row-major 3x4), -1 identity
     std::vector<int> obj_xf;
     std::vector<double> xf_fwd, xf_inv;
+    // textures, images (row-major RGB) and the noise permutations of each texture (3 x 512 entries)
+    std::vector<rayna_texture> textures;
+    struct Img {
+        uint32_t w, h;
+        std::vector<float> rgb;
+    };
+    std::vector<Img> images;
+    std::vector<std::vector<int32_t>> perms;
+    int sky_image = 0;
 
     // object graph
     std::vector<int> bounded, unbounded;  // object indices, input order
@@ -406,6 +415,27 @@ struct Scene {
     bool ok = false;
     const char *error = "";
 };
+
+// The permutation of an improved Perlin noise for a seed: 0..255 shuffled by Fisher-Yates with SplitMix64 draws
+// (Steele, Lea and Flood 2014), repeated to 512 entries.  (The engine's noise crate draws its own permutation.)
+inline void noise_permutation(uint32_t seed, int32_t *out) {
+    uint64_t state = seed;
+    auto next = [&]() {
+        uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    int32_t p[256];
+    for (int i = 0; i < 256; ++i) p[i] = i;
+    for (int i = 255; i > 0; --i) {
+        const int j = (int)(next() % (uint64_t)(i + 1));
+        const int32_t t = p[i];
+        p[i] = p[j];
+        p[j] = t;
+    }
+    for (int i = 0; i < 512; ++i) out[i] = p[i & 255];
+}
 
 // ---- mesh AABBs (reference and tight) -------------------------------------------------------------
 
@@ -545,6 +575,18 @@ inline void build_scene(Scene &s, const rayna_scene_desc &d, bool literal) {
     s.boxes.assign(d.boxes, d.boxes + 6 * (size_t)d.n_boxes);
     s.sdfs.assign(d.sdfs, d.sdfs + d.n_sdfs);
     s.skybox = d.skybox;
+    s.sky_image = d.sky_image;
+    for (int i = 0; i < d.n_images; ++i) {
+        const rayna_image &im = d.images[i];
+        s.images.push_back(Scene::Img{im.width, im.height,
+                                      std::vector<float>(im.rgb, im.rgb + 3 * (size_t)im.width * im.height)});
+    }
+    s.textures.assign(d.textures, d.textures + d.n_textures);
+    for (const rayna_texture &t : s.textures) {
+        std::vector<int32_t> p(3 * 512, 0);
+        for (int k = 0; k < 3; ++k) noise_permutation(t.noise_seed[k], p.data() + 512 * k);
+        s.perms.push_back(std::move(p));
+    }
 
     const int no = d.n_objects;
     s.obj_xf.assign(no, -1);
@@ -697,6 +739,7 @@ struct Hit {
     R t;
     V3<R> pos, normal, ray_normal;
     R uv[2] = {R(0), R(0)};  // Intersection::uv (shared/intersect.rs:7-43)
+    V3<R> pos_l{R(0), R(0), R(0)};  // Intersection::pos_l
     int side = 0;            // Intersection::side
     bool front_face;
     int object;
@@ -772,6 +815,7 @@ inline bool sphere_intersect(const double *sp, const Ray<R> &ray, const Interval
     const R phi = Math<R>::atan2(-local.z, local.x) + PI;
     h.uv[0] = phi / (R(2) * PI);
     h.uv[1] = theta / PI;
+    h.pos_l = local;  // sphere.rs:91
     h.side = 0;
     return true;
 }
@@ -808,6 +852,7 @@ inline bool triangle_intersect(const double *tv, const double *tn, const Ray<R> 
     h.ray_normal = n * -signum(det);
     h.uv[0] = u;  // triangle.rs:104
     h.uv[1] = v;
+    h.pos_l = V3<R>{b0, u, v};  // triangle.rs:95-101: (1 - u - v, u, v)
     h.side = 0;
     return true;
 }
@@ -840,6 +885,7 @@ inline bool parallelogram_intersect(const double *pp, const Ray<R> &ray, const I
     h.ray_normal = (-n) * signum(denom);
     h.uv[0] = alpha;  // planar/mod.rs:191
     h.uv[1] = beta;
+    h.pos_l = pl;  // planar/mod.rs:176
     h.side = 0;
     return true;
 }
@@ -884,6 +930,7 @@ inline bool axis_box_intersect(const double *bp, const Ray<R> &ray, const Interv
             h.uv[0] = (ub / radius[b] + R(1)) / R(2);
             h.uv[1] = (uc / radius[c] + R(1)) / R(2);
             h.side = 2 * a + (sgn[a] > R(0) ? 1 : 0);
+            h.pos_l = h.pos - centre;  // axis_box.rs:122
             return true;
         }
     }
@@ -969,6 +1016,7 @@ inline bool sdf_intersect(const Scene &s, const rayna_sdf &f, const Ray<R> &ray,
             h.uv[0] = R(0);  // raymarched.rs:106,109: uv zero, side = the march iteration
             h.uv[1] = R(0);
             h.side = i;
+            h.pos_l = p;  // raymarched.rs:105
             h.normal = n;
             h.ray_normal = n;
             h.front_face = !std::signbit(dist);
@@ -1081,6 +1129,7 @@ inline bool object_intersect_local(const Ctx<R> &c, int o, const Ray<R> &ray, co
         draw4<R>(c.key, c.pixel, c.sample, c.bounce, SLOT_MEDIUM_UV, b);
         h.uv[0] = b[0];
         h.uv[1] = b[1];
+        h.pos_l = h.pos;  // volumetric.rs:141-142
         h.side = 0;
         h.front_face = true;
         h.rank = s.obj_rank[o];
@@ -1109,6 +1158,7 @@ inline bool object_intersect(const Ctx<R> &c, int o, const Ray<R> &ray, const In
     h.normal = normalize(affine_vector<R>(fwd, h.normal));
     h.ray_normal = normalize(affine_vector<R>(fwd, h.ray_normal));
     h.pos = affine_point<R>(fwd, h.pos);
+    h.pos_l = affine_point<R>(fwd, h.pos_l);  // transform.rs:120-121 maps pos_l as a point too
     h.t = length(h.pos - ray.pos);
     return !iv.has_end || h.t <= iv.end;
 }
@@ -1214,16 +1264,17 @@ inline bool scatter(const rayna_material &m, const Ray<R> &ray, const Hit<R> &h,
     }
 }
 
-// Material::emitted_light (light.rs:247-249; default black mod.rs:100-103)
-inline Col emitted(const rayna_material &m) {
-    if (m.kind == RAYNA_MAT_LIGHT) return Col{m.colour[0], m.colour[1], m.colour[2]};
+// Material::emitted_light (light.rs:247-249; default black mod.rs:100-103), with the material's colour at the hit.
+inline Col emitted(const rayna_material &m, const Col &colour) {
+    if (m.kind == RAYNA_MAT_LIGHT) return colour;
     return Col{0, 0, 0};
 }
 
-// Material::reflected_light as applied to the future colour: out = future * albedo [* exp(-density d)].
+// Material::reflected_light as applied to the future colour: out = future * albedo [* exp(-density d)], with the
+// material's colour at the hit.
 template <class R>
-inline Col reflected(const rayna_material &m, const Ray<R> &ray, const Hit<R> &h, const Col &future) {
-    const Col albedo{m.colour[0], m.colour[1], m.colour[2]};
+inline Col reflected(const rayna_material &m, const Ray<R> &ray, const Hit<R> &h, const Col &albedo,
+                     const Col &future) {
     switch (m.kind) {
         case RAYNA_MAT_LAMBERTIAN:
         case RAYNA_MAT_METAL: return future * albedo;
@@ -1252,6 +1303,146 @@ inline Col sky(int kind, const Ray<R> &ray) {
         }
         default: return Col{0, 0, 0};
     }
+}
+
+// ================================================================================================
+// Textures (texture/*.rs) and the image skybox (skybox/hdri.rs)
+// ================================================================================================
+
+// Image::get_bilinear (core/image.rs:86-110): floor and ceil clamped to the image, the fraction from the clamped
+// floor (a NaN coordinate reads index 0, Rust's saturating cast); Colour::lerp = a + (b - a) * t with the
+// channel difference times t in the coordinate's precision (colour.rs:177-181, shared/math.rs:9-12).
+template <class R>
+inline Col image_bilinear(const Scene::Img &im, R px, R py) {
+    auto clampf = [](R v, uint32_t n) { return std::fmin(std::fmax(v, R(0)), (R)(n - 1u)); };
+    auto index = [&](R v, uint32_t n) {
+        const R c = clampf(v, n);
+        return c == c ? (uint32_t)c : 0u;
+    };
+    const R fx = std::floor(px), fy = std::floor(py);
+    const uint32_t x1 = index(fx, im.w), x2 = index(std::ceil(px), im.w);
+    const uint32_t y1 = index(fy, im.h), y2 = index(std::ceil(py), im.h);
+    const R xl = px - clampf(fx, im.w), yl = py - clampf(fy, im.h);
+    auto texel = [&](uint32_t x, uint32_t y) {
+        const float *t = &im.rgb[3 * ((size_t)y * im.w + x)];
+        return Col{t[0], t[1], t[2]};
+    };
+    auto lerp = [](const Col &a, const Col &b, R t) {
+        auto ch = [&](float x, float y) { return x + (float)((R)(y - x) * t); };
+        return Col{ch(a.r, b.r), ch(a.g, b.g), ch(a.b, b.b)};
+    };
+    const Col c11 = texel(x1, y1), c12 = texel(x1, y2), c21 = texel(x2, y1), c22 = texel(x2, y2);
+    return lerp(lerp(c11, c21, xl), lerp(c12, c22, xl), yl);
+}
+
+// sphere_uv (sphere.rs:114-121) of a unit vector.
+template <class R>
+inline void unit_sphere_uv(const V3<R> &p, R uv[2]) {
+    const R PI = sizeof(R) == 4 ? R(3.14159274101257324f) : R(3.14159265358979323846);
+    const R theta = Math<R>::acos(-p.y);
+    const R phi = Math<R>::atan2(-p.z, p.x) + PI;
+    uv[0] = phi / (R(2) * PI);
+    uv[1] = theta / PI;
+}
+
+// Improved Perlin noise (K. Perlin, "Improving Noise", SIGGRAPH 2002) over a 512-entry permutation p.
+template <class R>
+inline R perlin(const int32_t *p, R x, R y, R z) {
+    auto lattice = [](R f) { return std::isfinite(f) ? (int)(f - R(256) * std::floor(f / R(256))) : 0; };
+    auto fade = [](R t) { return t * t * t * (t * (t * R(6) - R(15)) + R(10)); };
+    auto lerp = [](R t, R a, R b) { return a + t * (b - a); };
+    auto grad = [](int hash, R gx, R gy, R gz) {
+        const int h = hash & 15;
+        const R u = h < 8 ? gx : gy;
+        const R v = h < 4 ? gy : (h == 12 || h == 14 ? gx : gz);
+        return ((h & 1) == 0 ? u : -u) + ((h & 2) == 0 ? v : -v);
+    };
+    const R fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
+    const int X = lattice(fx), Y = lattice(fy), Z = lattice(fz);
+    x -= fx;
+    y -= fy;
+    z -= fz;
+    const R u = fade(x), v = fade(y), w = fade(z);
+    const int A = p[X] + Y, AA = p[A] + Z, AB = p[A + 1] + Z, B = p[X + 1] + Y, BA = p[B] + Z, BB = p[B + 1] + Z;
+    const R one(1);
+    return lerp(w,
+                lerp(v, lerp(u, grad(p[AA], x, y, z), grad(p[BA], x - one, y, z)),
+                     lerp(u, grad(p[AB], x, y - one, z), grad(p[BB], x - one, y - one, z))),
+                lerp(v, lerp(u, grad(p[AA + 1], x, y, z - one), grad(p[BA + 1], x - one, y, z - one)),
+                     lerp(u, grad(p[AB + 1], x, y - one, z - one), grad(p[BB + 1], x - one, y - one, z - one))));
+}
+
+// ColourSource::get (texture/noise.rs:36-45) of ScalePoint(Perlin).set_scale(scale): the noise as a channel,
+// mapped from [-1, 1] to [0, 1] in channel precision.
+template <class R>
+inline Col noise_colour(const Scene &s, int ti, const rayna_texture &t, V3<R> q) {
+    q = q * (R)(float)t.scale;
+    const int32_t *p = s.perms[ti].data();
+    const float n0 = (float)perlin<R>(p, q.x, q.y, q.z);
+    float n1 = n0, n2 = n0;
+    if (t.noise_rgb) {
+        n1 = (float)perlin<R>(p + 512, q.x, q.y, q.z);
+        n2 = (float)perlin<R>(p + 1024, q.x, q.y, q.z);
+    }
+    return Col{n0 / 2.0f + 0.5f, n1 / 2.0f + 0.5f, n2 / 2.0f + 0.5f};
+}
+
+// do_checker (texture/checker.rs:44-63): floors summed from 0; even when rem_euclid(sum, 2) < 1.
+template <class R>
+inline bool checker_even(R sum) {
+    R r = std::fmod(sum, R(2));
+    if (r < R(0)) r = r + R(2);
+    return r < R(1);
+}
+
+// Texture::value of texture ti (0-based) at a hit.  Parameters are rounded to float, as the device holds them.
+template <class R>
+inline Col texture_value(const Scene &s, int ti, const Hit<R> &h) {
+    for (;;) {
+        const rayna_texture &t = s.textures[ti];
+        const R scale = (R)(float)t.scale;
+        const R ox = (R)(float)t.offset[0], oy = (R)(float)t.offset[1], oz = (R)(float)t.offset[2];
+        switch (t.kind) {
+            case RAYNA_TEX_SOLID: return Col{t.colour[0], t.colour[1], t.colour[2]};
+            case RAYNA_TEX_WORLD_CHECKER: {  // checker.rs:20-25
+                const R sum = ((R(0) + std::floor(h.pos.x / scale + ox)) + std::floor(h.pos.y / scale + oy)) +
+                              std::floor(h.pos.z / scale + oz);
+                ti = (checker_even(sum) ? t.even : t.odd) - 1;
+                break;
+            }
+            case RAYNA_TEX_UV_CHECKER: {  // checker.rs:36-41
+                const R sum = (R(0) + std::floor(h.uv[0] / scale + ox)) + std::floor(h.uv[1] / scale + oy);
+                ti = (checker_even(sum) ? t.even : t.odd) - 1;
+                break;
+            }
+            case RAYNA_TEX_IMAGE: {  // image.rs:29-39
+                const Scene::Img &im = s.images[t.image];
+                const R u = ox + h.uv[0] * (R)(float)t.image_scale[0];
+                const R v = R(1) - (oy + h.uv[1] * (R)(float)t.image_scale[1]);
+                return image_bilinear<R>(im, u * (R)im.w, v * (R)im.h);
+            }
+            case RAYNA_TEX_UV_NOISE: return noise_colour<R>(s, ti, t, V3<R>{h.uv[0], h.uv[1], R(0)});
+            case RAYNA_TEX_WORLD_NOISE: return noise_colour<R>(s, ti, t, h.pos);
+            default: return noise_colour<R>(s, ti, t, h.pos_l);  // RAYNA_TEX_LOCAL_NOISE
+        }
+    }
+}
+
+// The material's colour at a hit: the solid colour, or the texture's value.
+template <class R>
+inline Col material_colour(const Scene &s, const rayna_material &m, const Hit<R> &h) {
+    if (m.texture <= 0) return Col{m.colour[0], m.colour[1], m.colour[2]};
+    return texture_value<R>(s, m.texture - 1, h);
+}
+
+// The sky for a missed ray (skybox/*.rs), HdrImageSkybox included (hdri.rs:17-27).
+template <class R>
+inline Col sky_colour(const Scene &s, const Ray<R> &ray) {
+    if (s.skybox != RAYNA_SKY_HDRI) return sky<R>(s.skybox, ray);
+    R uv[2];
+    unit_sphere_uv<R>(ray.dir, uv);
+    const Scene::Img &im = s.images[s.sky_image];
+    return image_bilinear<R>(im, uv[0] * (R)im.w, (R(1) - uv[1]) * (R)im.h);
 }
 
 // ================================================================================================
@@ -1359,9 +1550,10 @@ struct Tracer {
     Col recursive(const Ray<R> &ray, uint32_t depth) {
         if (depth > depth_max) return Col{0, 0, 0};
         Hit<R> h;
-        if (!intersect(ray, depth, h)) return sky<R>(s->skybox, ray);
+        if (!intersect(ray, depth, h)) return sky_colour<R>(*s, ray);
         const rayna_material &m = s->materials[h.material];
-        const Col le = emitted(m);
+        const Col colour = material_colour<R>(*s, m, h);
+        const Col le = emitted(m, colour);
         R u[4];
         draw4<R>(key, pixel, sample, depth, SLOT_SCATTER, u);
         V3<R> dir;
@@ -1369,7 +1561,7 @@ struct Tracer {
         if (scatter(m, ray, h, u, &dir)) {
             const Ray<R> next = Ray<R>::make(h.pos, dir);
             const Col future = recursive(next, depth + 1);
-            scattered = reflected(m, ray, h, future);
+            scattered = reflected(m, ray, h, colour, future);
         }
         const Col mean = (Col{0, 0, 0} + scattered) / 1.0f;
         return le + mean;
@@ -1381,16 +1573,17 @@ struct Tracer {
         for (uint32_t depth = 0; depth <= depth_max; ++depth) {
             Hit<R> h;
             if (!intersect(ray, depth, h)) {
-                L = L + T * sky<R>(s->skybox, ray);
+                L = L + T * sky_colour<R>(*s, ray);
                 break;
             }
             const rayna_material &m = s->materials[h.material];
-            L = L + T * emitted(m);
+            const Col colour = material_colour<R>(*s, m, h);
+            L = L + T * emitted(m, colour);
             R u[4];
             draw4<R>(key, pixel, sample, depth, SLOT_SCATTER, u);
             V3<R> dir;
             if (!scatter(m, ray, h, u, &dir)) break;
-            T = reflected(m, ray, h, T);
+            T = reflected(m, ray, h, colour, T);
             ray = Ray<R>::make(h.pos, dir);
         }
         return L;
@@ -1400,7 +1593,7 @@ struct Tracer {
     // on a miss.  mode: RAYNA_MODE_* (render_opts.rs:32-52).
     Col mode_colour(const Ray<R> &ray, int mode) {
         Hit<R> h;
-        if (!intersect(ray, 0, h)) return sky<R>(s->skybox, ray);
+        if (!intersect(ray, 0, h)) return sky_colour<R>(*s, ray);
         static const float COLOURS[13][3] = {{1.0f, 1.0f, 1.0f}, {1.0f, 0.0f, 0.0f}, {1.0f, 0.5f, 0.0f},
                                              {1.0f, 1.0f, 0.0f}, {0.5f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f},
                                              {0.0f, 1.0f, 0.5f}, {0.0f, 1.0f, 1.0f}, {0.0f, 0.5f, 1.0f},
@@ -1456,11 +1649,14 @@ struct Tracer {
             if (!intersect(ray, depth, h)) break;
             const rayna_material &m = s->materials[h.material];
             double *g = grad + 3 * (size_t)h.material;
+            const Col colour = material_colour<R>(*s, m, h);
+            const float col[3] = {colour.r, colour.g, colour.b};
+            const bool graded = m.texture <= 0;  // a textured material's colour is not a parameter
             if (m.kind == RAYNA_MAT_LIGHT) {
                 const float t[3] = {T.r, T.g, T.b};
                 for (int c = 0; c < 3; ++c) {
-                    g[c] += d[c] * (double)t[c];
-                    rem[c] -= (double)t[c] * (double)m.colour[c];
+                    if (graded) g[c] += d[c] * (double)t[c];
+                    rem[c] -= (double)t[c] * (double)col[c];
                 }
             }
             R u[4];
@@ -1470,10 +1666,10 @@ struct Tracer {
             const bool albedo_scales = m.kind == RAYNA_MAT_LAMBERTIAN || m.kind == RAYNA_MAT_METAL ||
                                        m.kind == RAYNA_MAT_ISOTROPIC ||
                                        (m.kind == RAYNA_MAT_DIELECTRIC && !h.front_face);
-            if (albedo_scales && depth < depth_max)
+            if (albedo_scales && graded && depth < depth_max)
                 for (int c = 0; c < 3; ++c)
                     if (m.colour[c] != 0.0f) g[c] += d[c] * rem[c] / (double)m.colour[c];
-            T = reflected(m, ray, h, T);
+            T = reflected(m, ray, h, colour, T);
             ray = Ray<R>::make(h.pos, dir);
         }
     }

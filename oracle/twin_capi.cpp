@@ -236,7 +236,7 @@ int32_t twin_scatter_f64(const rayna_material *m, const double ray_in[6], const 
     V3<double> dir;
     const int ok = scatter<double>(*m, ray, h, u, &dir) ? 1 : 0;
     if (ok) dir = normalize(dir);
-    const Col a = reflected<double>(*m, ray, h, Col{1, 1, 1});
+    const Col a = reflected<double>(*m, ray, h, Col{m->colour[0], m->colour[1], m->colour[2]}, Col{1, 1, 1});
     out[0] = dir.x;
     out[1] = dir.y;
     out[2] = dir.z;
@@ -247,6 +247,42 @@ int32_t twin_scatter_f64(const rayna_material *m, const double ray_in[6], const 
 }
 
 // Sky colour of a direction (normalised here).
+// The value of texture `tex` (0-based) at a hit with world position pos, local position pos_l and uv.
+void twin_texture_value(const twin_scene *t, int32_t tex, int32_t precision, const double pos[3],
+                        const double pos_l[3], const double uv[2], float out[3]) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        Hit<R> h{};
+        h.pos = cast3<R>(pos);
+        h.pos_l = cast3<R>(pos_l);
+        h.uv[0] = (R)uv[0];
+        h.uv[1] = (R)uv[1];
+        const Col c = texture_value<R>(t->s, tex, h);
+        out[0] = c.r;
+        out[1] = c.g;
+        out[2] = c.b;
+    };
+    if (precision == 32) run(0.0f);
+    else run(0.0);
+}
+
+// The scene's sky colour (any skybox, the image sky included) for a direction (normalised as Ray::new does).
+void twin_scene_sky(const twin_scene *t, int32_t precision, const double d[3], float out[3]) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        const Ray<R> ray = Ray<R>::make(V3<R>{R(0), R(0), R(0)}, cast3<R>(d));
+        const Col c = sky_colour<R>(t->s, ray);
+        out[0] = c.r;
+        out[1] = c.g;
+        out[2] = c.b;
+    };
+    if (precision == 32) run(0.0f);
+    else run(0.0);
+}
+
+// The permutation table of an improved Perlin noise seed (512 entries).
+void twin_noise_permutation(uint32_t seed, int32_t out[512]) { noise_permutation(seed, out); }
+
 void twin_sky(int32_t kind, const double d[3], float out[3]) {
     const Ray<double> ray = Ray<double>::make(V3<double>{0, 0, 0}, cast3<double>(d));
     const Col c = sky<double>(kind, ray);

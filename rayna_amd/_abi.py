@@ -12,15 +12,27 @@ MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_LIGHT = 0, 1, 2, 3
 MESH_SPHERE, MESH_TRIANGLE, MESH_TRIANGLE_LIST, MESH_PARALLELOGRAM, MESH_AXIS_BOX, MESH_SDF, MESH_BVH = 0, 1, 2, 3, 4, 5, 6
 SDF_SPHERE, SDF_SMIN_SPHERES, SDF_MANDELBULB = 0, 1, 2
 OBJ_SIMPLE, OBJ_VOLUMETRIC = 0, 1
-SKY_NONE, SKY_SIMPLE, SKY_WHITE = 0, 1, 2
+SKY_NONE, SKY_SIMPLE, SKY_WHITE, SKY_HDRI = 0, 1, 2, 3
+TEX_SOLID, TEX_WORLD_CHECKER, TEX_UV_CHECKER, TEX_IMAGE, TEX_UV_NOISE, TEX_WORLD_NOISE, TEX_LOCAL_NOISE = range(7)
 MODE_PBR = 0
 MODE_RAY_NORMAL, MODE_OUTWARD_NORMAL, MODE_SCATTER, MODE_FRONT_FACE, MODE_DISTANCE, MODE_UV, MODE_SIDE = range(1, 8)
 LAYOUT_XMAJOR, LAYOUT_ROWMAJOR = 0, 1
 
 
 class Material(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("colour", C.c_float * 3), ("fuzz", C.c_double),
+    _fields_ = [("kind", C.c_int32), ("colour", C.c_float * 3), ("texture", C.c_int32), ("fuzz", C.c_double),
                 ("refractive_index", C.c_double), ("density", C.c_double)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("colour", C.c_float * 3), ("odd", C.c_int32), ("even", C.c_int32),
+                ("image", C.c_int32), ("noise_rgb", C.c_int32), ("noise_seed", C.c_uint32 * 3),
+                ("reserved", C.c_int32), ("scale", C.c_double), ("offset", C.c_double * 3),
+                ("image_scale", C.c_double * 2)]
+
+
+class Image(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.POINTER(C.c_float))]
 
 
 class Mesh(C.Structure):
@@ -50,7 +62,9 @@ class SceneDesc(C.Structure):
                 ("boxes", C.POINTER(C.c_double)), ("n_boxes", C.c_int32),
                 ("sdfs", C.POINTER(Sdf)), ("n_sdfs", C.c_int32),
                 ("skybox", C.c_int32),
-                ("transforms", C.POINTER(C.c_double)), ("n_transforms", C.c_int32)]
+                ("transforms", C.POINTER(C.c_double)), ("n_transforms", C.c_int32),
+                ("textures", C.POINTER(Texture)), ("n_textures", C.c_int32),
+                ("images", C.POINTER(Image)), ("n_images", C.c_int32), ("sky_image", C.c_int32)]
 
 
 class CameraDesc(C.Structure):

@@ -290,6 +290,7 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
             R.first_sdf = sc.first_sdf;
             R.n_sdfs = sc.n_sdfs;
             R.skybox = sc.skybox;
+            R.sky_image = sc.sky_image;
             R.sum_offset = (uint32_t)pl.out_offset[pc.req];
             R.layout = q.layout;
             R.first_piece = pc.s0 == 0 ? 1u : 0u;
@@ -428,6 +429,10 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     UP(sdfs, sdfs)
     UP(materials, materials)
     UP(xforms, xforms)
+    UP(textures, textures)
+    UP(images, images)
+    UP(texels, texels)
+    UP(perms, perms)
 #undef UP
     for (const SceneSlot &sl : b->slots) b->n_sdf_max = std::max<uint32_t>(b->n_sdf_max, (uint32_t)sl.n_sdfs);
     clamp_chunk(b);
@@ -436,6 +441,9 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     v.skybox = RAYNA_SKY_NONE;
     v.stack_need = merged.stack_need;
     v.instances = has_instances(merged) ? 1 : 0;
+    v.textured = has_textures(merged) ? 1 : 0;
+    for (const SceneSlot &sl : b->slots)
+        if (sl.skybox == RAYNA_SKY_HDRI) v.textured = 1;
     const size_t deep = (size_t)std::max(0, merged.stack_need - extend_lds_stack());
     if (deep > 0) {
         void *p = nullptr;

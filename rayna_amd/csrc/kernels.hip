@@ -454,43 +454,59 @@ __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint
 // the box face's uv and side (axis_box.rs:121-129: side = 2 axis + (sgn > 0), `-1.0 as u32` saturating to 0),
 // the volume's random uv (volumetric.rs:153) and the SDF's march iteration (raymarched.rs:106-109, carried in the
 // hit flags from bit 8).  A transform leaves both as the object-space test found them.
+// sphere_uv (sphere.rs:114-121) of a unit vector.
+__device__ __forceinline__ void unit_sphere_uv(f3 p, float uv[2]) {
+    const float PI = 3.14159274101257324f;
+    const float theta = dm::acos_f(-p.y);
+    const float phi = dm::atan2_f(-p.z, p.x) + PI;
+    uv[0] = phi / (2.0f * PI);
+    uv[1] = theta / PI;
+}
+
+// pos_l (optional): Intersection::pos_l -- the sphere's unit local point (sphere.rs:91), the barycentrics
+// (1 - u - v, u, v) (triangle.rs:95-101), the position relative to the plane's origin (planar/mod.rs:176) or the
+// box centre (axis_box.rs:122), else the position; a transform maps it as a point (transform.rs:120-121).
 __device__ void hit_uv_side(const SceneView &sv, const DRay &world_ray, float world_t, uint32_t ref, uint32_t aux,
                             const uint32_t *sub, const dm::Key &key, uint32_t pixel, uint32_t sample,
-                            uint32_t bounce, float uv[2], int *side) {
+                            uint32_t bounce, float uv[2], int *side, f3 *pos_l = nullptr) {
     const HitPrim p = hit_prim(sv, world_ray, world_t, ref, aux, sub);
     const DRay &ray = p.ray;
     const float t = p.t;
     const int32_t idx = p.prim;
+    const f3 pos = at(ray, t);
+    f3 pl = pos;
     uv[0] = uv[1] = 0.0f;
     *side = 0;
     switch (p.kind) {
         case ITEM_TRIANGLE: {
             float tt, det;
             tri_t(sv.tris[idx], ray, t, t, &tt, &uv[0], &uv[1], &det);
+            pl = f3{(1.0f - uv[0]) - uv[1], uv[0], uv[1]};
             break;
         }
         case ITEM_SPHERE: {
             const GSphere s = sv.spheres[idx];
-            const f3 local = (at(ray, t) - f3{s.cx, s.cy, s.cz}) / s.radius;
-            const float PI = 3.14159274101257324f;
-            const float theta = dm::acos_f(-local.y);
-            const float phi = dm::atan2_f(-local.z, local.x) + PI;
-            uv[0] = phi / (2.0f * PI);
-            uv[1] = theta / PI;
+            const f3 local = (pos - f3{s.cx, s.cy, s.cz}) / s.radius;
+            unit_sphere_uv(local, uv);
+            pl = local;
             break;
         }
         case ITEM_PARALLELOGRAM: {
             float tt;
-            para_t(sv.paras[idx], ray, t, t, &tt, uv);
+            const GPara g = sv.paras[idx];
+            para_t(g, ray, t, t, &tt, uv);
+            pl = pos - f3{g.px, g.py, g.pz};
             break;
         }
         case ITEM_AXIS_BOX: {
             float tt, w;
             int axis = 0;
             f3 sg{0.0f, 0.0f, 0.0f};
-            box_t(sv.boxes[idx], ray, t, t, &tt, &axis, &w, &sg, uv);
+            const GBox g = sv.boxes[idx];
+            box_t(g, ray, t, t, &tt, &axis, &w, &sg, uv);
             const float sa = axis == 0 ? sg.x : axis == 1 ? sg.y : sg.z;
             *side = 2 * axis + (sa > 0.0f ? 1 : 0);
+            pl = pos - f3{g.cx, g.cy, g.cz};
             break;
         }
         case ITEM_VOLUME: {
@@ -504,6 +520,22 @@ __device__ void hit_uv_side(const SceneView &sv, const DRay &world_ray, float wo
             *side = (int)(aux >> HIT_AUX_SIDE_SHIFT);
             break;
     }
+    if (pos_l) *pos_l = p.x ? affine_point(p.x->fwd, pl) : pl;
+}
+
+#include "textures.hpp"
+
+// The material's colour at a hit: its solid colour, or its texture's value (TEX: the scene has textures).
+template <bool TEX>
+__device__ __forceinline__ Col material_colour(const SceneView &sv, const GMaterial &m, const DRay &ray, float t,
+                                               uint32_t ref, uint32_t aux, const uint32_t *sub, const dm::Key &key,
+                                               uint32_t pixel, uint32_t sample, uint32_t bounce, const SHit &h) {
+    if (!TEX || m.texture < 0) return Col{m.r, m.g, m.b};
+    float uv[2];
+    int side;
+    f3 pl;
+    hit_uv_side(sv, ray, t, ref, aux, sub, key, pixel, sample, bounce, uv, &side, &pl);
+    return texture_value(sv, m.texture, h.pos, pl, uv);
 }
 
 // Material::scatter (material/*.rs); false for "no scatter".
@@ -544,9 +576,9 @@ __device__ __forceinline__ bool scatter(const GMaterial &m, const DRay &ray, con
     }
 }
 
-// Material::reflected_light applied to the path throughput (future colour) T.
-__device__ __forceinline__ Col reflected(const GMaterial &m, const DRay &ray, const SHit &h, Col T) {
-    const Col albedo{m.r, m.g, m.b};
+// Material::reflected_light applied to the path throughput (future colour) T, with the material's colour at the
+// hit.
+__device__ __forceinline__ Col reflected(const GMaterial &m, const DRay &ray, const SHit &h, Col albedo, Col T) {
     switch (m.kind) {
         case RAYNA_MAT_LAMBERTIAN:
         case RAYNA_MAT_METAL: return T * albedo;
@@ -565,7 +597,7 @@ __device__ __forceinline__ Col reflected(const GMaterial &m, const DRay &ray, co
     }
 }
 
-__device__ __forceinline__ Col sky(int kind, f3 d) {  // skybox/simple.rs, skybox/none.rs
+__device__ __forceinline__ Col sky(int kind, f3 d) {  // skybox/simple.rs, skybox/none.rs (HDRI: sky_image)
     if (kind == RAYNA_SKY_WHITE) return Col{1.0f, 1.0f, 1.0f};
     if (kind == RAYNA_SKY_SIMPLE) {
         const float a = 0.5f * d.y + 0.5f;
@@ -671,7 +703,10 @@ __device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy,
 // decision, so the replay follows the primal path exactly.
 //
 // Register budget: 96 VGPRs, so that five waves share each SIMD (the kernel is latency-bound).
-template <bool ADJOINT, bool BATCH>
+//
+// TEX: the scene has textured materials or an image sky (without, their code is compiled out).  A textured
+// material's colour receives no gradient.
+template <bool ADJOINT, bool BATCH, bool TEX>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_shade(
     SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue_in, const uint32_t *count_in,
     int32_t *queue_out, uint32_t *count_out, uint32_t depth, AdjointParams ap, BatchView bv) {
@@ -691,7 +726,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
             const uint32_t ref = pb.hit_ref[slot];
             dm::Key key{fp.key0, fp.key1};
             uint32_t pixel, sample, ray_depth = fp.ray_depth;
-            int32_t skybox = sv.skybox;
+            int32_t skybox = sv.skybox, sky_img = sv.sky_image;
             if (BATCH) {
                 const RequestParams &R = bv.reqs[bv.slot_req[slot]];
                 const uint32_t l = slot - R.slot_base;
@@ -700,17 +735,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
                 key = dm::Key{R.key0, R.key1};
                 ray_depth = R.ray_depth;
                 skybox = R.skybox;
+                sky_img = R.sky_image;
             } else {
                 pixel = slot % fp.n_pixels;
                 sample = fp.s0 + slot / fp.n_pixels;
             }
             if (ref == 0u) {
-                if (!ADJOINT) L = L + T * sky(skybox, ray.d);
+                if (!ADJOINT)
+                    L = L + T * ((TEX && skybox == RAYNA_SKY_HDRI) ? sky_image(sv, sky_img, ray.d) : sky(skybox, ray.d));
             } else {
                 SHit h;
-                hit_geometry(sv, ray, pb.hit_t[slot], ref, pb.hit_aux[slot], pb.hit_sub + slot, key, pixel, sample,
-                             depth, &h);
+                const float t = pb.hit_t[slot];
+                const uint32_t aux = pb.hit_aux[slot];
+                hit_geometry(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, &h);
                 const GMaterial m = sv.materials[h.material];
+                const Col colour =
+                    material_colour<TEX>(sv, m, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, h);
+                const bool graded = !TEX || m.texture < 0;
                 double d0 = 0.0, d1 = 0.0, d2 = 0.0;
                 if (ADJOINT) {
                     d0 = (double)ap.d_image[3 * pixel] * ap.inv_spp;
@@ -718,9 +759,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
                     d2 = (double)ap.d_image[3 * pixel + 2] * ap.inv_spp;
                 }
                 if (m.kind == RAYNA_MAT_LIGHT) {
-                    const Col le{m.r, m.g, m.b};
+                    const Col le = colour;
                     if (ADJOINT) {
-                        grad_add(ap, copy, h.material, d0 * T.r, d1 * T.g, d2 * T.b);
+                        if (graded) grad_add(ap, copy, h.material, d0 * T.r, d1 * T.g, d2 * T.b);
                         L = Col{L.r - T.r * le.r, L.g - T.g * le.g, L.b - T.b * le.b};
                     } else {
                         L = L + T * le;
@@ -732,12 +773,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
                 if (scatter(m, ray, h, u, &dir)) {
                     if (ADJOINT && depth < ray_depth) {
                         const bool albedo_scales = m.kind != RAYNA_MAT_DIELECTRIC || !h.front;
-                        if (albedo_scales)
+                        if (albedo_scales && graded)
                             grad_add(ap, copy, h.material, m.r != 0.0f ? d0 * (double)L.r / (double)m.r : 0.0,
                                      m.g != 0.0f ? d1 * (double)L.g / (double)m.g : 0.0,
                                      m.b != 0.0f ? d2 * (double)L.b / (double)m.b : 0.0);
                     }
-                    T = reflected(m, ray, h, T);
+                    T = reflected(m, ray, h, colour, T);
                     if (depth < ray_depth) {
                         const DRay nr = make_ray(h.pos, dir);
                         pb.ox[slot] = nr.o.x;
@@ -866,7 +907,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_mode(SceneView sv, FrameParams 
         const uint32_t ref = pb.hit_ref[slot];
         Col L;
         if (ref == 0u) {
-            L = sky(sv.skybox, ray.d);
+            L = sv.skybox == RAYNA_SKY_HDRI ? sky_image(sv, sv.sky_image, ray.d) : sky(sv.skybox, ray.d);
         } else {
             const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
             const float t = pb.hit_t[slot];
@@ -1001,15 +1042,12 @@ void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
                   const AdjointParams *adjoint, const BatchView &bv, hipStream_t s) {
-    if (bv.reqs)  // batches are forward renders
-        hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in,
-                           queue_out, count_out, depth, AdjointParams{}, bv);
-    else if (adjoint)
-        hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in,
-                           queue_out, count_out, depth, *adjoint, bv);
-    else
-        hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in,
-                           queue_out, count_out, depth, AdjointParams{}, bv);
+    const bool tex = sv.textured != 0;
+    auto kernel = bv.reqs   ? (tex ? k_shade<false, true, true> : k_shade<false, true, false>)  // batches: forward
+                  : adjoint ? (tex ? k_shade<true, false, true> : k_shade<true, false, false>)
+                            : (tex ? k_shade<false, false, true> : k_shade<false, false, false>);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in, queue_out, count_out,
+                       depth, adjoint ? *adjoint : AdjointParams{}, bv);
 }
 
 void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,

@@ -43,6 +43,7 @@ struct RequestParams {
     uint32_t key0, key1, ray_depth, n_pixels;
     uint32_t s0, slot_base, n_slots;
     int32_t root, first_sdf, n_sdfs, skybox;  // the request's scene within the merged scene set
+    int32_t sky_image;                        // RAYNA_SKY_HDRI: its image
     uint32_t sum_offset;                      // its 3 * n_pixels floats in the batch's sum and output buffers
     int32_t layout;                           // output layout (RAYNA_LAYOUT_*)
     uint32_t first_piece;                     // 1 if the chunk holds the request's first samples

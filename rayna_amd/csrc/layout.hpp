@@ -127,11 +127,32 @@ struct alignas(16) GXform {
 };
 static_assert(sizeof(GXform) == 112, "GXform must be 112 bytes");
 
-// Material (material/*.rs with SolidTexture), parameters rounded to float.
+// Material (material/*.rs), parameters rounded to float.  texture: -1 for the solid colour r, g, b, else the
+// index of its texture.
 struct alignas(16) GMaterial {
     int32_t kind;
     float r, g, b;
-    float fuzz, ior, density, pad;
+    float fuzz, ior, density;
+    int32_t texture;
+};
+
+// A texture (texture/*.rs), parameters rounded to float.  odd / even / image are indices into the texture and
+// image arrays; perm[i] is the offset of a 512-entry permutation table (noise).
+struct alignas(16) GTexture {
+    int32_t kind, odd, even, image;
+    float r, g, b, scale;
+    float ox, oy, oz, sx;
+    float sy;
+    int32_t perm[3];
+    int32_t rgb, pad0, pad1, pad2;
+};
+static_assert(sizeof(GTexture) == 80, "GTexture must be 80 bytes");
+
+// An image: width x height RGB texels (3 floats each, row-major) from `texel` on in the texel array.
+struct alignas(16) GImage {
+    int32_t texel;
+    uint32_t width, height;
+    int32_t pad;
 };
 
 // Hit record written by the extend kernel: distance, packed reference (kind << 28 | index), rank, aux bits.
@@ -159,11 +180,17 @@ struct SceneView {
     const GSdf *sdfs;
     const GMaterial *materials;
     const GXform *xforms;
+    const GTexture *textures;
+    const GImage *images;
+    const float *texels;
+    const int32_t *perms;
     int32_t root;        // -1: no bounded object
     int32_t n_sdfs;
     int32_t skybox;
     int32_t stack_need;  // deepest traversal stack the tree can require
     int32_t instances;   // 1 if some object is a transformed mesh BVH (an instance)
+    int32_t textured;    // 1 if some material has a texture or the sky is an image
+    int32_t sky_image;   // RAYNA_SKY_HDRI: the sky's image
 };
 
 }  // namespace rayna

@@ -135,12 +135,18 @@ rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
     UP(sdfs, sdfs)
     UP(materials, materials)
     UP(xforms, xforms)
+    UP(textures, textures)
+    UP(images, images)
+    UP(texels, texels)
+    UP(perms, perms)
 #undef UP
     v.root = ps.root;
     v.n_sdfs = (int32_t)ps.sdfs.size();
     v.skybox = ps.skybox;
     v.stack_need = ps.stack_need;
     v.instances = has_instances(ps) ? 1 : 0;
+    v.textured = has_textures(ps) ? 1 : 0;
+    v.sky_image = ps.sky_image;
     // traversal stack overflow beyond the LDS entries, one column per extend thread
     const size_t deep = (size_t)std::max(0, ps.stack_need - extend_lds_stack());
     if (deep > 0) {

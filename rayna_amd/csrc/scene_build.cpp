@@ -785,6 +785,98 @@ static void sdf_escape_sphere(GSdf &g, const std::vector<GSphere> &spheres) {
     g.escape_r = std::isfinite(Rm) ? Rf : INFINITY;
 }
 
+void noise_permutation(uint32_t seed, int32_t out[512]) {
+    uint64_t state = seed;
+    auto next = [&]() {  // SplitMix64 (Steele, Lea and Flood 2014)
+        uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    int32_t p[256];
+    for (int i = 0; i < 256; ++i) p[i] = i;
+    for (int i = 255; i > 0; --i) {
+        const int j = (int)(next() % (uint64_t)(i + 1));
+        std::swap(p[i], p[j]);
+    }
+    for (int i = 0; i < 512; ++i) out[i] = p[i & 255];
+}
+
+// Textures and images: validation and device records.  A texture's odd / even name earlier textures, so the
+// texture graph has no cycles and a lookup ends within n_textures steps.
+static rayna_status pack_textures(const rayna_scene_desc &d, PackedScene &P, std::string &error) {
+    if (d.n_textures < 0 || d.n_images < 0 || (d.n_textures && !d.textures) || (d.n_images && !d.images)) {
+        error = "bad texture or image array";
+        return RAYNA_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < d.n_images; ++i) {
+        const rayna_image &im = d.images[i];
+        const uint64_t n = (uint64_t)im.width * im.height;
+        if (im.width == 0 || im.height == 0 || !im.rgb || n > (1ull << 28) ||
+            P.texels.size() + 3 * n > (size_t)INT32_MAX) {
+            error = "bad image";
+            return RAYNA_INVALID_ARGUMENT;
+        }
+        P.images.push_back(GImage{(int32_t)(P.texels.size() / 3), im.width, im.height, 0});
+        P.texels.insert(P.texels.end(), im.rgb, im.rgb + 3 * n);
+    }
+    for (int i = 0; i < d.n_textures; ++i) {
+        const rayna_texture &t = d.textures[i];
+        GTexture g;
+        std::memset(&g, 0, sizeof(g));
+        g.kind = t.kind;
+        g.odd = t.odd - 1;
+        g.even = t.even - 1;
+        g.image = t.image;
+        g.r = t.colour[0];
+        g.g = t.colour[1];
+        g.b = t.colour[2];
+        g.scale = (float)t.scale;
+        g.ox = (float)t.offset[0];
+        g.oy = (float)t.offset[1];
+        g.oz = (float)t.offset[2];
+        g.sx = (float)t.image_scale[0];
+        g.sy = (float)t.image_scale[1];
+        g.rgb = t.noise_rgb ? 1 : 0;
+        switch (t.kind) {
+            case RAYNA_TEX_SOLID: break;
+            case RAYNA_TEX_WORLD_CHECKER:
+            case RAYNA_TEX_UV_CHECKER:
+                if (t.odd < 1 || t.odd > i || t.even < 1 || t.even > i) {
+                    error = "a checker texture must name earlier textures";
+                    return RAYNA_INVALID_ARGUMENT;
+                }
+                break;
+            case RAYNA_TEX_IMAGE:
+                if (t.image < 0 || t.image >= d.n_images) {
+                    error = "texture image index out of range";
+                    return RAYNA_INVALID_ARGUMENT;
+                }
+                break;
+            case RAYNA_TEX_UV_NOISE:
+            case RAYNA_TEX_WORLD_NOISE:
+            case RAYNA_TEX_LOCAL_NOISE:
+                for (int k = 0; k < (g.rgb ? 3 : 1); ++k) {
+                    g.perm[k] = (int32_t)P.perms.size();
+                    int32_t table[512];
+                    noise_permutation(t.noise_seed[k], table);
+                    P.perms.insert(P.perms.end(), table, table + 512);
+                }
+                break;
+            default:
+                error = "unknown texture kind";
+                return RAYNA_INVALID_ARGUMENT;
+        }
+        P.textures.push_back(g);
+    }
+    if (d.skybox == RAYNA_SKY_HDRI && (d.sky_image < 0 || d.sky_image >= d.n_images)) {
+        error = "sky image index out of range";
+        return RAYNA_INVALID_ARGUMENT;
+    }
+    P.sky_image = d.skybox == RAYNA_SKY_HDRI ? d.sky_image : 0;
+    return RAYNA_OK;
+}
+
 static bool finite_all(const double *p, size_t n) {
     for (size_t i = 0; i < n; ++i)
         if (!std::isfinite(p[i])) return false;
@@ -806,7 +898,7 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
         error = "null array in scene";
         return RAYNA_INVALID_ARGUMENT;
     }
-    if (d.skybox < RAYNA_SKY_NONE || d.skybox > RAYNA_SKY_WHITE) {
+    if (d.skybox < RAYNA_SKY_NONE || d.skybox > RAYNA_SKY_HDRI) {
         error = "unknown skybox";
         return RAYNA_INVALID_ARGUMENT;
     }
@@ -823,9 +915,17 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
             error = "unknown material kind";
             return RAYNA_INVALID_ARGUMENT;
         }
+        if (m.texture < 0 || m.texture > d.n_textures) {
+            error = "material texture index out of range";
+            return RAYNA_INVALID_ARGUMENT;
+        }
         GMaterial g{m.kind, m.colour[0], m.colour[1], m.colour[2], (float)m.fuzz, (float)m.refractive_index,
-                    (float)m.density, 0.0f};
+                    (float)m.density, m.texture - 1};
         P.materials.push_back(g);
+    }
+    {
+        const rayna_status ts = pack_textures(d, P, error);
+        if (ts != RAYNA_OK) return ts;
     }
     for (int i = 0; i < d.n_meshes; ++i) {
         const rayna_mesh &m = d.meshes[i];
@@ -1039,6 +1139,8 @@ bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vec
         const int32_t para_off = (int32_t)out.paras.size(), box_off = (int32_t)out.boxes.size();
         const int32_t vol_off = (int32_t)out.volumes.size(), sdf_off = (int32_t)out.sdfs.size();
         const int32_t mat_off = (int32_t)out.materials.size(), xf_off = (int32_t)out.xforms.size();
+        const int32_t tex_off = (int32_t)out.textures.size(), img_off = (int32_t)out.images.size();
+        const int32_t texel_off = (int32_t)(out.texels.size() / 3), perm_off = (int32_t)out.perms.size();
         for (GNode g : p.nodes) {
             for (int k = 0; k < 4; ++k) {
                 if (g.info[k] == INFO_INNER) g.child[k] += node_off;
@@ -1088,9 +1190,25 @@ bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vec
             f.material += mat_off;
             out.sdfs.push_back(f);
         }
-        out.materials.insert(out.materials.end(), p.materials.begin(), p.materials.end());
+        for (GMaterial m : p.materials) {
+            if (m.texture >= 0) m.texture += tex_off;
+            out.materials.push_back(m);
+        }
+        for (GTexture t : p.textures) {
+            t.odd += tex_off;
+            t.even += tex_off;
+            t.image += img_off;
+            for (int k = 0; k < 3; ++k) t.perm[k] += perm_off;
+            out.textures.push_back(t);
+        }
+        for (GImage im : p.images) {
+            im.texel += texel_off;
+            out.images.push_back(im);
+        }
+        out.texels.insert(out.texels.end(), p.texels.begin(), p.texels.end());
+        out.perms.insert(out.perms.end(), p.perms.begin(), p.perms.end());
         slots.push_back(SceneSlot{p.root >= 0 ? p.root + node_off : -1, sdf_off, (int32_t)p.sdfs.size(), p.skybox,
-                                  p.stack_need});
+                                  p.stack_need, p.sky_image + img_off});
         out.stack_need = std::max(out.stack_need, p.stack_need);
     }
     // leaf starts have 27 bits in a traversal stack code; hit references keep 28 bits of index

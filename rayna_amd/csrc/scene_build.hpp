@@ -47,9 +47,14 @@ struct PackedScene {
     std::vector<GSdf> sdfs;
     std::vector<GMaterial> materials;
     std::vector<GXform> xforms;
+    std::vector<GTexture> textures;
+    std::vector<GImage> images;
+    std::vector<float> texels;
+    std::vector<int32_t> perms;
     int32_t root = -1;
     int32_t stack_need = 0;
     int32_t skybox = RAYNA_SKY_NONE;
+    int32_t sky_image = 0;
     uint64_t tree_hash = 0;
     std::vector<int32_t> rank_object, rank_local;  // depth-first rank -> (object, index in its mesh)
 };
@@ -61,6 +66,18 @@ inline bool has_instances(const PackedScene &p) {
     return false;
 }
 
+// True if some material of the packed scene has a texture or its sky is an image.
+inline bool has_textures(const PackedScene &p) {
+    if (p.skybox == RAYNA_SKY_HDRI) return true;
+    for (const GMaterial &m : p.materials)
+        if (m.texture >= 0) return true;
+    return false;
+}
+
+// The permutation of an improved Perlin noise for `seed`: 0..255 shuffled by Fisher-Yates with SplitMix64 draws,
+// repeated to 512 entries.
+void noise_permutation(uint32_t seed, int32_t out[512]);
+
 // Validates and packs a scene.  Returns RAYNA_OK, or an error status with a message in `error`.
 rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &out, std::string &error);
 
@@ -71,6 +88,7 @@ struct SceneSlot {
     int32_t n_sdfs;
     int32_t skybox;
     int32_t stack_need;
+    int32_t sky_image;
 };
 
 // Concatenates packed scenes into one device layout: every index (child nodes, leaf ranges, item targets,

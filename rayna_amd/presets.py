@@ -13,7 +13,10 @@ Each function returns (Scene, Camera, RenderOpts) for one workload:
                    about their centres; NoSkybox
   transformed_mix  objects of every transformable kind under rotations, scalings and shears (parity workload)
   rttnw            the engine's RTTNW_DEMO (preset.rs:394-605): a BvhMesh floor of 400 boxes, a BvhMesh cube of
-                   1000 balls rotated as an instance, glass, metal and medium-filled spheres, a light; NoSkybox
+                   1000 balls rotated as an instance, glass, metal and medium-filled spheres, a noise-textured
+                   sphere, a light; NoSkybox
+  textured         every texture kind (world and uv checkers, image, uv / world / local noise) on every primitive,
+                   a textured light and an image sky (parity workload)
 All randomness comes from numpy's PCG64 seeded with `seed`, so every scene is a pure function of its arguments.
 """
 from __future__ import annotations
@@ -22,7 +25,8 @@ import math
 
 import numpy as np
 
-from .scene import (AxisBoxMesh, BvhMesh, Camera, DielectricMaterial, IsotropicMaterial, LambertianMaterial, LightMaterial,
+from .scene import (AxisBoxMesh, BvhMesh, Camera, ColourSource, HdrImageSkybox, ImageTexture, LocalNoiseTexture,
+                    Perlin, ScalePoint, UvCheckerTexture, UvNoiseTexture, WorldCheckerTexture, WorldNoiseTexture, DielectricMaterial, IsotropicMaterial, LambertianMaterial, LightMaterial,
                     MetalMaterial, NoSkybox, ParallelogramMesh, RaymarchedIsosurfaceMesh, RenderOpts, Scene,
                     SdfMandelbulb, SdfSminSpheres, SimpleObject, SimpleSkybox, SphereMesh, Transform3,
                     TriangleListMesh, TriangleMesh, VolumetricObject, WhiteSkybox)
@@ -300,8 +304,8 @@ def transformed_mix(seed: int = 6, width: int = 96, height: int = 64, samples: i
 def rttnw(seed: int = 0, width: int = 600, height: int = 600, samples: int = 32, ray_depth: int = 10,
           balls: int = 1000):
     """RTTNW_DEMO (preset.rs:394-605) with numpy's PCG64 for the engine's thread_rng: box heights U[0, 1] and ball
-    centres U[0, 1)^3 * 0.825.  The engine's noise-textured sphere (WorldNoiseTexture over Perlin noise) is given
-    a solid grey albedo of 0.5."""
+    centres U[0, 1)^3 * 0.825.  The noise sphere's WorldNoiseTexture(Greyscale(ScalePoint(Perlin(69), 4))) uses the
+    improved-Perlin restatement, whose permutation differs from the noise crate's."""
     rng = np.random.Generator(np.random.PCG64(seed))
     objects = []
     floor = []
@@ -319,7 +323,8 @@ def rttnw(seed: int = 0, width: int = 600, height: int = 600, samples: int = 32,
     objects.append(SimpleObject(SphereMesh((3.6, 1.5, 1.45), 0.7), DielectricMaterial((1.0, 1.0, 1.0), 1.5, 0.0)))
     objects.append(VolumetricObject(SphereMesh((3.6, 1.5, 1.45), 0.6999), IsotropicMaterial((0.2, 0.4, 0.9), 0.3),
                                     2.0))
-    objects.append(SimpleObject(SphereMesh((2.2, 2.8, 3.0), 0.8), LambertianMaterial((0.5, 0.5, 0.5))))
+    noise = WorldNoiseTexture(ColourSource.greyscale(ScalePoint(Perlin(69), 4.0)))
+    objects.append(SimpleObject(SphereMesh((2.2, 2.8, 3.0), 0.8), LambertianMaterial(noise)))
     spread = 0.825  # CUBE OF BALLS
     cube = [SphereMesh(tuple(float(x) for x in rng.random(3) * spread), 0.1) for _ in range(balls)]
     objects.append(SimpleObject(BvhMesh(cube), LambertianMaterial((0.85, 0.85, 0.85)),
@@ -331,6 +336,45 @@ def rttnw(seed: int = 0, width: int = 600, height: int = 600, samples: int = 32,
                  defocus_angle=0.0)
     opts = RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
     return Scene(objects, NoSkybox), cam, opts
+
+
+def test_image(width: int, height: int, seed: int) -> np.ndarray:
+    """A (height, width, 3) float image: smooth colour ramps crossed by seeded stripes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
+    img = np.stack([x / max(1, width - 1), y / max(1, height - 1), 0.5 + 0.5 * np.sin(0.7 * x + 0.3 * y)], axis=-1)
+    img[:, rng.integers(0, width, size=max(1, width // 8))] *= 0.25
+    return img.astype(np.float32)
+
+
+def textured(seed: int = 7, width: int = 96, height: int = 64, samples: int = 4, ray_depth: int = 6):
+    """Every texture kind: a world-checker ground, an image-textured sphere, uv checkers and uv noise on a box and
+    a parallelogram, a world-noise sphere, an rgb local-noise box under a rotation, an image-textured light, and an
+    image sky."""
+    checker = WorldCheckerTexture(even=(0.9, 0.9, 0.9), odd=(0.2, 0.3, 0.1), scale=0.5, offset=(0.25, 0.0, 0.1))
+    nested = UvCheckerTexture(even=checker, odd=(0.8, 0.2, 0.2), scale=0.25, offset=(0.1, 0.0))
+    img = ImageTexture(test_image(37, 23, seed), scale=(2.0, 1.0), offset=(0.1, 0.0))
+    rgb_noise = LocalNoiseTexture(ColourSource.rgb(ScalePoint(Perlin(seed), 3.0), ScalePoint(Perlin(seed + 1), 3.0),
+                                                   ScalePoint(Perlin(seed + 2), 3.0)))
+    objects = [
+        SimpleObject(ParallelogramMesh((-20.0, 0.0, -20.0), (40.0, 0.0, 0.0), (0.0, 0.0, 40.0)),
+                     LambertianMaterial(checker)),
+        SimpleObject(SphereMesh((-1.6, 0.8, 0.0), 0.8), LambertianMaterial(img)),
+        SimpleObject(AxisBoxMesh((-0.3, 0.0, -0.6), (0.5, 1.2, 0.2)), MetalMaterial(nested, 0.3)),
+        SimpleObject(SphereMesh((1.5, 0.7, 0.3), 0.7),
+                     LambertianMaterial(WorldNoiseTexture(ColourSource.greyscale(ScalePoint(Perlin(69), 4.0))))),
+        SimpleObject(AxisBoxMesh.new_centred((0.2, 0.5, 1.6), (0.8, 1.0, 0.8)), LambertianMaterial(rgb_noise),
+                     Transform3.from_axis_angle((0.0, 1.0, 0.0), 30.0)),
+        SimpleObject(ParallelogramMesh((2.2, 0.05, -1.5), (1.0, 0.0, 0.2), (0.0, 1.2, 0.0)),
+                     DielectricMaterial(UvNoiseTexture(ColourSource.greyscale(ScalePoint(Perlin(5), 6.0))), 1.4, 0.4)),
+        SimpleObject(TriangleMesh(((-2.5, 2.5, -2.0), (-0.5, 2.5, -2.2), (-1.5, 3.5, -2.1)),
+                                  ((0.0, 0.0, 1.0), (0.0, 0.0, 1.0), (0.0, 0.0, 1.0))),
+                     LightMaterial(ImageTexture(test_image(8, 8, seed + 3) * 6.0))),
+    ]
+    sky = HdrImageSkybox(test_image(64, 32, seed + 5) * 1.5)
+    cam = Camera(pos=(0.0, 2.0, 6.0), fwd=(0.0, -0.25, -1.0), v_fov=55.0, focus_dist=6.0, defocus_angle=0.0)
+    opts = RenderOpts(width=width, height=height, samples=samples, ray_depth=ray_depth, seed=seed)
+    return Scene(objects, sky), cam, opts
 
 
 def mixed_primitives(seed: int = 5, width: int = 96, height: int = 64, samples: int = 4, ray_depth: int = 6):

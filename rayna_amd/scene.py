@@ -23,52 +23,155 @@ from enum import IntEnum
 
 from . import _abi as A
 
+# ------------------------------------------------------------------------------------------ textures
+
+
+@dataclass
+class SolidTexture:
+    """SolidTexture { albedo } (texture/solid.rs:8-30); a plain colour tuple means the same."""
+    albedo: Sequence[float] = (0.0, 0.0, 0.0)
+
+
+@dataclass
+class WorldCheckerTexture:
+    """WorldCheckerTexture (texture/checker.rs:12-26): cells of `scale` in world space, shifted by `offset`."""
+    even: "TextureT"
+    odd: "TextureT"
+    scale: float = 1.0
+    offset: Sequence[float] = (0.0, 0.0, 0.0)
+
+
+@dataclass
+class UvCheckerTexture:
+    """UvCheckerTexture (texture/checker.rs:28-42): cells of `scale` in (u, v), shifted by `offset`."""
+    even: "TextureT"
+    odd: "TextureT"
+    scale: float = 1.0
+    offset: Sequence[float] = (0.0, 0.0)
+
+
+@dataclass
+class ImageTexture:
+    """ImageTexture (texture/image.rs:7-40): `image` is a (height, width, 3) float array, row-major (pixel (x, y)
+    at image[y, x]); uv is scaled by `scale`, offset by `offset`, and sampled bilinearly."""
+    image: np.ndarray
+    scale: Sequence[float] = (1.0, 1.0)
+    offset: Sequence[float] = (0.0, 0.0)
+
+
+@dataclass
+class Perlin:
+    """An improved Perlin noise (Perlin 2002) whose permutation is drawn from `seed` -- a restatement, not the
+    `noise` crate's Perlin (its permutation differs, so values differ for the same seed)."""
+    seed: int = 0
+
+
+@dataclass
+class ScalePoint:
+    """ScalePoint::new(source).set_scale(scale): the noise at the point times `scale`."""
+    source: Perlin
+    scale: float = 1.0
+
+
+@dataclass
+class ColourSource:
+    """ColourSource (texture/noise.rs:18-45): Greyscale (one noise) or Rgb (three), mapped from [-1, 1] to
+    [0, 1]."""
+    noises: List[Union[Perlin, ScalePoint]]
+
+    @staticmethod
+    def greyscale(noise):
+        return ColourSource([noise])
+
+    @staticmethod
+    def rgb(r, g, b):
+        return ColourSource([r, g, b])
+
+
+@dataclass
+class UvNoiseTexture:
+    source: ColourSource
+
+
+@dataclass
+class WorldNoiseTexture:
+    source: ColourSource
+
+
+@dataclass
+class LocalNoiseTexture:
+    source: ColourSource
+
+
+TextureT = Union[Sequence[float], SolidTexture, WorldCheckerTexture, UvCheckerTexture, ImageTexture, UvNoiseTexture,
+                 WorldNoiseTexture, LocalNoiseTexture]
+
+
+def _solid(t) -> Optional[Sequence[float]]:
+    """The colour of a solid texture (a colour tuple or SolidTexture), else None."""
+    if isinstance(t, SolidTexture):
+        return tuple(t.albedo)
+    if isinstance(t, (WorldCheckerTexture, UvCheckerTexture, ImageTexture, UvNoiseTexture, WorldNoiseTexture,
+                      LocalNoiseTexture)):
+        return None
+    return tuple(t)
+
+
 # ------------------------------------------------------------------------------------------ materials
+
+
+def _material(kind, tex, t, fuzz=0.0, ior=0.0, density=0.0):
+    col = _solid(t)
+    if col is None and tex is None:
+        raise ValueError("a textured material is described within its scene")
+    k = 0 if col is not None else tex(t)
+    return A.Material(kind=kind, colour=(C.c_float * 3)(*(col if col is not None else (0.0, 0.0, 0.0))), texture=k,
+                      fuzz=fuzz, refractive_index=ior, density=density)
 
 
 @dataclass
 class LambertianMaterial:
-    albedo: Sequence[float] = (0.5, 0.5, 0.5)
+    albedo: TextureT = (0.5, 0.5, 0.5)
 
-    def _desc(self):
-        return A.Material(A.MAT_LAMBERTIAN, (C.c_float * 3)(*self.albedo), 0.0, 0.0, 0.0)
+    def _desc(self, tex=None):
+        return _material(A.MAT_LAMBERTIAN, tex, self.albedo)
 
 
 @dataclass
 class MetalMaterial:
-    albedo: Sequence[float]
+    albedo: TextureT
     fuzz: float = 0.0
 
-    def _desc(self):
-        return A.Material(A.MAT_METAL, (C.c_float * 3)(*self.albedo), float(self.fuzz), 0.0, 0.0)
+    def _desc(self, tex=None):
+        return _material(A.MAT_METAL, tex, self.albedo, fuzz=float(self.fuzz))
 
 
 @dataclass
 class DielectricMaterial:
-    albedo: Sequence[float] = (1.0, 1.0, 1.0)
+    albedo: TextureT = (1.0, 1.0, 1.0)
     refractive_index: float = 1.5
     density: float = 0.0
 
-    def _desc(self):
-        return A.Material(A.MAT_DIELECTRIC, (C.c_float * 3)(*self.albedo), 0.0, float(self.refractive_index),
-                          float(self.density))
+    def _desc(self, tex=None):
+        return _material(A.MAT_DIELECTRIC, tex, self.albedo, ior=float(self.refractive_index),
+                         density=float(self.density))
 
 
 @dataclass
 class IsotropicMaterial:
-    albedo: Sequence[float] = (0.5, 0.5, 0.5)
+    albedo: TextureT = (0.5, 0.5, 0.5)
     density: float = 1.0
 
-    def _desc(self):
-        return A.Material(A.MAT_ISOTROPIC, (C.c_float * 3)(*self.albedo), 0.0, 0.0, float(self.density))
+    def _desc(self, tex=None):
+        return _material(A.MAT_ISOTROPIC, tex, self.albedo, density=float(self.density))
 
 
 @dataclass
 class LightMaterial:
-    emissive: Sequence[float]
+    emissive: TextureT
 
-    def _desc(self):
-        return A.Material(A.MAT_LIGHT, (C.c_float * 3)(*self.emissive), 0.0, 0.0, 0.0)
+    def _desc(self, tex=None):
+        return _material(A.MAT_LIGHT, tex, self.emissive)
 
 
 MaterialT = Union[LambertianMaterial, MetalMaterial, DielectricMaterial, IsotropicMaterial, LightMaterial]
@@ -281,9 +384,16 @@ NoSkybox, SimpleSkybox, WhiteSkybox = A.SKY_NONE, A.SKY_SIMPLE, A.SKY_WHITE
 
 
 @dataclass
+class HdrImageSkybox:
+    """HdrImageSkybox (skybox/hdri.rs:8-27): the image, (height, width, 3) row-major, looked up by the ray
+    direction's sphere uv and sampled bilinearly."""
+    image: np.ndarray
+
+
+@dataclass
 class Scene:
     objects: List[Union[SimpleObject, VolumetricObject]]
-    skybox: int = SimpleSkybox
+    skybox: Union[int, HdrImageSkybox] = SimpleSkybox
     _cache: Optional[tuple] = field(default=None, repr=False, compare=False)
 
     def desc(self) -> A.SceneDesc:
@@ -371,11 +481,74 @@ def _build_desc(scene: Scene):
     objects: List[A.Object] = []
     transforms: List[np.ndarray] = []
 
+    textures: List[A.Texture] = []
+    tex_ids = {}
+    images: List[np.ndarray] = []
+    img_ids = {}
+
+    def img_index(arr) -> int:
+        key = id(arr)
+        if key not in img_ids:
+            a = np.ascontiguousarray(arr, np.float32)
+            if a.ndim != 3 or a.shape[2] != 3:
+                raise ValueError("an image is a (height, width, 3) array")
+            img_ids[key] = len(images)
+            images.append(a)
+        return img_ids[key]
+
+    def noise_params(d, source):
+        ns = source.noises
+        scales = {float(n.scale) if isinstance(n, ScalePoint) else 1.0 for n in ns}
+        if len(ns) not in (1, 3) or len(scales) != 1:
+            raise ValueError("a colour source is one noise or three with one scale")
+        seeds = [(n.source if isinstance(n, ScalePoint) else n).seed for n in ns]
+        d.noise_rgb = 1 if len(ns) == 3 else 0
+        d.noise_seed = (C.c_uint32 * 3)(*(seeds + [0, 0])[:3])
+        d.scale = scales.pop()
+
+    def tex_index(t) -> int:
+        """The 1-based texture index of a non-solid texture (its sub-textures registered first)."""
+        key = id(t)
+        if key in tex_ids:
+            return tex_ids[key]
+        d = A.Texture()
+        if isinstance(t, (WorldCheckerTexture, UvCheckerTexture)):
+            odd, even = tex_ref(t.odd), tex_ref(t.even)
+            d.kind = A.TEX_WORLD_CHECKER if isinstance(t, WorldCheckerTexture) else A.TEX_UV_CHECKER
+            d.odd, d.even, d.scale = odd, even, float(t.scale)
+            off = list(t.offset) + [0.0] * (3 - len(t.offset))
+            d.offset = (C.c_double * 3)(*off[:3])
+        elif isinstance(t, ImageTexture):
+            d.kind = A.TEX_IMAGE
+            d.image = img_index(t.image)
+            d.image_scale = (C.c_double * 2)(*t.scale)
+            d.offset = (C.c_double * 3)(t.offset[0], t.offset[1], 0.0)
+        elif isinstance(t, (UvNoiseTexture, WorldNoiseTexture, LocalNoiseTexture)):
+            d.kind = {UvNoiseTexture: A.TEX_UV_NOISE, WorldNoiseTexture: A.TEX_WORLD_NOISE,
+                      LocalNoiseTexture: A.TEX_LOCAL_NOISE}[type(t)]
+            noise_params(d, t.source)
+        else:
+            raise TypeError(f"unsupported texture {type(t).__name__}")
+        textures.append(d)
+        tex_ids[key] = len(textures)
+        return tex_ids[key]
+
+    def tex_ref(t) -> int:
+        """Any texture as a 1-based index (a solid colour becomes a SOLID texture entry)."""
+        col = _solid(t)
+        if col is None:
+            return tex_index(t)
+        d = A.Texture()
+        d.kind = A.TEX_SOLID
+        d.colour = (C.c_float * 3)(*col)
+        textures.append(d)
+        return len(textures)
+
     def mat_index(m):
         key = id(m)
         if key not in mat_ids:
             mat_ids[key] = len(materials)
-            materials.append(m._desc())
+            materials.append(m._desc(tex_index))
             mat_objs.append(m)
         return mat_ids[key]
 
@@ -475,6 +648,14 @@ def _build_desc(scene: Scene):
     d.parallelograms, d.n_parallelograms = _ptr(pa, C.c_double), len(pa)
     d.boxes, d.n_boxes = _ptr(bx, C.c_double), len(bx)
     d.sdfs, d.n_sdfs = C.cast(sdf_arr, C.POINTER(A.Sdf)), len(sdfs)
-    d.skybox = int(scene.skybox)
+    if isinstance(scene.skybox, HdrImageSkybox):
+        d.skybox, d.sky_image = A.SKY_HDRI, img_index(scene.skybox.image)
+    else:
+        d.skybox, d.sky_image = int(scene.skybox), 0
     d.transforms, d.n_transforms = _ptr(xf, C.c_double), len(xf)
+    tex_arr = (A.Texture * max(1, len(textures)))(*textures)
+    img_arr = (A.Image * max(1, len(images)))(*[A.Image(a.shape[1], a.shape[0], _ptr(a, C.c_float)) for a in images])
+    keep.update(tex=tex_arr, img=img_arr, img_data=images)
+    d.textures, d.n_textures = C.cast(tex_arr, C.POINTER(A.Texture)), len(textures)
+    d.images, d.n_images = C.cast(img_arr, C.POINTER(A.Image)), len(images)
     return d, keep
