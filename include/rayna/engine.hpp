@@ -13,7 +13,11 @@
 //              RaymarchedIsosurfaceMesh, BvhMesh (of primitive meshes)  mesh/mod.rs:68-85
 //   materials  LambertianMaterial, MetalMaterial, DielectricMaterial, IsotropicMaterial, LightMaterial
 //                                                                     material/mod.rs:168-175
-//   skyboxes   NoSkybox, SimpleSkybox, WhiteSkybox                     skybox/mod.rs:28-34
+//   textures   SolidTexture, WorldCheckerTexture, UvCheckerTexture, ImageTexture, UvNoiseTexture,
+//              WorldNoiseTexture, LocalNoiseTexture (over ScalePoint(Perlin), an improved-Perlin restatement)
+//                                                                     texture/mod.rs:26-37
+//   skyboxes   NoSkybox, SimpleSkybox, WhiteSkybox, HdrImageSkybox(image)
+//                                                                     skybox/mod.rs:28-34
 //   camera     Camera { pos, fwd, v_fov, focus_dist, defocus_angle } scene/camera.rs:10-38
 //   options    RenderOpts { width, height, samples, mode, ray_depth, ray_branching }
 //                                                                     render/render_opts.rs:8-29
@@ -30,6 +34,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -120,12 +125,52 @@ struct Transform3 {
     }
 };
 
+// ---------------------------------------------------------------------------------------------- images
+// core/image.rs: pixel (x, y) at x * height + y (x-major), as the engine indexes Image[(x, y)].
+struct Image {
+    std::size_t width = 0, height = 0;
+    std::vector<ColourRgb> pixels;  // x-major: pixel (x, y) at x * height + y
+    const ColourRgb &operator()(std::size_t x, std::size_t y) const { return pixels[x * height + y]; }
+    auto begin() const { return pixels.begin(); }
+    auto end() const { return pixels.end(); }
+};
+
+// ---------------------------------------------------------------------------------------------- textures
+struct TextureInstance;
+using TexturePtr = std::shared_ptr<const TextureInstance>;
+struct Vector2 { Number x = 0, y = 0; };
+struct SolidTexture { ColourRgb albedo; };
+struct WorldCheckerTexture { Vector3 offset; TexturePtr even, odd; Number scale = 1; };
+struct UvCheckerTexture { Vector2 offset; TexturePtr even, odd; Number scale = 1; };
+struct ImageTexture { std::shared_ptr<const Image> image; Vector2 scale{1, 1}; Vector2 offset; };
+// An improved Perlin noise whose permutation is drawn from `seed` (not the noise crate's permutation).
+struct Perlin { std::uint32_t seed = 0; };
+struct ScalePoint { Perlin source; Number scale = 1; };
+// ColourSource::Greyscale (one noise) or ::Rgb (three noises sharing one scale).
+struct ColourSource {
+    std::vector<ScalePoint> noises;
+    static ColourSource Greyscale(ScalePoint n) { return {{n}}; }
+    static ColourSource Rgb(ScalePoint r, ScalePoint g, ScalePoint b) { return {{r, g, b}}; }
+};
+struct UvNoiseTexture { ColourSource source; };
+struct WorldNoiseTexture { ColourSource source; };
+struct LocalNoiseTexture { ColourSource source; };
+struct TextureInstance : std::variant<SolidTexture, WorldCheckerTexture, UvCheckerTexture, ImageTexture,
+                                      UvNoiseTexture, WorldNoiseTexture, LocalNoiseTexture> {
+    using variant::variant;
+    TextureInstance() : variant(SolidTexture{}) {}
+    TextureInstance(ColourRgb c) : variant(SolidTexture{c}) {}  // impl From<Colour> for TextureInstance
+    TextureInstance(Channel r, Channel g, Channel b) : variant(SolidTexture{{r, g, b}}) {}
+};
+template <class T>
+inline TexturePtr texture(T t) { return std::make_shared<const TextureInstance>(std::move(t)); }
+
 // ---------------------------------------------------------------------------------------------- materials
-struct LambertianMaterial { ColourRgb albedo; };
-struct MetalMaterial { ColourRgb albedo; Number fuzz = 0; };
-struct DielectricMaterial { ColourRgb albedo = ColourRgb::WHITE(); Number refractive_index = 1.5; Number density = 0; };
-struct IsotropicMaterial { ColourRgb albedo; Number density = 0; };
-struct LightMaterial { ColourRgb emissive; };
+struct LambertianMaterial { TextureInstance albedo; };
+struct MetalMaterial { TextureInstance albedo; Number fuzz = 0; };
+struct DielectricMaterial { TextureInstance albedo = ColourRgb::WHITE(); Number refractive_index = 1.5; Number density = 0; };
+struct IsotropicMaterial { TextureInstance albedo; Number density = 0; };
+struct LightMaterial { TextureInstance emissive; };
 using MaterialInstance =
     std::variant<LambertianMaterial, MetalMaterial, DielectricMaterial, IsotropicMaterial, LightMaterial>;
 
@@ -219,7 +264,16 @@ struct VolumetricObject {
 };
 using ObjectInstance = std::variant<SimpleObject, VolumetricObject>;
 
-enum class SkyboxInstance { NoSkybox = RAYNA_SKY_NONE, SimpleSkybox = RAYNA_SKY_SIMPLE, WhiteSkybox = RAYNA_SKY_WHITE };
+// SkyboxInstance (skybox/mod.rs:28-34): a kind (RAYNA_SKY_*) and, for HdrImageSkybox, its image.
+struct SkyboxInstance {
+    int32_t kind = RAYNA_SKY_SIMPLE;
+    std::shared_ptr<const Image> image;
+    static const SkyboxInstance NoSkybox, SimpleSkybox, WhiteSkybox;
+    static SkyboxInstance HdrImageSkybox(std::shared_ptr<const Image> image) { return {RAYNA_SKY_HDRI, image}; }
+};
+inline const SkyboxInstance SkyboxInstance::NoSkybox{RAYNA_SKY_NONE, nullptr};
+inline const SkyboxInstance SkyboxInstance::SimpleSkybox{RAYNA_SKY_SIMPLE, nullptr};
+inline const SkyboxInstance SkyboxInstance::WhiteSkybox{RAYNA_SKY_WHITE, nullptr};
 
 struct StandardScene {
     std::vector<ObjectInstance> objects;
@@ -253,13 +307,6 @@ struct RenderOpts {
 };
 
 // ---------------------------------------------------------------------------------------------- results
-struct Image {
-    std::size_t width = 0, height = 0;
-    std::vector<ColourRgb> pixels;  // x-major: pixel (x, y) at x * height + y
-    const ColourRgb &operator()(std::size_t x, std::size_t y) const { return pixels[x * height + y]; }
-    auto begin() const { return pixels.begin(); }
-    auto end() const { return pixels.end(); }
-};
 
 struct RenderStats {
     double duration = 0;           // seconds
@@ -290,7 +337,70 @@ struct SceneArrays {
     std::vector<rayna_material> materials;
     std::vector<double> spheres, tri_v, tri_n, paras, boxes, transforms;
     std::vector<rayna_sdf> sdfs;
+    std::vector<rayna_texture> textures;
+    std::vector<std::vector<float>> image_data;  // row-major RGB
+    std::vector<rayna_image> images;
+    std::vector<const Image *> image_keys;
     rayna_scene_desc desc{};
+
+    int32_t image(const Image *im) {
+        for (std::size_t i = 0; i < image_keys.size(); ++i)
+            if (image_keys[i] == im) return (int32_t)i;
+        std::vector<float> rgb(3 * im->width * im->height);
+        for (std::size_t y = 0; y < im->height; ++y)
+            for (std::size_t x = 0; x < im->width; ++x) {
+                const ColourRgb &c = (*im)(x, y);
+                float *o = &rgb[3 * (y * im->width + x)];
+                o[0] = c.r;
+                o[1] = c.g;
+                o[2] = c.b;
+            }
+        image_data.push_back(std::move(rgb));
+        image_keys.push_back(im);
+        images.push_back({(uint32_t)im->width, (uint32_t)im->height, nullptr});
+        return (int32_t)images.size() - 1;
+    }
+
+    // A texture as a 1-based index into `textures` (sub-textures first, so checkers name earlier entries).
+    int32_t texture_ref(const TextureInstance &t) {
+        rayna_texture r{};
+        std::visit([&](const auto &x) {
+            using T = std::decay_t<decltype(x)>;
+            auto noise = [&](const ColourSource &s) {
+                r.noise_rgb = s.noises.size() == 3 ? 1 : 0;
+                for (std::size_t k = 0; k < s.noises.size() && k < 3; ++k) r.noise_seed[k] = s.noises[k].source.seed;
+                r.scale = s.noises.empty() ? 1.0 : s.noises[0].scale;
+            };
+            if constexpr (std::is_same_v<T, SolidTexture>) {
+                r.kind = RAYNA_TEX_SOLID;
+                r.colour[0] = x.albedo.r;
+                r.colour[1] = x.albedo.g;
+                r.colour[2] = x.albedo.b;
+            }
+            if constexpr (std::is_same_v<T, WorldCheckerTexture> || std::is_same_v<T, UvCheckerTexture>) {
+                r.kind = std::is_same_v<T, WorldCheckerTexture> ? RAYNA_TEX_WORLD_CHECKER : RAYNA_TEX_UV_CHECKER;
+                r.odd = texture_ref(x.odd ? *x.odd : TextureInstance{});
+                r.even = texture_ref(x.even ? *x.even : TextureInstance{});
+                r.scale = x.scale;
+                r.offset[0] = x.offset.x;
+                r.offset[1] = x.offset.y;
+                if constexpr (std::is_same_v<T, WorldCheckerTexture>) r.offset[2] = x.offset.z;
+            }
+            if constexpr (std::is_same_v<T, ImageTexture>) {
+                r.kind = RAYNA_TEX_IMAGE;
+                r.image = image(x.image.get());
+                r.image_scale[0] = x.scale.x;
+                r.image_scale[1] = x.scale.y;
+                r.offset[0] = x.offset.x;
+                r.offset[1] = x.offset.y;
+            }
+            if constexpr (std::is_same_v<T, UvNoiseTexture>) { r.kind = RAYNA_TEX_UV_NOISE; noise(x.source); }
+            if constexpr (std::is_same_v<T, WorldNoiseTexture>) { r.kind = RAYNA_TEX_WORLD_NOISE; noise(x.source); }
+            if constexpr (std::is_same_v<T, LocalNoiseTexture>) { r.kind = RAYNA_TEX_LOCAL_NOISE; noise(x.source); }
+        }, static_cast<const TextureInstance::variant &>(t));
+        textures.push_back(r);
+        return (int32_t)textures.size();
+    }
 
     void put3(std::vector<double> &out, const Vector3 &v) {
         out.push_back(v.x);
@@ -300,10 +410,14 @@ struct SceneArrays {
 
     rayna_material material(const MaterialInstance &m) {
         rayna_material r{};
-        auto colour = [&](const ColourRgb &c) {
-            r.colour[0] = c.r;
-            r.colour[1] = c.g;
-            r.colour[2] = c.b;
+        auto colour = [&](const TextureInstance &t) {  // a solid colour in place, any other texture by index
+            if (const SolidTexture *s = std::get_if<SolidTexture>(&t)) {
+                r.colour[0] = s->albedo.r;
+                r.colour[1] = s->albedo.g;
+                r.colour[2] = s->albedo.b;
+            } else {
+                r.texture = texture_ref(t);
+            }
         };
         std::visit([&](const auto &x) {
             using T = std::decay_t<decltype(x)>;
@@ -432,9 +546,15 @@ struct SceneArrays {
         desc.n_boxes = (int32_t)(boxes.size() / 6);
         desc.sdfs = sdfs.data();
         desc.n_sdfs = (int32_t)sdfs.size();
-        desc.skybox = (int32_t)sc.skybox;
+        desc.skybox = sc.skybox.kind;
         desc.transforms = transforms.data();
         desc.n_transforms = (int32_t)(transforms.size() / 12);
+        if (sc.skybox.kind == RAYNA_SKY_HDRI && sc.skybox.image) desc.sky_image = image(sc.skybox.image.get());
+        for (std::size_t i = 0; i < images.size(); ++i) images[i].rgb = image_data[i].data();
+        desc.textures = textures.data();
+        desc.n_textures = (int32_t)textures.size();
+        desc.images = images.data();
+        desc.n_images = (int32_t)images.size();
     }
 };
 

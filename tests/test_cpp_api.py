@@ -8,7 +8,8 @@ import subprocess
 import numpy as np
 
 from rayna_amd import prepare_scene
-from rayna_amd.scene import (AxisBoxMesh, BvhMesh, DielectricMaterial, IsotropicMaterial, LambertianMaterial, LightMaterial,
+from rayna_amd.scene import (AxisBoxMesh, BvhMesh, ColourSource, DielectricMaterial, HdrImageSkybox, ImageTexture,
+                             Perlin, ScalePoint, WorldCheckerTexture, WorldNoiseTexture, IsotropicMaterial, LambertianMaterial, LightMaterial,
                              MetalMaterial, ParallelogramMesh, RaymarchedIsosurfaceMesh, Scene, SdfMandelbulb,
                              SdfSminSpheres, SimpleObject, SimpleSkybox, SphereMesh, Transform3, TriangleListMesh,
                              TriangleMesh, VolumetricObject, object_matrix)
@@ -57,8 +58,26 @@ int main() {
                                                       Transform3::from_scale({2, 1, 0.5}).then_translate({0, 3, 0})));
     s.objects.push_back(VolumetricObject::make(SphereMesh{{3, 1, 3}, 0.5}, IsotropicMaterial{{0.5f, 0.5f, 0.5f}, 0.1}, 2.0,
                                                Transform3::from_scale({1.5, 1, 1})));
-    s.skybox = SkyboxInstance::SimpleSkybox;
+    auto img = std::make_shared<Image>();
+    img->width = 4;
+    img->height = 3;
+    for (int i = 0; i < 12; ++i) img->pixels.push_back(ColourRgb{0.1f * i, 0.05f * i, 1.0f - 0.08f * i});
+    const TextureInstance checker = WorldCheckerTexture{{0.25, 0, 0.1}, texture(ColourRgb{0.9f, 0.9f, 0.9f}),
+                                                        texture(ColourRgb{0.2f, 0.3f, 0.1f}), 0.5};
+    s.objects.push_back(SimpleObject::new_uncorrected(SphereMesh{{5, 1, 5}, 1}, LambertianMaterial{checker}));
+    s.objects.push_back(SimpleObject::new_uncorrected(SphereMesh{{7, 1, 5}, 1},
+                                                      MetalMaterial{ImageTexture{img, {2, 1}, {0.1, 0}}, 0.2}));
+    s.objects.push_back(SimpleObject::new_uncorrected(
+        SphereMesh{{9, 1, 5}, 1},
+        LambertianMaterial{WorldNoiseTexture{ColourSource::Greyscale(ScalePoint{Perlin{69}, 4.0})}}));
+    s.skybox = SkyboxInstance::HdrImageSkybox(img);
     detail::SceneArrays a(s);
+    std::printf("%d %d %d", a.desc.n_textures, a.desc.n_images, a.desc.sky_image);
+    for (const rayna_texture &t : a.textures)
+        std::printf(" %d:%d:%d:%d:%u:%.17g", t.kind, t.odd, t.even, t.image, t.noise_seed[0], t.scale);
+    for (int i = 0; i < a.desc.n_materials; ++i) std::printf(" m%d", a.materials[i].texture);
+    for (float v : a.image_data[0]) std::printf(" %.9g", v);
+    std::printf("\n");
     std::printf("%d", a.desc.n_transforms);
     for (double v : a.transforms) std::printf(" %.17g", v);
     std::printf("\n");
@@ -108,7 +127,16 @@ def _python_scene():
                              Transform3.from_scale((2, 1, 0.5)).then_translate((0, 3, 0)), corrected=False))
     objs.append(VolumetricObject(SphereMesh((3, 1, 3), 0.5), IsotropicMaterial((0.5, 0.5, 0.5), 0.1), 2.0,
                                  Transform3.from_scale((1.5, 1, 1))))
-    return Scene(objs, SimpleSkybox)
+    img = np.zeros((3, 4, 3), np.float32)  # (height, width, 3); the C++ image is x-major, pixel i = x * 3 + y
+    for i in range(12):
+        x, y = divmod(i, 3)
+        img[y, x] = (np.float32(0.1) * i, np.float32(0.05) * i, np.float32(1.0) - np.float32(0.08) * i)
+    checker = WorldCheckerTexture(even=(0.9, 0.9, 0.9), odd=(0.2, 0.3, 0.1), scale=0.5, offset=(0.25, 0, 0.1))
+    objs.append(SimpleObject(SphereMesh((5, 1, 5), 1), LambertianMaterial(checker)))
+    objs.append(SimpleObject(SphereMesh((7, 1, 5), 1), MetalMaterial(ImageTexture(img, (2, 1), (0.1, 0)), 0.2)))
+    objs.append(SimpleObject(SphereMesh((9, 1, 5), 1),
+                             LambertianMaterial(WorldNoiseTexture(ColourSource.greyscale(ScalePoint(Perlin(69), 4.0))))))
+    return Scene(objs, HdrImageSkybox(img))
 
 
 def test_cpp_scene_flattening_matches_python(tmp_path):
@@ -119,14 +147,25 @@ def test_cpp_scene_flattening_matches_python(tmp_path):
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", f"-I{os.path.join(ROOT, 'include')}", str(src),
                     f"-L{lib_dir}", "-lrayna_amd", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], check=True)
     lines = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines()
-    xf = lines[0].split()
+    xf = lines[1].split()
     sc = _python_scene()
     mats = [object_matrix(o) for o in sc.objects if o.transform is not None]
     assert int(xf[0]) == len(mats) == 4
     got = np.array([float(v) for v in xf[1:]]).reshape(-1, 12)
     for g, m in zip(got, mats):
         assert np.allclose(g, m[:3, :4].reshape(-1), rtol=0, atol=1e-15)
-    out = lines[1].split()
+    tex = lines[0].split()
+    d = sc.desc()
+    assert (int(tex[0]), int(tex[1]), int(tex[2])) == (d.n_textures, d.n_images, d.sky_image)
+    want = [f"{t.kind}:{t.odd}:{t.even}:{t.image}:{t.noise_seed[0]}:{t.scale!r}" for t in d.textures[:d.n_textures]]
+    got = tex[3:3 + d.n_textures]
+    assert [w.rsplit(":", 1)[0] for w in want] == [g.rsplit(":", 1)[0] for g in got]
+    assert [float(w.rsplit(":", 1)[1]) for w in want] == [float(g.rsplit(":", 1)[1]) for g in got]
+    mats = [f"m{d.materials[i].texture}" for i in range(d.n_materials)]
+    assert tex[3 + d.n_textures:3 + d.n_textures + d.n_materials] == mats
+    pix = np.array([float(v) for v in tex[3 + d.n_textures + d.n_materials:]], np.float32)
+    assert np.array_equal(pix, sc._cache[1]["img_data"][0].reshape(-1))
+    out = lines[2].split()
     h, n = int(out[0]), int(out[1])
     ph, obj, loc, _ = prepare_scene(sc)
     assert h == ph and n == len(obj)
