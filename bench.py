@@ -237,6 +237,8 @@ def main(argv=None):
         return 0
     from rayna_amd import Renderer
 
+    if torch.cuda.is_available():  # the framework's device context first, so that its setup is not timed
+        torch.cuda.synchronize()
     scene, cam, opts = workload(a.workload)
     t_build = time.time()
     r = Renderer(scene, cam, opts)
@@ -244,15 +246,19 @@ def main(argv=None):
     for _ in range(a.warmup):
         r.render(to_host=False)
     torch.cuda.synchronize() if torch.cuda.is_available() else None
-    ext_ms = sh_ms = kern_ms = 0.0
+    ext_ms = sh_ms = kern_ms = call_ms = 0.0
     segs = 0
     launches = 0
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
+        ts = time.perf_counter()
         st = r.render(to_host=False).stats
+        step_ms.append((time.perf_counter() - ts) * 1e3)
         ext_ms += st.extend_ms
         sh_ms += st.shade_ms
         kern_ms += st.kernel_ms
+        call_ms += st.duration * 1e3
         segs += st.segments
         launches += st.extend_launches
     torch.cuda.synchronize() if torch.cuda.is_available() else None
@@ -285,6 +291,8 @@ def main(argv=None):
         "extend_ms_per_frame": ext_ms / a.steps,
         "shade_ms_per_frame": sh_ms / a.steps,
         "kernel_ms_per_frame": kern_ms / a.steps,
+        "render_call_ms_per_frame": call_ms / a.steps,
+        "step_ms_min_median_max": [min(step_ms), sorted(step_ms)[len(step_ms) // 2], max(step_ms)],
         "extend_launches_per_frame": launches // max(1, a.steps),
         "frame_alg_gbps": frame_bytes / secs / 1e9,
         "scene_build_s": t_build,

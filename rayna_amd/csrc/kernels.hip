@@ -22,7 +22,7 @@ constexpr int BLOCK = 256;
 #define RAYNA_EXT_BLOCK 128
 #endif
 #ifndef RAYNA_EXT_LDS_STACK
-#define RAYNA_EXT_LDS_STACK 16
+#define RAYNA_EXT_LDS_STACK 12
 #endif
 constexpr int EXT_BLOCK = RAYNA_EXT_BLOCK;
 constexpr int EXT_LDS_STACK = RAYNA_EXT_LDS_STACK;  // traversal stack entries per lane held in LDS

@@ -10,6 +10,8 @@
 //  * while-while: leaf ranges are pushed on the stack like inner nodes; a lane first pops inner nodes until
 //    it reaches a leaf, then tests that leaf's primitives one per iteration, so the lanes of a wave run the
 //    box code together and the primitive code together;
+//  * speculative inner loop: a lane that has found a leaf range keeps expanding inner nodes until every lane of
+//    its wave holds one, so the primitive loop runs with more lanes busy;
 //  * persistent waves with dynamic fetch: each wave takes rays from a per-launch counter whenever enough of
 //    its lanes are idle, so lanes whose rays finish early pick up new rays instead of waiting.
 
@@ -44,19 +46,18 @@ __device__ __forceinline__ int32_t leaf_code(int32_t start, int count, bool item
                      ((uint32_t)start & LEAF_CODE_START));
 }
 
-// Traversal stack of one lane: the top EXT_LDS_STACK entries in LDS (one column per lane, conflict-free),
+// Traversal stack of one lane: the top EXT_LDS_STACK entries in LDS, each (code, distance) one 64-bit word (one
+// column per lane, conflict-free),
 // deeper entries in this thread's column of a global overflow array (`stride` = threads in the grid).  The
 // host sizes the overflow for the deepest stack the tree can need, so no push is ever dropped.
 struct LaneStack {
-    int32_t *s_node;
-    float *s_tent;
+    int2 *s_entry;
     int2 *g;
     uint32_t stride;
     int sp;
     __device__ __forceinline__ void push(int32_t n, float t) {
         if (sp < EXT_LDS_STACK) {
-            s_node[sp * EXT_BLOCK + threadIdx.x] = n;
-            s_tent[sp * EXT_BLOCK + threadIdx.x] = t;
+            s_entry[sp * EXT_BLOCK + threadIdx.x] = make_int2(n, __float_as_int(t));
         } else {
             g[(size_t)(sp - EXT_LDS_STACK) * stride] = make_int2(n, __float_as_int(t));
         }
@@ -65,8 +66,9 @@ struct LaneStack {
     __device__ __forceinline__ void pop(int32_t *n, float *t) {
         --sp;
         if (sp < EXT_LDS_STACK) {
-            *n = s_node[sp * EXT_BLOCK + threadIdx.x];
-            *t = s_tent[sp * EXT_BLOCK + threadIdx.x];
+            const int2 e = s_entry[sp * EXT_BLOCK + threadIdx.x];
+            *n = e.x;
+            *t = __int_as_float(e.y);
         } else {
             const int2 e = g[(size_t)(sp - EXT_LDS_STACK) * stride];
             *n = e.x;
@@ -256,20 +258,23 @@ __device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_
 
 // Closest bounded hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501); the
 // unbounded objects are marched afterwards by k_march.  BATCH: the slots belong to the requests of bv, each
-// with its own scene root and RNG key.  INST: the scene holds instances.  Register budget: 96 VGPRs, the most that
-// keeps the five waves per SIMD the LDS stack allows.
+// with its own scene root and RNG key.  INST: the scene holds instances.  Occupancy: a 128-lane block holds
+// 12 KiB of LDS stack, so 13 blocks fit a CU; the register budget of 80 VGPRs keeps six waves per SIMD (measured:
+// 8.5% faster than five waves with a 16-entry stack; seven waves spill).
 template <bool BATCH, bool INST>
-__global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_extend(
+#ifndef RAYNA_EXT_WAVES
+#define RAYNA_EXT_WAVES 6
+#endif
+__global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_EXT_WAVES))) void k_extend(
     SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue, const uint32_t *count_ptr, uint32_t depth,
     int2 *overflow, uint32_t *work, ExtendTuning tune, BatchView bv) {
-    __shared__ int32_t s_node[EXT_LDS_STACK * EXT_BLOCK];
-    __shared__ float s_tent[EXT_LDS_STACK * EXT_BLOCK];
+    __shared__ int2 s_entry[EXT_LDS_STACK * EXT_BLOCK];
     const uint32_t count = queue ? *count_ptr : fp.n_slots;
     const uint32_t refill_lanes = tune.refill_lanes, wave_batch = tune.wave_batch;
     dm::Key key{fp.key0, fp.key1};
     const uint32_t gid = blockIdx.x * EXT_BLOCK + threadIdx.x;
     const uint32_t lane = __lane_id();
-    LaneStack st{s_node, s_tent, overflow + gid, gridDim.x * EXT_BLOCK, 0};
+    LaneStack st{s_entry, overflow + gid, gridDim.x * EXT_BLOCK, 0};
     bool active = false, exhausted = false;
     uint32_t slot = 0, pixel = 0, sample = 0;
     DRay ray{};
@@ -340,21 +345,38 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) 
         }
         RAYNA_STAT(iterations, 1);
         if (active) {
-            while (leaf == 0 && st.sp > 0) {  // inner nodes until a leaf range comes up
+            // Inner nodes until every lane holds a leaf range.  A lane that holds one keeps expanding inner nodes
+            // while others search (speculative traversal, Aila & Laine 2009); a second leaf range, or an
+            // instance's end, goes back on the stack and the lane waits.
+            bool stalled = false;
+            while (__any(leaf == 0 && st.sp > 0)) {
+                if (stalled || st.sp == 0) continue;
                 int32_t code;
                 float tent;
                 st.pop(&code, &tent);
                 RAYNA_STAT(pops, 1);
                 if (!(tent <= b.t * BOX_SLACK)) {  // culled, or an instance's marker (a NaN distance)
-                    if (INST && tent != tent) finish_instance(sv, pb, slot, code, ray, b, st);
+                    if (INST && tent != tent) {
+                        if (leaf != 0) {
+                            st.push(code, tent);
+                            stalled = true;
+                        } else {
+                            finish_instance(sv, pb, slot, code, ray, b, st);
+                        }
+                    }
                     continue;
                 }
                 if (code < 0) {
+                    if (leaf != 0) {
+                        st.push(code, tent);
+                        stalled = true;
+                        continue;
+                    }
                     leaf = code;
                     leaf_i = 0;
                     leaf_n = (int)(((uint32_t)code >> 27) & 7u) + 1;
                     leaf_t = tent;
-                    break;
+                    continue;
                 }
                 RAYNA_STAT(expands, 1);
                 expand(sv, ray, b.t, code, st);
