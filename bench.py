@@ -11,6 +11,7 @@ renderer, timed on this host on a bounded pixel sample).
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -33,6 +34,18 @@ FRAME_BYTES_PER_SAMPLE = 48 + 12
 # (x2.0: a 256 MiB copy reads 131,088 KiB) plus WRITE_SIZE 8.485e6 KiB (x1.0: the same copy writes 262,144 KiB).
 # Node and triangle fetches that miss L2 make up the difference to the 40 algorithmic bytes.
 EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (4.039e7 * 2.0 + 8.485e6) * 1024 / 263045425}
+
+
+class _NoGc:
+    """Collects, then pauses Python's cyclic collector for a timed region: a collection walks the whole heap
+    (torch's objects included) and can stall the host, and so the GPU, for tens of milliseconds."""
+
+    def __enter__(self):
+        gc.collect()
+        gc.disable()
+
+    def __exit__(self, *exc):
+        gc.enable()
 
 
 def workload(name: str):
@@ -166,14 +179,16 @@ def batch_leg(tune: bool = True):
 
     def one_by_one(requests):
         lat = []
-        t0 = time.perf_counter()
-        for q in requests:
-            r = rends[q.scene]
-            r.set_camera(q.camera)
-            r.set_options(q.opts)
-            r.render(layout=q.layout)
-            lat.append(time.perf_counter() - t0)
-        return time.perf_counter() - t0, np.array(lat) * 1e3
+        with _NoGc():
+            t0 = time.perf_counter()
+            for q in requests:
+                r = rends[q.scene]
+                r.set_camera(q.camera)
+                r.set_options(q.opts)
+                r.render(layout=q.layout)
+                lat.append(time.perf_counter() - t0)
+            total_s = time.perf_counter() - t0
+        return total_s, np.array(lat) * 1e3
 
     one_by_one(warm)
     seq_s, seq_lat = one_by_one(reqs)
@@ -182,9 +197,11 @@ def batch_leg(tune: bool = True):
 
     def batched(schedule):
         b.set_tuning(schedule=schedule, use_graph=False, **shape)  # eager: every completion is timed
-        t0 = time.perf_counter()
-        _, st = b.render(reqs)
-        return time.perf_counter() - t0, st
+        with _NoGc():
+            t0 = time.perf_counter()
+            _, st = b.render(reqs)
+            total_s = time.perf_counter() - t0
+        return total_s, st
 
     bat_s, st = batched("by_scene")
     sjf_s, st2 = batched("shortest_first")
@@ -250,6 +267,8 @@ def main(argv=None):
     segs = 0
     launches = 0
     step_ms = []
+    nogc = _NoGc()
+    nogc.__enter__()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         ts = time.perf_counter()
@@ -263,6 +282,7 @@ def main(argv=None):
         launches += st.extend_launches
     torch.cuda.synchronize() if torch.cuda.is_available() else None
     t1 = time.perf_counter()
+    nogc.__exit__(None)
     secs = t1 - t0
     samples_per_frame = opts.width * opts.height * opts.samples
     value = samples_per_frame * a.steps / secs / 1e6

@@ -84,6 +84,11 @@ __device__ __forceinline__ DRay ray_from_dir(f3 o, f3 unit_d) {
     r.inv = f3{1.0f / unit_d.x, 1.0f / unit_d.y, 1.0f / unit_d.z};
     return r;
 }
+// A ray whose reciprocal direction is left unset (shading needs it only for box hits: with_inv).
+__device__ __forceinline__ DRay ray_lazy(f3 o, f3 unit_d) { return DRay{o, unit_d, f3{0.0f, 0.0f, 0.0f}}; }
+__device__ __forceinline__ DRay with_inv(const DRay &r) {
+    return DRay{r.o, r.d, f3{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z}};
+}
 __device__ __forceinline__ f3 at(const DRay &r, float t) { return r.o + r.d * t; }
 
 // ------------------------------------------------------------------------------------------ primitives
@@ -416,7 +421,7 @@ __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint
             float tt, w;
             int axis = 0;
             f3 sg{0.0f, 0.0f, 0.0f};
-            box_t(sv.boxes[p.prim], r, p.t, p.t, &tt, &axis, &w, &sg);
+            box_t(sv.boxes[p.prim], with_inv(r), p.t, p.t, &tt, &axis, &w, &sg);
             f3 rn{0.0f, 0.0f, 0.0f};
             if (axis == 0) rn.x = sg.x;
             if (axis == 1) rn.y = sg.y;
@@ -503,7 +508,7 @@ __device__ void hit_uv_side(const SceneView &sv, const DRay &world_ray, float wo
             int axis = 0;
             f3 sg{0.0f, 0.0f, 0.0f};
             const GBox g = sv.boxes[idx];
-            box_t(g, ray, t, t, &tt, &axis, &w, &sg, uv);
+            box_t(g, with_inv(ray), t, t, &tt, &axis, &w, &sg, uv);
             const float sa = axis == 0 ? sg.x : axis == 1 ? sg.y : sg.z;
             *side = 2 * axis + (sa > 0.0f ? 1 : 0);
             pl = pos - f3{g.cx, g.cy, g.cz};
@@ -707,7 +712,10 @@ __device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy,
 // TEX: the scene has textured materials or an image sky (without, their code is compiled out).  A textured
 // material's colour receives no gradient.
 template <bool ADJOINT, bool BATCH, bool TEX>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_shade(
+#ifndef RAYNA_SHADE_WAVES
+#define RAYNA_SHADE_WAVES 5
+#endif
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHADE_WAVES))) void k_shade(
     SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue_in, const uint32_t *count_in,
     int32_t *queue_out, uint32_t *count_out, uint32_t depth, AdjointParams ap, BatchView bv) {
     const uint32_t count = queue_in ? *count_in : fp.n_slots;
@@ -719,7 +727,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
         uint32_t slot = 0;
         if (q < count) {
             slot = queue_in ? (uint32_t)queue_in[q] : q;
-            const DRay ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
+            const DRay ray = ray_lazy(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
                                           f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
             Col T{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
             Col L{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
@@ -903,7 +911,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_batch(BatchView bv, const ui
 __global__ __launch_bounds__(BLOCK) void k_shade_mode(SceneView sv, FrameParams fp, PathBuffers pb) {
     const dm::Key key{fp.key0, fp.key1};
     for (uint32_t slot = blockIdx.x * BLOCK + threadIdx.x; slot < fp.n_slots; slot += gridDim.x * BLOCK) {
-        const DRay ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+        const DRay ray = ray_lazy(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
         const uint32_t ref = pb.hit_ref[slot];
         Col L;
         if (ref == 0u) {
