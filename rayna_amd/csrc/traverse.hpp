@@ -55,16 +55,30 @@ struct LaneStack {
     int2 *g;
     uint32_t stride;
     int sp;
+    // The top entry stays in registers until the next push: the nearest child an expansion pushes last is popped
+    // next without touching LDS.
+    int32_t top_n = 0;
+    float top_t = 0.0f;
+    bool top_valid = false;
+    __device__ __forceinline__ void store(int i, int32_t n, float t) {
+        if (i < EXT_LDS_STACK) s_entry[i * EXT_BLOCK + threadIdx.x] = make_int2(n, __float_as_int(t));
+        else g[(size_t)(i - EXT_LDS_STACK) * stride] = make_int2(n, __float_as_int(t));
+    }
     __device__ __forceinline__ void push(int32_t n, float t) {
-        if (sp < EXT_LDS_STACK) {
-            s_entry[sp * EXT_BLOCK + threadIdx.x] = make_int2(n, __float_as_int(t));
-        } else {
-            g[(size_t)(sp - EXT_LDS_STACK) * stride] = make_int2(n, __float_as_int(t));
-        }
+        if (top_valid) store(sp - 1, top_n, top_t);
+        top_n = n;
+        top_t = t;
+        top_valid = true;
         ++sp;
     }
     __device__ __forceinline__ void pop(int32_t *n, float *t) {
         --sp;
+        if (top_valid) {
+            top_valid = false;
+            *n = top_n;
+            *t = top_t;
+            return;
+        }
         if (sp < EXT_LDS_STACK) {
             const int2 e = s_entry[sp * EXT_BLOCK + threadIdx.x];
             *n = e.x;
@@ -333,6 +347,7 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
                     }
                     b = Best{FLT_MAX, -1, 0u, 0u, 0u};
                     st.sp = 0;
+                    st.top_valid = false;
                     if (root >= 0) st.push(root, -__builtin_inff());
                     leaf = 0;
                     active = true;
