@@ -148,9 +148,9 @@ def adjoint_leg(steps: int):
 def batch_leg(tune: bool = True):
     """The batched-rendering configuration: 64 requests (log-uniform 128^2 .. 1920x1080, 1..256 spp, scenes
     drawn from random_spheres / triangle_mesh / sdf_volume, seeds 100..163).  The launch shape comes from the
-    autotuner, run on the same requests at 1/16 of the samples, and serves both sides: one render call per
+    autotuner, run on the same requests at 1/2 of the samples, and serves both sides: one render call per
     request (a Renderer per scene, set_camera + set_options + render, in request order) and one batch call
-    (scheduled by scene, and shortest first).  Throughput from wall time; latency of request i: its
+    (scheduled by scene, and shortest first; its chunks traced on the tuned number of concurrent pipelines).  Throughput from wall time; latency of request i: its
     completion time from the start."""
     import numpy as np
     from rayna_amd import Renderer, presets
@@ -167,7 +167,7 @@ def batch_leg(tune: bool = True):
     cfg = dict(AT.DEFAULTS)
     tuned = None
     if tune:
-        tr = AT.tune(b, AT.scaled_requests(reqs, 16), repeats=1)
+        tr = AT.tune(b, AT.scaled_requests(reqs, 2), repeats=1)
         cfg = dict(tr.best)
         tuned = {"best": tr.best, "best_ms": tr.best_ms, "default_ms": tr.default_ms, "trials": len(tr.trials)}
     shape = {k: cfg[k] for k in ("chunk_slots", "refill_lanes", "wave_batch", "shade_blocks_per_cu")}
@@ -196,7 +196,7 @@ def batch_leg(tune: bool = True):
         r.close()
 
     def batched(schedule):
-        b.set_tuning(schedule=schedule, use_graph=False, **shape)  # eager: every completion is timed
+        b.set_tuning(schedule=schedule, use_graph=False, pipelines=cfg["pipelines"], **shape)  # eager: timed
         with _NoGc():
             t0 = time.perf_counter()
             _, st = b.render(reqs)
@@ -219,7 +219,7 @@ def batch_leg(tune: bool = True):
                            "batched_by_scene": {"p50": pct(st.request_ms, 50), "p99": pct(st.request_ms, 99)},
                            "batched_shortest_first": {"p50": pct(st2.request_ms, 50),
                                                       "p99": pct(st2.request_ms, 99)}},
-            "launch_shape": shape, "tuning": tuned}
+            "launch_shape": {**shape, "pipelines": cfg["pipelines"]}, "tuning": tuned}
 
 
 def main(argv=None):

@@ -2,9 +2,11 @@
 
 The tunables are those of rayna_tuning (include/rayna_gpu.h): paths per chunk (the queue size), the extend
 kernel's refill threshold (idle lanes before a wave takes new rays: the regeneration threshold) and its claim
-size, shade blocks per compute unit (the block shape of the shade grid), and hipGraph capture.  The search is
-coordinate descent from the defaults: each tunable in turn is swept with the others held, and the fastest value
-(median device time over `repeats` runs of the tuning batch) is kept.  Results are returned, not stored.
+size, shade blocks per compute unit (the block shape of the shade grid), the chunks in flight at once, and
+hipGraph capture.  The search is coordinate descent from the defaults: each tunable in turn is swept with the
+others held, and the fastest value (median device time over `repeats` runs of the tuning batch) is kept if it
+beats the current best by more than `margin` (a fraction; run-to-run noise should not move the shape).  Results
+are returned, not stored.
 """
 from __future__ import annotations
 
@@ -15,14 +17,15 @@ from typing import Dict, List, Sequence
 from .batch import BatchRenderer, Request
 
 SPACE: Dict[str, Sequence] = {
-    "chunk_slots": (1 << 23, 1 << 25, 1 << 26, 1 << 27),
+    "chunk_slots": (1 << 25, 1 << 26, 1 << 27, 1 << 28),
     "refill_lanes": (8, 16, 32, 48),
-    "wave_batch": (32, 64, 128),
-    "shade_blocks_per_cu": (4, 8, 16),
+    "wave_batch": (64, 128, 192, 256),
+    "shade_blocks_per_cu": (8, 16, 32),
+    "pipelines": (1, 2, 3, 4),
     "use_graph": (False, True),
 }
-DEFAULTS = {"chunk_slots": 1 << 27, "refill_lanes": 16, "wave_batch": 64, "shade_blocks_per_cu": 8,
-            "use_graph": False}
+DEFAULTS = {"chunk_slots": 1 << 28, "refill_lanes": 16, "wave_batch": 128, "shade_blocks_per_cu": 16,
+            "pipelines": 3, "use_graph": False}
 
 
 @dataclass
@@ -39,7 +42,7 @@ def scaled_requests(requests: List[Request], spp_divisor: int) -> List[Request]:
 
 
 def tune(batch: BatchRenderer, requests: List[Request], repeats: int = 2, schedule: str = "by_scene",
-         space: Dict[str, Sequence] = SPACE) -> TuneResult:
+         space: Dict[str, Sequence] = SPACE, margin: float = 0.01) -> TuneResult:
     cache: Dict[tuple, float] = {}
 
     def measure(cfg) -> float:
@@ -65,7 +68,7 @@ def tune(batch: BatchRenderer, requests: List[Request], repeats: int = 2, schedu
         for v in values:
             cfg = {**best, name: v}
             ms = measure(cfg)
-            if ms < best_ms:
+            if ms < best_ms * (1.0 - margin):
                 best, best_ms = cfg, ms
     res.best, res.best_ms = best, best_ms
     batch.set_tuning(schedule=schedule, **best)

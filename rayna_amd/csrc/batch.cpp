@@ -19,9 +19,14 @@ using namespace rayna;
 namespace {
 
 std::string g_batch_create_error;
-// Paths in flight per chunk: 2^27 (a 1080p frame of 64 samples in one chunk; ~10 GB of path state, sized for
-// the 288 GB of HBM of one MI355X).  Whole-frame chunks keep every depth's launch full.
-constexpr uint64_t DEFAULT_CHUNK_SLOTS = 1ull << 27;
+// Paths in flight per chunk: at most 2^28 (two 1080p frames of 64 samples; ~25 GB of path state per pipeline,
+// sized for the 288 GB of HBM of one MI355X).  Large chunks keep every depth's launch full and pay a depth's
+// drain once for many requests.  A pipeline's buffers grow to the largest chunk of a call (a power of two).
+constexpr uint64_t DEFAULT_CHUNK_SLOTS = 1ull << 28;
+// Chunks in flight at once, each on its own stream with its own path state: while one chunk's depth drains
+// (the last long paths of a launch hold a few waves), the other's launches fill the rest of the GPU.
+constexpr int DEFAULT_PIPELINES = 3;
+constexpr int MAX_PIPELINES = 4;
 
 // One request's share of a chunk: samples [s0, s0 + n) of request `req`.
 struct Piece {
@@ -46,6 +51,23 @@ struct Plan {
     uint64_t key = 0;                          // hash of the request list (graph reuse)
 };
 
+// One chunk in flight: its stream and the path state of up to `capacity` paths.  Chunk c of a call runs on
+// pipeline c % pipelines; a pipeline's chunks follow each other on its stream.
+struct Pipeline {
+    hipStream_t stream = nullptr;  // pipeline 0 runs on the batch's own stream
+    hipEvent_t done = nullptr;     // after the pipeline's last chunk of a call (joins the batch's stream)
+    uint64_t capacity = 0;
+    std::vector<void *> allocs;
+    PathBuffers pb{};
+    uint32_t *slot_req = nullptr;
+    int32_t *queue[2] = {nullptr, nullptr};
+    uint32_t *queue_count = nullptr;
+    uint32_t queue_count_len = 0;
+    unsigned long long *segments = nullptr;
+    float *mt = nullptr;           // march outcomes, 2 * capacity * n_sdf_max
+    int2 *overflow = nullptr;      // traversal stack entries beyond the LDS part
+};
+
 }  // namespace
 
 struct rayna_batch {
@@ -54,25 +76,17 @@ struct rayna_batch {
     std::vector<void *> scene_allocs;
     SceneView view{};
     std::vector<SceneSlot> slots;
-    int2 *overflow = nullptr;
+    size_t overflow_entries = 0;  // per pipeline: extend grid * block * stack entries beyond the LDS part
     int cus = 1, grid_extend = 1024, grid_shade = 1024, grid_march = 1024;
     uint32_t n_sdf_max = 0;  // most unbounded objects of one scene
-    float *mt = nullptr;     // march outcomes, capacity * n_sdf_max
-    uint64_t mt_cap = 0;
     uint64_t chunk_slots = DEFAULT_CHUNK_SLOTS;
     ExtendTuning tune;
     bool use_graph = false;
     int schedule = 0;
-
-    // path state for chunk_slots slots
-    uint64_t capacity = 0;
-    std::vector<void *> path_allocs;
-    PathBuffers pb{};
-    uint32_t *slot_req = nullptr;
-    int32_t *queue[2] = {nullptr, nullptr};
-    uint32_t *queue_count = nullptr;
-    uint32_t queue_count_len = 0;
-    unsigned long long *segments = nullptr;
+    int pipelines = DEFAULT_PIPELINES;
+    Pipeline pipes[MAX_PIPELINES];
+    hipEvent_t fork = nullptr;         // the call's start, which the other pipelines' streams wait for
+    std::vector<hipEvent_t> resolved;  // per chunk: its resolve done (resolves run in chunk order)
 
     // per call
     RequestParams *table = nullptr;
@@ -125,48 +139,62 @@ void free_graph(rayna_batch *b) {
     b->graph_key = 0;
 }
 
-void free_paths(rayna_batch *b) {
-    for (void *p : b->path_allocs) (void)hipFree(p);
-    b->path_allocs.clear();
-    b->pb = PathBuffers{};
-    b->slot_req = nullptr;
-    b->mt = nullptr;
-    b->queue[0] = b->queue[1] = nullptr;
-    b->queue_count = nullptr;
-    b->queue_count_len = 0;
-    b->segments = nullptr;
-    b->capacity = 0;
+void free_pipe(rayna_batch *b, Pipeline &P) {
+    for (void *p : P.allocs) (void)hipFree(p);
+    P.allocs.clear();
+    P.pb = PathBuffers{};
+    P.slot_req = nullptr;
+    P.queue[0] = P.queue[1] = nullptr;
+    P.queue_count = nullptr;
+    P.queue_count_len = 0;
+    P.segments = nullptr;
+    P.mt = nullptr;
+    P.overflow = nullptr;
+    P.capacity = 0;
     free_graph(b);
 }
 
-rayna_status alloc_paths(rayna_batch *b, uint64_t slots, uint32_t max_depth) {
-    if (b->capacity == slots && b->queue_count_len >= 3 * (max_depth + 2)) return RAYNA_OK;
-    free_paths(b);
+void free_paths(rayna_batch *b) {
+    for (Pipeline &P : b->pipes) free_pipe(b, P);
+}
+
+// Path state for `slots` paths (and its stream) in pipeline p.
+rayna_status alloc_pipe(rayna_batch *b, int p, uint64_t slots, uint32_t max_depth) {
+    Pipeline &P = b->pipes[p];
+    if (!P.stream) {
+        if (p == 0) P.stream = b->stream;
+        else BHIP(b, hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+    }
+    if (!P.done) BHIP(b, hipEventCreateWithFlags(&P.done, hipEventDisableTiming));
+    if (P.capacity >= slots && P.queue_count_len >= 3 * (max_depth + 2)) return RAYNA_OK;
+    free_pipe(b, P);
     auto alloc = [&](size_t bytes, void **out) -> rayna_status {
         BHIP(b, hipMalloc(out, std::max<size_t>(bytes, 4)));
-        b->path_allocs.push_back(*out);
+        P.allocs.push_back(*out);
         return RAYNA_OK;
     };
     rayna_status st;
-    float **fields[] = {&b->pb.ox, &b->pb.oy, &b->pb.oz, &b->pb.dx, &b->pb.dy, &b->pb.dz, &b->pb.tr,
-                        &b->pb.tg, &b->pb.tb, &b->pb.lr, &b->pb.lg, &b->pb.lb, &b->pb.hit_t};
+    float **fields[] = {&P.pb.ox, &P.pb.oy, &P.pb.oz, &P.pb.dx, &P.pb.dy, &P.pb.dz, &P.pb.tr,
+                        &P.pb.tg, &P.pb.tb, &P.pb.lr, &P.pb.lg, &P.pb.lb, &P.pb.hit_t};
     for (float **f : fields)
         if ((st = alloc(slots * sizeof(float), (void **)f)) != RAYNA_OK) return st;
-    if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->pb.hit_ref)) != RAYNA_OK) return st;
-    if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->pb.hit_aux)) != RAYNA_OK) return st;
-    if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->pb.hit_sub)) != RAYNA_OK) return st;
-    if ((st = alloc(slots * sizeof(uint32_t), (void **)&b->slot_req)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&P.pb.hit_ref)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&P.pb.hit_aux)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&P.pb.hit_sub)) != RAYNA_OK) return st;
+    if ((st = alloc(slots * sizeof(uint32_t), (void **)&P.slot_req)) != RAYNA_OK) return st;
     for (int q = 0; q < 2; ++q)
-        if ((st = alloc(slots * sizeof(int32_t), (void **)&b->queue[q])) != RAYNA_OK) return st;
+        if ((st = alloc(slots * sizeof(int32_t), (void **)&P.queue[q])) != RAYNA_OK) return st;
     // [0, D+2) paths entering each depth; [D+2, 2(D+2)) extend counters; [2(D+2), 3(D+2)) march counters
     const uint32_t qlen = 3 * (std::max<uint32_t>(max_depth, 16) + 2);
-    if ((st = alloc(qlen * sizeof(uint32_t), (void **)&b->queue_count)) != RAYNA_OK) return st;
-    if ((st = alloc(sizeof(unsigned long long), (void **)&b->segments)) != RAYNA_OK) return st;
-    if (b->n_sdf_max && (st = alloc(2 * slots * b->n_sdf_max * sizeof(float), (void **)&b->mt)) != RAYNA_OK)
+    if ((st = alloc(qlen * sizeof(uint32_t), (void **)&P.queue_count)) != RAYNA_OK) return st;
+    if ((st = alloc(sizeof(unsigned long long), (void **)&P.segments)) != RAYNA_OK) return st;
+    if (b->n_sdf_max && (st = alloc(2 * slots * b->n_sdf_max * sizeof(float), (void **)&P.mt)) != RAYNA_OK)
         return st;
-    b->pb.hit_rank = nullptr;
-    b->capacity = slots;
-    b->queue_count_len = qlen;
+    if (b->overflow_entries && (st = alloc(b->overflow_entries * sizeof(int2), (void **)&P.overflow)) != RAYNA_OK)
+        return st;
+    P.pb.hit_rank = nullptr;
+    P.capacity = slots;
+    P.queue_count_len = qlen;
     return RAYNA_OK;
 }
 
@@ -310,42 +338,67 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
     pl.first_final.push_back((uint32_t)pl.finals.size());
 }
 
-// Enqueues every chunk of the plan.  `ev` (eager runs only): an event after the start and after each chunk.
-uint32_t enqueue(rayna_batch *b, const Plan &pl, std::vector<hipEvent_t> *ev) {
-    hipStream_t s = b->stream;
-    uint32_t launches = 0;
-    launch_fill_u32(reinterpret_cast<uint32_t *>(b->segments), 0u, 2, s);
-    ++launches;
-    if (ev) (void)hipEventRecord((*ev)[0], s);
+// Launches of a plan traced on `used` pipelines.
+uint32_t plan_launches(const Plan &pl, int used) {
+    uint32_t n = (uint32_t)used;  // segment counters
     for (size_t c = 0; c < pl.chunks.size(); ++c) {
-        BatchView bv{b->table + pl.first_entry[c], (uint32_t)pl.chunks[c].size(), b->slot_req};
+        const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
+        n += 2 + (pl.n_sdf[c] ? 4 : 2) * (pl.depth[c] + 1) + 2 + (nf ? 1 : 0);
+    }
+    return n;
+}
+
+// Enqueues every chunk of the plan, chunk c on pipeline c % used.  The pipelines' streams start after the
+// batch's stream (which holds the uploads) and the batch's stream waits for all of them at the end.  Resolves
+// add each chunk's samples into the requests' sums in chunk order, so the sums are those of one pipeline.
+// `timing` (eager runs only): events[0] at the start and events[1 + c] after chunk c.
+hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
+    hipError_t err = hipSuccess;
+    auto check = [&](hipError_t e) {
+        if (err == hipSuccess) err = e;
+    };
+    hipStream_t s0 = b->stream;
+    if (timing) check(hipEventRecord(b->events[0], s0));
+    if (used > 1) {
+        check(hipEventRecord(b->fork, s0));
+        for (int p = 1; p < used; ++p) check(hipStreamWaitEvent(b->pipes[p].stream, b->fork, 0));
+    }
+    for (int p = 0; p < used; ++p)
+        launch_fill_u32(reinterpret_cast<uint32_t *>(b->pipes[p].segments), 0u, 2, b->pipes[p].stream);
+    for (size_t c = 0; c < pl.chunks.size(); ++c) {
+        Pipeline &P = b->pipes[c % used];
+        hipStream_t s = P.stream;
+        BatchView bv{b->table + pl.first_entry[c], (uint32_t)pl.chunks[c].size(), P.slot_req};
         FrameParams fp{};
         fp.n_slots = pl.slots[c];
         fp.ray_depth = pl.depth[c];
         const uint32_t D = pl.depth[c];
-        launch_fill_u32(b->queue_count, 0u, 3 * (D + 2), s);
-        launch_generate(fp, b->pb, false, bv, s);
-        launches += 2;
+        launch_fill_u32(P.queue_count, 0u, 3 * (D + 2), s);
+        launch_generate(fp, P.pb, false, bv, s);
         for (uint32_t depth = 0; depth <= D; ++depth) {
-            const int32_t *qin = depth == 0 ? nullptr : b->queue[depth & 1];
-            const uint32_t *cin = depth == 0 ? nullptr : b->queue_count + depth;
-            int32_t *qout = b->queue[(depth + 1) & 1];
-            uint32_t *cout = b->queue_count + depth + 1;
-            uint32_t *work = b->queue_count + (D + 2) + depth;
-            launch_extend(b->view, fp, b->pb, qin, cin, depth, b->grid_extend, b->overflow, work, b->tune, bv, s);
-            launch_march(b->view, fp, b->pb, qin, cin, pl.n_sdf[c], b->mt, b->capacity * b->n_sdf_max,
-                         b->queue_count + 2 * (D + 2) + depth, b->grid_march, b->grid_shade, b->tune, bv, s);
-            launch_shade(b->view, fp, b->pb, qin, cin, qout, cout, depth, b->grid_shade, nullptr, bv, s);
-            launches += pl.n_sdf[c] ? 4 : 2;
+            const int32_t *qin = depth == 0 ? nullptr : P.queue[depth & 1];
+            const uint32_t *cin = depth == 0 ? nullptr : P.queue_count + depth;
+            int32_t *qout = P.queue[(depth + 1) & 1];
+            uint32_t *cout = P.queue_count + depth + 1;
+            uint32_t *work = P.queue_count + (D + 2) + depth;
+            launch_extend(b->view, fp, P.pb, qin, cin, depth, b->grid_extend, P.overflow, work, b->tune, bv, s);
+            launch_march(b->view, fp, P.pb, qin, cin, pl.n_sdf[c], P.mt, P.capacity * b->n_sdf_max,
+                         P.queue_count + 2 * (D + 2) + depth, b->grid_march, b->grid_shade, b->tune, bv, s);
+            launch_shade(b->view, fp, P.pb, qin, cin, qout, cout, depth, b->grid_shade, nullptr, bv, s);
         }
-        launch_count_segments(b->queue_count, D + 1, fp.n_slots, b->segments, s);
-        launch_resolve_batch(bv, pl.max_pixels[c], b->pb, b->sum, s);
+        launch_count_segments(P.queue_count, D + 1, fp.n_slots, P.segments, s);
+        if (used > 1 && c > 0) check(hipStreamWaitEvent(s, b->resolved[c - 1], 0));
+        launch_resolve_batch(bv, pl.max_pixels[c], P.pb, b->sum, s);
         const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
         launch_finalize_batch(bv, b->finals + pl.first_final[c], nf, pl.max_pixels[c], b->sum, b->out, s);
-        launches += 2 + (nf ? 1 : 0);
-        if (ev) (void)hipEventRecord((*ev)[1 + c], s);
+        if (used > 1) check(hipEventRecord(b->resolved[c], s));
+        if (timing) check(hipEventRecord(b->events[1 + c], s));
     }
-    return launches;
+    for (int p = 1; p < used; ++p) {
+        check(hipEventRecord(b->pipes[p].done, b->pipes[p].stream));
+        check(hipStreamWaitEvent(s0, b->pipes[p].done, 0));
+    }
+    return err;
 }
 
 void write_checkerboard(uint32_t w, uint32_t h, float *out, int layout) {  // renderer.rs:269-292
@@ -395,7 +448,7 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     b->cus = std::max(1, prop.multiProcessorCount);
     b->grid_extend = b->cus * extend_blocks_per_cu();
     b->grid_march = b->cus * march_blocks_per_cu();
-    b->grid_shade = b->cus * 8;
+    b->grid_shade = b->cus * SHADE_BLOCKS_PER_CU;
     std::vector<PackedScene> packed(n_scenes);
     for (int i = 0; i < n_scenes; ++i) {
         std::string err;
@@ -445,14 +498,10 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     for (const SceneSlot &sl : b->slots)
         if (sl.skybox == RAYNA_SKY_HDRI) v.textured = 1;
     const size_t deep = (size_t)std::max(0, merged.stack_need - extend_lds_stack());
-    if (deep > 0) {
-        void *p = nullptr;
-        if (hipMalloc(&p, deep * (size_t)b->grid_extend * extend_block() * sizeof(int2)) != hipSuccess) {
-            b->error = "overflow stack allocation failed";
-            return bail(RAYNA_OUT_OF_MEMORY);
-        }
-        b->scene_allocs.push_back(p);
-        b->overflow = static_cast<int2 *>(p);
+    b->overflow_entries = deep * (size_t)b->grid_extend * extend_block();
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+        b->error = "hipEventCreate failed";
+        return bail(RAYNA_DEVICE_ERROR);
     }
     *out = b;
     return RAYNA_OK;
@@ -461,13 +510,18 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
 rayna_status rayna_batch_set_tuning(rayna_batch *b, const rayna_tuning *t) {
     if (!b || !t) return bfail(b, RAYNA_INVALID_ARGUMENT, "null argument");
     if (t->refill_lanes > 64 || t->wave_batch > 4096 || t->shade_blocks_per_cu > 64 ||
-        (t->chunk_slots && t->chunk_slots < (1u << 16)) || t->schedule < 0 || t->schedule > 2)
+        (t->chunk_slots && t->chunk_slots < (1u << 16)) || t->schedule < 0 || t->schedule > 2 ||
+        t->pipelines < 0 || t->pipelines > MAX_PIPELINES)
         return bfail(b, RAYNA_INVALID_ARGUMENT, "tuning out of range");
     b->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
     b->tune.wave_batch = t->wave_batch ? t->wave_batch : ExtendTuning{}.wave_batch;
-    b->grid_shade = b->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : 8);
+    b->grid_shade = b->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : SHADE_BLOCKS_PER_CU);
+    const uint64_t chunk = b->chunk_slots;
     b->chunk_slots = t->chunk_slots ? t->chunk_slots : DEFAULT_CHUNK_SLOTS;
     clamp_chunk(b);
+    if (b->chunk_slots != chunk) free_paths(b);
+    b->pipelines = t->pipelines ? t->pipelines : DEFAULT_PIPELINES;
+    for (int p = b->pipelines; p < MAX_PIPELINES; ++p) free_pipe(b, b->pipes[p]);
     b->use_graph = t->use_graph != 0;
     b->schedule = t->schedule;
     free_graph(b);
@@ -490,7 +544,13 @@ rayna_status rayna_render_batch(rayna_batch *b, const rayna_request *reqs, int32
     Plan pl;
     make_plan(b, reqs, n, vps, valid, pl);
     rayna_status st;
-    if ((st = alloc_paths(b, b->chunk_slots, pl.max_depth)) != RAYNA_OK) return st;
+    const int used = (int)std::max<size_t>(1, std::min<size_t>(b->pipelines, pl.chunks.size()));
+    uint64_t need = 1ull << 16;
+    for (uint32_t sl : pl.slots)
+        while (need < sl) need <<= 1;
+    need = std::min(need, b->chunk_slots);
+    for (int p = 0; p < used; ++p)
+        if ((st = alloc_pipe(b, p, need, pl.max_depth)) != RAYNA_OK) return st;
     if ((st = ensure(b, &b->table, &b->table_cap, pl.table.size())) != RAYNA_OK) return st;
     if ((st = ensure(b, &b->finals, &b->finals_cap, pl.finals.size())) != RAYNA_OK) return st;
     if (pl.out_floats >= (1ull << 32))
@@ -510,31 +570,34 @@ rayna_status rayna_render_batch(rayna_batch *b, const rayna_request *reqs, int32
         BHIP(b, hipEventCreate(&e));
         b->events.push_back(e);
     }
-    uint32_t launches = 0;
+    while (b->resolved.size() < pl.chunks.size()) {
+        hipEvent_t e;
+        BHIP(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->resolved.push_back(e);
+    }
+    const uint32_t launches = plan_launches(pl, used);
     if (graph) {
-        if (!b->graph_exec || b->graph_key != pl.key) {
+        const uint64_t key = mix(pl.key, (uint64_t)used);
+        if (!b->graph_exec || b->graph_key != key) {
             free_graph(b);
             BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            launches = enqueue(b, pl, nullptr);
+            const hipError_t eq = enqueue(b, pl, used, false);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(b->stream, &g);
-            if (e != hipSuccess) return bfail(b, RAYNA_DEVICE_ERROR, std::string("graph capture: ") + hipGetErrorString(e));
+            if (eq != hipSuccess || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return bfail(b, RAYNA_DEVICE_ERROR,
+                             std::string("graph capture: ") + hipGetErrorString(eq != hipSuccess ? eq : e));
+            }
             b->graph = g;
             BHIP(b, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
-            b->graph_key = pl.key;
-        } else {
-            // the launch count of the same plan
-            for (size_t c = 0; c < pl.chunks.size(); ++c) {
-                const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
-                launches += 2 + (pl.n_sdf[c] ? 4 : 2) * (pl.depth[c] + 1) + 2 + (nf ? 1 : 0);
-            }
-            launches += 1;
+            b->graph_key = key;
         }
         BHIP(b, hipEventRecord(b->events[0], b->stream));
         BHIP(b, hipGraphLaunch(b->graph_exec, b->stream));
         BHIP(b, hipEventRecord(b->events[1], b->stream));
     } else {
-        launches = enqueue(b, pl, &b->events);
+        BHIP(b, enqueue(b, pl, used, true));
     }
     BHIP(b, hipGetLastError());
     for (int i = 0; i < n; ++i) {
@@ -543,9 +606,12 @@ rayna_status rayna_render_batch(rayna_batch *b, const rayna_request *reqs, int32
                                3 * (size_t)reqs[i].opts.width * reqs[i].opts.height * sizeof(float),
                                hipMemcpyDeviceToHost, b->stream));
     }
-    unsigned long long segs = 0;
-    BHIP(b, hipMemcpyAsync(&segs, b->segments, sizeof(segs), hipMemcpyDeviceToHost, b->stream));
+    unsigned long long seg[MAX_PIPELINES] = {};
+    for (int p = 0; p < used; ++p)
+        BHIP(b, hipMemcpyAsync(&seg[p], b->pipes[p].segments, sizeof(seg[p]), hipMemcpyDeviceToHost, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
+    unsigned long long segs = 0;
+    for (int p = 0; p < used; ++p) segs += seg[p];
     for (int i = 0; i < n; ++i) {
         if (valid[i] || !reqs[i].out_rgb) continue;
         write_checkerboard(ilog2(reqs[i].opts.width), ilog2(reqs[i].opts.height), reqs[i].out_rgb, reqs[i].layout);
@@ -580,7 +646,15 @@ void rayna_batch_destroy(rayna_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (Pipeline &P : b->pipes)
+        if (P.stream && P.stream != b->stream) (void)hipStreamSynchronize(P.stream);
     free_paths(b);
+    for (Pipeline &P : b->pipes) {
+        if (P.stream && P.stream != b->stream) (void)hipStreamDestroy(P.stream);
+        if (P.done) (void)hipEventDestroy(P.done);
+    }
+    for (hipEvent_t e : b->resolved) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     for (void *p : b->scene_allocs) (void)hipFree(p);
     if (b->table) (void)hipFree(b->table);
     if (b->finals) (void)hipFree(b->finals);

@@ -69,8 +69,11 @@ struct AdjointParams {
 // claiming `wave_batch` queue positions per atomic.
 struct ExtendTuning {
     uint32_t refill_lanes = 16;
-    uint32_t wave_batch = 64;
+    uint32_t wave_batch = 128;
 };
+
+// Shade grid: blocks of 256 threads per compute unit (grid-stride over the queue).
+constexpr int SHADE_BLOCKS_PER_CU = 16;
 
 // keep_radiance: leave the radiance field as it is (the adjoint replay keeps the primal radiance there).
 // With bv.reqs, fp.n_slots is the chunk's slot count and the requests come from bv.

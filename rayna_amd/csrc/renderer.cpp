@@ -262,7 +262,7 @@ rayna_status set_grid(rayna_renderer *r) {
     r->cus = cus;
     r->grid_extend = cus * extend_blocks_per_cu();
     r->grid_march = cus * march_blocks_per_cu();
-    r->grid_shade = cus * 8;
+    r->grid_shade = cus * SHADE_BLOCKS_PER_CU;
     return RAYNA_OK;
 }
 
@@ -553,7 +553,7 @@ rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t) {
         return fail(r, RAYNA_INVALID_ARGUMENT, "tuning out of range");
     r->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
     r->tune.wave_batch = t->wave_batch ? t->wave_batch : ExtendTuning{}.wave_batch;
-    r->grid_shade = r->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : 8);
+    r->grid_shade = r->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : SHADE_BLOCKS_PER_CU);
     const uint64_t target = t->chunk_slots ? t->chunk_slots : TARGET_SLOTS;
     if (target != r->chunk_target) {
         r->chunk_target = target;

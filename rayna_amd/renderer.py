@@ -201,7 +201,7 @@ class Renderer:
                    shade_blocks_per_cu: int = 0):
         """Launch shape of the wavefront kernels (0 keeps a default).  A new chunk_slots re-plans the path
         buffers and clears accumulation."""
-        t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 0, 0)
+        t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 0, 0, 0)
         _check(lib().rayna_set_tuning(self._h, C.byref(t)), self._h)
 
     def clear_accumulation(self):

@@ -30,7 +30,9 @@ def _single(scenes, q):
 
 @pytest.mark.parametrize("tuning", [dict(), dict(chunk_slots=1 << 16), dict(chunk_slots=1 << 16, use_graph=True),
                                     dict(schedule="shortest_first", refill_lanes=8, wave_batch=32, shade_blocks_per_cu=4),
-                                    dict(schedule="by_scene", chunk_slots=1 << 17)])
+                                    dict(schedule="by_scene", chunk_slots=1 << 17),
+                                    dict(chunk_slots=1 << 16, pipelines=1), dict(chunk_slots=1 << 16, pipelines=3),
+                                    dict(chunk_slots=1 << 16, pipelines=4, use_graph=True)])
 def test_batch_matches_single_renders(gpu, tuning):
     scenes, reqs = _requests()
     print("case batch", tuning, flush=True)
@@ -38,9 +40,11 @@ def test_batch_matches_single_renders(gpu, tuning):
         b.set_tuning(**tuning)
         imgs, st = b.render(reqs)
         if tuning.get("use_graph"):
-            imgs2, _ = b.render(reqs)  # replay of the captured graph
-            for a, c in zip(imgs, imgs2):
-                assert np.array_equal(a, c)
+            for k in range(3):  # replays of the captured graph, with eager work of the program in between
+                imgs2, _ = b.render(reqs)
+                for a, c in zip(imgs, imgs2):
+                    assert np.array_equal(a, c), f"replay {k}"
+                _single(scenes, reqs[k])
     segs = 0
     for i, (q, img) in enumerate(zip(reqs, imgs)):
         ref, s = _single(scenes, q)
