@@ -311,6 +311,7 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
             R.ray_depth = q.opts.ray_depth;
             R.n_pixels = P;
             R.s0 = pc.s0;
+            R.n_spp = pc.n;
             R.slot_base = slot;
             R.n_slots = pc.n * P;
             const SceneSlot &sc = b->slots[q.scene];

@@ -636,12 +636,12 @@ __global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers 
         fp.sdim = R.sdim;
         fp.key0 = R.key0;
         fp.key1 = R.key1;
-        fp.n_pixels = R.n_pixels;
         fp.s0 = R.s0;
+        fp.n_spp = R.n_spp;
         l = i - R.slot_base;
     }
-    const uint32_t pixel = l % fp.n_pixels;
-    const uint32_t k = fp.s0 + l / fp.n_pixels;
+    const uint32_t pixel = l / fp.n_spp;
+    const uint32_t k = fp.s0 + l % fp.n_spp;
     const uint32_t x = pixel % fp.width, y = pixel / fp.width;
     float u[4];
     dm::draw4(dm::Key{fp.key0, fp.key1}, pixel, k, 0, dm::SLOT_CAMERA, u);
@@ -738,15 +738,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
             if (BATCH) {
                 const RequestParams &R = bv.reqs[bv.slot_req[slot]];
                 const uint32_t l = slot - R.slot_base;
-                pixel = l % R.n_pixels;
-                sample = R.s0 + l / R.n_pixels;
+                pixel = l / R.n_spp;
+                sample = R.s0 + l % R.n_spp;
                 key = dm::Key{R.key0, R.key1};
                 ray_depth = R.ray_depth;
                 skybox = R.skybox;
                 sky_img = R.sky_image;
             } else {
-                pixel = slot % fp.n_pixels;
-                sample = fp.s0 + slot / fp.n_pixels;
+                pixel = slot / fp.n_spp;
+                sample = fp.s0 + slot % fp.n_spp;
             }
             if (ref == 0u) {
                 if (!ADJOINT)
@@ -838,7 +838,7 @@ __global__ __launch_bounds__(BLOCK) void k_resolve(FrameParams fp, PathBuffers p
     const uint32_t P = fp.n_pixels;
     float r = first ? 0.0f : sum[p], g = first ? 0.0f : sum[P + p], b = first ? 0.0f : sum[2 * P + p];
     for (uint32_t s = 0; s < chunk_samples; ++s) {
-        const uint32_t slot = s * P + p;
+        const uint32_t slot = p * chunk_samples + s;
         r = r + pb.lr[slot];
         g = g + pb.lg[slot];
         b = b + pb.lb[slot];
@@ -875,9 +875,9 @@ __global__ __launch_bounds__(BLOCK) void k_resolve_batch(BatchView bv, PathBuffe
     float *su = sum + R.sum_offset;
     const bool first = R.first_piece != 0;
     float r = first ? 0.0f : su[p], g = first ? 0.0f : su[P + p], b = first ? 0.0f : su[2 * P + p];
-    const uint32_t n = R.n_slots / P;
+    const uint32_t n = R.n_spp;
     for (uint32_t s = 0; s < n; ++s) {
-        const uint32_t slot = R.slot_base + s * P + p;
+        const uint32_t slot = R.slot_base + p * n + s;
         r = r + pb.lr[slot];
         g = g + pb.lg[slot];
         b = b + pb.lb[slot];
@@ -917,7 +917,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_mode(SceneView sv, FrameParams 
         if (ref == 0u) {
             L = sv.skybox == RAYNA_SKY_HDRI ? sky_image(sv, sv.sky_image, ray.d) : sky(sv.skybox, ray.d);
         } else {
-            const uint32_t pixel = slot % fp.n_pixels, sample = fp.s0 + slot / fp.n_pixels;
+            const uint32_t pixel = slot / fp.n_spp, sample = fp.s0 + slot % fp.n_spp;
             const float t = pb.hit_t[slot];
             const uint32_t aux = pb.hit_aux[slot];
             SHit h;

@@ -8,8 +8,9 @@
 
 namespace rayna {
 
-// Path state of one chunk of samples, structure-of-arrays in HBM.  Slot i of a chunk is sample
-// s0 + i / n_pixels of pixel i % n_pixels, so consecutive slots are neighbouring pixels.
+// Path state of one chunk of samples, structure-of-arrays in HBM.  A chunk holds samples [s0, s0 + n_spp) of
+// every pixel; slot i is sample s0 + i % n_spp of pixel i / n_spp, so a pixel's samples are neighbours (a wave
+// traces 64 samples of one pixel, or all the samples of a few neighbouring pixels).
 struct PathBuffers {
     float *ox, *oy, *oz;     // ray origin
     float *dx, *dy, *dz;     // ray direction (unit)
@@ -31,17 +32,18 @@ struct FrameParams {
     uint32_t ray_depth;
     uint32_t n_pixels;
     uint32_t s0;             // first sample of the chunk
-    uint32_t n_slots;        // slots in the chunk
+    uint32_t n_spp;          // samples per pixel in the chunk
+    uint32_t n_slots;        // slots in the chunk: n_pixels * n_spp
     int32_t mode;            // RAYNA_MODE_*
 };
 
 // One request of a batch, as seen by one chunk of paths: its slots are [slot_base, slot_base + n_slots), sample
-// s0 + l / n_pixels of pixel l % n_pixels for local slot l (sample-major, like a single frame).
+// s0 + l % n_spp of pixel l / n_spp for local slot l (pixel-major, like a single frame).
 struct RequestParams {
     GViewport vp;
     uint32_t width, height, spp, sdim;
     uint32_t key0, key1, ray_depth, n_pixels;
-    uint32_t s0, slot_base, n_slots;
+    uint32_t s0, n_spp, slot_base, n_slots;
     int32_t root, first_sdf, n_sdfs, skybox;  // the request's scene within the merged scene set
     int32_t sky_image;                        // RAYNA_SKY_HDRI: its image
     uint32_t sum_offset;                      // its 3 * n_pixels floats in the batch's sum and output buffers

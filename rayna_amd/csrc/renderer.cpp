@@ -278,6 +278,7 @@ FrameParams frame_params(const rayna_renderer *r, uint64_t frame_index) {
     fp.ray_depth = r->opts.ray_depth;
     fp.n_pixels = r->opts.width * r->opts.height;
     fp.s0 = 0;
+    fp.n_spp = 1;
     fp.n_slots = 0;
     fp.mode = r->opts.mode;
     return fp;
@@ -288,6 +289,7 @@ FrameParams frame_params(const rayna_renderer *r, uint64_t frame_index) {
 void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, std::vector<hipEvent_t> *ev, size_t *evi,
                  uint64_t *extend_launches, const AdjointParams *adjoint = nullptr) {
     fp.s0 = s0;
+    fp.n_spp = n;
     fp.n_slots = fp.n_pixels * n;
     const uint32_t D2 = r->opts.ray_depth + 2;
     launch_fill_u32(r->queue_count, 0u, 3 * D2, r->stream);
@@ -686,6 +688,7 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     HIP_TRY(r, hipMalloc((void **)&rank_dev, (size_t)P * sizeof(int32_t)));
     FrameParams fp = frame_params(r, 0);
     fp.s0 = sample_index;
+    fp.n_spp = 1;
     fp.n_slots = P;
     PathBuffers pb = r->pb;
     pb.hit_rank = rank_dev;

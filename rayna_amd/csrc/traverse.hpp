@@ -337,13 +337,13 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
                     if (BATCH) {
                         const RequestParams &R = bv.reqs[bv.slot_req[slot]];
                         const uint32_t l = slot - R.slot_base;
-                        pixel = l % R.n_pixels;
-                        sample = R.s0 + l / R.n_pixels;
+                        pixel = l / R.n_spp;
+                        sample = R.s0 + l % R.n_spp;
                         key = dm::Key{R.key0, R.key1};
                         root = R.root;
                     } else {
-                        pixel = slot % fp.n_pixels;
-                        sample = fp.s0 + slot / fp.n_pixels;
+                        pixel = slot / fp.n_spp;
+                        sample = fp.s0 + slot % fp.n_spp;
                     }
                     b = Best{FLT_MAX, -1, 0u, 0u, 0u};
                     st.sp = 0;
