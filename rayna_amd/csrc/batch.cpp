@@ -41,6 +41,7 @@ struct Plan {
     std::vector<uint32_t> finals;              // per chunk, entries (chunk-local) whose request ends there
     std::vector<uint32_t> first_final;         // chunk c's finals at [first_final[c], first_final[c + 1])
     std::vector<uint32_t> max_pixels;          // per chunk
+    std::vector<uint32_t> resolve_blocks;      // per chunk: the most resolve blocks of one of its requests
     std::vector<uint32_t> depth;               // per chunk: deepest ray_depth of its requests
     std::vector<uint32_t> n_sdf;               // per chunk: most unbounded objects of its requests' scenes
     std::vector<uint32_t> slots;               // per chunk
@@ -295,7 +296,7 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
     for (size_t c = 0; c < pl.chunks.size(); ++c) {
         pl.first_entry.push_back((uint32_t)pl.table.size());
         pl.first_final.push_back((uint32_t)pl.finals.size());
-        uint32_t slot = 0, maxp = 0, depth = 0, nsdf = 0;
+        uint32_t slot = 0, maxp = 0, depth = 0, nsdf = 0, rblocks = 1;
         for (size_t e = 0; e < pl.chunks[c].size(); ++e) {
             const Piece &pc = pl.chunks[c][e];
             const rayna_request &q = reqs[pc.req];
@@ -327,10 +328,13 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
             if (pc.s0 + pc.n == q.opts.samples) pl.finals.push_back((uint32_t)e);
             slot += R.n_slots;
             maxp = std::max(maxp, P);
+            const uint32_t per = resolve_pixels_per_block(pc.n);
+            rblocks = std::max(rblocks, (P + per - 1) / per);
             depth = std::max(depth, q.opts.ray_depth);
             nsdf = std::max(nsdf, (uint32_t)sc.n_sdfs);
         }
         pl.max_pixels.push_back(maxp);
+        pl.resolve_blocks.push_back(rblocks);
         pl.depth.push_back(depth);
         pl.n_sdf.push_back(nsdf);
         pl.slots.push_back(slot);
@@ -389,7 +393,7 @@ hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
         }
         launch_count_segments(P.queue_count, D + 1, fp.n_slots, P.segments, s);
         if (used > 1 && c > 0) check(hipStreamWaitEvent(s, b->resolved[c - 1], 0));
-        launch_resolve_batch(bv, pl.max_pixels[c], P.pb, b->sum, s);
+        launch_resolve_batch(bv, pl.resolve_blocks[c], P.pb, b->sum, s);
         const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
         launch_finalize_batch(bv, b->finals + pl.first_final[c], nf, pl.max_pixels[c], b->sum, b->out, s);
         if (used > 1) check(hipEventRecord(b->resolved[c], s));

@@ -830,22 +830,50 @@ __global__ __launch_bounds__(BLOCK) void k_sum_grad(const double *grad, uint32_t
     out[i] = s;
 }
 
+// Running sums, in sample order, of the n samples (slots base + p * n + k, k < n) of each pixel p of
+// [p0, p0 + np) into sum (three planes of P floats; first: start from 0).  The whole block calls it.  The samples
+// are staged in LDS with coalesced loads (one row of n | 1 floats per pixel and channel: an odd stride keeps the
+// summing threads on distinct banks), then one thread per (pixel, channel) adds its row in order.
+__device__ void resolve_pixels(const PathBuffers &pb, uint32_t base, uint32_t p0, uint32_t np, uint32_t n, uint32_t P,
+                               float *sum, bool first) {
+    __shared__ float stage[3 * RESOLVE_STAGE];
+    const uint32_t t = threadIdx.x;
+    const float *src[3] = {pb.lr + base + (size_t)p0 * n, pb.lg + base + (size_t)p0 * n, pb.lb + base + (size_t)p0 * n};
+    const uint32_t w = n | 1u;
+    if (w > RESOLVE_STAGE) {  // a thread per pixel, straight from HBM
+        for (uint32_t px = t; px < np; px += BLOCK)
+            for (int c = 0; c < 3; ++c) {
+                float s = first ? 0.0f : sum[c * P + p0 + px];
+                for (uint32_t k = 0; k < n; ++k) s = s + src[c][(size_t)px * n + k];
+                sum[c * P + p0 + px] = s;
+            }
+        return;
+    }
+    const uint32_t m = np * n;
+    for (uint32_t i = t; i < m; i += BLOCK) {
+        const uint32_t px = i / n, k = i - px * n;
+        const uint32_t o = px * w + k;
+        stage[o] = src[0][i];
+        stage[RESOLVE_STAGE + o] = src[1][i];
+        stage[2 * RESOLVE_STAGE + o] = src[2][i];
+    }
+    __syncthreads();
+    for (uint32_t u = t; u < 3 * np; u += BLOCK) {
+        const uint32_t c = u / np, px = u - c * np;
+        const float *row = stage + c * RESOLVE_STAGE + px * w;
+        float s = first ? 0.0f : sum[c * P + p0 + px];
+        for (uint32_t k = 0; k < n; ++k) s = s + row[k];
+        sum[c * P + p0 + px] = s;
+    }
+}
+
 // Per-pixel running sum of the chunk's samples in sample order (render_px_msaa's sum, renderer.rs:398-402).
 __global__ __launch_bounds__(BLOCK) void k_resolve(FrameParams fp, PathBuffers pb, float *sum, uint32_t chunk_samples,
                                                    int first) {
-    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
-    if (p >= fp.n_pixels) return;
-    const uint32_t P = fp.n_pixels;
-    float r = first ? 0.0f : sum[p], g = first ? 0.0f : sum[P + p], b = first ? 0.0f : sum[2 * P + p];
-    for (uint32_t s = 0; s < chunk_samples; ++s) {
-        const uint32_t slot = p * chunk_samples + s;
-        r = r + pb.lr[slot];
-        g = g + pb.lg[slot];
-        b = b + pb.lb[slot];
-    }
-    sum[p] = r;
-    sum[P + p] = g;
-    sum[2 * P + p] = b;
+    const uint32_t per = resolve_pixels_per_block(chunk_samples);
+    const uint32_t p0 = blockIdx.x * per;
+    const uint32_t np = fp.n_pixels - p0 < per ? fp.n_pixels - p0 : per;
+    resolve_pixels(pb, 0u, p0, np, chunk_samples, fp.n_pixels, sum, first != 0);
 }
 
 // Sample mean, accumulation buffer update and output image (accum_buffer.rs:31-45).
@@ -869,22 +897,11 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FrameParams fp, const float 
 // k_resolve for the requests of a batch chunk: blockIdx.y selects the request.
 __global__ __launch_bounds__(BLOCK) void k_resolve_batch(BatchView bv, PathBuffers pb, float *sum) {
     const RequestParams &R = bv.reqs[blockIdx.y];
-    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
-    const uint32_t P = R.n_pixels;
-    if (p >= P) return;
-    float *su = sum + R.sum_offset;
-    const bool first = R.first_piece != 0;
-    float r = first ? 0.0f : su[p], g = first ? 0.0f : su[P + p], b = first ? 0.0f : su[2 * P + p];
-    const uint32_t n = R.n_spp;
-    for (uint32_t s = 0; s < n; ++s) {
-        const uint32_t slot = R.slot_base + p * n + s;
-        r = r + pb.lr[slot];
-        g = g + pb.lg[slot];
-        b = b + pb.lb[slot];
-    }
-    su[p] = r;
-    su[P + p] = g;
-    su[2 * P + p] = b;
+    const uint32_t per = resolve_pixels_per_block(R.n_spp);
+    const uint32_t p0 = blockIdx.x * per;
+    if (p0 >= R.n_pixels) return;  // the whole block
+    const uint32_t np = R.n_pixels - p0 < per ? R.n_pixels - p0 : per;
+    resolve_pixels(pb, R.slot_base, p0, np, R.n_spp, R.n_pixels, sum + R.sum_offset, R.first_piece != 0);
 }
 
 // The sample mean of each finished request of a batch (a first frame: the accumulation is the mean itself),
@@ -1075,10 +1092,8 @@ void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers 
     }
 }
 
-void launch_resolve_batch(const BatchView &bv, uint32_t max_pixels, const PathBuffers &pb, float *sum,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(max_pixels, BLOCK), bv.n_reqs), dim3(BLOCK), 0, s, bv, pb,
-                       sum);
+void launch_resolve_batch(const BatchView &bv, uint32_t blocks, const PathBuffers &pb, float *sum, hipStream_t s) {
+    hipLaunchKernelGGL(k_resolve_batch, dim3(blocks, bv.n_reqs), dim3(BLOCK), 0, s, bv, pb, sum);
 }
 
 void launch_finalize_batch(const BatchView &bv, const uint32_t *final_reqs, uint32_t n_final, uint32_t max_pixels,
@@ -1103,8 +1118,8 @@ void launch_to_srgb8(const float *img, uint32_t n_pixels, uint8_t *rgba, hipStre
 
 void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, uint32_t chunk_samples, int first,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_resolve, dim3(grid_for(fp.n_pixels, BLOCK)), dim3(BLOCK), 0, s, fp, pb, sum, chunk_samples,
-                       first);
+    hipLaunchKernelGGL(k_resolve, dim3(grid_for(fp.n_pixels, (int)resolve_pixels_per_block(chunk_samples))),
+                       dim3(BLOCK), 0, s, fp, pb, sum, chunk_samples, first);
 }
 
 void launch_finalize(const FrameParams &fp, const float *sum, float *acc, double frames, float *out, int layout,

@@ -97,10 +97,17 @@ void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers 
 void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
                   const uint32_t *count, uint32_t n_sdf, float *mt, uint64_t mt_stride, uint32_t *work, int grid,
                   int merge_grid, const ExtendTuning &tune, const BatchView &bv, hipStream_t s);
+// Pixels per block of the resolve kernels for n samples per pixel in the chunk: a block stages its pixels'
+// samples (three channels, rows of n | 1 floats) in LDS so that its loads are coalesced; from 4160 samples per
+// pixel on, a block sums 256 pixels straight from HBM.
+constexpr uint32_t RESOLVE_STAGE = 4160;  // floats per channel
+constexpr uint32_t resolve_pixels_per_block(uint32_t n) {
+    return (n | 1u) > RESOLVE_STAGE ? 256u : (RESOLVE_STAGE / (n | 1u) > 256u ? 256u : RESOLVE_STAGE / (n | 1u));
+}
 // Batch chunks: per request, the running per-pixel sum of the chunk's samples (in sample order) and, at the
-// request's last chunk, its sample mean in its layout at out + sum_offset.
-void launch_resolve_batch(const BatchView &bv, uint32_t max_pixels, const PathBuffers &pb, float *sum,
-                          hipStream_t s);
+// request's last chunk, its sample mean in its layout at out + sum_offset.  blocks: the most resolve blocks of
+// one request of the chunk (its pixels / resolve_pixels_per_block(its n_spp), rounded up).
+void launch_resolve_batch(const BatchView &bv, uint32_t blocks, const PathBuffers &pb, float *sum, hipStream_t s);
 void launch_finalize_batch(const BatchView &bv, const uint32_t *final_reqs, uint32_t n_final, uint32_t max_pixels,
                            const float *sum, float *out, hipStream_t s);
 void launch_sum_grad(const double *grad, uint32_t n, uint32_t copies, double *out, hipStream_t s);
