@@ -3,6 +3,7 @@
 #include "scene_build.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -252,6 +253,136 @@ class Builder {
 };
 
 // ---------------------------------------------------------------------------------------------------------
+// The traversal tree of a triangle list on the device.  The closest hit does not depend on the tree: culling is
+// conservative and ties go to the higher depth-first rank of the engine's tree, which every triangle carries.
+// So the kernels traverse a tree built for them.  The engine's SAH grows every candidate box from the origin
+// (Aabb::encompass_iter, aabb.rs:64-67), so for a mesh away from the origin its costs barely depend on the
+// split, and it stops at clusters of up to eight items.  This one is a binned SAH over centroids (32 bins per
+// axis, tight boxes, a node costing two triangle tests) with leaves of at most RAYNA_LEAF_MAX triangles; its
+// binary tree becomes 4-wide nodes when it is emitted, by opening the child of largest surface area until a
+// node has four.
+#ifndef RAYNA_LEAF_MAX
+#define RAYNA_LEAF_MAX 4
+#endif
+#ifndef RAYNA_SAH_NODE_COST
+#define RAYNA_SAH_NODE_COST 2.0  // a node's traversal cost in triangle tests
+#endif
+class TraversalTree {
+   public:
+    struct Node {
+        Aabb64 box;
+        int32_t left = -1, right = -1;  // children; left < 0: a leaf of items order[first, first + count)
+        int32_t first = 0, count = 0;
+    };
+    std::vector<Node> nodes;
+    std::vector<int32_t> order;
+
+    explicit TraversalTree(const std::vector<Aabb64> &boxes) : box_(boxes) {
+        const int n = (int)boxes.size();
+        order.resize(n);
+        cen_.resize(n);
+        for (int i = 0; i < n; ++i) {
+            order[i] = i;
+            for (int a = 0; a < 3; ++a) cen_[i][a] = 0.5 * (boxes[i].mn[a] + boxes[i].mx[a]);
+        }
+        if (n > 0) build(0, n);
+    }
+
+   private:
+    static constexpr int BINS = 32;
+    const std::vector<Aabb64> &box_;
+    std::vector<std::array<double, 3>> cen_;
+
+    static Aabb64 empty_box() { return Aabb64{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}}; }
+    static void add(Aabb64 &a, const Aabb64 &b) {
+        for (int k = 0; k < 3; ++k) {
+            a.mn[k] = std::min(a.mn[k], b.mn[k]);
+            a.mx[k] = std::max(a.mx[k], b.mx[k]);
+        }
+    }
+    static double half_area(const Aabb64 &b) {
+        const double x = b.mx[0] - b.mn[0], y = b.mx[1] - b.mn[1], z = b.mx[2] - b.mn[2];
+        return x * y + y * z + z * x;
+    }
+
+    int build(int lo, int hi) {
+        const int id = (int)nodes.size();
+        nodes.emplace_back();
+        Aabb64 box = empty_box(), cb = empty_box();
+        for (int i = lo; i < hi; ++i) {
+            add(box, box_[order[i]]);
+            const Aabb64 c{{cen_[order[i]][0], cen_[order[i]][1], cen_[order[i]][2]},
+                           {cen_[order[i]][0], cen_[order[i]][1], cen_[order[i]][2]}};
+            add(cb, c);
+        }
+        nodes[id].box = box;
+        const int n = hi - lo;
+        // best binned split
+        int best_axis = -1, best_bin = 0;
+        double best_cost = INFINITY;
+        for (int a = 0; a < 3; ++a) {
+            const double ext = cb.mx[a] - cb.mn[a];
+            if (!(ext > 0)) continue;
+            Aabb64 bb[BINS];
+            int cnt[BINS] = {};
+            for (int b = 0; b < BINS; ++b) bb[b] = empty_box();
+            const double scale = BINS / ext;
+            for (int i = lo; i < hi; ++i) {
+                const int b = std::min(BINS - 1, (int)((cen_[order[i]][a] - cb.mn[a]) * scale));
+                ++cnt[b];
+                add(bb[b], box_[order[i]]);
+            }
+            double right_area[BINS];
+            int right_cnt[BINS];
+            Aabb64 acc = empty_box();
+            int c = 0;
+            for (int b = BINS - 1; b >= 1; --b) {
+                add(acc, bb[b]);
+                c += cnt[b];
+                right_area[b] = half_area(acc);
+                right_cnt[b] = c;
+            }
+            acc = empty_box();
+            c = 0;
+            for (int b = 1; b < BINS; ++b) {
+                add(acc, bb[b - 1]);
+                c += cnt[b - 1];
+                if (c == 0 || right_cnt[b] == 0) continue;
+                const double cost = half_area(acc) * c + right_area[b] * right_cnt[b];
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = a;
+                    best_bin = b;
+                }
+            }
+        }
+        const double parent = half_area(box);
+        const double leaf_cost = (double)n;
+        const double split_cost = parent > 0 ? RAYNA_SAH_NODE_COST + best_cost / parent : INFINITY;
+        if (n <= RAYNA_LEAF_MAX && !(split_cost < leaf_cost)) {
+            nodes[id].first = lo;
+            nodes[id].count = n;
+            return id;
+        }
+        int mid;
+        if (best_axis >= 0) {
+            const double ext = cb.mx[best_axis] - cb.mn[best_axis], scale = BINS / ext;
+            const int32_t *m = std::partition(order.data() + lo, order.data() + hi, [&](int32_t it) {
+                return std::min(BINS - 1, (int)((cen_[it][best_axis] - cb.mn[best_axis]) * scale)) < best_bin;
+            });
+            mid = (int)(m - order.data());
+        } else {
+            mid = lo + n / 2;  // coincident centroids: halves in input order
+        }
+        const int l = build(lo, mid);
+        const int r = build(mid, hi);
+        nodes[id].left = l;
+        nodes[id].right = r;
+        return id;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------
 
 struct Packer {
     const rayna_scene_desc &d;
@@ -343,8 +474,16 @@ struct Packer {
         return g;
     }
 
-    // One triangle of the scene into the device arrays; returns its index.
+    // One triangle of the scene into the device arrays, ranked next in depth-first order; returns its index.
     int32_t add_triangle(int tri, int32_t material, int32_t object, int32_t local) {
+        const int32_t i = push_triangle(tri, material, rank++);
+        P.rank_object.push_back(object);
+        P.rank_local.push_back(local);
+        return i;
+    }
+
+    // One triangle into the device arrays with a given rank.
+    int32_t push_triangle(int tri, int32_t material, int32_t tri_rank) {
         const double *v = &d.tri_vertices[9 * (size_t)tri];
         const double *nn = &d.tri_normals[9 * (size_t)tri];
         GTri g;
@@ -360,7 +499,7 @@ struct Packer {
         g.e2x = v2[0] - v0[0];
         g.e2y = v2[1] - v0[1];
         g.e2z = v2[2] - v0[2];
-        g.rank = rank++;
+        g.rank = tri_rank;
         g.material = material;
         // Can sum_i b_i n_i vanish for barycentrics b?  Not if every n_i leans towards the mean direction.
         double c[3] = {nn[0] + nn[3] + nn[6], nn[1] + nn[4] + nn[7], nn[2] + nn[5] + nn[8]};
@@ -387,8 +526,6 @@ struct Packer {
         tn.n2z = (float)nn[8];
         tn.pad0 = tn.pad1 = tn.pad2 = 0;
         P.tri_normals.push_back(tn);
-        P.rank_object.push_back(object);
-        P.rank_local.push_back(local);
         return (int32_t)P.tris.size() - 1;
     }
 
@@ -495,7 +632,103 @@ struct Packer {
         g.info[k] = (int32_t)items.size() | (LEAF_TRIS << 4);
     }
 
+    // The items of an engine tree in the order the engine's depth-first walk meets them (the rank order).
+    static void engine_item_order(const RefTree &t, std::vector<int32_t> &out) {
+        std::function<void(int, bool)> walk = [&](int node, bool wrap) {
+            std::vector<int32_t> slots;
+            if (wrap) slots.push_back(node);
+            else {
+                const RefTree::Node &nd = t.nodes[node];
+                for (int i = 0; i < nd.n_children; ++i) slots.push_back(t.children[nd.first_child + i]);
+            }
+            for (int32_t ch : slots) {
+                if (!is_nested(t, ch)) out.push_back(t.nodes[ch].item);
+                else if (all_items(t, ch)) {
+                    const RefTree::Node &nd = t.nodes[ch];
+                    for (int i = 0; i < nd.n_children; ++i) out.push_back(t.nodes[t.children[nd.first_child + i]].item);
+                } else walk(ch, false);
+            }
+        };
+        if (t.root >= 0) walk(t.root, !is_nested(t, t.root) || all_items(t, t.root));
+    }
+
+    // A triangle list: ranks from the engine's tree, nodes from the traversal tree.
+    int32_t emit_triangle_tree(int mesh, int32_t object, int32_t material) {
+        const rayna_mesh &m = d.meshes[mesh];
+        std::vector<int32_t> ranked;
+        engine_item_order(blas[mesh], ranked);
+        std::vector<int32_t> rank_of(m.count, -1);
+        for (int32_t it : ranked)
+            if (it >= 0 && it < m.count) rank_of[it] = 0;
+        if ((int32_t)ranked.size() != m.count || std::count(rank_of.begin(), rank_of.end(), 0) != m.count)
+            return -1;  // not one rank per triangle: the caller emits the engine's tree
+        for (size_t k = 0; k < ranked.size(); ++k) {
+            rank_of[ranked[k]] = rank + (int32_t)k;
+            P.rank_object.push_back(object);
+            P.rank_local.push_back(ranked[k]);
+        }
+        rank += (int32_t)ranked.size();
+        std::vector<Aabb64> boxes(m.count);
+        for (int32_t i = 0; i < m.count; ++i) boxes[i] = tri_box(m.index + i);
+        const TraversalTree tt(boxes);
+        auto leaf = [&](GNode &g, int k, const TraversalTree::Node &nd) {
+            g.child[k] = (int32_t)P.tris.size();
+            g.info[k] = nd.count | (LEAF_TRIS << 4);
+            for (int32_t i = 0; i < nd.count; ++i) {
+                const int32_t it = tt.order[nd.first + i];
+                push_triangle(m.index + it, material, rank_of[it]);
+            }
+        };
+        std::function<int32_t(int, bool)> emit = [&](int node, bool wrap) -> int32_t {
+            const int32_t gi = (int32_t)P.nodes.size();
+            P.nodes.push_back(empty_node());
+            std::vector<int32_t> slots;
+            if (wrap) slots.push_back(node);
+            else {
+                slots = {tt.nodes[node].left, tt.nodes[node].right};
+                for (;;) {  // open the inner child of largest surface area while there is room
+                    int open = -1;
+                    double best = -1.0;
+                    for (int k = 0; k < (int)slots.size(); ++k) {
+                        const TraversalTree::Node &c = tt.nodes[slots[k]];
+                        if (c.left < 0) continue;
+                        const Aabb64 &b = c.box;
+                        const double x = b.mx[0] - b.mn[0], y = b.mx[1] - b.mn[1], z = b.mx[2] - b.mn[2];
+                        const double a = x * y + y * z + z * x;
+                        if (a > best) {
+                            best = a;
+                            open = k;
+                        }
+                    }
+                    if (open < 0 || slots.size() >= 4) break;
+                    const int32_t c = slots[open];
+                    slots[open] = tt.nodes[c].left;
+                    slots.insert(slots.begin() + open + 1, tt.nodes[c].right);
+                }
+            }
+            for (int k = 0; k < (int)slots.size(); ++k) {
+                const TraversalTree::Node &c = tt.nodes[slots[k]];
+                set_slot_box(P.nodes[gi], k, c.box);
+                if (c.left < 0) {
+                    GNode g = P.nodes[gi];
+                    leaf(g, k, c);
+                    P.nodes[gi] = g;
+                } else {
+                    const int32_t sub = emit(slots[k], false);
+                    P.nodes[gi].child[k] = sub;
+                    P.nodes[gi].info[k] = INFO_INNER;
+                }
+            }
+            return gi;
+        };
+        return emit(0, tt.nodes[0].left < 0);
+    }
+
     int32_t emit_blas(int mesh, int32_t object, int32_t material) {
+        if (d.meshes[mesh].kind == RAYNA_MESH_TRIANGLE_LIST && d.meshes[mesh].count > 0) {
+            const int32_t root = emit_triangle_tree(mesh, object, material);
+            if (root >= 0) return root;
+        }
         const RefTree &t = blas[mesh];
         const std::vector<Aabb64> &tight = blas_tight[mesh];
         const rayna_mesh &m = d.meshes[mesh];
