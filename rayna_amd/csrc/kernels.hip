@@ -613,8 +613,43 @@ __device__ __forceinline__ Col sky(int kind, f3 d) {  // skybox/simple.rs, skybo
 
 // ------------------------------------------------------------------------------------------ kernels
 
-// Camera ray of every slot of the chunk: render_px_msaa's stratified jitter (renderer.rs:370-389) and
-// Viewport::calc_ray (camera.rs:362-384); throughput 1, radiance 0.  BATCH: each slot finds its request (the
+// The camera ray of sample k of pixel `pixel`: render_px_msaa's stratified jitter (renderer.rs:370-389) and
+// Viewport::calc_ray (camera.rs:362-384).  k_generate stores it; k_shade recomputes it at the first bounce
+// instead of loading it (the shade kernel waits on memory, not on arithmetic).
+__device__ __forceinline__ DRay camera_ray(const GViewport &vp, uint32_t width, uint32_t height, uint32_t sdim,
+                                           dm::Key key, uint32_t pixel, uint32_t k) {
+    const uint32_t x = pixel % width, y = pixel / width;
+    float u[4];
+    dm::draw4(key, pixel, k, 0, dm::SLOT_CAMERA, u);
+    const uint32_t s = sdim;
+    const float jx = u[0] - 0.5f, jy = u[1] - 0.5f;
+    float cx, cy;
+    if (k < s * s) {
+        const float inv = 1.0f / (float)s;
+        const uint32_t ii = k % s, jj = k / s;
+        cx = ((float)x + jx * inv) + (float)ii * inv;
+        cy = ((float)y + jy * inv) + (float)jj * inv;
+    } else {
+        cx = (float)x + jx;
+        cy = (float)y + jy;
+    }
+    const float W = (float)width, H = (float)height;
+    const float uu = (cx - W / 2.0f) / H;
+    const float vv = (cy - H / 2.0f) / H;
+    const f3 pc{vp.pixel_center[0], vp.pixel_center[1], vp.pixel_center[2]};
+    const f3 vu{vp.vu[0], vp.vu[1], vp.vu[2]}, vvv{vp.vv[0], vp.vv[1], vp.vv[2]};
+    const f3 pix = (pc + vu * uu) + vvv * vv;
+    float sn, cs;
+    dm::sincos_turns(u[3], &sn, &cs);
+    const float rr = sqrtf(u[2]);
+    const float dx = rr * cs, dy = rr * sn;
+    const f3 pos{vp.pos[0], vp.pos[1], vp.pos[2]};
+    const f3 du{vp.du[0], vp.du[1], vp.du[2]}, dv{vp.dv[0], vp.dv[1], vp.dv[2]};
+    const f3 org = (pos + du * dx) + dv * dy;
+    return make_ray(org, pix - org);
+}
+
+// Camera ray of every slot of the chunk; throughput 1, radiance 0.  BATCH: each slot finds its request (the
 // last with slot_base <= slot) and records it for the later kernels.
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers pb, int keep_radiance, BatchView bv) {
@@ -640,38 +675,8 @@ __global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers 
         fp.n_spp = R.n_spp;
         l = i - R.slot_base;
     }
-    const uint32_t pixel = l / fp.n_spp;
-    const uint32_t k = fp.s0 + l % fp.n_spp;
-    const uint32_t x = pixel % fp.width, y = pixel / fp.width;
-    float u[4];
-    dm::draw4(dm::Key{fp.key0, fp.key1}, pixel, k, 0, dm::SLOT_CAMERA, u);
-    const uint32_t s = fp.sdim;
-    const float jx = u[0] - 0.5f, jy = u[1] - 0.5f;
-    float cx, cy;
-    if (k < s * s) {
-        const float inv = 1.0f / (float)s;
-        const uint32_t ii = k % s, jj = k / s;
-        cx = ((float)x + jx * inv) + (float)ii * inv;
-        cy = ((float)y + jy * inv) + (float)jj * inv;
-    } else {
-        cx = (float)x + jx;
-        cy = (float)y + jy;
-    }
-    const float W = (float)fp.width, H = (float)fp.height;
-    const float uu = (cx - W / 2.0f) / H;
-    const float vv = (cy - H / 2.0f) / H;
-    const GViewport &vp = fp.vp;
-    const f3 pc{vp.pixel_center[0], vp.pixel_center[1], vp.pixel_center[2]};
-    const f3 vu{vp.vu[0], vp.vu[1], vp.vu[2]}, vvv{vp.vv[0], vp.vv[1], vp.vv[2]};
-    const f3 pix = (pc + vu * uu) + vvv * vv;
-    float sn, cs;
-    dm::sincos_turns(u[3], &sn, &cs);
-    const float rr = sqrtf(u[2]);
-    const float dx = rr * cs, dy = rr * sn;
-    const f3 pos{vp.pos[0], vp.pos[1], vp.pos[2]};
-    const f3 du{vp.du[0], vp.du[1], vp.du[2]}, dv{vp.dv[0], vp.dv[1], vp.dv[2]};
-    const f3 org = (pos + du * dx) + dv * dy;
-    const DRay r = make_ray(org, pix - org);
+    const DRay r = camera_ray(fp.vp, fp.width, fp.height, fp.sdim, dm::Key{fp.key0, fp.key1}, l / fp.n_spp,
+                              fp.s0 + l % fp.n_spp);
     pb.ox[i] = r.o.x;
     pb.oy[i] = r.o.y;
     pb.oz[i] = r.o.z;
@@ -711,6 +716,10 @@ __device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy,
 //
 // TEX: the scene has textured materials or an image sky (without, their code is compiled out).  A textured
 // material's colour receives no gradient.
+#ifndef RAYNA_SHADE_STRIP
+#define RAYNA_SHADE_STRIP 16
+#endif
+constexpr uint32_t SHADE_STRIP = RAYNA_SHADE_STRIP;
 template <bool ADJOINT, bool BATCH, bool TEX>
 #ifndef RAYNA_SHADE_WAVES
 #define RAYNA_SHADE_WAVES 5
@@ -721,20 +730,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
     const uint32_t count = queue_in ? *count_in : fp.n_slots;
     const uint32_t lane = __lane_id();
     const uint32_t copy = ADJOINT ? blockIdx.x % ap.copies : 0u;
-    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
-        const uint32_t q = base + threadIdx.x;
+    // Each wave shades strips of SHADE_STRIP * 64 consecutive queue positions; the slots that go on are staged
+    // in the wave's part of LDS and appended to the next queue with one atomic per strip.
+    __shared__ uint32_t staged[BLOCK * SHADE_STRIP];
+    uint32_t *mine = staged + (threadIdx.x / 64) * 64 * SHADE_STRIP;
+    const uint32_t n_waves = gridDim.x * (BLOCK / 64);
+    const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t base = wave * 64 * SHADE_STRIP; base < count; base += n_waves * 64 * SHADE_STRIP) {
+      uint32_t n_kept = 0;
+      for (uint32_t j = 0; j < SHADE_STRIP; ++j) {
+        const uint32_t q = base + j * 64 + lane;
         bool keep = false;
         uint32_t slot = 0;
         if (q < count) {
             slot = queue_in ? (uint32_t)queue_in[q] : q;
-            const DRay ray = ray_lazy(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
-                                          f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
-            Col T{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
-            Col L{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
             const uint32_t ref = pb.hit_ref[slot];
             dm::Key key{fp.key0, fp.key1};
             uint32_t pixel, sample, ray_depth = fp.ray_depth;
             int32_t skybox = sv.skybox, sky_img = sv.sky_image;
+            DRay ray;
             if (BATCH) {
                 const RequestParams &R = bv.reqs[bv.slot_req[slot]];
                 const uint32_t l = slot - R.slot_base;
@@ -744,10 +759,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                 ray_depth = R.ray_depth;
                 skybox = R.skybox;
                 sky_img = R.sky_image;
+                if (!queue_in) ray = camera_ray(R.vp, R.width, R.height, R.sdim, key, pixel, sample);
             } else {
                 pixel = slot / fp.n_spp;
                 sample = fp.s0 + slot % fp.n_spp;
+                if (!queue_in) ray = camera_ray(fp.vp, fp.width, fp.height, fp.sdim, key, pixel, sample);
             }
+            // The first bounce (identity queue) starts from the camera ray with throughput 1 and radiance 0 (the
+            // adjoint's radiance field holds the primal radiance), as k_generate stored them.
+            Col T{1.0f, 1.0f, 1.0f}, L{0.0f, 0.0f, 0.0f};
+            if (queue_in) {
+                ray = ray_lazy(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+                T = Col{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
+            } else {
+                ray = ray_lazy(ray.o, ray.d);
+            }
+            if (queue_in || ADJOINT) L = Col{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
             if (ref == 0u) {
                 if (!ADJOINT)
                     L = L + T * ((TEX && skybox == RAYNA_SKY_HDRI) ? sky_image(sv, sky_img, ray.d) : sky(skybox, ray.d));
@@ -807,17 +834,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
             pb.lb[slot] = L.b;
         }
         const unsigned long long mask = __ballot(keep);
-        if (mask != 0ull) {
-            const uint32_t n = (uint32_t)__popcll(mask);
-            const int leader = __ffsll((long long)mask) - 1;
-            uint32_t wbase = 0;
-            if ((int)lane == leader) wbase = atomicAdd(count_out, n);
-            wbase = __shfl(wbase, leader);
-            if (keep) {
-                const uint32_t off = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-                queue_out[wbase + off] = (int32_t)slot;
-            }
-        }
+        if (keep) mine[n_kept + (uint32_t)__popcll(mask & below)] = slot;
+        n_kept += (uint32_t)__popcll(mask);
+      }
+      if (n_kept != 0u) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t wbase = 0;
+        if (lane == 0u) wbase = atomicAdd(count_out, n_kept);
+        wbase = __shfl(wbase, 0);
+        for (uint32_t i = lane; i < n_kept; i += 64) queue_out[wbase + i] = (int32_t)mine[i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
     }
 }
 
