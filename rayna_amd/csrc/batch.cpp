@@ -64,6 +64,7 @@ struct Pipeline {
     int32_t *queue[2] = {nullptr, nullptr};
     uint32_t *queue_count = nullptr;
     uint32_t queue_count_len = 0;
+    uint32_t *claims = nullptr;    // extend claim counters, EXTEND_CLAIM_WORDS per depth
     unsigned long long *segments = nullptr;
     float *mt = nullptr;           // march outcomes, 2 * capacity * n_sdf_max
     int2 *overflow = nullptr;      // traversal stack entries beyond the LDS part
@@ -147,6 +148,7 @@ void free_pipe(rayna_batch *b, Pipeline &P) {
     P.slot_req = nullptr;
     P.queue[0] = P.queue[1] = nullptr;
     P.queue_count = nullptr;
+    P.claims = nullptr;
     P.queue_count_len = 0;
     P.segments = nullptr;
     P.mt = nullptr;
@@ -188,6 +190,8 @@ rayna_status alloc_pipe(rayna_batch *b, int p, uint64_t slots, uint32_t max_dept
     // [0, D+2) paths entering each depth; [D+2, 2(D+2)) extend counters; [2(D+2), 3(D+2)) march counters
     const uint32_t qlen = 3 * (std::max<uint32_t>(max_depth, 16) + 2);
     if ((st = alloc(qlen * sizeof(uint32_t), (void **)&P.queue_count)) != RAYNA_OK) return st;
+    if ((st = alloc((size_t)(qlen / 3) * EXTEND_CLAIM_WORDS * sizeof(uint32_t), (void **)&P.claims)) != RAYNA_OK)
+        return st;
     if ((st = alloc(sizeof(unsigned long long), (void **)&P.segments)) != RAYNA_OK) return st;
     if (b->n_sdf_max && (st = alloc(2 * slots * b->n_sdf_max * sizeof(float), (void **)&P.mt)) != RAYNA_OK)
         return st;
@@ -348,7 +352,7 @@ uint32_t plan_launches(const Plan &pl, int used) {
     uint32_t n = (uint32_t)used;  // segment counters
     for (size_t c = 0; c < pl.chunks.size(); ++c) {
         const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
-        n += 2 + (pl.n_sdf[c] ? 4 : 2) * (pl.depth[c] + 1) + 2 + (nf ? 1 : 0);
+        n += 3 + (pl.n_sdf[c] ? 4 : 2) * (pl.depth[c] + 1) + 2 + (nf ? 1 : 0);
     }
     return n;
 }
@@ -379,13 +383,14 @@ hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
         fp.ray_depth = pl.depth[c];
         const uint32_t D = pl.depth[c];
         launch_fill_u32(P.queue_count, 0u, 3 * (D + 2), s);
+        launch_fill_u32(P.claims, 0u, (D + 1) * EXTEND_CLAIM_WORDS, s);
         launch_generate(fp, P.pb, false, bv, s);
         for (uint32_t depth = 0; depth <= D; ++depth) {
             const int32_t *qin = depth == 0 ? nullptr : P.queue[depth & 1];
             const uint32_t *cin = depth == 0 ? nullptr : P.queue_count + depth;
             int32_t *qout = P.queue[(depth + 1) & 1];
             uint32_t *cout = P.queue_count + depth + 1;
-            uint32_t *work = P.queue_count + (D + 2) + depth;
+            uint32_t *work = P.claims + (size_t)depth * EXTEND_CLAIM_WORDS;
             launch_extend(b->view, fp, P.pb, qin, cin, depth, b->grid_extend, P.overflow, work, b->tune, bv, s);
             launch_march(b->view, fp, P.pb, qin, cin, pl.n_sdf[c], P.mt, P.capacity * b->n_sdf_max,
                          P.queue_count + 2 * (D + 2) + depth, b->grid_march, b->grid_shade, b->tune, bv, s);

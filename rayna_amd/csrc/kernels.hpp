@@ -81,9 +81,17 @@ constexpr int SHADE_BLOCKS_PER_CU = 16;
 // With bv.reqs, fp.n_slots is the chunk's slot count and the requests come from bv.
 void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, const BatchView &bv,
                      hipStream_t s);
+// Claim counters of one extend launch: the queue is cut into EXTEND_PARTS contiguous parts, each with its own
+// counter, one 128-byte line apart (EXTEND_CLAIM_WORDS words in all, zeroed before the launch).
+#ifndef RAYNA_EXTEND_PARTS
+#define RAYNA_EXTEND_PARTS 4
+#endif
+constexpr uint32_t EXTEND_PARTS = RAYNA_EXTEND_PARTS;
+constexpr uint32_t EXTEND_CLAIM_WORDS = EXTEND_PARTS * 32;
+
 // queue == nullptr: identity queue of fp.n_slots entries; else queue[0 .. *count)
 // overflow: traversal stack entries beyond the LDS part, extend_block() * grid * (stack_need - extend_lds_stack());
-// work: a zeroed counter the launch's waves take rays from
+// work: EXTEND_CLAIM_WORDS zeroed words, the counters the launch's waves take rays from
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
                    const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work,
                    const ExtendTuning &tune, const BatchView &bv, hipStream_t s);

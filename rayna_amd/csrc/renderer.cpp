@@ -51,6 +51,7 @@ struct rayna_renderer {
     PathBuffers pb{};
     int32_t *queue[2] = {nullptr, nullptr};
     uint32_t *queue_count = nullptr;
+    uint32_t *claims = nullptr;  // extend claim counters, EXTEND_CLAIM_WORDS per depth and one set for queries
     unsigned long long *segments = nullptr;
     float *sum = nullptr, *acc = nullptr, *out_dev = nullptr;
     uint8_t *rgba_dev = nullptr;  // the 8-bit output of rayna_render_srgb8
@@ -173,6 +174,7 @@ void free_paths(rayna_renderer *r) {
     r->pb = PathBuffers{};
     r->queue[0] = r->queue[1] = nullptr;
     r->queue_count = nullptr;
+    r->claims = nullptr;
     r->segments = nullptr;
     r->sum = r->acc = r->out_dev = nullptr;
     r->dimg = nullptr;
@@ -232,6 +234,9 @@ rayna_status alloc_paths(rayna_renderer *r) {
     for (int q = 0; q < 2; ++q)
         if ((st = alloc(slots * sizeof(int32_t), (void **)&r->queue[q])) != RAYNA_OK) return st;
     if ((st = alloc(qlen * sizeof(uint32_t), (void **)&r->queue_count)) != RAYNA_OK) return st;
+    if ((st = alloc((size_t)(o.ray_depth + 3) * EXTEND_CLAIM_WORDS * sizeof(uint32_t), (void **)&r->claims)) !=
+        RAYNA_OK)
+        return st;
     if ((st = alloc(sizeof(unsigned long long), (void **)&r->segments)) != RAYNA_OK) return st;
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->sum)) != RAYNA_OK) return st;
     if ((st = alloc(3 * P * sizeof(float), (void **)&r->acc)) != RAYNA_OK) return st;
@@ -293,6 +298,7 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
     fp.n_slots = fp.n_pixels * n;
     const uint32_t D2 = r->opts.ray_depth + 2;
     launch_fill_u32(r->queue_count, 0u, 3 * D2, r->stream);
+    launch_fill_u32(r->claims, 0u, D2 * EXTEND_CLAIM_WORDS, r->stream);
     launch_generate(fp, r->pb, adjoint != nullptr, BatchView{}, r->stream);
     for (uint32_t depth = 0; depth <= r->opts.ray_depth; ++depth) {
         const int32_t *qin = depth == 0 ? nullptr : r->queue[depth & 1];
@@ -300,7 +306,7 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
         int32_t *qout = r->queue[(depth + 1) & 1];
         uint32_t *cout = r->queue_count + depth + 1;
         if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
-        uint32_t *work = r->queue_count + D2 + depth;
+        uint32_t *work = r->claims + (size_t)depth * EXTEND_CLAIM_WORDS;
         launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, work, r->tune,
                       BatchView{}, r->stream);
         launch_march(r->dev.view, fp, r->pb, qin, cin, (uint32_t)r->dev.view.n_sdfs, r->mt, r->mt_cap,
@@ -695,8 +701,10 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     r->primal_frame = -1;
     launch_generate(fp, pb, false, BatchView{}, r->stream);
     uint32_t *work = r->queue_count + 3 * (r->opts.ray_depth + 2);
+    uint32_t *claims = r->claims + (size_t)(r->opts.ray_depth + 2) * EXTEND_CLAIM_WORDS;
     launch_fill_u32(work, 0u, 2, r->stream);
-    launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, work, r->tune,
+    launch_fill_u32(claims, 0u, EXTEND_CLAIM_WORDS, r->stream);
+    launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, claims, r->tune,
                   BatchView{}, r->stream);
     launch_march(r->dev.view, fp, pb, nullptr, nullptr, (uint32_t)r->dev.view.n_sdfs, r->mt, r->mt_cap, work + 1,
                  r->grid_march, r->grid_shade, r->tune, BatchView{}, r->stream);
