@@ -481,6 +481,7 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     SceneView &v = b->view;
 #define UP(vec, field) \
     if ((st = bupload(b, merged.vec, &v.field)) != RAYNA_OK) return bail(st);
+    merged.nodes = device_nodes(merged.nodes);
     UP(nodes, nodes)
     UP(tris, tris)
     UP(tri_normals, tri_normals)

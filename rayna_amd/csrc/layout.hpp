@@ -23,6 +23,24 @@ namespace rayna {
 enum : int32_t { INFO_EMPTY = -1, INFO_INNER = 0 };
 enum : int32_t { LEAF_TRIS = 1, LEAF_ITEMS = 2 };
 
+// Traversal stack entry codes of the extend kernel: >= 0 an inner node; < 0 a leaf range (bit 30: item range,
+// bits 27-29: count - 1, bits 0-26: first triangle or item); ENTRY_NONE marks an unused child slot.  The device
+// copy of the nodes holds each child slot's code in `info` (see encode_child_codes), so an expansion reads
+// boxes and codes only.
+constexpr uint32_t LEAF_CODE_FLAG = 0x80000000u;
+constexpr uint32_t LEAF_CODE_ITEMS = 0x40000000u;
+constexpr uint32_t LEAF_CODE_START = 0x07ffffffu;
+constexpr int32_t ENTRY_NONE = 0x7fffffff;
+constexpr int32_t leaf_code(int32_t start, int count, bool items) {
+    return (int32_t)(LEAF_CODE_FLAG | (items ? LEAF_CODE_ITEMS : 0u) | ((uint32_t)(count - 1) << 27) |
+                     ((uint32_t)start & LEAF_CODE_START));
+}
+constexpr int32_t child_code(int32_t child, int32_t info) {
+    return info == INFO_EMPTY   ? ENTRY_NONE
+           : info == INFO_INNER ? child
+                                : leaf_code(child, info & 0xf, (info >> 4) == LEAF_ITEMS);
+}
+
 struct alignas(16) GNode {
     float lo_x[4], lo_y[4], lo_z[4];
     float hi_x[4], hi_y[4], hi_z[4];

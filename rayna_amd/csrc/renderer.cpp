@@ -125,6 +125,7 @@ rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
     rayna_status s2;
 #define UP(vec, field) \
     if ((s2 = upload(r, ds, ps.vec, &v.field)) != RAYNA_OK) { free_scene(ds); return s2; }
+    ps.nodes = device_nodes(ps.nodes);
     UP(nodes, nodes)
     UP(tris, tris)
     UP(tri_normals, tri_normals)

@@ -1363,6 +1363,13 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
     return RAYNA_OK;
 }
 
+std::vector<GNode> device_nodes(const std::vector<GNode> &nodes) {
+    std::vector<GNode> out = nodes;
+    for (GNode &g : out)
+        for (int k = 0; k < 4; ++k) g.info[k] = child_code(g.child[k], g.info[k]);
+    return out;
+}
+
 bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vector<SceneSlot> &slots) {
     out = PackedScene{};
     slots.clear();

@@ -96,4 +96,8 @@ struct SceneSlot {
 // only order hits within one traversal).  Returns false if the merged arrays overflow a leaf or hit index.
 bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vector<SceneSlot> &slots);
 
+// The device copy of a node array: each child slot's `info` replaced by its traversal stack entry code
+// (child_code: the inner node, the leaf range's code, or ENTRY_NONE).
+std::vector<GNode> device_nodes(const std::vector<GNode> &nodes);
+
 }  // namespace rayna

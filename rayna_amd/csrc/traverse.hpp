@@ -15,11 +15,6 @@
 //  * persistent waves with dynamic fetch: each wave takes rays from a per-launch counter whenever enough of
 //    its lanes are idle, so lanes whose rays finish early pick up new rays instead of waiting.
 
-// Stack entry codes: >= 0 an inner node; < 0 a leaf range (bit 30: item range, bits 27-29: count - 1,
-// bits 0-26: first triangle or item).
-constexpr uint32_t LEAF_CODE_FLAG = 0x80000000u;
-constexpr uint32_t LEAF_CODE_ITEMS = 0x40000000u;
-constexpr uint32_t LEAF_CODE_START = 0x07ffffffu;
 
 // Traversal counters, compiled in only for measurement builds (-DRAYNA_TRAVERSAL_STATS): per lane, node pops,
 // node expansions, primitive tests and loop iterations, summed into a device array read by
@@ -41,10 +36,6 @@ __device__ unsigned long long g_trav_stats[4];
 #define RAYNA_STAT_FLUSH() ((void)0)
 #endif
 
-__device__ __forceinline__ int32_t leaf_code(int32_t start, int count, bool items) {
-    return (int32_t)(LEAF_CODE_FLAG | (items ? LEAF_CODE_ITEMS : 0u) | ((uint32_t)(count - 1) << 27) |
-                     ((uint32_t)start & LEAF_CODE_START));
-}
 
 // Traversal stack of one lane: the top EXT_LDS_STACK entries in LDS, each (code, distance) one 64-bit word (one
 // column per lane, conflict-free),
@@ -96,15 +87,12 @@ struct LaneStack {
 __device__ __forceinline__ void expand(const SceneView &sv, const DRay &ray, float best, int32_t node, LaneStack &st) {
     const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
     const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
-    const int4 ch = reinterpret_cast<const int4 *>(np)[6];
-    const int4 inf = reinterpret_cast<const int4 *>(np)[7];
+    const int4 cc = reinterpret_cast<const int4 *>(np)[7];  // the slots' entry codes (device node copy)
     const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, lo_y[4] = {ly.x, ly.y, ly.z, ly.w},
                 lo_z[4] = {lz.x, lz.y, lz.z, lz.w};
     const float hi_x[4] = {hx.x, hx.y, hx.z, hx.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w},
                 hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
-    const int32_t child[4] = {ch.x, ch.y, ch.z, ch.w};
-    const int32_t info[4] = {inf.x, inf.y, inf.z, inf.w};
-    int32_t code[4];
+    int32_t code[4] = {cc.x, cc.y, cc.z, cc.w};
     float key[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -117,11 +105,9 @@ __device__ __forceinline__ void expand(const SceneView &sv, const DRay &ray, flo
         t0 = fmaxf(t0, fminf(tz1, tz2));
         t1 = fminf(t1, fmaxf(tz1, tz2));
         const float lo = fmaxf(t0, T_MIN), hi = fminf(t1, best);
-        const bool hit = info[k] != INFO_EMPTY && lo <= hi * BOX_SLACK;
-        code[k] = info[k] == INFO_INNER ? child[k]
-                                        : leaf_code(child[k], info[k] & 0xf, (info[k] >> 4) == LEAF_ITEMS);
+        const bool hit = code[k] != ENTRY_NONE && lo <= hi * BOX_SLACK;
         key[k] = hit ? t0 : -__builtin_inff();
-        if (!hit) code[k] = 0x7fffffff;  // marks "not pushed" (no node has this index)
+        if (!hit) code[k] = ENTRY_NONE;  // not pushed
     }
 #define RAYNA_CSWAP(a, b)              \
     if (key[a] < key[b]) {             \
@@ -140,7 +126,7 @@ __device__ __forceinline__ void expand(const SceneView &sv, const DRay &ray, flo
 #undef RAYNA_CSWAP
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        if (code[k] != 0x7fffffff) st.push(code[k], key[k]);
+        if (code[k] != ENTRY_NONE) st.push(code[k], key[k]);
 }
 
 // Instances (an object's mesh BVH under a non-identity transform, SimpleObject::full_intersect around
