@@ -334,7 +334,7 @@ rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_ima
 /* Launch shape of the wavefront kernels.  Zero fields keep the default. */
 typedef struct rayna_tuning {
     uint32_t chunk_slots;          /* paths in flight per chunk (default 2^27 for a renderer, 2^28 for a batch) */
-    uint32_t refill_lanes;         /* extend: idle lanes of a wave before it takes new rays, 1..64 (default 16) */
+    uint32_t refill_lanes;         /* extend: idle lanes of a wave before it takes new rays, 1..64 (default 32) */
     uint32_t wave_batch;           /* extend: queue positions a wave claims per atomic, 1..4096 (default 128) */
     uint32_t shade_blocks_per_cu;  /* shade: blocks of 256 threads per compute unit, 1..64 (default 16) */
     int32_t use_graph;             /* batches: capture the launch sequence in a hipGraph and replay it (1) */

@@ -24,7 +24,7 @@ SPACE: Dict[str, Sequence] = {
     "pipelines": (1, 2, 3, 4),
     "use_graph": (False, True),
 }
-DEFAULTS = {"chunk_slots": 1 << 28, "refill_lanes": 16, "wave_batch": 128, "shade_blocks_per_cu": 16,
+DEFAULTS = {"chunk_slots": 1 << 28, "refill_lanes": 32, "wave_batch": 128, "shade_blocks_per_cu": 16,
             "pipelines": 3, "use_graph": False}
 
 

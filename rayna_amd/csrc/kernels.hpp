@@ -70,7 +70,7 @@ struct AdjointParams {
 // Traversal tuning of the extend kernel: a wave takes new rays once `refill_lanes` of its lanes are idle,
 // claiming `wave_batch` queue positions per atomic.
 struct ExtendTuning {
-    uint32_t refill_lanes = 16;
+    uint32_t refill_lanes = 32;
     uint32_t wave_batch = 128;
 };
 

@@ -21,7 +21,7 @@ class _Batch:
 
 
 def test_descent_finds_the_separable_optimum():
-    want = {"chunk_slots": 1 << 26, "refill_lanes": 32, "wave_batch": 192, "shade_blocks_per_cu": 8,
+    want = {"chunk_slots": 1 << 26, "refill_lanes": 48, "wave_batch": 192, "shade_blocks_per_cu": 8,
             "pipelines": 4, "use_graph": True}
     cost = lambda c: 100.0 + sum(10.0 for k, v in want.items() if c[k] != v)
     b = _Batch(cost)
