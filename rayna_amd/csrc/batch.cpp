@@ -384,7 +384,7 @@ hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
         const uint32_t D = pl.depth[c];
         launch_fill_u32(P.queue_count, 0u, 3 * (D + 2), s);
         launch_fill_u32(P.claims, 0u, (D + 1) * EXTEND_CLAIM_WORDS, s);
-        launch_generate(fp, P.pb, false, bv, s);
+        launch_generate(fp, P.pb, bv, s);
         for (uint32_t depth = 0; depth <= D; ++depth) {
             const int32_t *qin = depth == 0 ? nullptr : P.queue[depth & 1];
             const uint32_t *cin = depth == 0 ? nullptr : P.queue_count + depth;

@@ -649,10 +649,12 @@ __device__ __forceinline__ DRay camera_ray(const GViewport &vp, uint32_t width, 
     return make_ray(org, pix - org);
 }
 
-// Camera ray of every slot of the chunk; throughput 1, radiance 0.  BATCH: each slot finds its request (the
-// last with slot_base <= slot) and records it for the later kernels.
+// Camera ray of every slot of the chunk.  Throughput and radiance get no initial values: the first k_shade takes
+// them as 1 and 0 (the adjoint replay reads the primal radiance the field still holds) and writes the radiance
+// of every slot, as k_shade_mode does.  BATCH: each slot finds its request (the last with slot_base <= slot) and
+// records it for the later kernels.
 template <bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers pb, int keep_radiance, BatchView bv) {
+__global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers pb, BatchView bv) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= fp.n_slots) return;
     uint32_t l = i;  // slot within the request
@@ -683,14 +685,6 @@ __global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers 
     pb.dx[i] = r.d.x;
     pb.dy[i] = r.d.y;
     pb.dz[i] = r.d.z;
-    pb.tr[i] = 1.0f;
-    pb.tg[i] = 1.0f;
-    pb.tb[i] = 1.0f;
-    if (!keep_radiance) {
-        pb.lr[i] = 0.0f;
-        pb.lg[i] = 0.0f;
-        pb.lb[i] = 0.0f;
-    }
 }
 // Adds g to the gradient of material `mat` in gradient copy `copy` (float64 hardware atomics).
 __device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy, int32_t mat, double g0, double g1,
@@ -766,7 +760,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                 if (!queue_in) ray = camera_ray(fp.vp, fp.width, fp.height, fp.sdim, key, pixel, sample);
             }
             // The first bounce (identity queue) starts from the camera ray with throughput 1 and radiance 0 (the
-            // adjoint's radiance field holds the primal radiance), as k_generate stored them.
+            // adjoint's radiance field holds the primal radiance); k_generate stores only the ray.
             Col T{1.0f, 1.0f, 1.0f}, L{0.0f, 0.0f, 0.0f};
             if (queue_in) {
                 ray = ray_lazy(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
@@ -1076,14 +1070,11 @@ int march_blocks_per_cu() {
 }
 int march_block() { return MARCH_BLOCK; }
 
-void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, const BatchView &bv,
-                     hipStream_t s) {
+void launch_generate(const FrameParams &fp, const PathBuffers &pb, const BatchView &bv, hipStream_t s) {
     if (bv.reqs)
-        hipLaunchKernelGGL(k_generate<true>, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb,
-                           keep_radiance ? 1 : 0, bv);
+        hipLaunchKernelGGL(k_generate<true>, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb, bv);
     else
-        hipLaunchKernelGGL(k_generate<false>, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb,
-                           keep_radiance ? 1 : 0, bv);
+        hipLaunchKernelGGL(k_generate<false>, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb, bv);
 }
 
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,

@@ -77,10 +77,10 @@ struct ExtendTuning {
 // Shade grid: blocks of 256 threads per compute unit (grid-stride over the queue).
 constexpr int SHADE_BLOCKS_PER_CU = 16;
 
-// keep_radiance: leave the radiance field as it is (the adjoint replay keeps the primal radiance there).
-// With bv.reqs, fp.n_slots is the chunk's slot count and the requests come from bv.
-void launch_generate(const FrameParams &fp, const PathBuffers &pb, bool keep_radiance, const BatchView &bv,
-                     hipStream_t s);
+// The camera ray of every slot (the throughput and radiance fields are left as they are: the first shade takes
+// them as 1 and 0, and the adjoint replay reads the primal radiance still held there).  With bv.reqs,
+// fp.n_slots is the chunk's slot count and the requests come from bv.
+void launch_generate(const FrameParams &fp, const PathBuffers &pb, const BatchView &bv, hipStream_t s);
 // Claim counters of one extend launch: the queue is cut into EXTEND_PARTS contiguous parts, each with its own
 // counter, one 128-byte line apart (EXTEND_CLAIM_WORDS words in all, zeroed before the launch).
 #ifndef RAYNA_EXTEND_PARTS

@@ -300,7 +300,7 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
     const uint32_t D2 = r->opts.ray_depth + 2;
     launch_fill_u32(r->queue_count, 0u, 3 * D2, r->stream);
     launch_fill_u32(r->claims, 0u, D2 * EXTEND_CLAIM_WORDS, r->stream);
-    launch_generate(fp, r->pb, adjoint != nullptr, BatchView{}, r->stream);
+    launch_generate(fp, r->pb, BatchView{}, r->stream);
     for (uint32_t depth = 0; depth <= r->opts.ray_depth; ++depth) {
         const int32_t *qin = depth == 0 ? nullptr : r->queue[depth & 1];
         const uint32_t *cin = depth == 0 ? nullptr : r->queue_count + depth;
@@ -700,7 +700,7 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     PathBuffers pb = r->pb;
     pb.hit_rank = rank_dev;
     r->primal_frame = -1;
-    launch_generate(fp, pb, false, BatchView{}, r->stream);
+    launch_generate(fp, pb, BatchView{}, r->stream);
     uint32_t *work = r->queue_count + 3 * (r->opts.ray_depth + 2);
     uint32_t *claims = r->claims + (size_t)(r->opts.ray_depth + 2) * EXTEND_CLAIM_WORDS;
     launch_fill_u32(work, 0u, 2, r->stream);
