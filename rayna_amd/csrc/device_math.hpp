@@ -139,16 +139,14 @@ __device__ __forceinline__ float atan2_f(float y, float x) {
         const float base = signbit(x) ? PI : 0.0f;
         return signbit(y) ? -base : base;
     }
-    float a;
-    if (ay <= ax) {
-        const float t = ay / ax;
-        if (t > 0x1.a8279ap-2f) a = PI_4 + atan_small((t - 1.0f) / (t + 1.0f));
-        else a = atan_small(t);
-    } else {
-        const float t = ax / ay;
-        if (t > 0x1.a8279ap-2f) a = PI_4 - atan_small((t - 1.0f) / (t + 1.0f));
-        else a = PI_2 - atan_small(t);
-    }
+    // Four reductions: t = ay / ax or ax / ay (the smaller over the larger), shifted by (t - 1) / (t + 1) when t is
+    // large, then a = p, PI_4 + p, PI_4 - p or PI_2 - p.  They share one quotient, one shifted quotient and one
+    // polynomial, chosen by selects, so a wave whose lanes take different reductions pays for one.
+    const bool swap = !(ay <= ax);
+    const float t = (swap ? ax : ay) / (swap ? ay : ax);
+    const bool big = t > 0x1.a8279ap-2f;
+    const float p = atan_small(big ? (t - 1.0f) / (t + 1.0f) : t);
+    float a = swap ? (big ? PI_4 - p : PI_2 - p) : (big ? PI_4 + p : p);
     if (signbit(x)) a = PI - a;
     return signbit(y) ? -a : a;
 }

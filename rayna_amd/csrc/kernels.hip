@@ -261,6 +261,19 @@ __device__ __forceinline__ float ipow(float x, int n) {  // binary powering in c
     return r;
 }
 
+// x^(n-1) and x^n for the Mandelbulb estimate, each as ipow computes it.  For the usual power 8 the products are
+// written out (ipow's 1 * a is exact): x^7 = (x * x^2) * x^4 and x^8 = x^4 * x^4, sharing x^2 and x^4.
+__device__ __forceinline__ void bulb_powers(float x, int n, float *x_nm1, float *x_n) {
+    if (n == 8) {
+        const float x2 = x * x, x4 = x2 * x2;
+        *x_nm1 = (x * x2) * x4;
+        *x_n = x4 * x4;
+    } else {
+        *x_nm1 = ipow(x, n - 1);
+        *x_n = ipow(x, n);
+    }
+}
+
 __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
     if (f.kind == RAYNA_SDF_SPHERE) {
         const GSphere s = sv.spheres[f.first_sphere];
@@ -286,8 +299,9 @@ __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
         if (r > f.bailout) break;
         const float theta = dm::acos_f(z.z / r);
         const float phi = dm::atan2_f(z.y, z.x);
-        dr = ipow(r, f.power - 1) * (float)f.power * dr + 1.0f;
-        const float zr = ipow(r, f.power);
+        float rp1, zr;
+        bulb_powers(r, f.power, &rp1, &zr);
+        dr = rp1 * (float)f.power * dr + 1.0f;
         float st, ct, sp, cp;
         dm::sincos_rad(theta * (float)f.power, &st, &ct);
         dm::sincos_rad(phi * (float)f.power, &sp, &cp);

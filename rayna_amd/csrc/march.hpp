@@ -133,8 +133,9 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
                 } else {
                     const float theta = dm::acos_f(z.z / r);
                     const float phi = dm::atan2_f(z.y, z.x);
-                    dr = ipow(r, f.power - 1) * (float)f.power * dr + 1.0f;
-                    const float zr = ipow(r, f.power);
+                    float rp1, zr;
+                    bulb_powers(r, f.power, &rp1, &zr);
+                    dr = rp1 * (float)f.power * dr + 1.0f;
                     float st, ct, sp, cp;
                     dm::sincos_rad(theta * (float)f.power, &st, &ct);
                     dm::sincos_rad(phi * (float)f.power, &sp, &cp);
