@@ -29,14 +29,11 @@ __device__ __forceinline__ void sincos_turns(float r, float *s_out, float *c_out
                 C10 = -0x1.a6d1f2p-16f;
     const float sp = f * (S1 + f2 * (S3 + f2 * (S5 + f2 * (S7 + f2 * S9))));
     const float cp = 1.0f + f2 * (C2 + f2 * (C4 + f2 * (C6 + f2 * (C8 + f2 * C10))));
+    // quadrant q: (s, c) = (sp, cp), (cp, -sp), (-sp, -cp), (-cp, sp), by selects and exact negations
     const int quadrant = ((int)k) & 3;
-    float s, c;
-    if (quadrant == 0) { s = sp; c = cp; }
-    else if (quadrant == 1) { s = cp; c = -sp; }
-    else if (quadrant == 2) { s = -sp; c = -cp; }
-    else { s = -cp; c = sp; }
-    *s_out = s;
-    *c_out = c;
+    const float s = (quadrant & 1) ? cp : sp, c = (quadrant & 1) ? sp : cp;
+    *s_out = (quadrant & 2) ? -s : s;
+    *c_out = ((quadrant + 1) & 2) ? -c : c;
 }
 
 __device__ __forceinline__ float log_f(float x) {
@@ -111,14 +108,11 @@ __device__ __forceinline__ void sincos_rad(float x, float *s_out, float *c_out) 
                 C10 = -0x1.27e4fcp-22f;
     const float sp = r + r * (r2 * (S3 + r2 * (S5 + r2 * (S7 + r2 * S9))));
     const float cp = 1.0f + r2 * (C2 + r2 * (C4 + r2 * (C6 + r2 * (C8 + r2 * C10))));
+    // quadrant q: (s, c) = (sp, cp), (cp, -sp), (-sp, -cp), (-cp, sp), by selects and exact negations
     const int quadrant = ((int)k) & 3;
-    float s, c;
-    if (quadrant == 0) { s = sp; c = cp; }
-    else if (quadrant == 1) { s = cp; c = -sp; }
-    else if (quadrant == 2) { s = -sp; c = -cp; }
-    else { s = -cp; c = sp; }
-    *s_out = s;
-    *c_out = c;
+    const float s = (quadrant & 1) ? cp : sp, c = (quadrant & 1) ? sp : cp;
+    *s_out = (quadrant & 2) ? -s : s;
+    *c_out = ((quadrant + 1) & 2) ? -c : c;
 }
 
 __device__ __forceinline__ float atan_small(float z) {
