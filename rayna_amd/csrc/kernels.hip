@@ -767,7 +767,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                 ray_depth = R.ray_depth;
                 skybox = R.skybox;
                 sky_img = R.sky_image;
-                if (!queue_in) ray = camera_ray(R.vp, R.width, R.height, R.sdim, key, pixel, sample);
             } else {
                 pixel = slot / fp.n_spp;
                 sample = fp.s0 + slot % fp.n_spp;
@@ -775,13 +774,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
             }
             // The first bounce (identity queue) starts from the camera ray with throughput 1 and radiance 0 (the
             // adjoint's radiance field holds the primal radiance); k_generate stores only the ray.
+            // (A batch chunk's first bounce loads the ray k_generate stored: recomputing it would read the
+            // request's viewport per lane.)
             Col T{1.0f, 1.0f, 1.0f}, L{0.0f, 0.0f, 0.0f};
-            if (queue_in) {
+            if (queue_in || BATCH)
                 ray = ray_lazy(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
-                T = Col{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
-            } else {
+            else
                 ray = ray_lazy(ray.o, ray.d);
-            }
+            if (queue_in) T = Col{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
             if (queue_in || ADJOINT) L = Col{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
             if (ref == 0u) {
                 if (!ADJOINT)
