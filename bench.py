@@ -150,8 +150,9 @@ def batch_leg(tune: bool = True):
     drawn from random_spheres / triangle_mesh / sdf_volume, seeds 100..163).  The launch shape comes from the
     autotuner, run on the same requests at 1/2 of the samples, and serves both sides: one render call per
     request (a Renderer per scene, set_camera + set_options + render, in request order) and one batch call
-    (scheduled by scene, and shortest first; its chunks traced on the tuned number of concurrent pipelines).  Throughput from wall time; latency of request i: its
-    completion time from the start."""
+    (with the tuned schedule, by scene or interleaved, and again shortest first; its chunks traced on the tuned
+    number of concurrent pipelines).  Throughput from wall time; latency of request i: its completion time
+    from the start."""
     import numpy as np
     from rayna_amd import Renderer, presets
     from rayna_amd import autotune as AT
@@ -161,13 +162,14 @@ def batch_leg(tune: bool = True):
     total = sum(q.opts.width * q.opts.height * q.opts.samples for q in reqs)
     warm = AT.scaled_requests(reqs, 1 << 16)  # one sample each: allocates the largest buffers up front
 
-    b = BatchRenderer(scenes)
-    b.set_tuning(schedule="by_scene")
-    b.render(warm)
+    half = AT.scaled_requests(reqs, 2)
     cfg = dict(AT.DEFAULTS)
     tuned = None
     if tune:
-        tr = AT.tune(b, AT.scaled_requests(reqs, 2), repeats=1)
+        with BatchRenderer(scenes) as b:
+            b.set_tuning(schedule="by_scene")
+            b.render(warm)
+            tr = AT.tune(b, half, repeats=1)
         cfg = dict(tr.best)
         tuned = {"best": tr.best, "best_ms": tr.best_ms, "default_ms": tr.default_ms, "trials": len(tr.trials)}
     shape = {k: cfg[k] for k in ("chunk_slots", "refill_lanes", "wave_batch", "shade_blocks_per_cu")}
@@ -195,6 +197,12 @@ def batch_leg(tune: bool = True):
     for r in rends.values():
         r.close()
 
+    # the batch's path state is allocated (by a run at half the samples, whose chunks are as large) only once the
+    # renderers have been closed, so that the two sides never hold their buffers at once
+    b = BatchRenderer(scenes)
+    b.set_tuning(schedule=cfg["schedule"], use_graph=False, pipelines=cfg["pipelines"], **shape)
+    b.render(half, to_host=False)
+
     def batched(schedule):
         b.set_tuning(schedule=schedule, use_graph=False, pipelines=cfg["pipelines"], **shape)  # eager: timed
         with _NoGc():
@@ -203,7 +211,7 @@ def batch_leg(tune: bool = True):
             total_s = time.perf_counter() - t0
         return total_s, st
 
-    bat_s, st = batched("by_scene")
+    bat_s, st = batched(cfg["schedule"])
     sjf_s, st2 = batched("shortest_first")
     b.close()
 
@@ -216,10 +224,10 @@ def batch_leg(tune: bool = True):
             "chunks": st.chunks, "launches": st.launches,
             "batched_shortest_first_msamples_per_s": total / sjf_s / 1e6,
             "latency_ms": {"one_request_per_launch": {"p50": pct(seq_lat, 50), "p99": pct(seq_lat, 99)},
-                           "batched_by_scene": {"p50": pct(st.request_ms, 50), "p99": pct(st.request_ms, 99)},
+                           "batched": {"p50": pct(st.request_ms, 50), "p99": pct(st.request_ms, 99)},
                            "batched_shortest_first": {"p50": pct(st2.request_ms, 50),
                                                       "p99": pct(st2.request_ms, 99)}},
-            "launch_shape": {**shape, "pipelines": cfg["pipelines"]}, "tuning": tuned}
+            "schedule": cfg["schedule"], "launch_shape": {**shape, "pipelines": cfg["pipelines"]}, "tuning": tuned}
 
 
 def main(argv=None):

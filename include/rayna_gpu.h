@@ -333,14 +333,17 @@ rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_ima
 
 /* Launch shape of the wavefront kernels.  Zero fields keep the default. */
 typedef struct rayna_tuning {
-    uint32_t chunk_slots;          /* paths in flight per chunk (default 2^27 for a renderer, 2^28 for a batch) */
+    uint32_t chunk_slots;          /* paths in flight per chunk (default 2^27 for a renderer, 2^29 for a batch) */
     uint32_t refill_lanes;         /* extend: idle lanes of a wave before it takes new rays, 1..64 (default 32) */
     uint32_t wave_batch;           /* extend: queue positions a wave claims per atomic, 1..4096 (default 128) */
     uint32_t shade_blocks_per_cu;  /* shade: blocks of 256 threads per compute unit, 1..64 (default 16) */
     int32_t use_graph;             /* batches: capture the launch sequence in a hipGraph and replay it (1) */
     int32_t schedule;              /* batches: 0 = requests in the given order, 1 = fewest samples first,
                                     * 2 = grouped by scene (groups by total samples, ascending), fewest
-                                    * samples first within a group: a wave then rarely mixes scenes */
+                                    * samples first within a group: a wave then rarely mixes scenes,
+                                    * 3 = interleaved: as 2, with chunks of one scene each, dealt in turn
+                                    * from the scene groups so that concurrent pipelines hold different
+                                    * scenes (eager only: with use_graph, RAYNA_UNSUPPORTED) */
     int32_t pipelines;             /* batches: chunks in flight at once, each on its own stream with its own
                                     * path state (chunk_slots paths each), 1..4 (default 3); the images do not
                                     * depend on it */

@@ -19,7 +19,7 @@ from . import _abi as A
 from .renderer import RayError, lib
 
 
-SCHEDULES = {"given": 0, "shortest_first": 1, "by_scene": 2}
+SCHEDULES = {"given": 0, "shortest_first": 1, "by_scene": 2, "interleaved": 3}
 
 
 @dataclass
@@ -60,8 +60,9 @@ class BatchRenderer:
                    shade_blocks_per_cu: int = 0, use_graph: bool = False, schedule: str = "given",
                    pipelines: int = 0):
         """Launch shape (0 keeps a default), graph capture, the order requests are traced in ("given",
-        "shortest_first", or "by_scene": scene groups by total samples, shortest first within a group) and the
-        chunks in flight at once (`pipelines`, 1..4, each on its own stream; default 3)."""
+        "shortest_first", "by_scene": scene groups by total samples, shortest first within a group, or
+        "interleaved": by scene, with one-scene chunks dealt in turn from the groups) and the chunks in flight
+        at once (`pipelines`, 1..4, each on its own stream; default 3)."""
         t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 1 if use_graph else 0,
                      SCHEDULES[schedule], pipelines)
         self._check(lib().rayna_batch_set_tuning(self._h, C.byref(t)))

@@ -19,10 +19,10 @@ using namespace rayna;
 namespace {
 
 std::string g_batch_create_error;
-// Paths in flight per chunk: at most 2^28 (two 1080p frames of 64 samples; ~25 GB of path state per pipeline,
+// Paths in flight per chunk: at most 2^29 (four 1080p frames of 64 samples; ~50 GB of path state per pipeline,
 // sized for the 288 GB of HBM of one MI355X).  Large chunks keep every depth's launch full and pay a depth's
 // drain once for many requests.  A pipeline's buffers grow to the largest chunk of a call (a power of two).
-constexpr uint64_t DEFAULT_CHUNK_SLOTS = 1ull << 28;
+constexpr uint64_t DEFAULT_CHUNK_SLOTS = 1ull << 29;
 // Chunks in flight at once, each on its own stream with its own path state: while one chunk's depth drains
 // (the last long paths of a launch hold a few waves), the other's launches fill the rest of the GPU.
 constexpr int DEFAULT_PIPELINES = 3;
@@ -45,6 +45,8 @@ struct Plan {
     std::vector<uint32_t> depth;               // per chunk: deepest ray_depth of its requests
     std::vector<uint32_t> n_sdf;               // per chunk: most unbounded objects of its requests' scenes
     std::vector<uint32_t> slots;               // per chunk
+    std::vector<int32_t> dep;                  // per chunk: the last earlier chunk holding a piece of one of its
+                                               // requests (its resolve must come first), or -1
     std::vector<int32_t> last_chunk;           // per request (-1: not traced, e.g. invalid camera)
     std::vector<uint64_t> out_offset;          // per request, floats into the output buffer
     uint64_t out_floats = 0;
@@ -251,13 +253,14 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
     pl.last_chunk.assign(n, -1);
     pl.out_offset.assign(n, 0);
     uint64_t fill = C;  // forces a new chunk
+    int32_t prev_scene = -1;
     uint64_t h = 1469598103934665603ull;
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
     const auto work = [&](int k) { return (uint64_t)reqs[k].opts.width * reqs[k].opts.height * reqs[k].opts.samples; };
     if (b->schedule == 1) {
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) < work(y); });
-    } else if (b->schedule == 2) {
+    } else if (b->schedule == 2 || b->schedule == 3) {
         std::vector<uint64_t> scene_work(b->slots.size(), 0);
         for (int i = 0; i < n; ++i) scene_work[reqs[i].scene] += work(i);
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -281,6 +284,8 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
         std::memcpy(cam_bits, &q.camera, sizeof(cam_bits));
         for (uint64_t w : cam_bits) h = mix(h, w);
         if (!valid[i]) continue;
+        if (b->schedule == 3 && q.scene != prev_scene) fill = C;  // interleaved: chunks hold one scene each
+        prev_scene = q.scene;
         uint32_t done = 0;
         while (done < q.opts.samples) {
             uint64_t fit = (C - fill) / P;
@@ -297,6 +302,42 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
         pl.last_chunk[i] = (int32_t)pl.chunks.size() - 1;
     }
     pl.key = h;
+    if (b->schedule == 3 && !pl.chunks.empty()) {
+        // interleaved: the by-scene chunks dealt round-robin over the scene groups (groups in the by-scene
+        // order), so that the chunks in flight on the pipelines at one time mostly belong to different scenes
+        std::vector<std::vector<std::vector<Piece>>> groups;
+        int32_t last_scene = -1;
+        for (auto &ch : pl.chunks) {
+            const int32_t sc = reqs[ch.front().req].scene;
+            if (groups.empty() || sc != last_scene) groups.emplace_back();
+            groups.back().push_back(std::move(ch));
+            last_scene = sc;
+        }
+        pl.chunks.clear();
+        for (size_t k = 0;; ++k) {
+            bool any = false;
+            for (auto &g : groups)
+                if (k < g.size()) {
+                    pl.chunks.push_back(std::move(g[k]));
+                    any = true;
+                }
+            if (!any) break;
+        }
+        for (size_t c = 0; c < pl.chunks.size(); ++c)
+            for (const Piece &pc : pl.chunks[c])
+                if (pc.s0 + pc.n == reqs[pc.req].opts.samples) pl.last_chunk[pc.req] = (int32_t)c;
+    }
+    {
+        std::vector<int32_t> seen(n, -1);  // per request: the last chunk so far with a piece of it
+        for (size_t c = 0; c < pl.chunks.size(); ++c) {
+            int32_t d = -1;
+            for (const Piece &pc : pl.chunks[c]) {
+                d = std::max(d, seen[pc.req]);
+                seen[pc.req] = (int32_t)c;
+            }
+            pl.dep.push_back(d);
+        }
+    }
     for (size_t c = 0; c < pl.chunks.size(); ++c) {
         pl.first_entry.push_back((uint32_t)pl.table.size());
         pl.first_final.push_back((uint32_t)pl.finals.size());
@@ -358,8 +399,9 @@ uint32_t plan_launches(const Plan &pl, int used) {
 }
 
 // Enqueues every chunk of the plan, chunk c on pipeline c % used.  The pipelines' streams start after the
-// batch's stream (which holds the uploads) and the batch's stream waits for all of them at the end.  Resolves
-// add each chunk's samples into the requests' sums in chunk order, so the sums are those of one pipeline.
+// batch's stream (which holds the uploads) and the batch's stream waits for all of them at the end.  A chunk's
+// resolve waits for the resolve of the last earlier chunk holding a piece of one of its requests, so every
+// request's sums are added in sample order, as on one pipeline.
 // `timing` (eager runs only): events[0] at the start and events[1 + c] after chunk c.
 hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
     hipError_t err = hipSuccess;
@@ -397,7 +439,9 @@ hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
             launch_shade(b->view, fp, P.pb, qin, cin, qout, cout, depth, b->grid_shade, nullptr, bv, s);
         }
         launch_count_segments(P.queue_count, D + 1, fp.n_slots, P.segments, s);
-        if (used > 1 && c > 0) check(hipStreamWaitEvent(s, b->resolved[c - 1], 0));
+        // (a dependency on the same pipeline is kept by its stream's order)
+        if (used > 1 && pl.dep[c] >= 0 && (size_t)pl.dep[c] % used != c % used)
+            check(hipStreamWaitEvent(s, b->resolved[pl.dep[c]], 0));
         launch_resolve_batch(bv, pl.resolve_blocks[c], P.pb, b->sum, s);
         const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
         launch_finalize_batch(bv, b->finals + pl.first_final[c], nf, pl.max_pixels[c], b->sum, b->out, s);
@@ -521,9 +565,11 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
 rayna_status rayna_batch_set_tuning(rayna_batch *b, const rayna_tuning *t) {
     if (!b || !t) return bfail(b, RAYNA_INVALID_ARGUMENT, "null argument");
     if (t->refill_lanes > 64 || t->wave_batch > 4096 || t->shade_blocks_per_cu > 64 ||
-        (t->chunk_slots && t->chunk_slots < (1u << 16)) || t->schedule < 0 || t->schedule > 2 ||
+        (t->chunk_slots && t->chunk_slots < (1u << 16)) || t->schedule < 0 || t->schedule > 3 ||
         t->pipelines < 0 || t->pipelines > MAX_PIPELINES)
         return bfail(b, RAYNA_INVALID_ARGUMENT, "tuning out of range");
+    if (t->use_graph && t->schedule == 3)
+        return bfail(b, RAYNA_UNSUPPORTED, "the interleaved schedule is not captured in a graph");
     b->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
     b->tune.wave_batch = t->wave_batch ? t->wave_batch : ExtendTuning{}.wave_batch;
     b->grid_shade = b->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : SHADE_BLOCKS_PER_CU);
@@ -630,9 +676,16 @@ rayna_status rayna_render_batch(rayna_batch *b, const rayna_request *reqs, int32
     if (stats) {
         rayna_batch_stats s{};
         s.request_ms = stats->request_ms;
-        const size_t last = graph ? 1 : pl.chunks.size();
         float ms = 0;
-        if (!pl.chunks.empty() || graph) BHIP(b, hipEventElapsedTime(&ms, b->events[0], b->events[last]));
+        if (graph) {
+            BHIP(b, hipEventElapsedTime(&ms, b->events[0], b->events[1]));
+        } else {  // the chunks may end out of order on the pipelines
+            for (size_t c = 0; c < pl.chunks.size(); ++c) {
+                float m = 0;
+                BHIP(b, hipEventElapsedTime(&m, b->events[0], b->events[1 + c]));
+                ms = std::max(ms, m);
+            }
+        }
         s.kernel_ms = ms;
         for (int i = 0; i < n; ++i) {
             if (valid[i]) s.samples += (uint64_t)reqs[i].opts.width * reqs[i].opts.height * reqs[i].opts.samples;

@@ -3,8 +3,10 @@ the best by more than the margin), driven by a stand-in batch whose device time 
 shape; and the request scaling it tunes on."""
 from types import SimpleNamespace
 
+from rayna_amd import _abi as A
 from rayna_amd import autotune as AT
 from rayna_amd.batch import Request
+from rayna_amd.renderer import RayError
 from rayna_amd.scene import Camera, RenderOpts
 
 
@@ -12,7 +14,9 @@ class _Batch:
     def __init__(self, cost):
         self.cost, self.cfg, self.calls = cost, None, 0
 
-    def set_tuning(self, schedule="given", **cfg):
+    def set_tuning(self, **cfg):
+        if cfg.get("use_graph") and cfg.get("schedule") == "interleaved":  # as the batch refuses it
+            raise RayError(A.RAYNA_UNSUPPORTED, "the interleaved schedule is not captured in a graph")
         self.cfg = cfg
 
     def render(self, requests, to_host=True):
@@ -22,12 +26,20 @@ class _Batch:
 
 def test_descent_finds_the_separable_optimum():
     want = {"chunk_slots": 1 << 26, "refill_lanes": 48, "wave_batch": 192, "shade_blocks_per_cu": 8,
-            "pipelines": 4, "use_graph": True}
+            "pipelines": 4, "use_graph": False, "schedule": "by_scene"}
     cost = lambda c: 100.0 + sum(10.0 for k, v in want.items() if c[k] != v)
     b = _Batch(cost)
     r = AT.tune(b, [], repeats=1)
     assert r.best == want and r.best_ms == 100.0 and r.default_ms == 100.0 + 10.0 * 6
     assert b.cfg == {k: v for k, v in want.items()}  # left on the best shape
+
+
+def test_refused_shapes_are_skipped():
+    # graph capture pays off, and the interleaved schedule more, but the two together are refused
+    cost = lambda c: 100.0 - 5.0 * c["use_graph"] - 8.0 * (c["schedule"] == "interleaved")
+    r = AT.tune(_Batch(cost), [], repeats=1)
+    assert r.best["schedule"] == "interleaved" and not r.best["use_graph"] and r.best_ms == 92.0
+    assert all(t["ms"] < float("inf") for t in r.trials)  # refused shapes are not trials
 
 
 def test_noise_below_the_margin_keeps_the_defaults():

@@ -32,7 +32,9 @@ def _single(scenes, q):
                                     dict(schedule="shortest_first", refill_lanes=8, wave_batch=32, shade_blocks_per_cu=4),
                                     dict(schedule="by_scene", chunk_slots=1 << 17),
                                     dict(chunk_slots=1 << 16, pipelines=1), dict(chunk_slots=1 << 16, pipelines=3),
-                                    dict(chunk_slots=1 << 16, pipelines=4, use_graph=True)])
+                                    dict(chunk_slots=1 << 16, pipelines=4, use_graph=True),
+                                    dict(chunk_slots=1 << 16, pipelines=3, schedule="interleaved"),
+                                    dict(chunk_slots=1 << 16, pipelines=2, schedule="interleaved")])
 def test_batch_matches_single_renders(gpu, tuning):
     scenes, reqs = _requests()
     print("case batch", tuning, flush=True)
@@ -55,6 +57,18 @@ def test_batch_matches_single_renders(gpu, tuning):
     assert st.samples == sum(q.opts.width * q.opts.height * q.opts.samples for q in reqs)
     if tuning.get("chunk_slots"):
         assert st.chunks > 1
+
+
+def test_interleaved_schedule_is_not_captured(gpu):
+    """Graph capture of the interleaved schedule is refused before anything is traced (a captured interleaved
+    batch once ended its process with a segmentation fault in the runtime)."""
+    from rayna_amd import RayError
+    from rayna_amd import _abi as A
+    scenes, _ = _requests(n=2)
+    with BatchRenderer(scenes) as b:
+        with pytest.raises(RayError) as e:
+            b.set_tuning(schedule="interleaved", use_graph=True)
+        assert e.value.status == A.RAYNA_UNSUPPORTED
 
 
 def test_batch_against_oracle_and_invalid_camera(gpu):
