@@ -26,9 +26,10 @@ HBM_PEAK_GBPS = 8000.0
 EXTEND_BYTES_PER_SEGMENT = 40
 # Algorithmic bytes of the whole frame per segment (extend + shade) and per sample (generate + resolve):
 # extend 40 B; shade reads queue 4, ray 24, hit 12, throughput 12, radiance 12 and writes ray 24,
-# throughput 12, radiance 12, queue entry 4 -> 116 B; generate writes 48 B per sample, resolve reads 12 B.
+# throughput 12, radiance 12, queue entry 4 -> 116 B; generate writes the 24 B camera ray per sample, resolve
+# reads 12 B.
 FRAME_BYTES_PER_SEGMENT = 40 + 116
-FRAME_BYTES_PER_SAMPLE = 48 + 12
+FRAME_BYTES_PER_SAMPLE = 24 + 12
 # HBM traffic of the extend kernel per closest-hit query on the 1080p triangle-mesh frame, from rocprofv3
 # counters over one frame (9 launches, 263,045,425 queries): FETCH_SIZE 2.0241e7 KiB read as 64-byte units
 # (x2.0: a 256 MiB copy reads 131,088 KiB) plus WRITE_SIZE 6.812e6 KiB (x1.0: the same copy writes 262,144 KiB).
