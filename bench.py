@@ -31,10 +31,10 @@ EXTEND_BYTES_PER_SEGMENT = 40
 FRAME_BYTES_PER_SEGMENT = 40 + 116
 FRAME_BYTES_PER_SAMPLE = 24 + 12
 # HBM traffic of the extend kernel per closest-hit query on the 1080p triangle-mesh frame, from rocprofv3
-# counters over one frame (9 launches, 263,045,425 queries): FETCH_SIZE 2.0241e7 KiB read as 64-byte units
-# (x2.0: a 256 MiB copy reads 131,088 KiB) plus WRITE_SIZE 6.812e6 KiB (x1.0: the same copy writes 262,144 KiB).
+# counters over one frame (9 launches, 263,045,425 queries): FETCH_SIZE 2.0191e7 KiB read as 64-byte units
+# (x2.0: a 256 MiB copy reads 131,088 KiB) plus WRITE_SIZE 6.809e6 KiB (x1.0: the same copy writes 262,144 KiB).
 # Node and triangle fetches that miss L2 make up the difference to the 40 algorithmic bytes.
-EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (2.0241e7 * 2.0 + 6.812e6) * 1024 / 263045425}
+EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (2.0191e7 * 2.0 + 6.809e6) * 1024 / 263045425}
 
 
 class _NoGc:
