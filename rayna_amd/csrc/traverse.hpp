@@ -12,9 +12,10 @@
 //    box code together and the primitive code together;
 //  * speculative inner loop: a lane that has found a leaf range keeps expanding inner nodes until every lane of
 //    its wave holds one, so the primitive loop runs with more lanes busy;
-//  * persistent waves with dynamic fetch: each wave takes rays from a per-launch counter whenever enough of
-//    its lanes are idle, so lanes whose rays finish early pick up new rays instead of waiting.
-
+//  * persistent waves with dynamic fetch: each wave takes rays from the launch's claim counters whenever enough
+//    of its lanes are idle, so lanes whose rays finish early pick up new rays instead of waiting.
+//
+// Stack entry codes and the node words that hold them are described in layout.hpp.
 
 // Traversal counters, compiled in only for measurement builds (-DRAYNA_TRAVERSAL_STATS): per lane, node pops,
 // node expansions, primitive tests and loop iterations, summed into a device array read by
