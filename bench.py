@@ -35,6 +35,10 @@ FRAME_BYTES_PER_SAMPLE = 24 + 12
 # (x2.0: a 256 MiB copy reads 131,088 KiB) plus WRITE_SIZE 6.809e6 KiB (x1.0: the same copy writes 262,144 KiB).
 # Node and triangle fetches that miss L2 make up the difference to the 40 algorithmic bytes.
 EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (2.0191e7 * 2.0 + 6.809e6) * 1024 / 263045425}
+# What bounds the extend kernel instead, from the same frame's counters (VALUBusy and VALUUtilization passes,
+# time-weighted over the 9 launches): the VALU issues every cycle and 37.9% of the lanes it issues for are active.
+EXTEND_ISSUE = {"triangle_mesh": {"valu_busy_pct": 104.0, "valu_lanes_active_pct": 37.9,
+                                  "achieved_hbm_gbps": (2.0191e7 * 2.0 + 6.809e6) * 1024 / 9 / 6.2463e-3 / 1e9}}
 
 
 class _NoGc:
@@ -329,6 +333,8 @@ def main(argv=None):
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": "k_extend", "bytes_per_segment": EXTEND_BYTES_PER_SEGMENT},
     }
+    if a.workload in EXTEND_ISSUE:  # profiled, not measured by this run
+        line["extend_issue_profiled"] = EXTEND_ISSUE[a.workload]
     tw = None
     if not a.no_cpu_baseline:
         import oracle
