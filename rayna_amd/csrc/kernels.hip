@@ -782,10 +782,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
             else
                 ray = ray_lazy(ray.o, ray.d);
             if (queue_in) T = Col{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
-            if (queue_in || ADJOINT) L = Col{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
+            // Forward: the radiance this bounce adds, L += add.  After the first bounce the radiance field is read
+            // and written only when add is not zero in every channel: L + 0 is L (L is never -0: it starts as
+            // 0 + add, and an exact cancellation rounds to +0).
+            Col add{0.0f, 0.0f, 0.0f};
+            if (ADJOINT) L = Col{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
             if (ref == 0u) {
                 if (!ADJOINT)
-                    L = L + T * ((TEX && skybox == RAYNA_SKY_HDRI) ? sky_image(sv, sky_img, ray.d) : sky(skybox, ray.d));
+                    add = T * ((TEX && skybox == RAYNA_SKY_HDRI) ? sky_image(sv, sky_img, ray.d) : sky(skybox, ray.d));
             } else {
                 SHit h;
                 const float t = pb.hit_t[slot];
@@ -807,7 +811,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                         if (graded) grad_add(ap, copy, h.material, d0 * T.r, d1 * T.g, d2 * T.b);
                         L = Col{L.r - T.r * le.r, L.g - T.g * le.g, L.b - T.b * le.b};
                     } else {
-                        L = L + T * le;
+                        add = T * le;
                     }
                 }
                 float u[4];
@@ -837,9 +841,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                     }
                 }
             }
-            pb.lr[slot] = L.r;
-            pb.lg[slot] = L.g;
-            pb.lb[slot] = L.b;
+            const bool adds = !(add.r == 0.0f && add.g == 0.0f && add.b == 0.0f);
+            if (!ADJOINT && queue_in && adds) L = Col{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
+            if (!ADJOINT) L = L + add;
+            if (ADJOINT || !queue_in || adds) {
+                pb.lr[slot] = L.r;
+                pb.lg[slot] = L.g;
+                pb.lb[slot] = L.b;
+            }
         }
         const unsigned long long mask = __ballot(keep);
         if (keep) mine[n_kept + (uint32_t)__popcll(mask & below)] = slot;
