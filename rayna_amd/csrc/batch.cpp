@@ -69,6 +69,8 @@ struct Pipeline {
     uint32_t *claims = nullptr;    // extend claim counters, EXTEND_CLAIM_WORDS per depth
     unsigned long long *segments = nullptr;
     float *mt = nullptr;           // march outcomes, 2 * capacity * n_sdf_max
+    int32_t *sdf_queue = nullptr;  // scenes with unbounded objects: SDF-hit slots (normals in pb.sdf_n*)
+    uint32_t *sdf_count = nullptr;
     int2 *overflow = nullptr;      // traversal stack entries beyond the LDS part
 };
 
@@ -154,6 +156,8 @@ void free_pipe(rayna_batch *b, Pipeline &P) {
     P.queue_count_len = 0;
     P.segments = nullptr;
     P.mt = nullptr;
+    P.sdf_queue = nullptr;
+    P.sdf_count = nullptr;
     P.overflow = nullptr;
     P.capacity = 0;
     free_graph(b);
@@ -195,8 +199,14 @@ rayna_status alloc_pipe(rayna_batch *b, int p, uint64_t slots, uint32_t max_dept
     if ((st = alloc((size_t)(qlen / 3) * EXTEND_CLAIM_WORDS * sizeof(uint32_t), (void **)&P.claims)) != RAYNA_OK)
         return st;
     if ((st = alloc(sizeof(unsigned long long), (void **)&P.segments)) != RAYNA_OK) return st;
-    if (b->n_sdf_max && (st = alloc(2 * slots * b->n_sdf_max * sizeof(float), (void **)&P.mt)) != RAYNA_OK)
-        return st;
+    if (b->n_sdf_max) {
+        if ((st = alloc(2 * slots * b->n_sdf_max * sizeof(float), (void **)&P.mt)) != RAYNA_OK) return st;
+        float **planes[] = {&P.pb.sdf_nx, &P.pb.sdf_ny, &P.pb.sdf_nz};
+        for (float **f : planes)
+            if ((st = alloc(slots * sizeof(float), (void **)f)) != RAYNA_OK) return st;
+        if ((st = alloc(slots * sizeof(int32_t), (void **)&P.sdf_queue)) != RAYNA_OK) return st;
+        if ((st = alloc(sizeof(uint32_t), (void **)&P.sdf_count)) != RAYNA_OK) return st;
+    }
     if (b->overflow_entries && (st = alloc(b->overflow_entries * sizeof(int2), (void **)&P.overflow)) != RAYNA_OK)
         return st;
     P.pb.hit_rank = nullptr;
@@ -393,7 +403,7 @@ uint32_t plan_launches(const Plan &pl, int used) {
     uint32_t n = (uint32_t)used;  // segment counters
     for (size_t c = 0; c < pl.chunks.size(); ++c) {
         const uint32_t nf = pl.first_final[c + 1] - pl.first_final[c];
-        n += 3 + (pl.n_sdf[c] ? 4 : 2) * (pl.depth[c] + 1) + 2 + (nf ? 1 : 0);
+        n += 3 + (pl.n_sdf[c] ? 6 : 2) * (pl.depth[c] + 1) + 2 + (nf ? 1 : 0);
     }
     return n;
 }
@@ -435,7 +445,8 @@ hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
             uint32_t *work = P.claims + (size_t)depth * EXTEND_CLAIM_WORDS;
             launch_extend(b->view, fp, P.pb, qin, cin, depth, b->grid_extend, P.overflow, work, b->tune, bv, s);
             launch_march(b->view, fp, P.pb, qin, cin, pl.n_sdf[c], P.mt, P.capacity * b->n_sdf_max,
-                         P.queue_count + 2 * (D + 2) + depth, b->grid_march, b->grid_shade, b->tune, bv, s);
+                         P.queue_count + 2 * (D + 2) + depth, b->grid_march, b->grid_shade, b->tune, bv,
+                         P.sdf_queue, P.sdf_count, s);
             launch_shade(b->view, fp, P.pb, qin, cin, qout, cout, depth, b->grid_shade, nullptr, bv, s);
         }
         launch_count_segments(P.queue_count, D + 1, fp.n_slots, P.segments, s);

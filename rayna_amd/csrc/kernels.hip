@@ -396,7 +396,7 @@ __device__ __forceinline__ HitPrim hit_prim(const SceneView &sv, const DRay &ray
 // (object/transform.rs): the position by the object-to-world map, both normals by its linear part, normalised.
 __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint32_t ref, uint32_t aux,
                              const uint32_t *sub, const dm::Key &key, uint32_t pixel, uint32_t sample,
-                             uint32_t bounce, SHit *h) {
+                             uint32_t bounce, SHit *h, bool have_sdf_n = false, f3 sdf_n = f3{0.0f, 0.0f, 0.0f}) {
     const HitPrim p = hit_prim(sv, ray, t, ref, aux, sub);
     const DRay &r = p.ray;
     h->pos = at(r, p.t);
@@ -453,8 +453,8 @@ __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint
             h->front = true;
             break;
         }
-        default: {  // ITEM_SDF
-            const f3 n = sdf_normal(sv, sv.sdfs[p.prim], h->pos);
+        default: {  // ITEM_SDF: computed by k_sdf_normal when given
+            const f3 n = have_sdf_n ? sdf_n : sdf_normal(sv, sv.sdfs[p.prim], h->pos);
             h->normal = n;
             h->ray_normal = n;
             h->front = !(aux & HIT_AUX_BACK);
@@ -794,7 +794,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                 SHit h;
                 const float t = pb.hit_t[slot];
                 const uint32_t aux = pb.hit_aux[slot];
-                hit_geometry(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, &h);
+                f3 sdf_n{0.0f, 0.0f, 0.0f};
+                const bool sdf_hit = (ref >> HIT_KIND_SHIFT) == (uint32_t)ITEM_SDF && pb.sdf_nx;
+                if (sdf_hit) sdf_n = f3{pb.sdf_nx[slot], pb.sdf_ny[slot], pb.sdf_nz[slot]};
+                hit_geometry(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, &h, sdf_hit, sdf_n);
                 const GMaterial m = sv.materials[h.material];
                 const Col colour =
                     material_colour<TEX>(sv, m, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, h);
@@ -986,7 +989,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade_mode(SceneView sv, FrameParams 
             const float t = pb.hit_t[slot];
             const uint32_t aux = pb.hit_aux[slot];
             SHit h;
-            hit_geometry(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, 0, &h);
+            f3 sdf_n{0.0f, 0.0f, 0.0f};
+            const bool sdf_hit = (ref >> HIT_KIND_SHIFT) == (uint32_t)ITEM_SDF && pb.sdf_nx;
+            if (sdf_hit) sdf_n = f3{pb.sdf_nx[slot], pb.sdf_ny[slot], pb.sdf_nz[slot]};
+            hit_geometry(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, 0, &h, sdf_hit, sdf_n);
             const float PAL[13][3] = {{1.0f, 1.0f, 1.0f}, {1.0f, 0.0f, 0.0f}, {1.0f, 0.5f, 0.0f}, {1.0f, 1.0f, 0.0f},
                                       {0.5f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.5f}, {0.0f, 1.0f, 1.0f},
                                       {0.0f, 0.5f, 1.0f}, {0.0f, 0.0f, 1.0f}, {0.5f, 0.0f, 1.0f}, {1.0f, 0.0f, 1.0f},
@@ -1122,19 +1128,22 @@ void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers 
 
 void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
                   const uint32_t *count, uint32_t n_sdf, float *mt, uint64_t mt_stride, uint32_t *work, int grid,
-                  int merge_grid, const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
+                  int merge_grid, const ExtendTuning &tune, const BatchView &bv, int32_t *sdf_queue,
+                  uint32_t *sdf_count, hipStream_t s) {
     if (n_sdf == 0) return;
+    launch_fill_u32(sdf_count, 0u, 1, s);
     if (bv.reqs) {
         hipLaunchKernelGGL(k_march<true>, dim3(grid), dim3(MARCH_BLOCK), 0, s, sv, fp, pb, queue, count, n_sdf, mt,
                            mt_stride, work, tune, bv);
         hipLaunchKernelGGL(k_march_merge<true>, dim3(merge_grid), dim3(BLOCK), 0, s, sv, fp, pb, queue, count,
-                           n_sdf, mt, mt_stride, bv);
+                           n_sdf, mt, mt_stride, sdf_queue, sdf_count, bv);
     } else {
         hipLaunchKernelGGL(k_march<false>, dim3(grid), dim3(MARCH_BLOCK), 0, s, sv, fp, pb, queue, count, n_sdf,
                            mt, mt_stride, work, tune, bv);
         hipLaunchKernelGGL(k_march_merge<false>, dim3(merge_grid), dim3(BLOCK), 0, s, sv, fp, pb, queue, count,
-                           n_sdf, mt, mt_stride, bv);
+                           n_sdf, mt, mt_stride, sdf_queue, sdf_count, bv);
     }
+    hipLaunchKernelGGL(k_sdf_normal, dim3(merge_grid), dim3(BLOCK), 0, s, sv, pb, sdf_queue, sdf_count);
 }
 
 void launch_resolve_batch(const BatchView &bv, uint32_t blocks, const PathBuffers &pb, float *sum, hipStream_t s) {

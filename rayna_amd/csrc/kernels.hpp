@@ -21,6 +21,7 @@ struct PathBuffers {
     uint32_t *hit_aux;       // HIT_AUX_* bits; the object-space distance of a transformed object's hit
     uint32_t *hit_sub;       // an instance's hit: the reference of its primitive inside the instanced mesh BVH
     int32_t *hit_rank;       // optional (parity queries): rank of the closest primitive
+    float *sdf_nx, *sdf_ny, *sdf_nz;  // scenes with unbounded objects: the shading normal of an SDF hit
 };
 
 struct FrameParams {
@@ -99,12 +100,14 @@ void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers 
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
                   const AdjointParams *adjoint, const BatchView &bv, hipStream_t s);
 // The unbounded SDF objects of each queued path (n_sdf per path; in a batch, each request's own), marched
-// after launch_extend with its hit as the cut, then merged into the hit record.  mt: two planes of mt_stride
-// >= count * n_sdf floats (outcome, march iteration).
-// work: a zeroed counter.  No-op for n_sdf == 0.
+// after launch_extend with its hit as the cut, then merged into the hit record; the paths whose closest hit
+// becomes an SDF get their shading normal in pb.sdf_n*.  mt: two planes of mt_stride >= count * n_sdf floats
+// (outcome, march iteration).  work: a zeroed counter.  sdf_queue: room for every queued path; sdf_count: one
+// counter (zeroed here).  No-op for n_sdf == 0.
 void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
                   const uint32_t *count, uint32_t n_sdf, float *mt, uint64_t mt_stride, uint32_t *work, int grid,
-                  int merge_grid, const ExtendTuning &tune, const BatchView &bv, hipStream_t s);
+                  int merge_grid, const ExtendTuning &tune, const BatchView &bv, int32_t *sdf_queue,
+                  uint32_t *sdf_count, hipStream_t s);
 // Pixels per block of the resolve kernels for n samples per pixel in the chunk: a block stages its pixels'
 // samples (three channels, rows of n | 1 floats) in LDS so that its loads are coalesced; from 4160 samples per
 // pixel on, a block sums 256 pixels straight from HBM.

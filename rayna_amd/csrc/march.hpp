@@ -166,43 +166,93 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
 }
 
 // The closest hit of each path over its bounded hit and its unbounded objects' marches, in list order with
-// strict < (object/list.rs:358-363).
+// strict < (object/list.rs:358-363).  The paths whose closest hit becomes an unbounded object are appended to
+// sdf_queue for k_sdf_normal: each wave takes strips of MERGE_STRIP * 64 queue positions, stages those slots in
+// its part of LDS and appends them with one atomic per strip.
+constexpr uint32_t MERGE_STRIP = 16;
+
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_march_merge(SceneView sv, FrameParams fp, PathBuffers pb,
                                                        const int32_t *queue, const uint32_t *count_ptr,
                                                        uint32_t n_sdf, const float *mt, uint64_t mt_stride,
-                                                       BatchView bv) {
+                                                       int32_t *sdf_queue, uint32_t *sdf_count, BatchView bv) {
     const uint32_t count = queue ? *count_ptr : fp.n_slots;
-    for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < count; q += gridDim.x * BLOCK) {
-        const uint32_t slot = queue ? (uint32_t)queue[q] : q;
-        int32_t first = 0, n_own = (int32_t)n_sdf;
-        if (BATCH) {
-            const RequestParams &R = bv.reqs[bv.slot_req[slot]];
-            first = R.first_sdf;
-            n_own = R.n_sdfs;
-        }
-        float bt = pb.hit_t[slot];
-        int32_t rank = pb.hit_rank ? pb.hit_rank[slot] : 0;
-        uint32_t ref = pb.hit_ref[slot], aux = pb.hit_aux[slot];
+    __shared__ uint32_t staged[BLOCK * MERGE_STRIP];
+    uint32_t *mine = staged + (threadIdx.x / 64) * 64 * MERGE_STRIP;
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t n_waves = gridDim.x * (BLOCK / 64);
+    const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64;
+    for (uint32_t base = wave * 64 * MERGE_STRIP; base < count; base += n_waves * 64 * MERGE_STRIP) {
+      uint32_t n_kept = 0;
+      for (uint32_t jx = 0; jx < MERGE_STRIP; ++jx) {
+        const uint32_t q = base + jx * 64 + lane;
         bool changed = false;
-        for (int32_t jj = 0; jj < n_own; ++jj) {
-            const float v = mt[(size_t)jj * count + q];
-            const float t = fabsf(v);
-            if (t < bt) {
-                const int32_t j = first + jj;
-                bt = t;
-                rank = -2 - sv.sdfs[j].list_pos;
-                ref = make_ref(ITEM_SDF, j);
-                aux = (signbit(v) ? HIT_AUX_BACK : 0u) |
-                      ((uint32_t)__float_as_int(mt[mt_stride + (size_t)jj * count + q]) << HIT_AUX_SIDE_SHIFT);
-                changed = true;
+        uint32_t slot = 0;
+        if (q < count) {
+            slot = queue ? (uint32_t)queue[q] : q;
+            int32_t first = 0, n_own = (int32_t)n_sdf;
+            if (BATCH) {
+                const RequestParams &R = bv.reqs[bv.slot_req[slot]];
+                first = R.first_sdf;
+                n_own = R.n_sdfs;
+            }
+            float bt = pb.hit_t[slot];
+            int32_t rank = pb.hit_rank ? pb.hit_rank[slot] : 0;
+            uint32_t ref = pb.hit_ref[slot], aux = pb.hit_aux[slot];
+            for (int32_t jj = 0; jj < n_own; ++jj) {
+                const float v = mt[(size_t)jj * count + q];
+                const float t = fabsf(v);
+                if (t < bt) {
+                    const int32_t j = first + jj;
+                    bt = t;
+                    rank = -2 - sv.sdfs[j].list_pos;
+                    ref = make_ref(ITEM_SDF, j);
+                    aux = (signbit(v) ? HIT_AUX_BACK : 0u) |
+                          ((uint32_t)__float_as_int(mt[mt_stride + (size_t)jj * count + q]) << HIT_AUX_SIDE_SHIFT);
+                    changed = true;
+                }
+            }
+            if (changed) {
+                pb.hit_t[slot] = bt;
+                pb.hit_ref[slot] = ref;
+                pb.hit_aux[slot] = aux;
+                if (pb.hit_rank) pb.hit_rank[slot] = rank;
             }
         }
-        if (changed) {
-            pb.hit_t[slot] = bt;
-            pb.hit_ref[slot] = ref;
-            pb.hit_aux[slot] = aux;
-            if (pb.hit_rank) pb.hit_rank[slot] = rank;
-        }
+        const unsigned long long mask = __ballot(changed);
+        if (changed) mine[n_kept + (uint32_t)__popcll(mask & below)] = slot;
+        n_kept += (uint32_t)__popcll(mask);
+      }
+      if (n_kept != 0u) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t wbase = 0;
+        if (lane == 0u) wbase = atomicAdd(sdf_count, n_kept);
+        wbase = __shfl(wbase, 0);
+        for (uint32_t i = lane; i < n_kept; i += 64) sdf_queue[wbase + i] = (int32_t)mine[i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+}
+
+// The shading normal of every path whose closest hit is an unbounded object: sdf_normal (six estimates) at the
+// hit point, computed as hit_geometry would compute it, into the path's sdf_n fields.  A dense pass over the
+// queue k_march_merge built, so that the estimates run on full waves instead of on the few lanes of each shade
+// wave whose path hit an SDF.
+__global__ __launch_bounds__(BLOCK) void k_sdf_normal(SceneView sv, PathBuffers pb, const int32_t *sdf_queue,
+                                                      const uint32_t *sdf_count) {
+    const uint32_t n = *sdf_count;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const uint32_t slot = (uint32_t)sdf_queue[i];
+        const DRay ray = ray_lazy(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+        const uint32_t ref = pb.hit_ref[slot];
+        const f3 nrm = sdf_normal(sv, sv.sdfs[ref & HIT_INDEX_MASK], at(ray, pb.hit_t[slot]));
+        pb.sdf_nx[slot] = nrm.x;
+        pb.sdf_ny[slot] = nrm.y;
+        pb.sdf_nz[slot] = nrm.z;
     }
 }

@@ -72,6 +72,10 @@ struct rayna_renderer {
     int grid_extend = 1024, grid_shade = 1024, grid_march = 1024;
     int cus = 1;
     float *mt = nullptr;  // unbounded-object march outcomes, slot_capacity * n_sdfs
+    void *sdf_buf = nullptr;  // scenes with unbounded objects: SDF-hit normals (three planes), their queue, a counter
+    uint64_t sdf_cap = 0;
+    int32_t *sdf_queue = nullptr;
+    uint32_t *sdf_count = nullptr;
     uint64_t mt_cap = 0;
     uint64_t chunk_target = TARGET_SLOTS;
     ExtendTuning tune;
@@ -185,15 +189,37 @@ void free_paths(rayna_renderer *r) {
     r->queue_count_len = 0;
 }
 
-// The march outcome buffer: two planes (outcome, march iteration) of one float per (slot, unbounded object).
+// The march outcome buffer: two planes (outcome, march iteration) of one float per (slot, unbounded object);
+// and for scenes with unbounded objects the SDF-hit normals (three planes of one float per slot), the queue of
+// SDF-hit slots and its counter.  Sets the path buffers' normal planes (free_paths clears them).
 rayna_status ensure_mt(rayna_renderer *r) {
     const uint64_t need = r->slot_capacity * (uint64_t)std::max(0, r->dev.view.n_sdfs);
-    if (need <= r->mt_cap) return RAYNA_OK;
-    if (r->mt) (void)hipFree(r->mt);
-    r->mt = nullptr;
-    r->mt_cap = 0;
-    HIP_TRY(r, hipMalloc((void **)&r->mt, 2 * need * sizeof(float)));
-    r->mt_cap = need;
+    if (need > r->mt_cap) {
+        if (r->mt) (void)hipFree(r->mt);
+        r->mt = nullptr;
+        r->mt_cap = 0;
+        HIP_TRY(r, hipMalloc((void **)&r->mt, 2 * need * sizeof(float)));
+        r->mt_cap = need;
+    }
+    const uint64_t cap = r->slot_capacity;
+    if (r->dev.view.n_sdfs > 0 && cap > r->sdf_cap) {
+        if (r->sdf_buf) (void)hipFree(r->sdf_buf);
+        r->sdf_buf = nullptr;
+        r->sdf_cap = 0;
+        HIP_TRY(r, hipMalloc(&r->sdf_buf, cap * (3 * sizeof(float) + sizeof(int32_t)) + sizeof(uint32_t)));
+        r->sdf_cap = cap;
+    }
+    if (r->dev.view.n_sdfs > 0) {
+        float *f = static_cast<float *>(r->sdf_buf);
+        const uint64_t c = r->sdf_cap;
+        r->pb.sdf_nx = f;
+        r->pb.sdf_ny = f + c;
+        r->pb.sdf_nz = f + 2 * c;
+        r->sdf_queue = reinterpret_cast<int32_t *>(f + 3 * c);
+        r->sdf_count = reinterpret_cast<uint32_t *>(r->sdf_queue + c);
+    } else {
+        r->pb.sdf_nx = r->pb.sdf_ny = r->pb.sdf_nz = nullptr;
+    }
     return RAYNA_OK;
 }
 
@@ -311,7 +337,8 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
         launch_extend(r->dev.view, fp, r->pb, qin, cin, depth, r->grid_extend, r->dev.overflow, work, r->tune,
                       BatchView{}, r->stream);
         launch_march(r->dev.view, fp, r->pb, qin, cin, (uint32_t)r->dev.view.n_sdfs, r->mt, r->mt_cap,
-                     r->queue_count + 2 * D2 + depth, r->grid_march, r->grid_shade, r->tune, BatchView{}, r->stream);
+                     r->queue_count + 2 * D2 + depth, r->grid_march, r->grid_shade, r->tune, BatchView{},
+                     r->sdf_queue, r->sdf_count, r->stream);
         if (fp.mode != RAYNA_MODE_PBR) {  // render_px_once: the camera ray's hit coloured by the mode
             if (ev) (void)hipEventRecord((*ev)[(*evi)++], r->stream);
             launch_shade_mode(r->dev.view, fp, r->pb, r->grid_shade, r->stream);
@@ -708,7 +735,7 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
     launch_extend(r->dev.view, fp, pb, nullptr, nullptr, 0, r->grid_extend, r->dev.overflow, claims, r->tune,
                   BatchView{}, r->stream);
     launch_march(r->dev.view, fp, pb, nullptr, nullptr, (uint32_t)r->dev.view.n_sdfs, r->mt, r->mt_cap, work + 1,
-                 r->grid_march, r->grid_shade, r->tune, BatchView{}, r->stream);
+                 r->grid_march, r->grid_shade, r->tune, BatchView{}, r->sdf_queue, r->sdf_count, r->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(prim_rank, rank_dev, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(t, pb.hit_t, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
@@ -764,6 +791,7 @@ void rayna_destroy(rayna_renderer *r) {
     if (r->grad) (void)hipFree(r->grad);
     if (r->grad_sum) (void)hipFree(r->grad_sum);
     if (r->mt) (void)hipFree(r->mt);
+    if (r->sdf_buf) (void)hipFree(r->sdf_buf);
     if (r->rgba_dev) (void)hipFree(r->rgba_dev);
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if (r->stream) (void)hipStreamDestroy(r->stream);
