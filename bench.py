@@ -31,14 +31,14 @@ EXTEND_BYTES_PER_SEGMENT = 40
 FRAME_BYTES_PER_SEGMENT = 40 + 116
 FRAME_BYTES_PER_SAMPLE = 24 + 12
 # HBM traffic of the extend kernel per closest-hit query on the 1080p triangle-mesh frame, from rocprofv3
-# counters over one frame (9 launches, 263,045,425 queries): FETCH_SIZE 2.0191e7 KiB read as 64-byte units
+# counters over one frame (9 launches, 263,045,425 queries): FETCH_SIZE 2.0251e7 KiB read as 64-byte units
 # (x2.0: a 256 MiB copy reads 131,088 KiB) plus WRITE_SIZE 6.809e6 KiB (x1.0: the same copy writes 262,144 KiB).
 # Node and triangle fetches that miss L2 make up the difference to the 40 algorithmic bytes.
-EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (2.0191e7 * 2.0 + 6.809e6) * 1024 / 263045425}
+EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (2.0251e7 * 2.0 + 6.809e6) * 1024 / 263045425}
 # What bounds the extend kernel instead, from the same frame's counters (VALUBusy and VALUUtilization passes,
-# time-weighted over the 9 launches): the VALU issues every cycle and 37.9% of the lanes it issues for are active.
-EXTEND_ISSUE = {"triangle_mesh": {"valu_busy_pct": 104.0, "valu_lanes_active_pct": 37.9,
-                                  "achieved_hbm_gbps": (2.0191e7 * 2.0 + 6.809e6) * 1024 / 9 / 6.2463e-3 / 1e9}}
+# time-weighted over the 9 launches): the VALU issues every cycle and 38.0% of the lanes it issues for are active.
+EXTEND_ISSUE = {"triangle_mesh": {"valu_busy_pct": 103.8, "valu_lanes_active_pct": 38.0,
+                                  "achieved_hbm_gbps": (2.0251e7 * 2.0 + 6.809e6) * 1024 / 9 / 6.3051e-3 / 1e9}}
 
 
 class _NoGc:
