@@ -3,9 +3,10 @@
 //
 // Work item = (path, unbounded object), SDF-major (item = j * count + q), so the items a wave takes mostly
 // belong to one SDF.  A lane runs its march as a state machine: each pass of the persistent loop either starts
-// a march step (cut and escape tests; a sphere or smooth-min estimate is evaluated at once) or performs one
-// iteration of the Mandelbulb estimate, so every lane of a wave executes the same iteration body whatever its
-// march length or iteration count.  A lane whose march ends takes the next item of its wave's claimed batch in
+// a march step (cut and escape tests; a sphere or smooth-min estimate is evaluated at once) or performs
+// iterations of the Mandelbulb estimate (one, then more while at least MARCH_ITERATE_LANES lanes of the wave
+// still iterate), so every lane of a wave executes the same iteration body whatever its march length or
+// iteration count.  A lane whose march ends takes the next item of its wave's claimed batch in
 // the next pass.  The operations of each march are exactly those of RaymarchedIsosurfaceMesh::intersect
 // (raymarched.rs:228-277) and sdf_eval, in the same order, so the outcome is the same bit for bit: the march
 // starts at total = T_MIN; each step evaluates the estimate at the ray point of `total`, advances total by its
@@ -21,6 +22,7 @@
 // hit wins.
 
 constexpr int MARCH_BLOCK = 256;
+constexpr uint32_t MARCH_ITERATE_LANES = 32;
 
 template <bool BATCH>
 __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams fp, PathBuffers pb,
@@ -124,24 +126,31 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
                 have_dist = true;
             }
         }
-        if (in_eval) {  // one iteration of the Mandelbulb estimate (sdf_eval's loop)
-            bool done = it >= f.bulb_iterations;
-            if (!done) {
-                r = length(z);
-                if (r > f.bailout) {
-                    done = true;
-                } else {
-                    const float theta = dm::acos_f(z.z / r);
-                    const float phi = dm::atan2_f(z.y, z.x);
-                    float rp1, zr;
-                    bulb_powers(r, f.power, &rp1, &zr);
-                    dr = rp1 * (float)f.power * dr + 1.0f;
-                    float st, ct, sp, cp;
-                    dm::sincos_rad(theta * (float)f.power, &st, &ct);
-                    dm::sincos_rad(phi * (float)f.power, &sp, &cp);
-                    z = f3{st * cp, sp * st, ct} * zr + p0;
-                    done = ++it >= f.bulb_iterations;
+        if (in_eval) {  // iterations of the Mandelbulb estimate (sdf_eval's loop)
+            bool done;
+            for (;;) {
+                done = it >= f.bulb_iterations;
+                if (!done) {
+                    r = length(z);
+                    if (r > f.bailout) {
+                        done = true;
+                    } else {
+                        const float theta = dm::acos_f(z.z / r);
+                        const float phi = dm::atan2_f(z.y, z.x);
+                        float rp1, zr;
+                        bulb_powers(r, f.power, &rp1, &zr);
+                        dr = rp1 * (float)f.power * dr + 1.0f;
+                        float st, ct, sp, cp;
+                        dm::sincos_rad(theta * (float)f.power, &st, &ct);
+                        dm::sincos_rad(phi * (float)f.power, &sp, &cp);
+                        z = f3{st * cp, sp * st, ct} * zr + p0;
+                        done = ++it >= f.bulb_iterations;
+                    }
                 }
+                if (done) break;
+                // the lanes whose estimate goes on iterate again at once while at least MARCH_ITERATE_LANES
+                // of the wave do; the rest of the pass (step starts and ends) waits for fewer of them
+                if ((uint32_t)__popcll(__ballot(1)) < MARCH_ITERATE_LANES) break;
             }
             if (done) {
                 dist = 0.5f * dm::log_f(r) * r / dr;
