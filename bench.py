@@ -30,15 +30,17 @@ EXTEND_BYTES_PER_SEGMENT = 40
 # reads 12 B.
 FRAME_BYTES_PER_SEGMENT = 40 + 116
 FRAME_BYTES_PER_SAMPLE = 24 + 12
-# HBM traffic of the extend kernel per closest-hit query on the 1080p triangle-mesh frame, from rocprofv3
-# counters over one frame (9 launches, 263,045,425 queries): FETCH_SIZE 2.0251e7 KiB read as 64-byte units
-# (x2.0: a 256 MiB copy reads 131,088 KiB) plus WRITE_SIZE 6.809e6 KiB (x1.0: the same copy writes 262,144 KiB).
-# Node and triangle fetches that miss L2 make up the difference to the 40 algorithmic bytes.
-EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (2.0251e7 * 2.0 + 6.809e6) * 1024 / 263045425}
-# What bounds the extend kernel instead, from the same frame's counters (VALUBusy and VALUUtilization passes,
-# time-weighted over the 9 launches): the VALU issues every cycle and 38.0% of the lanes it issues for are active.
-EXTEND_ISSUE = {"triangle_mesh": {"valu_busy_pct": 103.8, "valu_lanes_active_pct": 38.0,
-                                  "achieved_hbm_gbps": (2.0251e7 * 2.0 + 6.809e6) * 1024 / 9 / 6.3051e-3 / 1e9}}
+# HBM traffic of the extend stage per closest-hit query on the 1080p triangle-mesh frame, from rocprofv3
+# counters over one frame (9 launches -- k_extend_packet for the camera rays, k_extend for depths 1-8 --
+# 263,044,862 queries): FETCH_SIZE 2.0200e7 KiB read as 64-byte units (x2.0: a 256 MiB copy reads 131,088 KiB)
+# plus WRITE_SIZE 6.4708e6 KiB (x1.0: the same copy writes 262,144 KiB).  Node and triangle fetches that miss L2
+# make up the difference to the 40 algorithmic bytes.
+EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (2.0200e7 * 2.0 + 6.4708e6) * 1024 / 263044862}
+# What bounds the extend stage instead, from the same frame's counters (VALUBusy and VALUUtilization passes,
+# time-weighted over the 9 launches, 49.997 ms in all): the VALU issues every cycle and 41.8% of the lanes it
+# issues for are active (camera-ray packets 89.7%, the later depths 30.5%).
+EXTEND_ISSUE = {"triangle_mesh": {"valu_busy_pct": 102.7, "valu_lanes_active_pct": 41.8,
+                                  "achieved_hbm_gbps": (2.0200e7 * 2.0 + 6.4708e6) * 1024 / 9 / 5.5552e-3 / 1e9}}
 
 
 class _NoGc:
@@ -331,7 +333,8 @@ def main(argv=None):
         "scene_build_s": t_build,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": "k_extend", "bytes_per_segment": EXTEND_BYTES_PER_SEGMENT},
+                     "kernel": "extend: k_extend_packet (camera rays) + k_extend (later depths)",
+                     "bytes_per_segment": EXTEND_BYTES_PER_SEGMENT},
     }
     if a.workload in EXTEND_ISSUE:  # profiled, not measured by this run
         line["extend_issue_profiled"] = EXTEND_ISSUE[a.workload]
