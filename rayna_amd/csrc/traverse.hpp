@@ -427,7 +427,10 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
 // its own traversal would and, culling being conservative, its closest hit is the same.  Scenes with
 // instances, and trees that need more than PACKET_STACK entries, take k_extend.
 constexpr int PACKET_STACK = 64;
-constexpr uint32_t PACKET_MIN_SPP = 16;  // fewer samples per pixel spread a wave's lanes over several pixels
+#ifndef RAYNA_PACKET_MIN_SPP
+#define RAYNA_PACKET_MIN_SPP 16
+#endif
+constexpr uint32_t PACKET_MIN_SPP = RAYNA_PACKET_MIN_SPP;  // fewer samples per pixel spread a wave over pixels
 
 // Least value over the 64 lanes (every lane active): within rows of 16 by DPP, then over the four rows.
 __device__ __forceinline__ float wave_min(float v) {
