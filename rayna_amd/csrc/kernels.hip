@@ -1109,9 +1109,14 @@ void launch_generate(const FrameParams &fp, const PathBuffers &pb, const BatchVi
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
                    const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work,
                    const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
-    if (!bv.reqs && !queue && depth == 0 && !sv.instances && sv.stack_need <= PACKET_STACK &&
-        fp.n_spp >= PACKET_MIN_SPP) {  // camera rays of one request: wave packets
-        hipLaunchKernelGGL(k_extend_packet, dim3(grid_for(fp.n_slots, EXT_BLOCK)), dim3(EXT_BLOCK), 0, s, sv, fp, pb);
+    if (!queue && depth == 0 && !sv.instances && sv.stack_need <= PACKET_STACK &&
+        (bv.reqs || fp.n_spp >= PACKET_MIN_SPP)) {  // camera rays: wave packets
+        if (bv.reqs)
+            hipLaunchKernelGGL(k_extend_packet<true>, dim3(grid_for(fp.n_slots, EXT_BLOCK)), dim3(EXT_BLOCK), 0, s, sv,
+                               fp, pb, bv);
+        else
+            hipLaunchKernelGGL(k_extend_packet<false>, dim3(grid_for(fp.n_slots, EXT_BLOCK)), dim3(EXT_BLOCK), 0, s,
+                               sv, fp, pb, bv);
         return;
     }
     auto kernel = bv.reqs ? (sv.instances ? k_extend<true, true> : k_extend<true, false>)

@@ -424,8 +424,10 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
 // code, the lanes that take it and the least entry distance among those lanes; node and primitive addresses are
 // wave-uniform.  Each lane still tests every box against its own ray and its own closest hit, and every
 // primitive of a visited leaf range against its own ray (leaf_test), so a lane visits a superset of the nodes
-// its own traversal would and, culling being conservative, its closest hit is the same.  Scenes with
-// instances, and trees that need more than PACKET_STACK entries, take k_extend.
+// its own traversal would and, culling being conservative, its closest hit is the same.  In a batch (BATCH) a
+// wave's lanes can belong to requests on different scenes: the wave walks once per scene root among its lanes,
+// each walk with the lanes of that root.  Scenes with instances, and trees that need more than PACKET_STACK
+// entries, take k_extend.
 constexpr int PACKET_STACK = 64;
 #ifndef RAYNA_PACKET_MIN_SPP
 #define RAYNA_PACKET_MIN_SPP 16
@@ -444,7 +446,9 @@ __device__ __forceinline__ float wave_min(float v) {
     return fminf(fminf(r0, r1), fminf(r2, r3));
 }
 
-__global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, FrameParams fp, PathBuffers pb) {
+template <bool BATCH>
+__global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, FrameParams fp, PathBuffers pb,
+                                                             BatchView bv) {
     __shared__ int32_t s_code[EXT_BLOCK / 64][PACKET_STACK];
     __shared__ float s_key[EXT_BLOCK / 64][PACKET_STACK];
     __shared__ unsigned long long s_mask[EXT_BLOCK / 64][PACKET_STACK];
@@ -453,8 +457,20 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
     const bool valid = slot < fp.n_slots;
     const uint32_t ls = valid ? slot : 0u;
     DRay ray = ray_from_dir(f3{pb.ox[ls], pb.oy[ls], pb.oz[ls]}, f3{pb.dx[ls], pb.dy[ls], pb.dz[ls]});
-    const uint32_t pixel = ls / fp.n_spp, sample = fp.s0 + ls % fp.n_spp;
-    const dm::Key key{fp.key0, fp.key1};
+    uint32_t pixel, sample;
+    dm::Key key{fp.key0, fp.key1};
+    int32_t root = sv.root;
+    if (BATCH) {
+        const RequestParams &R = bv.reqs[bv.slot_req[ls]];
+        const uint32_t l = ls - R.slot_base;
+        pixel = l / R.n_spp;
+        sample = R.s0 + l % R.n_spp;
+        key = dm::Key{R.key0, R.key1};
+        root = R.root;
+    } else {
+        pixel = ls / fp.n_spp;
+        sample = fp.s0 + ls % fp.n_spp;
+    }
     Best b{FLT_MAX, -1, 0u, 0u, 0u};
     LaneStack unused{nullptr, nullptr, 0, 0};  // leaf_test<false> pushes only for mesh-BVH links, handled here
     int sp = 0;
@@ -464,76 +480,86 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
         s_mask[w][sp] = m;
         ++sp;
     };
-    const unsigned long long lanes = __ballot(valid);
-    if (sv.root >= 0 && lanes != 0ull) push(sv.root, -__builtin_inff(), lanes);
-    while (sp > 0) {
-        --sp;
-        const int32_t code = __builtin_amdgcn_readfirstlane(s_code[w][sp]);
-        const float kt = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s_key[w][sp])));
-        const unsigned long long m = s_mask[w][sp];
-        const bool take = ((m >> lane) & 1ull) != 0ull && kt <= b.t * BOX_SLACK;
-        const unsigned long long tm = __ballot(take);
-        if (tm == 0ull) continue;
-        if (code >= 0) {  // inner node: each child the lanes' boxes hit, far to near
-            const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + code);
-            const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
-            const int4 cc = reinterpret_cast<const int4 *>(np)[7];
-            const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, lo_y[4] = {ly.x, ly.y, ly.z, ly.w},
-                        lo_z[4] = {lz.x, lz.y, lz.z, lz.w};
-            const float hi_x[4] = {hx.x, hx.y, hx.z, hx.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w},
-                        hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
-            int32_t ccode[4] = {cc.x, cc.y, cc.z, cc.w};
-            float ckey[4];
-            unsigned long long cmask[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float tx1 = (lo_x[k] - ray.o.x) * ray.inv.x, tx2 = (hi_x[k] - ray.o.x) * ray.inv.x;
-                float t0 = fminf(tx1, tx2), t1 = fmaxf(tx1, tx2);
-                const float ty1 = (lo_y[k] - ray.o.y) * ray.inv.y, ty2 = (hi_y[k] - ray.o.y) * ray.inv.y;
-                t0 = fmaxf(t0, fminf(ty1, ty2));
-                t1 = fminf(t1, fmaxf(ty1, ty2));
-                const float tz1 = (lo_z[k] - ray.o.z) * ray.inv.z, tz2 = (hi_z[k] - ray.o.z) * ray.inv.z;
-                t0 = fmaxf(t0, fminf(tz1, tz2));
-                t1 = fminf(t1, fmaxf(tz1, tz2));
-                const float lo = fmaxf(t0, T_MIN), hi = fminf(t1, b.t);
-                const bool hit = take && ccode[k] != ENTRY_NONE && lo <= hi * BOX_SLACK;
-                cmask[k] = __ballot(hit);
-                ckey[k] = cmask[k] != 0ull ? wave_min(hit ? t0 : __builtin_inff()) : -__builtin_inff();
-            }
-#define RAYNA_PSWAP(a, b)                          \
-    if (ckey[a] < ckey[b]) {                       \
-        const float tt_ = ckey[a];                 \
-        ckey[a] = ckey[b];                         \
-        ckey[b] = tt_;                             \
-        const int32_t cc_ = ccode[a];              \
-        ccode[a] = ccode[b];                       \
-        ccode[b] = cc_;                            \
-        const unsigned long long mm_ = cmask[a];   \
-        cmask[a] = cmask[b];                       \
-        cmask[b] = mm_;                            \
-    }
-            RAYNA_PSWAP(0, 1)
-            RAYNA_PSWAP(2, 3)
-            RAYNA_PSWAP(0, 2)
-            RAYNA_PSWAP(1, 3)
-            RAYNA_PSWAP(1, 2)
-#undef RAYNA_PSWAP
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (cmask[k] != 0ull) push(ccode[k], ckey[k], cmask[k]);
-            continue;
+    unsigned long long todo = __ballot(valid);
+    while (todo != 0ull) {  // one walk per scene root among the wave's lanes (a single one outside batches)
+        int32_t groot = sv.root;
+        unsigned long long group = todo;
+        if (BATCH) {
+            const int leader = __ffsll((long long)todo) - 1;
+            groot = __builtin_amdgcn_readlane(root, leader);
+            group = __ballot(((todo >> lane) & 1ull) != 0ull && root == groot);
         }
-        // leaf range: mesh-BVH links go on the wave's stack for the lanes that took the range; the other
-        // entries are tested by those lanes
-        const int n = (int)(((uint32_t)code >> 27) & 7u) + 1;
-        const int32_t start = (int32_t)((uint32_t)code & LEAF_CODE_START);
-        const bool items = ((uint32_t)code & LEAF_CODE_ITEMS) != 0u;
-        for (int i = 0; i < n; ++i) {
-            if (items && sv.items[start + i].kind == ITEM_LINK) {
-                push(sv.items[start + i].index, kt, tm);
+        todo &= ~group;
+        if (groot >= 0) push(groot, -__builtin_inff(), group);
+        while (sp > 0) {
+            --sp;
+            const int32_t code = __builtin_amdgcn_readfirstlane(s_code[w][sp]);
+            const float kt = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s_key[w][sp])));
+            const unsigned long long m = s_mask[w][sp];
+            const bool take = ((m >> lane) & 1ull) != 0ull && kt <= b.t * BOX_SLACK;
+            const unsigned long long tm = __ballot(take);
+            if (tm == 0ull) continue;
+            if (code >= 0) {  // inner node: each child the lanes' boxes hit, far to near
+                const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + code);
+                const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+                const int4 cc = reinterpret_cast<const int4 *>(np)[7];
+                const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, lo_y[4] = {ly.x, ly.y, ly.z, ly.w},
+                            lo_z[4] = {lz.x, lz.y, lz.z, lz.w};
+                const float hi_x[4] = {hx.x, hx.y, hx.z, hx.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w},
+                            hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+                int32_t ccode[4] = {cc.x, cc.y, cc.z, cc.w};
+                float ckey[4];
+                unsigned long long cmask[4];
+    #pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float tx1 = (lo_x[k] - ray.o.x) * ray.inv.x, tx2 = (hi_x[k] - ray.o.x) * ray.inv.x;
+                    float t0 = fminf(tx1, tx2), t1 = fmaxf(tx1, tx2);
+                    const float ty1 = (lo_y[k] - ray.o.y) * ray.inv.y, ty2 = (hi_y[k] - ray.o.y) * ray.inv.y;
+                    t0 = fmaxf(t0, fminf(ty1, ty2));
+                    t1 = fminf(t1, fmaxf(ty1, ty2));
+                    const float tz1 = (lo_z[k] - ray.o.z) * ray.inv.z, tz2 = (hi_z[k] - ray.o.z) * ray.inv.z;
+                    t0 = fmaxf(t0, fminf(tz1, tz2));
+                    t1 = fminf(t1, fmaxf(tz1, tz2));
+                    const float lo = fmaxf(t0, T_MIN), hi = fminf(t1, b.t);
+                    const bool hit = take && ccode[k] != ENTRY_NONE && lo <= hi * BOX_SLACK;
+                    cmask[k] = __ballot(hit);
+                    ckey[k] = cmask[k] != 0ull ? wave_min(hit ? t0 : __builtin_inff()) : -__builtin_inff();
+                }
+    #define RAYNA_PSWAP(a, b)                          \
+        if (ckey[a] < ckey[b]) {                       \
+            const float tt_ = ckey[a];                 \
+            ckey[a] = ckey[b];                         \
+            ckey[b] = tt_;                             \
+            const int32_t cc_ = ccode[a];              \
+            ccode[a] = ccode[b];                       \
+            ccode[b] = cc_;                            \
+            const unsigned long long mm_ = cmask[a];   \
+            cmask[a] = cmask[b];                       \
+            cmask[b] = mm_;                            \
+        }
+                RAYNA_PSWAP(0, 1)
+                RAYNA_PSWAP(2, 3)
+                RAYNA_PSWAP(0, 2)
+                RAYNA_PSWAP(1, 3)
+                RAYNA_PSWAP(1, 2)
+    #undef RAYNA_PSWAP
+    #pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (cmask[k] != 0ull) push(ccode[k], ckey[k], cmask[k]);
                 continue;
             }
-            if (take) (void)leaf_test<false>(sv, ray, code, i, kt, b, unused, key, pixel, sample, 0u);
+            // leaf range: mesh-BVH links go on the wave's stack for the lanes that took the range; the other
+            // entries are tested by those lanes
+            const int n = (int)(((uint32_t)code >> 27) & 7u) + 1;
+            const int32_t start = (int32_t)((uint32_t)code & LEAF_CODE_START);
+            const bool items = ((uint32_t)code & LEAF_CODE_ITEMS) != 0u;
+            for (int i = 0; i < n; ++i) {
+                if (items && sv.items[start + i].kind == ITEM_LINK) {
+                    push(sv.items[start + i].index, kt, tm);
+                    continue;
+                }
+                if (take) (void)leaf_test<false>(sv, ray, code, i, kt, b, unused, key, pixel, sample, 0u);
+            }
         }
     }
     if (valid) {
