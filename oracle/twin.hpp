@@ -951,6 +951,54 @@ inline R ipow(R x, int n) {  // binary powering, the order of compiler-rt's __po
     return r;
 }
 
+// A complex number, for the Mandelbulb's angle multiples: (cos a + i sin a)^n = cos na + i sin na.
+template <class R>
+struct Cx {
+    R re, im;
+};
+template <class R>
+inline Cx<R> cmul(const Cx<R> &a, const Cx<R> &b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// a^n (n >= 1) by binary powering; the lowest factor is taken as is instead of being multiplied into 1.
+template <class R>
+inline Cx<R> cpow(Cx<R> a, int n) {
+    Cx<R> r{R(1), R(0)};
+    bool have = false;
+    while (n > 0) {
+        if (n & 1) {
+            r = have ? cmul(r, a) : a;
+            have = true;
+        }
+        n >>= 1;
+        if (n) a = cmul(a, a);
+    }
+    return r;
+}
+
+// One step z <- z^n + p0 of the power-n Mandelbulb (White and Nylander's spherical power: radius r^n, polar
+// angle n theta, azimuth n phi, with theta = acos(z.z / r) and phi = atan2(z.y, z.x)), written without
+// trigonometry: with rho = |(z.x, z.y)|, u = (z.z + i rho) / r = e^{i theta} and w = (z.x + i z.y) / rho =
+// e^{i phi}, the spherical power is r^n (Im u^n Re w^n, Im u^n Im w^n, Re u^n).  On the z axis (rho = 0) w is
+// taken as 1, and at r = 0 u is taken as 1: both factors then meet a zero of the other (sin n theta = 0 on the
+// axis; r^n = 0 at the origin), so the step is the limit of the spherical formula there.  `rho2` is
+// z.x^2 + z.y^2 as |z| summed it.
+template <class R>
+inline V3<R> bulb_step(const V3<R> &z, R rho2, R r, R zr, int power, const V3<R> &p0) {
+    const R rho = Math<R>::sqrt(rho2);
+    Cx<R> u{R(1), R(0)}, w{R(1), R(0)};
+    if (r > R(0)) {
+        const R ir = R(1) / r;
+        u = Cx<R>{z.z * ir, rho * ir};
+    }
+    if (rho > R(0)) {
+        const R iq = R(1) / rho;
+        w = Cx<R>{z.x * iq, z.y * iq};
+    }
+    const Cx<R> un = cpow(u, power), wn = cpow(w, power);
+    return V3<R>{un.im * wn.re, un.im * wn.im, un.re} * zr + p0;
+}
+
 template <class R>
 inline R sdf_eval(const Scene &s, const rayna_sdf &f, const V3<R> &p) {
     switch (f.kind) {
@@ -970,7 +1018,7 @@ inline R sdf_eval(const Scene &s, const rayna_sdf &f, const V3<R> &p) {
             }
             return d;
         }
-        default: {  // RAYNA_SDF_MANDELBULB
+        default: {  // RAYNA_SDF_MANDELBULB: the distance estimate 0.5 ln(r) r / dr of the escape-time iteration
             const int power = (int)f.power;
             const R bail = (R)f.bailout;
             const V3<R> c = cast3<R>(f.centre);
@@ -978,18 +1026,14 @@ inline R sdf_eval(const Scene &s, const rayna_sdf &f, const V3<R> &p) {
             V3<R> z = p0;
             R dr = R(1), r = R(0);
             for (int i = 0; i < f.bulb_iterations; ++i) {
-                r = length(z);
+                const R rho2 = z.x * z.x + z.y * z.y;
+                r = Math<R>::sqrt(rho2 + z.z * z.z);  // length(z), in its summation order
                 if (r > bail) break;
-                const R theta = Math<R>::acos(z.z / r);
-                const R phi = Math<R>::atan2(z.y, z.x);
                 dr = ipow(r, power - 1) * (R)power * dr + R(1);
-                const R zr = ipow(r, power);
-                R st, ct, sp, cp;
-                Math<R>::sincos(theta * (R)power, &st, &ct);
-                Math<R>::sincos(phi * (R)power, &sp, &cp);
-                z = V3<R>{st * cp, sp * st, ct} * zr + p0;
+                z = bulb_step(z, rho2, r, ipow(r, power), power, p0);
             }
-            return R(0.5) * Math<R>::log(r) * r / dr;
+            // at the centre itself (r = 0) the estimate takes its limit, 0
+            return r == R(0) ? R(0) : R(0.5) * Math<R>::log(r) * r / dr;
         }
     }
 }

@@ -274,6 +274,64 @@ __device__ __forceinline__ void bulb_powers(float x, int n, float *x_nm1, float 
     }
 }
 
+// A complex number, for the Mandelbulb's angle multiples: (cos a + i sin a)^n = cos na + i sin na.
+struct cx {
+    float re, im;
+};
+__device__ __forceinline__ cx cmul(cx a, cx b) { return cx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+// a^n (n >= 1) by binary powering, the lowest factor taken as is; for the usual n = 8 three squarings.
+__device__ __forceinline__ cx cpow(cx a, int n) {
+    if (n == 8) {
+        a = cmul(a, a);
+        a = cmul(a, a);
+        return cmul(a, a);
+    }
+    cx r{1.0f, 0.0f};
+    bool have = false;
+    while (n > 0) {
+        if (n & 1) {
+            r = have ? cmul(r, a) : a;
+            have = true;
+        }
+        n >>= 1;
+        if (n) a = cmul(a, a);
+    }
+    return r;
+}
+
+// One iteration of the Mandelbulb estimate: r = |z|; unless r exceeds the bail-out radius, dr <- n r^(n-1) dr + 1
+// and z <- z^n + p0, the spherical power (radius r^n, polar angle n theta, azimuth n phi) written without
+// trigonometry: with rho = |(z.x, z.y)|, u = (z.z + i rho) / r = e^{i theta} and w = (z.x + i z.y) / rho =
+// e^{i phi}, z^n = r^n (Im u^n Re w^n, Im u^n Im w^n, Re u^n); w is 1 on the z axis and u is 1 at r = 0, where
+// the other factor vanishes.  Returns false when r exceeds the bail-out radius (z and dr unchanged).
+__device__ __forceinline__ bool bulb_iterate(const GSdf &f, f3 p0, f3 &z, float &dr, float &r) {
+    const float rho2 = z.x * z.x + z.y * z.y;
+    r = sqrtf(rho2 + z.z * z.z);  // length(z), in its summation order
+    if (r > f.bailout) return false;
+    float rp1, zr;
+    bulb_powers(r, f.power, &rp1, &zr);
+    dr = rp1 * (float)f.power * dr + 1.0f;
+    const float rho = sqrtf(rho2);
+    cx u{1.0f, 0.0f}, w{1.0f, 0.0f};
+    if (r > 0.0f) {
+        const float ir = 1.0f / r;
+        u = cx{z.z * ir, rho * ir};
+    }
+    if (rho > 0.0f) {
+        const float iq = 1.0f / rho;
+        w = cx{z.x * iq, z.y * iq};
+    }
+    const cx un = cpow(u, f.power), wn = cpow(w, f.power);
+    z = f3{un.im * wn.re, un.im * wn.im, un.re} * zr + p0;
+    return true;
+}
+
+// The estimate 0.5 ln(r) r / dr; at the centre itself (r = 0) its limit, 0.
+__device__ __forceinline__ float bulb_estimate(float r, float dr) {
+    return r == 0.0f ? 0.0f : 0.5f * dm::log_f(r) * r / dr;
+}
+
 __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
     if (f.kind == RAYNA_SDF_SPHERE) {
         const GSphere s = sv.spheres[f.first_sphere];
@@ -294,20 +352,9 @@ __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
     const f3 p0 = p - f3{f.cx, f.cy, f.cz};
     f3 z = p0;
     float dr = 1.0f, r = 0.0f;
-    for (int i = 0; i < f.bulb_iterations; ++i) {
-        r = length(z);
-        if (r > f.bailout) break;
-        const float theta = dm::acos_f(z.z / r);
-        const float phi = dm::atan2_f(z.y, z.x);
-        float rp1, zr;
-        bulb_powers(r, f.power, &rp1, &zr);
-        dr = rp1 * (float)f.power * dr + 1.0f;
-        float st, ct, sp, cp;
-        dm::sincos_rad(theta * (float)f.power, &st, &ct);
-        dm::sincos_rad(phi * (float)f.power, &sp, &cp);
-        z = f3{st * cp, sp * st, ct} * zr + p0;
-    }
-    return 0.5f * dm::log_f(r) * r / dr;
+    for (int i = 0; i < f.bulb_iterations; ++i)
+        if (!bulb_iterate(f, p0, z, dr, r)) break;
+    return bulb_estimate(r, dr);
 }
 
 __device__ f3 sdf_normal(const SceneView &sv, const GSdf &f, f3 p) {

@@ -130,30 +130,14 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
             bool done;
             for (;;) {
                 done = it >= f.bulb_iterations;
-                if (!done) {
-                    r = length(z);
-                    if (r > f.bailout) {
-                        done = true;
-                    } else {
-                        const float theta = dm::acos_f(z.z / r);
-                        const float phi = dm::atan2_f(z.y, z.x);
-                        float rp1, zr;
-                        bulb_powers(r, f.power, &rp1, &zr);
-                        dr = rp1 * (float)f.power * dr + 1.0f;
-                        float st, ct, sp, cp;
-                        dm::sincos_rad(theta * (float)f.power, &st, &ct);
-                        dm::sincos_rad(phi * (float)f.power, &sp, &cp);
-                        z = f3{st * cp, sp * st, ct} * zr + p0;
-                        done = ++it >= f.bulb_iterations;
-                    }
-                }
+                if (!done) done = !bulb_iterate(f, p0, z, dr, r) || ++it >= f.bulb_iterations;
                 if (done) break;
                 // the lanes whose estimate goes on iterate again at once while at least MARCH_ITERATE_LANES
                 // of the wave do; the rest of the pass (step starts and ends) waits for fewer of them
                 if ((uint32_t)__popcll(__ballot(1)) < MARCH_ITERATE_LANES) break;
             }
             if (done) {
-                dist = 0.5f * dm::log_f(r) * r / dr;
+                dist = bulb_estimate(r, dr);
                 in_eval = false;
                 have_dist = true;
             }

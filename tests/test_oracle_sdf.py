@@ -1,0 +1,254 @@
+"""Pins the oracle's signed distance functions (the configuration's power-8 Mandelbulb and smooth-min spheres)
+against what the mathematics fixes, independently of the oracle's own formulas.
+
+The engine takes SDFs as closures (rayna_engine/src/mesh/isosurface/mod.rs:7-9) and ships no Mandelbulb or
+smooth-min one, so no engine fixture covers them; the march around them is RaymarchedIsosurfaceMesh::intersect
+(mesh/isosurface/raymarched.rs:73-122).  The oracle evaluates the Mandelbulb without trigonometry (complex
+powers of e^{i theta} and e^{i phi}); the checks below use the spherical definition with numpy's libm
+trigonometry instead, closed forms on the z axis and beyond the bail-out radius, and the bulb's symmetries, so
+that a wrong angle convention, power, dr update or blend fails one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from rayna_amd import (LambertianMaterial, RaymarchedIsosurfaceMesh, Scene, SdfMandelbulb, SdfSminSpheres,
+                       SimpleObject, WhiteSkybox)
+
+
+def _bulb(iters=10, power=8, centre=(0.0, 0.0, 0.0), bail=2.0, eps=1e-4, max_iter=256):
+    m = RaymarchedIsosurfaceMesh(SdfMandelbulb(centre, power, iters, bail), max_iter, eps)
+    return oracle.TwinScene(Scene([SimpleObject(m, LambertianMaterial())], WhiteSkybox))
+
+
+def _smin(centres, radii, k, eps=1e-9, max_iter=400):
+    m = RaymarchedIsosurfaceMesh(SdfSminSpheres(np.asarray(centres, float), np.asarray(radii, float), k), max_iter,
+                                 eps)
+    return oracle.TwinScene(Scene([SimpleObject(m, LambertianMaterial())], WhiteSkybox))
+
+
+# ------------------------------------------------------------------------------------------ Mandelbulb
+
+
+def bulb_spherical(p, n=8, iters=10, bail=2.0):
+    """The power-n Mandelbulb distance estimate in its spherical definition (White and Nylander's power:
+    r -> r^n, theta -> n theta, phi -> n phi; estimate 0.5 ln(r) r / dr, dr <- n r^(n-1) dr + 1, the radius
+    taken at the start of the last iteration), with libm's acos / atan2 / sin / cos."""
+    p0 = np.asarray(p, float)
+    z = p0.copy()
+    dr, r = 1.0, 0.0
+    for _ in range(iters):
+        r = math.sqrt(float(z @ z))
+        if r > bail:
+            break
+        th, ph = math.acos(z[2] / r), math.atan2(z[1], z[0])
+        dr = r ** (n - 1) * n * dr + 1.0
+        z = r ** n * np.array([math.sin(n * th) * math.cos(n * ph), math.sin(n * ph) * math.sin(n * th),
+                               math.cos(n * th)]) + p0
+    return 0.5 * math.log(r) * r / dr
+
+
+def bulb_axis(t, n=8, iters=10, bail=2.0):
+    """On the z axis the power-n map is the real map x <- x^n + t (for even n; |x|^n carries the sign of
+    cos(n theta) = 1 at theta = 0 and pi) with dr <- n |x|^(n-1) dr + 1."""
+    x, dr, r = t, 1.0, 0.0
+    for _ in range(iters):
+        r = abs(x)
+        if r > bail:
+            break
+        dr = n * r ** (n - 1) * dr + 1.0
+        x = x ** n + t
+    return 0.5 * math.log(r) * r / dr
+
+
+def test_mandelbulb_on_the_z_axis_is_the_real_map():
+    tw = _bulb()
+    ts = np.concatenate([np.linspace(-1.9, 1.9, 191), [-1.0864, 0.6501, 0.65, 1e-3, -1e-3]])
+    ts = ts[ts != 0.0]
+    got = tw.sdf_eval(0, np.stack([np.zeros_like(ts), np.zeros_like(ts), ts], 1))
+    want = np.array([bulb_axis(t) for t in ts])
+    assert np.all(np.abs(got - want) <= 1e-11 * np.abs(want) + 1e-15)
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3, 10])
+def test_mandelbulb_equals_the_spherical_definition(iters):
+    tw = _bulb(iters=iters)
+    P = np.random.default_rng(iters).uniform(-1.3, 1.3, (3000, 3))
+    got = tw.sdf_eval(0, P)
+    want = np.array([bulb_spherical(p, iters=iters) for p in P])
+    rel = np.abs(got - want) / np.abs(want)
+    assert rel.max() <= 1e-8, rel.max()
+
+
+@pytest.mark.parametrize("power", [2, 3, 5, 9])
+def test_other_integer_powers_equal_the_spherical_definition(power):
+    tw = _bulb(iters=4, power=power)
+    P = np.random.default_rng(power).uniform(-1.2, 1.2, (1500, 3))
+    got = tw.sdf_eval(0, P)
+    want = np.array([bulb_spherical(p, n=power, iters=4) for p in P])
+    assert np.max(np.abs(got - want) / np.abs(want)) <= 1e-8
+
+
+def _rot_z(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def test_mandelbulb_symmetries():
+    """z -> z^8 multiplies the azimuth by 8, so a rotation by a about z commutes with the map when 8a = a
+    (mod 2 pi): the power-8 bulb has 7-fold symmetry about z; phi -> -phi commutes too (mirror y -> -y).  A
+    rotation by 2 pi / 8 is not a symmetry, which shows the check can fail."""
+    tw = _bulb()
+    P = np.random.default_rng(7).uniform(-1.2, 1.2, (2000, 3))
+    d = tw.sdf_eval(0, P)
+    for k in (1, 2, 3):
+        dr = tw.sdf_eval(0, P @ _rot_z(2 * math.pi * k / 7).T)
+        assert np.quantile(np.abs(dr - d) / np.abs(d), 0.999) <= 1e-8
+    dm = tw.sdf_eval(0, P * [1.0, -1.0, 1.0])
+    assert np.quantile(np.abs(dm - d) / np.abs(d), 0.999) <= 1e-8
+    d8 = tw.sdf_eval(0, P @ _rot_z(2 * math.pi / 8).T)
+    assert np.median(np.abs(d8 - d) / np.abs(d)) > 1e-3
+
+
+def test_mandelbulb_beyond_the_bailout_radius():
+    """|p - c| > bailout: the iteration stops at once with dr = 1, so the estimate is 0.5 ln r r."""
+    c = np.array([0.25, -0.5, 1.0])
+    tw = _bulb(centre=tuple(c))
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(500, 3))
+    r = rng.uniform(2.0 + 1e-9, 40.0, 500)
+    P = c + v / np.linalg.norm(v, axis=1)[:, None] * r[:, None]
+    got = tw.sdf_eval(0, P)
+    rr = np.linalg.norm(P - c, axis=1)
+    assert np.allclose(got, 0.5 * np.log(rr) * rr, rtol=1e-14, atol=0)
+
+
+def test_mandelbulb_centre_is_the_limit():
+    """At the centre the estimate's limit 0.5 ln(r) r / dr -> 0 is taken (the spherical formula is 0/0 there);
+    nearby the orbit stays at ~p0 and the estimate is ~0.5 ln|p| |p|."""
+    for prec in (64, 32):
+        tw = _bulb(centre=(0.5, 0.0, -0.5))
+        assert tw.sdf_eval(0, [[0.5, 0.0, -0.5]], precision=prec)[0] == 0.0
+    tw = _bulb()
+    p = np.array([[1e-3, -2e-3, 5e-4]])
+    r = np.linalg.norm(p)
+    assert math.isclose(tw.sdf_eval(0, p)[0], 0.5 * math.log(r) * r, rel_tol=1e-6)
+
+
+def test_mandelbulb_single_precision_follows_double():
+    tw = _bulb()
+    P = np.random.default_rng(5).uniform(-1.3, 1.3, (3000, 3)).astype(np.float32).astype(np.float64)
+    d64, d32 = tw.sdf_eval(0, P, precision=64), tw.sdf_eval(0, P, precision=32)
+    rel = np.abs(d32 - d64) / np.abs(d64)
+    assert np.quantile(rel, 0.9) <= 1e-4
+
+
+def _bisect(f, lo, hi, n=200):
+    flo = f(lo)
+    for _ in range(n):
+        mid = 0.5 * (lo + hi)
+        if (f(mid) < 0) == (flo < 0):
+            lo, flo = mid, f(mid)
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def test_mandelbulb_march_down_the_z_axis_meets_the_root():
+    """A ray from (0, 0, 3) down the z axis: the march (raymarched.rs:73-122) stops where the estimate drops
+    below epsilon, just above the root of the real-map estimate, which bisection finds."""
+    eps = 1e-4
+    tw = _bulb(eps=eps)
+    root = _bisect(bulb_axis, 0.5, 1.0)
+    h = tw.trace([0.0, 0.0, 3.0], [0.0, 0.0, -1.0])
+    assert h is not None
+    z_hit = 3.0 - h["t"]
+    slope = (bulb_axis(root + 1e-3) - bulb_axis(root)) / 1e-3
+    assert 0.0 <= z_hit - root <= 2.0 * eps / slope, (z_hit, root, slope)
+    assert 0.0 < bulb_axis(z_hit) < eps
+    assert h["front_face"]
+
+
+# ------------------------------------------------------------------------------------------ smooth minimum
+
+
+def _sphere_d(P, centres, radii):
+    diff = P[:, None, :] - centres[None, :, :]
+    return np.sqrt((diff[..., 0] ** 2 + diff[..., 1] ** 2) + diff[..., 2] ** 2) - radii[None, :]
+
+
+def _blob_set(seed=2, n=16):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.uniform(-2.0, 2.0, (n, 3)), rng.uniform(0.2, 0.6, n)
+
+
+def test_smin_bounds():
+    """Each fold subtracts at most k/4 below the smaller argument (h <= 1): min_i d_i - (n-1) k/4 <= d <=
+    min_i d_i."""
+    centres, radii = _blob_set()
+    k = 0.1
+    tw = _smin(centres, radii, k)
+    P = np.random.default_rng(11).uniform(-3, 3, (20000, 3))
+    d = tw.sdf_eval(0, P)
+    dmin = _sphere_d(P, centres, radii).min(axis=1)
+    assert np.all(d <= dmin + 1e-12)
+    assert np.all(d >= dmin - (len(radii) - 1) * k / 4 - 1e-12)
+    assert np.any(d < dmin - 1e-3)  # the blend does act somewhere
+
+
+def test_smin_far_apart_is_the_plain_minimum():
+    """Where every pair of sphere distances differs by at least k, every fold has h = 0 and the result is
+    the minimum exactly."""
+    centres, radii = _blob_set(n=5)
+    k = 0.1
+    tw = _smin(centres, radii, k)
+    P = np.random.default_rng(12).uniform(-3, 3, (20000, 3))
+    ds = _sphere_d(P, centres, radii)
+    srt = np.sort(ds, axis=1)
+    sep = np.all(np.diff(srt, axis=1) >= k + 1e-9, axis=1)
+    assert sep.sum() > 50
+    d = tw.sdf_eval(0, P[sep])
+    assert np.allclose(d, ds[sep].min(axis=1), rtol=0, atol=1e-14)
+
+
+def test_smin_two_spheres_closed_form():
+    """Two spheres: the polynomial smooth minimum min(a, b) - max(k - |a - b|, 0)^2 / (4 k); two coincident
+    equal spheres give exactly d - k/4."""
+    k = 0.3
+    tw = _smin([[0, 0, 0], [0, 0, 0]], [0.5, 0.5], k)
+    P = np.random.default_rng(13).uniform(-2, 2, (1000, 3))
+    d1 = np.linalg.norm(P, axis=1) - 0.5
+    assert np.allclose(tw.sdf_eval(0, P), d1 - k / 4, rtol=0, atol=1e-14)
+    c = np.array([[0.0, 0.0, 0.0], [0.7, 0.1, 0.0]])
+    r = np.array([0.4, 0.3])
+    tw = _smin(c, r, k)
+    ds = _sphere_d(P, c, r)
+    h = np.maximum(k - np.abs(ds[:, 0] - ds[:, 1]), 0.0)
+    assert np.allclose(tw.sdf_eval(0, P), ds.min(axis=1) - h * h / (4 * k), rtol=0, atol=1e-13)
+
+
+def test_smin_march_meets_the_analytic_sphere():
+    """Two spheres far apart (their distances differ by more than k near either surface): a march toward one
+    hits its analytic ray-sphere distance within epsilon."""
+    k, eps = 0.1, 1e-9
+    c = np.array([[0.0, 0.0, 0.0], [6.0, 0.0, 0.0]])
+    r = np.array([1.0, 0.5])
+    tw = _smin(c, r, k, eps=eps)
+    rng = np.random.default_rng(14)
+    hits = 0
+    for _ in range(60):
+        o = np.array([0.0, 0.0, -5.0]) + rng.normal(size=3) * 0.3
+        target = c[0] + rng.normal(size=3) * 0.5
+        dvec = (target - o) / np.linalg.norm(target - o)
+        h = tw.trace(o, dvec)
+        oc = o - c[0]
+        b = oc @ dvec
+        disc = b * b - (oc @ oc - 1.0)
+        if disc < 0:
+            continue
+        hits += 1
+        cos_in = math.sqrt(disc) / r[0]  # the march stops within epsilon of the surface: eps / cos along the ray
+        assert h is not None and 0.0 <= (-b - math.sqrt(disc)) - h["t"] <= 1.5 * eps / cos_in
+    assert hits > 30
