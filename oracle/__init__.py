@@ -19,6 +19,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib: Optional[C.CDLL] = None
 
 BOX_REFERENCE, BOX_TIGHT = 0, 1
+PATH_SKY, PATH_ABSORBED, PATH_DEPTH = -1000, -1001, -1002
 INTEGRATOR_RECURSIVE, INTEGRATOR_FORWARD = 0, 1
 
 
@@ -71,6 +72,8 @@ def lib() -> C.CDLL:
                                       C.c_uint32, C.POINTER(C.c_double)]
         L.twin_sdf_eval.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.c_int64,
                                     C.POINTER(C.c_double)]
+        L.twin_path_ranks.argtypes = [vp, C.POINTER(A.CameraDesc), C.POINTER(A.RenderOptsDesc), C.c_int32, C.c_int32,
+                                      C.c_uint32, C.c_uint32, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 
@@ -184,6 +187,19 @@ class TwinScene:
         out = np.zeros(3, np.float32)
         lib().twin_scene_sky(self.h, precision, _p(dd, C.c_double), _p(out, C.c_float))
         return out
+
+    def path_ranks(self, cam, opts, x: int, y: int, precision: int = 64, box_mode: Optional[int] = None):
+        """Per sample of pixel (x, y), frame 0: the tuple of closest-hit ranks along its path, ended by how the
+        path ended (PATH_SKY, PATH_ABSORBED or PATH_DEPTH)."""
+        if box_mode is None:
+            box_mode = BOX_REFERENCE if precision == 64 else BOX_TIGHT
+        cap = opts.ray_depth + 3
+        codes = np.zeros((opts.samples, cap), np.int32)
+        lens = np.zeros(opts.samples, np.int32)
+        cd, od = cam.desc(), opts.desc()
+        lib().twin_path_ranks(self.h, C.byref(cd), C.byref(od), precision, box_mode, x, y, _p(codes, C.c_int32), cap,
+                              _p(lens, C.c_int32))
+        return [tuple(int(v) for v in codes[k, :lens[k]]) for k in range(opts.samples)]
 
     def sdf_eval(self, sdf: int, points, precision: int = 64):
         p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)

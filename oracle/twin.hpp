@@ -785,21 +785,36 @@ inline bool box_hit(const Ctx<R> &c, const NodeBoxes &nb, const Ray<R> &ray, con
     return lo <= hi * (R)(1.0 + BOX_SLACK);
 }
 
-// SphereMesh::intersect (sphere.rs:140-195)
+// SphereMesh::intersect (sphere.rs:44-99), sphere_uv (sphere.rs:114-121)
 template <class R>
 inline bool sphere_intersect(const double *sp, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
     const V3<R> c = cast3<R>(sp);
     const R radius = (R)sp[3];
-    const R r2 = radius * radius;  // radius_sqr, precomputed in the engine (sphere.rs:128)
-    const V3<R> oc = ray.pos - c;
-    const R half_b = dot(oc, ray.dir);
-    const R cc = dot(oc, oc) - r2;
-    const R disc = (half_b * half_b) - cc;
-    if (disc < R(0)) return false;
-    const R sq = Math<R>::sqrt(disc);
-    R root = -half_b - sq;
+    // The quadratic is evaluated in double for either precision, from the precision's own centre, radius and
+    // ray; for double this is the engine's arithmetic unchanged.  Single precision differs in two ways, both
+    // for accuracy: (1) the engine's c = |oc|^2 - r^2 and b^2 - c cancel in single precision (for the r = 1000
+    // ground sphere the root would be off by up to 5e-4 relative), while in double o - c is exact and the rest
+    // rounds far below single precision; (2) the engine takes a = |d|^2 as 1 (sphere.rs:50-52), but a
+    // single-precision unit direction is unit only to ~6e-8, which c / (2 sqrt(disc)) magnifies to ~1e-4 for a
+    // small, distant sphere, so the single-precision quadratic keeps a: disc = b^2 - a c and the roots are
+    // (-b -+ sqrt(disc)) / a, the division taken as the product with 2 - a (exact to (1 - a)^2 ~ 1e-14).  The
+    // root, rounded once to single precision, then stays within 2e-6 of the double-precision engine's.  Each
+    // root is checked against the interval after that rounding.
+    const V3<double> ocd{(double)ray.pos.x - (double)c.x, (double)ray.pos.y - (double)c.y,
+                         (double)ray.pos.z - (double)c.z};
+    const V3<double> dd{(double)ray.dir.x, (double)ray.dir.y, (double)ray.dir.z};
+    const double r2 = (double)radius * (double)radius;  // radius_sqr, precomputed in the engine (sphere.rs:32)
+    const double half_b = dot(ocd, dd);
+    const double cc = dot(ocd, ocd) - r2;
+    const bool single = sizeof(R) == 4;
+    const double a = single ? dot(dd, dd) : 1.0;
+    const double disc = (half_b * half_b) - (single ? a * cc : cc);
+    if (disc < 0.0) return false;
+    const double sq = std::sqrt(disc);
+    const double inv_a = 2.0 - a;
+    R root = single ? (R)((-half_b - sq) * inv_a) : (R)(-half_b - sq);
     if (!iv.contains(root)) {
-        root = -half_b + sq;
+        root = single ? (R)((-half_b + sq) * inv_a) : (R)(-half_b + sq);
         if (!iv.contains(root)) return false;
     }
     h.t = root;
@@ -1631,6 +1646,40 @@ struct Tracer {
             ray = Ray<R>::make(h.pos, dir);
         }
         return L;
+    }
+
+    // The path of one sample as the forward integrator follows it: the depth-first rank of each vertex's closest
+    // hit (-2 - i for unbounded object i), then how the path ended: PATH_SKY, PATH_ABSORBED (no scatter) or
+    // PATH_DEPTH (the depth limit).  Returns the number of codes written (at most cap).
+    enum : int32_t { PATH_SKY = -1000, PATH_ABSORBED = -1001, PATH_DEPTH = -1002 };
+    int path_ranks(Ray<R> ray, int32_t *out, int cap) {
+        int n = 0;
+        auto put = [&](int32_t v) {
+            if (n < cap) out[n] = v;
+            ++n;
+        };
+        for (uint32_t depth = 0;; ++depth) {
+            if (depth > depth_max) {
+                put(PATH_DEPTH);
+                break;
+            }
+            Hit<R> h;
+            if (!intersect(ray, depth, h)) {
+                put(PATH_SKY);
+                break;
+            }
+            put(h.rank);
+            const rayna_material &m = s->materials[h.material];
+            R u[4];
+            draw4<R>(key, pixel, sample, depth, SLOT_SCATTER, u);
+            V3<R> dir;
+            if (!scatter(m, ray, h, u, &dir)) {
+                put(PATH_ABSORBED);
+                break;
+            }
+            ray = Ray<R>::make(h.pos, dir);
+        }
+        return n < cap ? n : cap;
     }
 
     // render_px_once for the non-PBR modes (renderer.rs:428-490): one closest hit, coloured by the mode; the sky

@@ -305,6 +305,23 @@ void twin_camera_ray(const rayna_camera *cam, const rayna_render_opts *opts, uin
     out[5] = r.dir.z;
 }
 
+// The paths of every sample of pixel (x, y) of frame 0 (Tracer::path_ranks): for sample k, codes
+// [k * cap, k * cap + n_k) and n_k in lens[k].  Used to name the cause where the precisions' radiance differs.
+void twin_path_ranks(const twin_scene *t, const rayna_camera *cam, const rayna_render_opts *opts, int32_t precision,
+                     int32_t box_mode, uint32_t x, uint32_t y, int32_t *codes, int32_t cap, int32_t *lens) {
+    const Viewport vp = calculate_viewport(*cam);
+    const RngKey key = make_key(opts->seed, 0);
+    for (uint32_t k = 0; k < opts->samples; ++k) {
+        if (precision == 32) {
+            Tracer<float> tr{&t->s, key, y * opts->width + x, k, opts->ray_depth, box_mode};
+            lens[k] = tr.path_ranks(camera_ray<float>(vp, *opts, key, x, y, k), codes + (size_t)k * cap, cap);
+        } else {
+            Tracer<double> tr{&t->s, key, y * opts->width + x, k, opts->ray_depth, box_mode};
+            lens[k] = tr.path_ranks(camera_ray<double>(vp, *opts, key, x, y, k), codes + (size_t)k * cap, cap);
+        }
+    }
+}
+
 // SDF value at points (n x 3) in the given precision.
 void twin_sdf_eval(const twin_scene *t, int32_t sdf, int32_t precision, const double *p, int64_t n, double *out) {
     const rayna_sdf &f = t->s.sdfs[sdf];

@@ -93,19 +93,29 @@ __device__ __forceinline__ f3 at(const DRay &r, float t) { return r.o + r.d * t;
 
 // ------------------------------------------------------------------------------------------ primitives
 
-// SphereMesh::intersect (sphere.rs:140-195)
+// SphereMesh::intersect (sphere.rs:44-99): the engine's quadratic, evaluated in double from the single-precision
+// centre, radius and ray, keeping a = |d|^2.  In single precision c = |oc|^2 - r^2 and b^2 - c cancel (the
+// r = 1000 ground sphere's roots would be off by up to 5e-4 relative), and the engine's a = 1 (sphere.rs:50-52)
+// holds only to ~6e-8 for a single-precision unit direction, which c / (2 sqrt(disc)) magnifies to ~1e-4 for a
+// small, distant sphere.  In double o - c is exact and the rest rounds far below single precision; the roots are
+// (-b -+ sqrt(b^2 - a c)) / a, the division taken as the product with 2 - a (exact to (1 - a)^2 ~ 1e-14).  The
+// root, rounded once to single precision, stays within 2e-6 of the double-precision engine's; each root is
+// checked against [t0, t1] after that rounding.
 __device__ __forceinline__ bool sphere_t(const GSphere &s, const DRay &r, float t0, float t1, float *t) {
-    const f3 c{s.cx, s.cy, s.cz};
-    const float r2 = s.radius * s.radius;
-    const f3 oc = r.o - c;
-    const float half_b = dot(oc, r.d);
-    const float cc = dot(oc, oc) - r2;
-    const float disc = (half_b * half_b) - cc;
-    if (disc < 0.0f) return false;
-    const float sq = sqrtf(disc);
-    float root = -half_b - sq;
+    const double ox = (double)r.o.x - (double)s.cx, oy = (double)r.o.y - (double)s.cy,
+                 oz = (double)r.o.z - (double)s.cz;
+    const double dx = r.d.x, dy = r.d.y, dz = r.d.z;
+    const double r2 = (double)s.radius * (double)s.radius;
+    const double half_b = (ox * dx + oy * dy) + oz * dz;
+    const double cc = ((ox * ox + oy * oy) + oz * oz) - r2;
+    const double a = (dx * dx + dy * dy) + dz * dz;
+    const double disc = (half_b * half_b) - a * cc;
+    if (disc < 0.0) return false;
+    const double sq = sqrt(disc);
+    const double inv_a = 2.0 - a;
+    float root = (float)((-half_b - sq) * inv_a);
     if (!(t0 <= root && root <= t1)) {
-        root = -half_b + sq;
+        root = (float)((-half_b + sq) * inv_a);
         if (!(t0 <= root && root <= t1)) return false;
     }
     *t = root;

@@ -1,5 +1,6 @@
 """The oracle reproduces its committed golden fixtures exactly (tests/golden/make_golden.py wrote them), and the
-fixtures hold the properties the path guarantees: equal-seed fp32/fp64 agreement on primary hits."""
+fixtures hold the equal-seed precision contract between the fp32 path and the fp64 engine precision
+(tests/precision.py)."""
 import os
 
 import numpy as np
@@ -18,17 +19,26 @@ def test_oracle_reproduces_golden(name):
         assert np.array_equal(g[k], now[k]), k
 
 
+# Per workload: the largest fraction of pixels whose radiance may fall outside 1e-4 of the engine precision's
+# (each such pixel must have a stated cause: tests/precision.py), and of primary hits whose distance may fall
+# outside 1e-5 (all at grazing incidence).  Measured on these fixtures: random spheres 4 of 2304 pixels (mirror
+# and glass chains), mixed primitives 3 of 1536 (paths that diverge), icosphere mesh 0 of 1296 (1 grazing hit of
+# 887), SDF + volume 28 of 576 (19 diverge, 9 meet an SDF first).
+RADIANCE_OFF_MAX = {"random_spheres": 0.01, "mixed_primitives": 0.01, "triangle_mesh_s4": 0.002, "sdf_volume": 0.08}
+GRAZING_MAX = 0.005
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_golden_primary_hits_agree_across_precisions(name):
-    """Equal-seed primary hits of the fp32 path against the fp64 engine precision.  Meshes and SDFs meet 1e-5
-    relative on distance at the 99th percentile.  Sphere scenes do not: the engine's c = |oc|^2 - r^2
-    (sphere.rs:151) cancels in fp32 for the r = 1000 ground sphere, leaving up to ~3e-4."""
+def test_golden_equal_seed_precision_contract(name):
+    """The fp32 path (the GPU's arithmetic) against the fp64 engine precision at equal seeds: primary-hit ranks,
+    distances within 1e-5 and radiance within 1e-4, with every exception of a stated cause and bounded."""
+    import oracle
+    from tests import precision as P
     g = np.load(os.path.join(HERE, f"{name}.npz"))
-    same = g["rank"] == g["rank64"]
-    assert same.mean() > 0.99
-    hit = same & (g["rank"] != -1)
-    rel = np.abs(g["t32"][hit] - g["t64"][hit]) / np.abs(g["t64"][hit])
-    if name in ("random_spheres", "mixed_primitives"):
-        assert np.median(rel) <= 1e-5 and rel.max() <= 1e-3
-    else:
-        assert np.quantile(rel, 0.99) <= 1e-5 and rel.max() <= 1e-4
+    sc, cam, opts = CASES[name]()
+    tw = oracle.TwinScene(sc)
+    prim = P.primary_check(tw, cam, opts, g["rank"], g["t32"], g["rank64"], g["t64"])
+    assert prim["rank_same"] >= P.RANK_MIN
+    assert prim["t_grazing"] <= GRAZING_MAX * prim["hits"]
+    rad = P.radiance_check(tw, sc, cam, opts, g["img32"], g["img64"])
+    assert rad["off"] <= RADIANCE_OFF_MAX[name] * rad["pixels"], rad
