@@ -409,6 +409,18 @@ rayna_status rayna_tree_hash(rayna_renderer *r, uint64_t *hash);
 rayna_status rayna_scene_prepare(const rayna_scene_desc *scene, uint64_t *tree_hash, int32_t *object,
                                  int32_t *local, int32_t capacity, int32_t *n_ranks, int32_t *stack_need);
 
+/* Host-only, as rayna_scene_prepare: the flattened traversal tree the extend kernels walk (no engine
+ * counterpart; for inspection of its stack bounds).  node_words: 8 words per node (the four child words, then
+ * the four info words: INFO_EMPTY = -1 unused, 0 inner node, else count | kind << 4 with kind 1 = triangles,
+ * 2 = items); item_words: 2 per item (kind, index: kind 6 = a mesh-BVH link to node `index`, 8 = a transformed
+ * object whose entry is in xform_words); xform_words: 2 per transformed object (inner kind, inner index).  Any
+ * array may be NULL (then only counted); counts[3] = nodes, items, transformed objects.  root: the root node
+ * (-1: none); stack_need: the deepest per-lane stack the per-lane traversal can need; packet_need: the deepest
+ * wave stack of the camera-ray packets. */
+rayna_status rayna_scene_tree(const rayna_scene_desc *scene, int32_t *node_words, int32_t node_cap,
+                              int32_t *item_words, int32_t item_cap, int32_t *xform_words, int32_t xform_cap,
+                              int32_t counts[3], int32_t *root, int32_t *stack_need, int32_t *packet_need);
+
 /* Renderer drop. */
 void rayna_destroy(rayna_renderer *r);
 

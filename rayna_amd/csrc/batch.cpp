@@ -559,6 +559,7 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     v.n_sdfs = (int32_t)merged.sdfs.size();
     v.skybox = RAYNA_SKY_NONE;
     v.stack_need = merged.stack_need;
+    v.packet_need = merged.packet_need;
     v.instances = has_instances(merged) ? 1 : 0;
     v.textured = has_textures(merged) ? 1 : 0;
     for (const SceneSlot &sl : b->slots)

@@ -1166,7 +1166,7 @@ void launch_generate(const FrameParams &fp, const PathBuffers &pb, const BatchVi
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
                    const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work,
                    const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
-    if (!queue && depth == 0 && !sv.instances && sv.stack_need <= PACKET_STACK &&
+    if (!queue && depth == 0 && !sv.instances && sv.packet_need <= PACKET_STACK &&
         (bv.reqs || fp.n_spp >= PACKET_MIN_SPP)) {  // camera rays: wave packets
         if (bv.reqs)
             hipLaunchKernelGGL(k_extend_packet<true>, dim3(grid_for(fp.n_slots, EXT_BLOCK)), dim3(EXT_BLOCK), 0, s, sv,

@@ -205,7 +205,8 @@ struct SceneView {
     int32_t root;        // -1: no bounded object
     int32_t n_sdfs;
     int32_t skybox;
-    int32_t stack_need;  // deepest traversal stack the tree can require
+    int32_t stack_need;   // deepest per-lane stack k_extend can need (speculative traversal)
+    int32_t packet_need;  // deepest wave stack k_extend_packet can need (depth-first, no speculation)
     int32_t instances;   // 1 if some object is a transformed mesh BVH (an instance)
     int32_t textured;    // 1 if some material has a texture or the sky is an image
     int32_t sky_image;   // RAYNA_SKY_HDRI: the sky's image

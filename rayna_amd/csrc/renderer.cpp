@@ -150,6 +150,7 @@ rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
     v.n_sdfs = (int32_t)ps.sdfs.size();
     v.skybox = ps.skybox;
     v.stack_need = ps.stack_need;
+    v.packet_need = ps.packet_need;
     v.instances = has_instances(ps) ? 1 : 0;
     v.textured = has_textures(ps) ? 1 : 0;
     v.sky_image = ps.sky_image;
@@ -778,6 +779,42 @@ rayna_status rayna_scene_prepare(const rayna_scene_desc *scene, uint64_t *tree_h
         for (int32_t i = 0; i < (int32_t)ps.rank_object.size() && i < capacity; ++i) {
             object[i] = ps.rank_object[i];
             local[i] = ps.rank_local[i];
+        }
+    return RAYNA_OK;
+}
+
+rayna_status rayna_scene_tree(const rayna_scene_desc *scene, int32_t *node_words, int32_t node_cap,
+                              int32_t *item_words, int32_t item_cap, int32_t *xform_words, int32_t xform_cap,
+                              int32_t counts[3], int32_t *root, int32_t *stack_need, int32_t *packet_need) {
+    g_create_error.clear();
+    if (!scene) return fail(nullptr, RAYNA_INVALID_ARGUMENT, "null scene");
+    PackedScene ps;
+    std::string err;
+    const rayna_status st = pack_scene(*scene, ps, err);
+    if (st != RAYNA_OK) return fail(nullptr, st, err);
+    if (counts) {
+        counts[0] = (int32_t)ps.nodes.size();
+        counts[1] = (int32_t)ps.items.size();
+        counts[2] = (int32_t)ps.xforms.size();
+    }
+    if (root) *root = ps.root;
+    if (stack_need) *stack_need = ps.stack_need;
+    if (packet_need) *packet_need = ps.packet_need;
+    if (node_words)
+        for (int32_t i = 0; i < (int32_t)ps.nodes.size() && i < node_cap; ++i)
+            for (int k = 0; k < 4; ++k) {
+                node_words[8 * i + k] = ps.nodes[i].child[k];
+                node_words[8 * i + 4 + k] = ps.nodes[i].info[k];
+            }
+    if (item_words)
+        for (int32_t i = 0; i < (int32_t)ps.items.size() && i < item_cap; ++i) {
+            item_words[2 * i] = ps.items[i].kind;
+            item_words[2 * i + 1] = ps.items[i].index;
+        }
+    if (xform_words)
+        for (int32_t i = 0; i < (int32_t)ps.xforms.size() && i < xform_cap; ++i) {
+            xform_words[2 * i] = ps.xforms[i].inner_kind;
+            xform_words[2 * i + 1] = ps.xforms[i].inner_index;
         }
     return RAYNA_OK;
 }

@@ -896,13 +896,24 @@ struct Packer {
         return st;
     }
 
-    // The deepest traversal stack the flattened tree can need, for any ray and any culling.  An inner node pushes
-    // its m non-empty children (inner nodes and leaf ranges alike) and they are popped one by one, so the height
-    // above the point where the node was popped peaks at max(m, m - 1 + the deepest child's need).  A leaf range
-    // pushes the root of each mesh-BVH item it tests; those are processed after the range, in any order.  An
-    // instance (a transformed mesh BVH) ends the range's pass: it pushes the rest of the range, three saved words
-    // of the closest hit, a marker and its root, so its tree runs 4 entries above the rest of the range.
-    int32_t stack_need() const {
+    // The deepest traversal stacks the flattened tree can need, for any ray and any culling.
+    //
+    // Depth-first (the packets' wave stack, and the bound `dfs` below): an inner node pushes its m non-empty
+    // children (inner nodes and leaf ranges alike) and they are popped one by one, so the height above the point
+    // where the node was popped peaks at max(m, m - 1 + the deepest child's need).  A leaf range pushes the root
+    // of each mesh-BVH item it tests; those are processed after the range, in any order.  An instance (a
+    // transformed mesh BVH) ends the range's pass: it pushes the rest of the range, three saved words of the
+    // closest hit, a marker and its root, so its tree runs 4 entries above the rest of the range.
+    //
+    // Speculative (k_extend's lane stack): a lane that holds a leaf range keeps popping and expanding inner nodes
+    // until it pops a second range (pushed back) or its stack empties, and only then tests the held range, whose
+    // mesh-BVH links and instance entries go on top of whatever the speculation pushed meanwhile.  The stack
+    // below them is the state of a depth-first traversal in which the held range pushes nothing, so it holds at
+    // most `dfs` entries; the held range's own entries and their trees add at most its leaf need, and no second
+    // range with such entries can be pending meanwhile (a lane holds one range; the ranges inside mesh BVHs and
+    // instances are triangle or primitive ranges, which push nothing).  So k_extend needs at most `dfs` plus the
+    // largest leaf need of a range holding a link or an instance.
+    void stack_needs(int32_t *speculative, int32_t *depth_first) const {
         std::vector<int32_t> memo(P.nodes.size(), -1);
         std::function<int32_t(int32_t)> node_need;
         std::function<int32_t(int32_t, int32_t)> leaf_need = [&](int32_t start, int32_t count) -> int32_t {
@@ -936,8 +947,24 @@ struct Packer {
             memo[g] = m == 0 ? 0 : std::max(m, m - 1 + deepest);
             return memo[g];
         };
-        if (P.root < 0) return 0;
-        return std::max<int32_t>(1, node_need(P.root));
+        *speculative = *depth_first = 0;
+        if (P.root < 0) return;
+        const int32_t dfs = std::max<int32_t>(1, node_need(P.root));
+        int32_t held = 0;  // the largest leaf need of a range with a link or an instance
+        for (const GNode &n : P.nodes)
+            for (int k = 0; k < 4; ++k) {
+                if (n.info[k] == INFO_EMPTY || n.info[k] == INFO_INNER || (n.info[k] >> 4) != LEAF_ITEMS) continue;
+                const int32_t start = n.child[k], count = n.info[k] & 0xf;
+                bool pushes = false;
+                for (int32_t i = 0; i < count; ++i) {
+                    const GItem &it = P.items[start + i];
+                    pushes = pushes || it.kind == ITEM_LINK ||
+                             (it.kind == ITEM_XFORM && P.xforms[it.index].inner_kind == ITEM_LINK);
+                }
+                if (pushes) held = std::max(held, leaf_need(start, count));
+            }
+        *depth_first = dfs;
+        *speculative = dfs + held;
     }
 };
 
@@ -1359,7 +1386,7 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
     for (int m = 0; m < d.n_meshes; ++m) hash_tree(K.blas[m], P.tree_hash);
     const rayna_status st = K.emit_tlas(error);
     if (st != RAYNA_OK) return st;
-    P.stack_need = K.stack_need();
+    K.stack_needs(&P.stack_need, &P.packet_need);
     return RAYNA_OK;
 }
 
@@ -1448,8 +1475,9 @@ bool merge_scenes(const std::vector<PackedScene> &in, PackedScene &out, std::vec
         out.texels.insert(out.texels.end(), p.texels.begin(), p.texels.end());
         out.perms.insert(out.perms.end(), p.perms.begin(), p.perms.end());
         slots.push_back(SceneSlot{p.root >= 0 ? p.root + node_off : -1, sdf_off, (int32_t)p.sdfs.size(), p.skybox,
-                                  p.stack_need, p.sky_image + img_off});
+                                  p.stack_need, p.packet_need, p.sky_image + img_off});
         out.stack_need = std::max(out.stack_need, p.stack_need);
+        out.packet_need = std::max(out.packet_need, p.packet_need);
     }
     // leaf starts have 27 bits in a traversal stack code; hit references keep 28 bits of index
     return out.tris.size() < (1u << 27) && out.items.size() < (1u << 27) && out.sdfs.size() < (1u << 28);

@@ -52,7 +52,8 @@ struct PackedScene {
     std::vector<float> texels;
     std::vector<int32_t> perms;
     int32_t root = -1;
-    int32_t stack_need = 0;
+    int32_t stack_need = 0;   // deepest per-lane stack of k_extend (speculative traversal)
+    int32_t packet_need = 0;  // deepest wave stack of k_extend_packet (depth-first, no speculation)
     int32_t skybox = RAYNA_SKY_NONE;
     int32_t sky_image = 0;
     uint64_t tree_hash = 0;
@@ -88,6 +89,7 @@ struct SceneSlot {
     int32_t n_sdfs;
     int32_t skybox;
     int32_t stack_need;
+    int32_t packet_need;
     int32_t sky_image;
 };
 

@@ -29,7 +29,7 @@ _lib: Optional[C.CDLL] = None
 EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_options", "rayna_clear_accumulation",
            "rayna_render", "rayna_render_srgb8", "rayna_set_material_colours", "rayna_adjoint", "rayna_set_tuning", "rayna_batch_create",
            "rayna_batch_set_tuning", "rayna_render_batch", "rayna_batch_destroy", "rayna_batch_last_error",
-           "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare", "rayna_destroy",
+           "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare", "rayna_scene_tree", "rayna_destroy",
            "rayna_last_error")
 
 
@@ -91,6 +91,11 @@ def lib() -> C.CDLL:
         L.rayna_scene_prepare.argtypes = [C.POINTER(A.SceneDesc), C.POINTER(C.c_uint64), C.POINTER(C.c_int32),
                                           C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32),
                                           C.POINTER(C.c_int32)]
+        L.rayna_scene_tree.restype = C.c_int
+        L.rayna_scene_tree.argtypes = [C.POINTER(A.SceneDesc), C.POINTER(C.c_int32), C.c_int32,
+                                       C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32)]
         L.rayna_destroy.restype = None
         L.rayna_destroy.argtypes = [vp]
         L.rayna_last_error.restype = C.c_char_p
@@ -122,6 +127,23 @@ class RenderStats:
 class Render:
     img: np.ndarray
     stats: RenderStats
+
+
+def scene_tree(scene):
+    """Host-only: the flattened traversal tree (rayna_scene_tree) as numpy arrays: node words (n, 8), item words
+    (n, 2), transformed-object words (n, 2), root, per-lane stack need, packet stack need."""
+    d = scene.desc()
+    counts = (C.c_int32 * 3)()
+    root, need, pneed = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+    _check(lib().rayna_scene_tree(C.byref(d), None, 0, None, 0, None, 0, counts, C.byref(root), C.byref(need),
+                                  C.byref(pneed)))
+    nodes = np.zeros((counts[0], 8), np.int32)
+    items = np.zeros((counts[1], 2), np.int32)
+    xforms = np.zeros((counts[2], 2), np.int32)
+    p = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    _check(lib().rayna_scene_tree(C.byref(d), p(nodes), len(nodes), p(items), len(items), p(xforms), len(xforms),
+                                  counts, C.byref(root), C.byref(need), C.byref(pneed)))
+    return nodes, items, xforms, int(root.value), int(need.value), int(pneed.value)
 
 
 def prepare_scene(scene):
