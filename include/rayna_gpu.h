@@ -352,6 +352,33 @@ typedef struct rayna_tuning {
 /* Applies a tuning to a renderer.  A new chunk_slots re-plans the path buffers and clears accumulation. */
 rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t);
 
+/* ---------------------------------------------------------------- debug validation (shared/validate.rs:13-198)
+ * The engine asserts these invariants in debug builds along every path (render/renderer.rs:395-561).  With
+ * validation on, the PBR frames of rayna_render count, per check, the rays, hits and colours that break them,
+ * with the engine's tolerances (a unit vector: |n.n - 1| <= 2e-4, glam's is_normalized; a hit on its ray:
+ * within 1e-6 absolute or 1e-3 relative, approx's relative_eq with validate.rs's EPSILON and RELATIVE); the
+ * kernels never trap.  The counts equal those of the oracle's fp32 restatement bit for bit. */
+enum {
+    RAYNA_CHECK_RAY = 0,         /* a traced ray: origin not finite, or direction not finite and unit */
+    RAYNA_CHECK_HIT_DIST = 1,    /* a hit's distance not finite or outside [0.001, FLT_MAX] */
+    RAYNA_CHECK_HIT_POS = 2,     /* a hit's position not finite, or not the ray's point at its distance */
+    RAYNA_CHECK_HIT_NORMAL = 3,  /* a hit's normal or ray normal not finite and unit */
+    RAYNA_CHECK_HIT_UV = 4,      /* a hit's uv NaN or outside [0, 1] */
+    RAYNA_CHECK_COLOUR = 5,      /* a material colour, emitted radiance (lights, sky) or path throughput NaN,
+                                  * infinite or negative */
+    RAYNA_CHECK_SAMPLE = 6,      /* a sample's radiance NaN, infinite or negative */
+    RAYNA_CHECK_PIXEL = 7,       /* a pixel's sample mean NaN, infinite or negative */
+    RAYNA_CHECKS = 8
+};
+
+/* Turns validation on (1) or off (0) for the renderer's later PBR frames (not batches, adjoints or the
+ * non-PBR modes); turning it on clears the counts.  Validation adds a kernel variant and per-hit work. */
+rayna_status rayna_set_validate(rayna_renderer *r, int32_t enable);
+
+/* The counts of each RAYNA_CHECK_* since validation was turned on or last reset (counts: RAYNA_CHECKS
+ * entries); reset != 0 clears them after reading. */
+rayna_status rayna_validate_counts(rayna_renderer *r, uint64_t *counts, int32_t reset);
+
 /* ---------------------------------------------------------------- batches
  * Many render requests (each its own scene, camera, resolution, samples and seed) traced together: the
  * requests' samples are packed into chunks of paths, and every chunk runs one generate / extend / shade

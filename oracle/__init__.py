@@ -20,6 +20,8 @@ _lib: Optional[C.CDLL] = None
 
 BOX_REFERENCE, BOX_TIGHT = 0, 1
 PATH_SKY, PATH_ABSORBED, PATH_DEPTH = -1000, -1001, -1002
+# the invariants of the engine's shared/validate.rs, in the order of twin::CHECK_* and RAYNA_CHECK_*
+CHECKS = ("ray", "hit_dist", "hit_pos", "hit_normal", "hit_uv", "colour", "sample", "pixel")
 INTEGRATOR_RECURSIVE, INTEGRATOR_FORWARD = 0, 1
 
 
@@ -72,6 +74,10 @@ def lib() -> C.CDLL:
                                       C.c_uint32, C.POINTER(C.c_double)]
         L.twin_sdf_eval.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.c_int64,
                                     C.POINTER(C.c_double)]
+        L.twin_render_checked.restype = C.c_uint64
+        L.twin_render_checked.argtypes = [vp, C.POINTER(A.CameraDesc), C.POINTER(A.RenderOptsDesc), C.c_int32,
+                                          C.c_int32, C.c_uint32, C.c_int32, C.POINTER(C.c_float),
+                                          C.POINTER(C.c_uint64)]
         L.twin_path_ranks.argtypes = [vp, C.POINTER(A.CameraDesc), C.POINTER(A.RenderOptsDesc), C.c_int32, C.c_int32,
                                       C.c_uint32, C.c_uint32, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
         _lib = L
@@ -135,6 +141,19 @@ class TwinScene:
         segs = lib().twin_render(self.h, C.byref(cd), C.byref(od), precision, box_mode, integrator, frame, row_start,
                                  row_step, col_step, threads, _p(out, C.c_float), C.byref(secs), C.byref(tests))
         return out, int(segs), secs.value, int(tests.value)
+
+    def render_checked(self, cam, opts, precision: int = 32, box_mode: Optional[int] = None, frame: int = 0,
+                       threads: int = 0):
+        """A frame by the forward integrator with the engine's debug invariants (shared/validate.rs) counted:
+        (image rows x cols x 3, counts per CHECKS entry)."""
+        if box_mode is None:
+            box_mode = BOX_TIGHT if precision == 32 else BOX_REFERENCE
+        out = np.zeros((opts.height, opts.width, 3), np.float32)
+        checks = np.zeros(len(CHECKS), np.uint64)
+        cd, od = cam.desc(), opts.desc()
+        lib().twin_render_checked(self.h, C.byref(cd), C.byref(od), precision, box_mode, frame, threads,
+                                  _p(out, C.c_float), _p(checks, C.c_uint64))
+        return out, dict(zip(CHECKS, (int(v) for v in checks)))
 
     def set_colours(self, rgb):
         """Replace the material colours (albedo, or emission for lights): an (n_materials, 3) array."""

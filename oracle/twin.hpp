@@ -1588,6 +1588,60 @@ inline Ray<R> camera_ray(const Viewport &vp, const rayna_render_opts &o, const R
 // Integrators
 // ================================================================================================
 
+// ---- shared/validate.rs (13-198): the invariants the engine asserts in debug builds at render/renderer.rs:
+// 395-561, counted here per check instead of asserted ----
+enum : int {
+    CHECK_RAY = 0,         // a traced ray: origin not finite, or direction not finite and unit (validate::ray)
+    CHECK_HIT_DIST = 1,    // a hit's distance not finite or outside the interval
+    CHECK_HIT_POS = 2,     // a hit's position not finite, or not at the ray's point of its distance
+    CHECK_HIT_NORMAL = 3,  // a hit's normal or ray normal not finite and unit
+    CHECK_HIT_UV = 4,      // a hit's uv NaN or outside [0, 1]
+    CHECK_COLOUR = 5,      // a material colour, emitted radiance or path throughput NaN, infinite or negative
+    CHECK_SAMPLE = 6,      // a sample's radiance NaN, infinite or negative
+    CHECK_PIXEL = 7,       // a pixel's sample mean NaN, infinite or negative
+    N_CHECKS = 8
+};
+
+template <class R>
+inline bool finite3(const V3<R> &v) {
+    return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z);
+}
+// validate::normal3: finite and glam's is_normalized, |n.n - 1| <= 2e-4 (glam 0.27)
+template <class R>
+inline bool unit3(const V3<R> &n) {
+    return finite3(n) && std::fabs(dot(n, n) - R(1)) <= R(2e-4);
+}
+// approx's relative_eq with validate.rs's EPSILON = 1e-6 and RELATIVE = 1e-3
+template <class R>
+inline bool approx_eq(R a, R b) {
+    if (a == b) return true;
+    if (std::isinf(a) || std::isinf(b)) return false;
+    const R d = std::fabs(a - b);
+    if (d <= R(1e-6)) return true;
+    return d <= std::fmax(std::fabs(a), std::fabs(b)) * R(1e-3);
+}
+// validate::channel for each channel: not NaN, not infinite, >= 0
+inline bool colour_ok(const Col &c) {
+    auto ok = [](float v) { return !std::isnan(v) && !std::isinf(v) && v >= 0.0f; };
+    return ok(c.r) && ok(c.g) && ok(c.b);
+}
+template <class R>
+inline bool ray_ok(const Ray<R> &r) {
+    return finite3(r.pos) && unit3(r.dir);
+}
+// validate::intersection (validate.rs:160-198): distance, position on the ray, normals, uv.  Each failing
+// group counts once.
+template <class R>
+inline void check_hit(uint64_t *c, const Ray<R> &ray, const Interval<R> &iv, const Hit<R> &h) {
+    if (!std::isfinite(h.t) || !iv.contains(h.t)) ++c[CHECK_HIT_DIST];
+    const V3<R> at = ray.at(h.t);
+    if (!finite3(h.pos) || !approx_eq(length(ray.pos - h.pos), h.t) || !approx_eq(h.pos.x, at.x) ||
+        !approx_eq(h.pos.y, at.y) || !approx_eq(h.pos.z, at.z))
+        ++c[CHECK_HIT_POS];
+    if (!unit3(h.normal) || !unit3(h.ray_normal)) ++c[CHECK_HIT_NORMAL];
+    if (!(h.uv[0] >= R(0) && h.uv[0] <= R(1) && h.uv[1] >= R(0) && h.uv[1] <= R(1))) ++c[CHECK_HIT_UV];
+}
+
 template <class R>
 struct Tracer {
     const Scene *s;
@@ -1596,6 +1650,7 @@ struct Tracer {
     int box_mode;
     uint64_t segments = 0;
     uint64_t prim_tests = 0;
+    uint64_t *checks = nullptr;  // N_CHECKS counters: validate the forward integrator's path (CHECK_*)
 
     Ctx<R> ctx(uint32_t bounce) { return Ctx<R>{s, key, pixel, sample, bounce, box_mode, &prim_tests}; }
 
@@ -1626,25 +1681,36 @@ struct Tracer {
         return le + mean;
     }
 
-    // The same estimator with forward throughput: L += T*Le at each vertex, T *= attenuation.
+    // The same estimator with forward throughput: L += T*Le at each vertex, T *= attenuation.  With `checks`,
+    // the invariants of validate.rs are counted: each traced ray, each hit, the material colour, a light's
+    // emitted radiance and the sky's, the throughput after each scatter, and the sample's radiance.
     Col forward(Ray<R> ray) {
         Col L{0, 0, 0}, T{1, 1, 1};
         for (uint32_t depth = 0; depth <= depth_max; ++depth) {
+            if (checks && !ray_ok(ray)) ++checks[CHECK_RAY];
             Hit<R> h;
             if (!intersect(ray, depth, h)) {
-                L = L + T * sky_colour<R>(*s, ray);
+                const Col add = T * sky_colour<R>(*s, ray);
+                if (checks && !colour_ok(add)) ++checks[CHECK_COLOUR];
+                L = L + add;
                 break;
             }
+            if (checks) check_hit(checks, ray, Interval<R>{true, true, R(1e-3), Math<R>::big()}, h);
             const rayna_material &m = s->materials[h.material];
             const Col colour = material_colour<R>(*s, m, h);
-            L = L + T * emitted(m, colour);
+            if (checks && !colour_ok(colour)) ++checks[CHECK_COLOUR];
+            const Col add = T * emitted(m, colour);
+            if (checks && m.kind == RAYNA_MAT_LIGHT && !colour_ok(add)) ++checks[CHECK_COLOUR];
+            L = L + add;
             R u[4];
             draw4<R>(key, pixel, sample, depth, SLOT_SCATTER, u);
             V3<R> dir;
             if (!scatter(m, ray, h, u, &dir)) break;
             T = reflected(m, ray, h, colour, T);
+            if (checks && !colour_ok(T)) ++checks[CHECK_COLOUR];
             ray = Ray<R>::make(h.pos, dir);
         }
+        if (checks && !colour_ok(L)) ++checks[CHECK_SAMPLE];
         return L;
     }
 
@@ -1780,6 +1846,7 @@ struct RenderArgs {
     uint32_t row_start, row_step;  // rows row_start, row_start + row_step, ...
     int threads;
     uint32_t col_step;             // columns 0, col_step, 2 col_step, ... of each selected row
+    uint64_t *checks = nullptr;    // N_CHECKS counters: count validate.rs's invariants (forward integrator)
 };
 
 // One frame of per-pixel sample means (render_px_msaa's mean, renderer.rs:398-402), row-major, for the
@@ -1794,8 +1861,10 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
     const uint32_t n_cols = (o.width + cstep - 1) / cstep;
     std::atomic<uint32_t> next{0};
     std::atomic<uint64_t> segs{0}, tests{0};
+    std::atomic<uint64_t> checks[N_CHECKS];
+    for (auto &c : checks) c = 0;
     auto worker = [&]() {
-        uint64_t my_segs = 0, my_tests = 0;
+        uint64_t my_segs = 0, my_tests = 0, my_checks[N_CHECKS] = {};
         for (;;) {
             const uint32_t r = next.fetch_add(1);
             if (r >= n_rows) break;
@@ -1803,6 +1872,7 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
             for (uint32_t cx = 0; cx < n_cols; ++cx) {
                 const uint32_t x = cx * cstep;
                 Tracer<R> tr{a.scene, key, y * o.width + x, 0, o.ray_depth, a.box_mode};
+                if (a.checks) tr.checks = my_checks;
                 Col sum{0, 0, 0};
                 for (uint32_t k = 0; k < o.samples; ++k) {
                     tr.sample = k;
@@ -1812,6 +1882,7 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
                     sum = sum + c;
                 }
                 const Col mean = sum / (float)o.samples;
+                if (a.checks && !colour_ok(mean)) ++my_checks[CHECK_PIXEL];
                 float *px = out + ((size_t)r * n_cols + cx) * 3;
                 px[0] = mean.r;
                 px[1] = mean.g;
@@ -1822,6 +1893,7 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
         }
         segs += my_segs;
         tests += my_tests;
+        for (int i = 0; i < N_CHECKS; ++i) checks[i] += my_checks[i];
     };
     int nt = a.threads > 0 ? a.threads : (int)std::thread::hardware_concurrency();
     if (nt < 1) nt = 1;
@@ -1830,6 +1902,8 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
     worker();
     for (auto &t : pool) t.join();
     if (prim_tests_out) *prim_tests_out = tests.load();
+    if (a.checks)
+        for (int i = 0; i < N_CHECKS; ++i) a.checks[i] = checks[i].load();
     return segs.load();
 }
 

@@ -81,6 +81,15 @@ uint64_t twin_render(const twin_scene *t, const rayna_camera *cam, const rayna_r
     return segs;
 }
 
+// twin_render with validate.rs's invariants counted along the forward integrator's paths (precision 32 or 64):
+// checks gets twin::N_CHECKS counters (twin::CHECK_*).
+uint64_t twin_render_checked(const twin_scene *t, const rayna_camera *cam, const rayna_render_opts *opts,
+                             int32_t precision, int32_t box_mode, uint32_t frame, int32_t threads, float *out,
+                             uint64_t *checks) {
+    RenderArgs a{&t->s, cam, opts, frame, box_mode, INTEGRATOR_FORWARD, 0, 1, threads, 1, checks};
+    return precision == 32 ? render_rows<float>(a, out, nullptr) : render_rows<double>(a, out, nullptr);
+}
+
 // The UI's 8-bit conversion (twin::to_srgb8) of an x-major width x height x 3 float image to RGBA, row-major.
 void twin_to_srgb8(const float *img, uint32_t w, uint32_t h, uint8_t *rgba) { to_srgb8(img, w, h, rgba); }
 

@@ -682,6 +682,40 @@ __device__ __forceinline__ Col sky(int kind, f3 d) {  // skybox/simple.rs, skybo
     return Col{0.0f, 0.0f, 0.0f};
 }
 
+// ------------------------------------------------------------------------------------------ validation
+
+// shared/validate.rs (13-198): the invariants the engine asserts in debug builds along every path
+// (render/renderer.rs:395-561), counted per check (RAYNA_CHECK_*) instead of asserted, with the engine's
+// tolerances.  The same checks, in the same arithmetic, as the oracle's fp32 restatement.
+__device__ __forceinline__ bool finite3(f3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+// validate::normal3: finite and glam's is_normalized, |n.n - 1| <= 2e-4
+__device__ __forceinline__ bool unit3(f3 n) { return finite3(n) && fabsf(dot(n, n) - 1.0f) <= 2e-4f; }
+// approx's relative_eq with validate.rs's EPSILON = 1e-6 and RELATIVE = 1e-3
+__device__ __forceinline__ bool approx_eq(float a, float b) {
+    if (a == b) return true;
+    if (isinf(a) || isinf(b)) return false;
+    const float d = fabsf(a - b);
+    if (d <= 1e-6f) return true;
+    return d <= fmaxf(fabsf(a), fabsf(b)) * 1e-3f;
+}
+// validate::colour: every channel not NaN, not infinite, >= 0
+__device__ __forceinline__ bool colour_ok(Col c) {
+    auto ok = [](float v) { return !isnan(v) && !isinf(v) && v >= 0.0f; };
+    return ok(c.r) && ok(c.g) && ok(c.b);
+}
+__device__ __forceinline__ void count_check(uint32_t *checks, int which) { atomicAdd(checks + which, 1u); }
+// validate::intersection (validate.rs:160-198) of a hit of `ray` at distance t in [0.001, FLT_MAX]
+__device__ __forceinline__ void validate_hit(uint32_t *checks, const DRay &ray, float t, const SHit &h,
+                                             const float uv[2]) {
+    if (!isfinite(t) || !(1e-3f <= t && t <= FLT_MAX)) count_check(checks, RAYNA_CHECK_HIT_DIST);
+    const f3 p = at(ray, t);
+    if (!finite3(h.pos) || !approx_eq(length(ray.o - h.pos), t) || !approx_eq(h.pos.x, p.x) ||
+        !approx_eq(h.pos.y, p.y) || !approx_eq(h.pos.z, p.z))
+        count_check(checks, RAYNA_CHECK_HIT_POS);
+    if (!unit3(h.normal) || !unit3(h.ray_normal)) count_check(checks, RAYNA_CHECK_HIT_NORMAL);
+    if (!(uv[0] >= 0.0f && uv[0] <= 1.0f && uv[1] >= 0.0f && uv[1] <= 1.0f)) count_check(checks, RAYNA_CHECK_HIT_UV);
+}
+
 // ------------------------------------------------------------------------------------------ kernels
 
 // The camera ray of sample k of pixel `pixel`: render_px_msaa's stratified jitter (renderer.rs:370-389) and
@@ -781,11 +815,15 @@ __device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy,
 //
 // TEX: the scene has textured materials or an image sky (without, their code is compiled out).  A textured
 // material's colour receives no gradient.
+//
+// VALIDATE (single PBR frames): the engine's debug invariants are counted into fp.checks (validate_hit and the
+// checks of validate.rs): the ray shaded, its hit, the material colour, emitted radiance, the throughput after
+// a scatter and, where the path ends, its radiance.
 #ifndef RAYNA_SHADE_STRIP
 #define RAYNA_SHADE_STRIP 16
 #endif
 constexpr uint32_t SHADE_STRIP = RAYNA_SHADE_STRIP;
-template <bool ADJOINT, bool BATCH, bool TEX>
+template <bool ADJOINT, bool BATCH, bool TEX, bool VALIDATE = false>
 #ifndef RAYNA_SHADE_WAVES
 #define RAYNA_SHADE_WAVES 5
 #endif
@@ -839,6 +877,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
             else
                 ray = ray_lazy(ray.o, ray.d);
             if (queue_in) T = Col{pb.tr[slot], pb.tg[slot], pb.tb[slot]};
+            if (VALIDATE && !(finite3(ray.o) && unit3(ray.d))) count_check(fp.checks, RAYNA_CHECK_RAY);
             // Forward: the radiance this bounce adds, L += add.  After the first bounce the radiance field is read
             // and written only when add is not zero in every channel: L + 0 is L (L is never -0: it starts as
             // 0 + add, and an exact cancellation rounds to +0).
@@ -847,6 +886,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
             if (ref == 0u) {
                 if (!ADJOINT)
                     add = T * ((TEX && skybox == RAYNA_SKY_HDRI) ? sky_image(sv, sky_img, ray.d) : sky(skybox, ray.d));
+                if (VALIDATE && !colour_ok(add)) count_check(fp.checks, RAYNA_CHECK_COLOUR);
             } else {
                 SHit h;
                 const float t = pb.hit_t[slot];
@@ -855,9 +895,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                 const bool sdf_hit = (ref >> HIT_KIND_SHIFT) == (uint32_t)ITEM_SDF && pb.sdf_nx;
                 if (sdf_hit) sdf_n = f3{pb.sdf_nx[slot], pb.sdf_ny[slot], pb.sdf_nz[slot]};
                 hit_geometry(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, &h, sdf_hit, sdf_n);
+                if (VALIDATE) {
+                    float uv[2];
+                    int side;
+                    hit_uv_side(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, uv, &side);
+                    validate_hit(fp.checks, ray, t, h, uv);
+                }
                 const GMaterial m = sv.materials[h.material];
                 const Col colour =
                     material_colour<TEX>(sv, m, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, h);
+                if (VALIDATE && !colour_ok(colour)) count_check(fp.checks, RAYNA_CHECK_COLOUR);
                 const bool graded = !TEX || m.texture < 0;
                 double d0 = 0.0, d1 = 0.0, d2 = 0.0;
                 if (ADJOINT) {
@@ -872,6 +919,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                         L = Col{L.r - T.r * le.r, L.g - T.g * le.g, L.b - T.b * le.b};
                     } else {
                         add = T * le;
+                        if (VALIDATE && !colour_ok(add)) count_check(fp.checks, RAYNA_CHECK_COLOUR);
                     }
                 }
                 float u[4];
@@ -886,6 +934,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                                      m.b != 0.0f ? d2 * (double)L.b / (double)m.b : 0.0);
                     }
                     T = reflected(m, ray, h, colour, T);
+                    if (VALIDATE && !colour_ok(T)) count_check(fp.checks, RAYNA_CHECK_COLOUR);
                     if (depth < ray_depth) {
                         const DRay nr = make_ray(h.pos, dir);
                         pb.ox[slot] = nr.o.x;
@@ -902,8 +951,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHA
                 }
             }
             const bool adds = !(add.r == 0.0f && add.g == 0.0f && add.b == 0.0f);
-            if (!ADJOINT && queue_in && adds) L = Col{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
+            if (!ADJOINT && queue_in && (adds || VALIDATE)) L = Col{pb.lr[slot], pb.lg[slot], pb.lb[slot]};
             if (!ADJOINT) L = L + add;
+            if (VALIDATE && !keep && !colour_ok(L)) count_check(fp.checks, RAYNA_CHECK_SAMPLE);
             if (ADJOINT || !queue_in || adds) {
                 pb.lr[slot] = L.r;
                 pb.lg[slot] = L.g;
@@ -992,6 +1042,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FrameParams fp, const float 
     const uint32_t P = fp.n_pixels;
     const float n = (float)fp.spp;
     const float mean[3] = {sum[p] / n, sum[P + p] / n, sum[2 * P + p] / n};
+    if (fp.checks && !colour_ok(Col{mean[0], mean[1], mean[2]})) count_check(fp.checks, RAYNA_CHECK_PIXEL);
     const uint32_t x = p % fp.width, y = p / fp.width;
     const size_t o = layout == RAYNA_LAYOUT_XMAJOR ? ((size_t)x * fp.height + y) * 3 : (size_t)p * 3;
 #pragma unroll
@@ -1186,9 +1237,11 @@ void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers 
                   const uint32_t *count_in, int32_t *queue_out, uint32_t *count_out, uint32_t depth, int grid,
                   const AdjointParams *adjoint, const BatchView &bv, hipStream_t s) {
     const bool tex = sv.textured != 0;
-    auto kernel = bv.reqs   ? (tex ? k_shade<false, true, true> : k_shade<false, true, false>)  // batches: forward
-                  : adjoint ? (tex ? k_shade<true, false, true> : k_shade<true, false, false>)
-                            : (tex ? k_shade<false, false, true> : k_shade<false, false, false>);
+    const bool validate = fp.checks && !bv.reqs && !adjoint;
+    auto kernel = bv.reqs    ? (tex ? k_shade<false, true, true> : k_shade<false, true, false>)  // batches: forward
+                  : adjoint  ? (tex ? k_shade<true, false, true> : k_shade<true, false, false>)
+                  : validate ? (tex ? k_shade<false, false, true, true> : k_shade<false, false, false, true>)
+                             : (tex ? k_shade<false, false, true> : k_shade<false, false, false>);
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in, queue_out, count_out,
                        depth, adjoint ? *adjoint : AdjointParams{}, bv);
 }

@@ -36,6 +36,7 @@ struct FrameParams {
     uint32_t n_spp;          // samples per pixel in the chunk
     uint32_t n_slots;        // slots in the chunk: n_pixels * n_spp
     int32_t mode;            // RAYNA_MODE_*
+    uint32_t *checks = nullptr;  // validation on: RAYNA_CHECKS violation counters (RAYNA_CHECK_*), else null
 };
 
 // One request of a batch, as seen by one chunk of paths: its slots are [slot_base, slot_base + n_slots), sample

@@ -80,6 +80,9 @@ struct rayna_renderer {
     uint64_t chunk_target = TARGET_SLOTS;
     ExtendTuning tune;
     std::vector<hipEvent_t> events;
+    // debug validation (shared/validate.rs): RAYNA_CHECKS counters, counted in PBR frames while `validate`
+    uint32_t *checks = nullptr;
+    bool validate = false;
     std::string error;
 };
 
@@ -506,6 +509,7 @@ rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uin
             r->acc_dirty = false;
         }
         FrameParams fp = frame_params(r, r->frames);
+        if (r->validate && r->opts.mode == RAYNA_MODE_PBR) fp.checks = r->checks;
         const int64_t frame_index = (int64_t)r->frames;
         r->frames += 1;
         HIP_TRY(r, hipMemsetAsync(r->segments, 0, sizeof(unsigned long long), r->stream));
@@ -597,6 +601,35 @@ rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t) {
         (void)hipSetDevice(r->device);
         return alloc_paths(r);  // clears accumulation
     }
+    return RAYNA_OK;
+}
+
+rayna_status rayna_set_validate(rayna_renderer *r, int32_t enable) {
+    if (!r) return fail(r, RAYNA_INVALID_ARGUMENT, "null renderer");
+    HIP_TRY(r, hipSetDevice(r->device));
+    if (enable && !r->checks) {
+        void *p = nullptr;
+        HIP_TRY(r, hipMalloc(&p, RAYNA_CHECKS * sizeof(uint32_t)));
+        r->checks = static_cast<uint32_t *>(p);
+    }
+    if (enable) {
+        launch_fill_u32(r->checks, 0u, RAYNA_CHECKS, r->stream);
+        HIP_TRY(r, hipStreamSynchronize(r->stream));
+    }
+    r->validate = enable != 0;
+    return RAYNA_OK;
+}
+
+rayna_status rayna_validate_counts(rayna_renderer *r, uint64_t *counts, int32_t reset) {
+    if (!r || !counts) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    uint32_t c[RAYNA_CHECKS] = {};
+    if (r->checks) {
+        HIP_TRY(r, hipSetDevice(r->device));
+        HIP_TRY(r, hipMemcpyAsync(c, r->checks, sizeof(c), hipMemcpyDeviceToHost, r->stream));
+        if (reset) launch_fill_u32(r->checks, 0u, RAYNA_CHECKS, r->stream);
+        HIP_TRY(r, hipStreamSynchronize(r->stream));
+    }
+    for (int i = 0; i < RAYNA_CHECKS; ++i) counts[i] = c[i];
     return RAYNA_OK;
 }
 
@@ -830,6 +863,7 @@ void rayna_destroy(rayna_renderer *r) {
     if (r->mt) (void)hipFree(r->mt);
     if (r->sdf_buf) (void)hipFree(r->sdf_buf);
     if (r->rgba_dev) (void)hipFree(r->rgba_dev);
+    if (r->checks) (void)hipFree(r->checks);
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;

@@ -27,10 +27,15 @@ _lib: Optional[C.CDLL] = None
 
 # every symbol declared in include/rayna_gpu.h
 EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_options", "rayna_clear_accumulation",
-           "rayna_render", "rayna_render_srgb8", "rayna_set_material_colours", "rayna_adjoint", "rayna_set_tuning", "rayna_batch_create",
+           "rayna_render", "rayna_render_srgb8", "rayna_set_material_colours", "rayna_adjoint", "rayna_set_tuning",
+           "rayna_set_validate", "rayna_validate_counts", "rayna_batch_create",
            "rayna_batch_set_tuning", "rayna_render_batch", "rayna_batch_destroy", "rayna_batch_last_error",
            "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare", "rayna_scene_tree", "rayna_destroy",
            "rayna_last_error")
+
+
+# RAYNA_CHECK_* of include/rayna_gpu.h, in order: the invariants of the engine's shared/validate.rs
+VALIDATE_CHECKS = ("ray", "hit_dist", "hit_pos", "hit_normal", "hit_uv", "colour", "sample", "pixel")
 
 
 class RayError(RuntimeError):
@@ -70,6 +75,10 @@ def lib() -> C.CDLL:
                                     C.c_uint32, C.POINTER(A.RenderStats)]
         L.rayna_set_tuning.restype = C.c_int
         L.rayna_set_tuning.argtypes = [vp, C.POINTER(A.Tuning)]
+        L.rayna_set_validate.restype = C.c_int
+        L.rayna_set_validate.argtypes = [vp, C.c_int32]
+        L.rayna_validate_counts.restype = C.c_int
+        L.rayna_validate_counts.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int32]
         L.rayna_batch_create.restype = C.c_int
         L.rayna_batch_create.argtypes = [C.POINTER(A.SceneDesc), C.c_int32, C.POINTER(vp)]
         L.rayna_batch_set_tuning.restype = C.c_int
@@ -225,6 +234,16 @@ class Renderer:
         buffers and clears accumulation."""
         t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 0, 0, 0)
         _check(lib().rayna_set_tuning(self._h, C.byref(t)), self._h)
+
+    def set_validate(self, enable: bool = True):
+        """Count the engine's debug invariants (shared/validate.rs) in later PBR frames; on clears the counts."""
+        _check(lib().rayna_set_validate(self._h, 1 if enable else 0), self._h)
+
+    def validate_counts(self, reset: bool = False) -> dict:
+        """Violations per check (VALIDATE_CHECKS) since validation was turned on or last reset."""
+        c = (C.c_uint64 * len(VALIDATE_CHECKS))()
+        _check(lib().rayna_validate_counts(self._h, c, 1 if reset else 0), self._h)
+        return dict(zip(VALIDATE_CHECKS, (int(v) for v in c)))
 
     def clear_accumulation(self):
         _check(lib().rayna_clear_accumulation(self._h), self._h)
