@@ -561,6 +561,7 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     v.stack_need = merged.stack_need;
     v.packet_need = merged.packet_need;
     v.instances = has_instances(merged) ? 1 : 0;
+    v.sphere_tests = has_spheres(merged) ? 1 : 0;
     v.textured = has_textures(merged) ? 1 : 0;
     for (const SceneSlot &sl : b->slots)
         if (sl.skybox == RAYNA_SKY_HDRI) v.textured = 1;

@@ -5,6 +5,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdint.h>
 
 #include "../../include/rayna_gpu.h"
@@ -211,9 +212,12 @@ __device__ __forceinline__ bool box_t(const GBox &g, const DRay &r, float t0, fl
     return false;
 }
 
+// SPH: the scene intersects spheres in traversal (without, the double-precision sphere test is compiled out: a
+// sphere boundary then never occurs).
+template <bool SPH>
 __device__ __forceinline__ bool boundary_t(const SceneView &sv, const GVolume &v, const DRay &r, float t0, float t1,
                                            float *t) {
-    if (v.boundary_kind == ITEM_SPHERE) return sphere_t(sv.spheres[v.boundary_index], r, t0, t1, t);
+    if (v.boundary_kind == ITEM_SPHERE) return SPH && sphere_t(sv.spheres[v.boundary_index], r, t0, t1, t);
     int axis;
     float w;
     f3 sg;
@@ -222,14 +226,15 @@ __device__ __forceinline__ bool boundary_t(const SceneView &sv, const GVolume &v
 
 // VolumetricObject::full_intersect (object/volumetric.rs:86-160): entry with an unbounded interval, exit after
 // entry + 0.001, free flight -ln(U)/density.
+template <bool SPH>
 __device__ __forceinline__ bool volume_t(const SceneView &sv, const GVolume &v, const DRay &r, float t_end,
                                          const dm::Key &key, uint32_t pixel, uint32_t sample, uint32_t bounce,
                                          float *t) {
     float entry;
-    if (!boundary_t(sv, v, r, -__builtin_inff(), __builtin_inff(), &entry)) return false;
+    if (!boundary_t<SPH>(sv, v, r, -__builtin_inff(), __builtin_inff(), &entry)) return false;
     entry = fmaxf(entry, T_MIN);
     float exit_d;
-    if (!boundary_t(sv, v, r, entry + 0.001f, __builtin_inff(), &exit_d)) return false;
+    if (!boundary_t<SPH>(sv, v, r, entry + 0.001f, __builtin_inff(), &exit_d)) return false;
     const float inside = exit_d - entry;
     float u[4];
     dm::draw4(key, pixel, sample, bounce, dm::SLOT_MEDIUM_BASE + (uint32_t)v.object, u);
@@ -1186,14 +1191,14 @@ int extend_lds_stack() { return EXT_LDS_STACK; }
 
 int extend_blocks_per_cu() {
     int n = 0, nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend<false, false>, EXT_BLOCK, 0) != hipSuccess || n < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend<false, false, false>, EXT_BLOCK, 0) != hipSuccess ||
+        n < 1)
         n = 4;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<true, false>, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
-        n = std::min(n, nb);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false, true>, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
-        n = std::min(n, nb);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<true, true>, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
-        n = std::min(n, nb);
+    for (auto k : {k_extend<true, false, false>, k_extend<false, true, false>, k_extend<true, true, false>,
+                   k_extend<false, false, true>, k_extend<true, false, true>, k_extend<false, true, true>,
+                   k_extend<true, true, true>})
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
+            n = std::min(n, nb);
     return n;
 }
 int extend_block() { return EXT_BLOCK; }
@@ -1219,16 +1224,16 @@ void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers
                    const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
     if (!queue && depth == 0 && !sv.instances && sv.packet_need <= PACKET_STACK &&
         (bv.reqs || fp.n_spp >= PACKET_MIN_SPP)) {  // camera rays: wave packets
-        if (bv.reqs)
-            hipLaunchKernelGGL(k_extend_packet<true>, dim3(grid_for(fp.n_slots, EXT_BLOCK)), dim3(EXT_BLOCK), 0, s, sv,
-                               fp, pb, bv);
-        else
-            hipLaunchKernelGGL(k_extend_packet<false>, dim3(grid_for(fp.n_slots, EXT_BLOCK)), dim3(EXT_BLOCK), 0, s,
-                               sv, fp, pb, bv);
+        auto packet = bv.reqs ? (sv.sphere_tests ? k_extend_packet<true, true> : k_extend_packet<true, false>)
+                              : (sv.sphere_tests ? k_extend_packet<false, true> : k_extend_packet<false, false>);
+        hipLaunchKernelGGL(packet, dim3(grid_for(fp.n_slots, EXT_BLOCK)), dim3(EXT_BLOCK), 0, s, sv, fp, pb, bv);
         return;
     }
-    auto kernel = bv.reqs ? (sv.instances ? k_extend<true, true> : k_extend<true, false>)
-                          : (sv.instances ? k_extend<false, true> : k_extend<false, false>);
+    auto kernel = sv.sphere_tests
+                      ? (bv.reqs ? (sv.instances ? k_extend<true, true, true> : k_extend<true, false, true>)
+                                 : (sv.instances ? k_extend<false, true, true> : k_extend<false, false, true>))
+                      : (bv.reqs ? (sv.instances ? k_extend<true, true, false> : k_extend<true, false, false>)
+                                 : (sv.instances ? k_extend<false, true, false> : k_extend<false, false, false>));
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth, overflow, work,
                        tune, bv);
 }
@@ -1309,6 +1314,33 @@ void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s) {
 void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
                            unsigned long long *segments, hipStream_t s) {
     hipLaunchKernelGGL(k_count_segments, dim3(1), dim3(64), 0, s, queue_count, n, first_count, segments);
+}
+
+// ------------------------------------------------------------------------------------------ static workspace
+
+// The gradient copies of rayna_adjoint_static live in this file-scope array instead of memory the library
+// allocates, so that entry point allocates nothing.  Every GPU the library runs on holds its own copy until the
+// process ends.  In host code `scratch` names a host-side stand-in, not device memory: the device address of
+// each GPU's copy comes from hipGetSymbolAddress, taken once per device with that device current and kept.
+constexpr size_t ADJOINT_SCRATCH_DOUBLES = size_t(1) << 19;  // 4 MiB
+static __device__ __align__(256) double scratch[ADJOINT_SCRATCH_DOUBLES];
+
+size_t adjoint_scratch_doubles() { return ADJOINT_SCRATCH_DOUBLES; }
+
+hipError_t adjoint_scratch(int device, double **ptr) {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static double *addr[MAX_DEVICES] = {};
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = static_cast<double *>(p);
+    }
+    *ptr = addr[device];
+    return hipSuccess;
 }
 
 }  // namespace rayna

@@ -67,6 +67,18 @@ inline bool has_instances(const PackedScene &p) {
     return false;
 }
 
+// True if traversal of the packed scene intersects spheres: a sphere item (also inside a BVH mesh), a medium
+// bounded by a sphere, or a transformed sphere or sphere-bounded medium.
+inline bool has_spheres(const PackedScene &p) {
+    for (const GItem &it : p.items)
+        if (it.kind == ITEM_SPHERE) return true;
+    for (const GVolume &v : p.volumes)
+        if (v.boundary_kind == ITEM_SPHERE) return true;
+    for (const GXform &x : p.xforms)
+        if (x.inner_kind == ITEM_SPHERE) return true;
+    return false;
+}
+
 // True if some material of the packed scene has a texture or its sky is an image.
 inline bool has_textures(const PackedScene &p) {
     if (p.skybox == RAYNA_SKY_HDRI) return true;

@@ -173,8 +173,9 @@ __device__ __forceinline__ void finish_instance(const SceneView &sv, const PathB
 
 // Tests entry `i` of leaf range `code` and keeps it if closer, or equally close with a higher rank.  Returns true
 // when the entry is an instance, which ends the pass over the range.  INST: the scene holds instances (without,
-// the instance code is compiled out).
-template <bool INST>
+// the instance code is compiled out).  SPH: the scene intersects spheres (without, the double-precision sphere
+// test is compiled out, which keeps the traversal loop's registers free; no sphere item then occurs).
+template <bool INST, bool SPH>
 __device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_t code, int i, float leaf_t,
                                           Best &b, LaneStack &st, const dm::Key &key, uint32_t pixel,
                                           uint32_t sample, uint32_t bounce) {
@@ -221,7 +222,7 @@ __device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_
     float t = 0.0f;
     bool ok = false;
     switch (kind) {
-        case ITEM_SPHERE: ok = sphere_t(sv.spheres[j], r, T_MIN, t_end, &t); break;
+        case ITEM_SPHERE: ok = SPH && sphere_t(sv.spheres[j], r, T_MIN, t_end, &t); break;
         case ITEM_TRIANGLE: {
             const GTri g = sv.tris[j];
             float u, v, det;
@@ -240,7 +241,7 @@ __device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_
             ok = box_t(sv.boxes[j], r, T_MIN, t_end, &t, &axis, &w, &sg);
             break;
         }
-        case ITEM_VOLUME: ok = volume_t(sv, sv.volumes[j], r, t_end, key, pixel, sample, bounce, &t); break;
+        case ITEM_VOLUME: ok = volume_t<SPH>(sv, sv.volumes[j], r, t_end, key, pixel, sample, bounce, &t); break;
         default: break;
     }
     if (!ok) return false;
@@ -259,10 +260,10 @@ __device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_
 
 // Closest bounded hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501); the
 // unbounded objects are marched afterwards by k_march.  BATCH: the slots belong to the requests of bv, each
-// with its own scene root and RNG key.  INST: the scene holds instances.  Occupancy: a 128-lane block holds
+// with its own scene root and RNG key.  INST: the scene holds instances.  SPH: it intersects spheres.  Occupancy: a 128-lane block holds
 // 12 KiB of LDS stack, so 13 blocks fit a CU; the register budget of 80 VGPRs keeps six waves per SIMD (measured:
 // 8.5% faster than five waves with a 16-entry stack; seven waves spill).
-template <bool BATCH, bool INST>
+template <bool BATCH, bool INST, bool SPH>
 #ifndef RAYNA_EXT_WAVES
 #define RAYNA_EXT_WAVES 6
 #endif
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
             }
             while (leaf != 0) {  // the leaf's primitives, one per iteration
                 RAYNA_STAT(prims, 1);
-                if (leaf_test<INST>(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth) ||
+                if (leaf_test<INST, SPH>(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth) ||
                     ++leaf_i == leaf_n)
                     leaf = 0;
             }
@@ -446,7 +447,7 @@ __device__ __forceinline__ float wave_min(float v) {
     return fminf(fminf(r0, r1), fminf(r2, r3));
 }
 
-template <bool BATCH>
+template <bool BATCH, bool SPH>
 __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, FrameParams fp, PathBuffers pb,
                                                              BatchView bv) {
     __shared__ int32_t s_code[EXT_BLOCK / 64][PACKET_STACK];
@@ -472,7 +473,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
         sample = fp.s0 + ls % fp.n_spp;
     }
     Best b{FLT_MAX, -1, 0u, 0u, 0u};
-    LaneStack unused{nullptr, nullptr, 0, 0};  // leaf_test<false> pushes only for mesh-BVH links, handled here
+    LaneStack unused{nullptr, nullptr, 0, 0};  // leaf_test<false, SPH> pushes only for mesh-BVH links, handled here
     int sp = 0;
     auto push = [&](int32_t code, float k, unsigned long long m) {
         s_code[w][sp] = code;
@@ -558,7 +559,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
                     push(sv.items[start + i].index, kt, tm);
                     continue;
                 }
-                if (take) (void)leaf_test<false>(sv, ray, code, i, kt, b, unused, key, pixel, sample, 0u);
+                if (take) (void)leaf_test<false, SPH>(sv, ray, code, i, kt, b, unused, key, pixel, sample, 0u);
             }
         }
     }
