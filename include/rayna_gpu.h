@@ -347,6 +347,10 @@ typedef struct rayna_tuning {
     int32_t pipelines;             /* batches: chunks in flight at once, each on its own stream with its own
                                     * path state (chunk_slots paths each), 1..4 (default 3); the images do not
                                     * depend on it */
+    int32_t packet_depths;         /* extend: bounces traced as wave packets when the chunk's samples of a pixel
+                                    * are neighbouring slots (at least 16 per pixel, or a batch): 1 = the camera
+                                    * rays only, d = the bounces before depth d, -1 = none, 0 = the default (2);
+                                    * up to 16; the images do not depend on it */
 } rayna_tuning;
 
 /* Applies a tuning to a renderer.  A new chunk_slots re-plans the path buffers and clears accumulation. */

@@ -87,7 +87,7 @@ class RenderStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("chunk_slots", C.c_uint32), ("refill_lanes", C.c_uint32), ("wave_batch", C.c_uint32),
                 ("shade_blocks_per_cu", C.c_uint32), ("use_graph", C.c_int32), ("schedule", C.c_int32),
-                ("pipelines", C.c_int32)]
+                ("pipelines", C.c_int32), ("packet_depths", C.c_int32)]
 
 
 class Request(C.Structure):

@@ -58,13 +58,13 @@ class BatchRenderer:
 
     def set_tuning(self, chunk_slots: int = 0, refill_lanes: int = 0, wave_batch: int = 0,
                    shade_blocks_per_cu: int = 0, use_graph: bool = False, schedule: str = "given",
-                   pipelines: int = 0):
+                   pipelines: int = 0, packet_depths: int = 0):
         """Launch shape (0 keeps a default), graph capture, the order requests are traced in ("given",
         "shortest_first", "by_scene": scene groups by total samples, shortest first within a group, or
         "interleaved": by scene, with one-scene chunks dealt in turn from the groups) and the chunks in flight
         at once (`pipelines`, 1..4, each on its own stream; default 3)."""
         t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 1 if use_graph else 0,
-                     SCHEDULES[schedule], pipelines)
+                     SCHEDULES[schedule], pipelines, packet_depths)
         self._check(lib().rayna_batch_set_tuning(self._h, C.byref(t)))
         self.use_graph = use_graph
 

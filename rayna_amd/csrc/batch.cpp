@@ -585,6 +585,9 @@ rayna_status rayna_batch_set_tuning(rayna_batch *b, const rayna_tuning *t) {
         return bfail(b, RAYNA_UNSUPPORTED, "the interleaved schedule is not captured in a graph");
     b->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
     b->tune.wave_batch = t->wave_batch ? t->wave_batch : ExtendTuning{}.wave_batch;
+    if (t->packet_depths < -1 || t->packet_depths > 16) return bfail(b, RAYNA_INVALID_ARGUMENT, "tuning out of range");
+    b->tune.packet_depths = t->packet_depths == 0 ? ExtendTuning{}.packet_depths
+                                                   : (uint32_t)std::max(0, t->packet_depths);
     b->grid_shade = b->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : SHADE_BLOCKS_PER_CU);
     const uint64_t chunk = b->chunk_slots;
     b->chunk_slots = t->chunk_slots ? t->chunk_slots : DEFAULT_CHUNK_SLOTS;

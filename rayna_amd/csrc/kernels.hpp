@@ -71,9 +71,12 @@ struct AdjointParams {
 
 // Traversal tuning of the extend kernel: a wave takes new rays once `refill_lanes` of its lanes are idle,
 // claiming `wave_batch` queue positions per atomic.
+// packet_depths: the bounces (depth < packet_depths) traced as wave packets (k_extend_packet) where a chunk's
+// samples of a pixel are neighbouring slots.
 struct ExtendTuning {
     uint32_t refill_lanes = 32;
     uint32_t wave_batch = 128;
+    uint32_t packet_depths = 2;
 };
 
 // Shade grid: blocks of 256 threads per compute unit (grid-stride over the queue).

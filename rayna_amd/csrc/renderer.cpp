@@ -595,6 +595,9 @@ rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t) {
         return fail(r, RAYNA_INVALID_ARGUMENT, "tuning out of range");
     r->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
     r->tune.wave_batch = t->wave_batch ? t->wave_batch : ExtendTuning{}.wave_batch;
+    if (t->packet_depths < -1 || t->packet_depths > 16) return fail(r, RAYNA_INVALID_ARGUMENT, "tuning out of range");
+    r->tune.packet_depths = t->packet_depths == 0 ? ExtendTuning{}.packet_depths
+                                                   : (uint32_t)std::max(0, t->packet_depths);
     r->grid_shade = r->cus * (int)(t->shade_blocks_per_cu ? t->shade_blocks_per_cu : SHADE_BLOCKS_PER_CU);
     const uint64_t target = t->chunk_slots ? t->chunk_slots : TARGET_SLOTS;
     if (target != r->chunk_target) {

@@ -231,10 +231,10 @@ class Renderer:
         self._options = options
 
     def set_tuning(self, chunk_slots: int = 0, refill_lanes: int = 0, wave_batch: int = 0,
-                   shade_blocks_per_cu: int = 0):
-        """Launch shape of the wavefront kernels (0 keeps a default).  A new chunk_slots re-plans the path
-        buffers and clears accumulation."""
-        t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 0, 0, 0)
+                   shade_blocks_per_cu: int = 0, packet_depths: int = 0):
+        """Launch shape of the wavefront kernels (0 keeps a default; packet_depths -1: no packets).  A new
+        chunk_slots re-plans the path buffers and clears accumulation."""
+        t = A.Tuning(chunk_slots, refill_lanes, wave_batch, shade_blocks_per_cu, 0, 0, 0, packet_depths)
         _check(lib().rayna_set_tuning(self._h, C.byref(t)), self._h)
 
     def set_validate(self, enable: bool = True):
