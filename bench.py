@@ -43,6 +43,14 @@ EXTEND_ISSUE = {"triangle_mesh": {"valu_busy_pct": 102.7, "valu_lanes_active_pct
                                   "achieved_hbm_gbps": (2.0200e7 * 2.0 + 6.4708e6) * 1024 / 9 / 5.5552e-3 / 1e9}}
 
 
+_T0 = time.perf_counter()
+
+
+def _progress(msg: str):
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"bench [{time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 class _NoGc:
     """Collects, then pauses Python's cyclic collector for a timed region: a collection walks the whole heap
     (torch's objects included) and can stall the host, and so the GPU, for tens of milliseconds."""
@@ -66,10 +74,30 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(tw, cam, opts, target_seconds: float, threads: int):
+def host_cpus():
+    """Threads for the CPU legs (the host's CPU share for this GPU: at most 16), and what the host shows."""
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return min(16, share), {"cpu_model": model, "cpus_visible": os.cpu_count(), "cpus_in_affinity": share}
+
+
+def cpu_baseline(tw, cam, opts, target_seconds: float, threads: int, host: dict):
     """The oracle's fp64 restatement of the engine renderer (engine BVH boxes, recursive integrator) on a
-    strided pixel sample of the same frame, sized to about `target_seconds` of work."""
+    strided pixel sample of the same frame: the median of three runs, each about a fifth of `target_seconds`
+    of work; beside it the same sample by the fp32 restatement (the GPU's arithmetic, tight boxes, forward
+    integrator) and by the fp64 one with tight boxes."""
+    import statistics
+
     import oracle
+    per_run = target_seconds / 5.0
     # calibration pass: a tiny sample
     row_step, col_step = max(1, opts.height // 4), max(1, opts.width // 8)
     _, _, secs, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_REFERENCE, row_step=row_step,
@@ -77,7 +105,7 @@ def cpu_baseline(tw, cam, opts, target_seconds: float, threads: int):
     n_px = ((opts.height + row_step - 1) // row_step) * ((opts.width + col_step - 1) // col_step)
     per_px = max(secs / n_px, 1e-7)
     for _ in range(2):  # a tiny sample overstates the cost per pixel (idle threads), so refine once
-        want_px = max(1.0, target_seconds / per_px)
+        want_px = max(1.0, per_run / per_px)
         scale = max(1.0, (opts.width * opts.height / want_px) ** 0.5)
         row_step = max(1, int(round(scale)))
         col_step = max(1, int(round(scale)))
@@ -85,21 +113,31 @@ def cpu_baseline(tw, cam, opts, target_seconds: float, threads: int):
                                        row_step=row_step, col_step=col_step, threads=threads)
         n_px = out.shape[0] * out.shape[1]
         per_px = max(secs / n_px, 1e-7)
-        if secs > 0.4 * target_seconds or scale == 1.0:
+        if secs > 0.4 * per_run or scale == 1.0:
             break
+    runs = [secs]
+    for _ in range(2):
+        runs.append(tw.render(cam, opts, precision=64, box_mode=oracle.BOX_REFERENCE, row_step=row_step,
+                              col_step=col_step, threads=threads)[2])
+    med = statistics.median(runs)
     samples = n_px * opts.samples
-    # the same sample with refitted tight boxes (identical image, the engine's culling removed)
+    # the same sample with refitted tight boxes (identical image, the engine's culling removed), and in fp32
     out2, _, secs2, _ = tw.render(cam, opts, precision=64, box_mode=oracle.BOX_TIGHT, row_step=row_step,
                                   col_step=col_step, threads=threads)
+    _, _, secs32, _ = tw.render(cam, opts, precision=32, box_mode=oracle.BOX_TIGHT, row_step=row_step,
+                                col_step=col_step, threads=threads)
     return {
-        "value": samples / secs / 1e6,
+        "value": samples / med / 1e6,
         "unit": "Msamples/s",
         "cores": threads,
         "kind": "port",
         "sample": f"fp64 oracle, engine BVH boxes, every {row_step}th row x every {col_step}th column at "
-                  f"{opts.samples} spp ({n_px} pixels, {samples} samples, {segs} closest-hit queries) in {secs:.1f} s",
+                  f"{opts.samples} spp ({n_px} pixels, {samples} samples, {segs} closest-hit queries): median of "
+                  f"3 runs ({', '.join(f'{x:.2f}' for x in runs)} s) on {threads} threads",
+        **host,
         "tight_boxes_value": samples / secs2 / 1e6,
         "tight_boxes_identical_image": bool((out == out2).all()),
+        "fp32_oracle_tight_boxes_value": samples / secs32 / 1e6,
     }
 
 
@@ -122,6 +160,45 @@ def rmse_leg(r, tw, cam, opts, threads: int, ref_spp: int = 16384, grid: int = 1
     return {"reference": f"fp64 oracle, {ref_spp} spp, seed {opts.seed + 1}, every {rs}th row x {cs}th column "
                          f"({ref.shape[0] * ref.shape[1]} pixels, {secs:.1f} s)",
             "gpu_rmse": rm(gpu), "fp64_oracle_rmse": rm(f64), "spp": opts.samples}
+
+
+def frame_leg(name: str, steps: int = 3, warmup: int = 1):
+    """Another workload's frames, timed as the headline is (device-resident, wall clock around the frames)."""
+    import torch
+    from rayna_amd import Renderer
+
+    scene, cam, opts = workload(name)
+    with Renderer(scene, cam, opts) as r:
+        for _ in range(warmup):
+            r.render(to_host=False)
+        torch.cuda.synchronize()
+        ext = sh = 0.0
+        segs = 0
+        with _NoGc():
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                st = r.render(to_host=False).stats
+                ext += st.extend_ms
+                sh += st.shade_ms
+                segs += st.segments
+            torch.cuda.synchronize()
+            secs = time.perf_counter() - t0
+    n = opts.width * opts.height * opts.samples
+    return {"workload": name, "width": opts.width, "height": opts.height, "spp": opts.samples,
+            "ray_depth": opts.ray_depth, "msamples_per_s": n * steps / secs / 1e6, "mrays_per_s": segs / secs / 1e6,
+            "ms_per_frame": secs * 1e3 / steps, "extend_ms_per_frame": ext / steps,
+            "shade_ms_per_frame": sh / steps, "frames": steps}
+
+
+def inverse_leg():
+    """The inverse-rendering configuration run once at its configured size (diff.recover_colours defaults:
+    256x256, a 1024-spp target, 16 spp forward + path-replay adjoint per step, Adam lr 0.02, 200 steps)."""
+    from rayna_amd import diff
+    res = diff.recover_colours()
+    return {"steps": len(res.losses), "loss_first": res.losses[0], "loss_last": res.losses[-1],
+            "param_rmse_first": res.param_rmse[0], "param_rmse_last": res.param_rmse[-1],
+            "identifiable_colours": res.identifiable, "adjoint_forward_ratio": res.adjoint_forward_ratio,
+            "seconds": res.seconds}
 
 
 def adjoint_leg(steps: int):
@@ -179,6 +256,7 @@ def batch_leg(tune: bool = True):
             tr = AT.tune(b, half, repeats=1)
         cfg = dict(tr.best)
         tuned = {"best": tr.best, "best_ms": tr.best_ms, "default_ms": tr.default_ms, "trials": len(tr.trials)}
+        _progress(f"batch: autotuned ({len(tr.trials)} trials)")
     shape = {k: cfg[k] for k in ("chunk_slots", "refill_lanes", "wave_batch", "shade_blocks_per_cu")}
 
     firsts = {q.scene: q for q in reversed(reqs)}
@@ -200,7 +278,12 @@ def batch_leg(tune: bool = True):
         return total_s, np.array(lat) * 1e3
 
     one_by_one(warm)
-    seq_s, seq_lat = one_by_one(reqs)
+    seq_runs = []
+    for i in range(3):
+        seq_runs.append(one_by_one(reqs))
+        _progress(f"batch: one request per launch, run {i + 1}: {seq_runs[-1][0]:.2f} s")
+    seq_runs.sort(key=lambda x: x[0])
+    seq_s, seq_lat = seq_runs[1]  # the median of three
     for r in rends.values():
         r.close()
 
@@ -220,12 +303,27 @@ def batch_leg(tune: bool = True):
 
     bat_s, st = batched(cfg["schedule"])
     sjf_s, st2 = batched("shortest_first")
+    _progress(f"batch: batched {bat_s:.2f} s, shortest first {sjf_s:.2f} s")
+    # the whole launch sequence captured in a hipGraph (the interleaved schedule is traced eager only): the
+    # first call captures, the next replays
+    graph_schedule = "by_scene" if cfg["schedule"] == "interleaved" else cfg["schedule"]
+    b.set_tuning(schedule=graph_schedule, use_graph=True, pipelines=cfg["pipelines"], **shape)
+    b.render(reqs, to_host=False)
+    with _NoGc():
+        t0 = time.perf_counter()
+        b.render(reqs)
+        graph_s = time.perf_counter() - t0
+    b.set_tuning(schedule=graph_schedule, use_graph=False, pipelines=cfg["pipelines"], **shape)
+    eager_same_s, _ = batched(graph_schedule)
     b.close()
 
     def pct(a, q):
         return float(np.percentile(a, q))
     return {"workload": "batch_requests", "requests": len(reqs), "samples": total,
             "one_request_per_launch_msamples_per_s": total / seq_s / 1e6,
+            "one_request_per_launch_runs_s": [x[0] for x in seq_runs],
+            "graph_replay": {"schedule": graph_schedule, "msamples_per_s": total / graph_s / 1e6, "s": graph_s,
+                             "eager_same_schedule_s": eager_same_s},
             "batched_msamples_per_s": total / bat_s / 1e6, "batched_over_one_per_launch": seq_s / bat_s,
             "one_request_per_launch_s": seq_s, "batched_s": bat_s, "batched_kernel_ms": st.kernel_ms,
             "chunks": st.chunks, "launches": st.launches,
@@ -243,12 +341,14 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="triangle_mesh")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU legs' threads (0: the host's share, <= 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering leg")
     ap.add_argument("--no-tune", action="store_true", help="batched leg with the default launch shape")
+    ap.add_argument("--no-workloads", action="store_true", help="skip the other workloads' frames")
+    ap.add_argument("--no-inverse", action="store_true", help="skip the inverse-rendering run")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB three times (the counter calibration kernel)")
     a = ap.parse_args(argv)
@@ -272,9 +372,11 @@ def main(argv=None):
     if torch.cuda.is_available():  # the framework's device context first, so that its setup is not timed
         torch.cuda.synchronize()
     scene, cam, opts = workload(a.workload)
+    _progress(f"{a.workload}: scene described")
     t_build = time.time()
     r = Renderer(scene, cam, opts)
     t_build = time.time() - t_build
+    _progress(f"{a.workload}: renderer created in {t_build:.1f} s")
     for _ in range(a.warmup):
         r.render(to_host=False)
     torch.cuda.synchronize() if torch.cuda.is_available() else None
@@ -297,8 +399,14 @@ def main(argv=None):
         launches += st.extend_launches
     torch.cuda.synchronize() if torch.cuda.is_available() else None
     t1 = time.perf_counter()
+    # the same frames with the image copied to the caller's host buffer (Renderer::render returns the image)
+    t2 = time.perf_counter()
+    for _ in range(a.steps):
+        r.render(layout="xmajor")
+    t3 = time.perf_counter()
     nogc.__exit__(None)
     secs = t1 - t0
+    _progress(f"{a.workload}: {a.steps} timed frames, {secs * 1e3 / a.steps:.2f} ms each")
     samples_per_frame = opts.width * opts.height * opts.samples
     value = samples_per_frame * a.steps / secs / 1e6
     ext_bytes = EXTEND_BYTES_PER_SEGMENT * segs
@@ -330,6 +438,9 @@ def main(argv=None):
         "step_ms_min_median_max": [min(step_ms), sorted(step_ms)[len(step_ms) // 2], max(step_ms)],
         "extend_launches_per_frame": launches // max(1, a.steps),
         "frame_alg_gbps": frame_bytes / secs / 1e9,
+        "to_host": {"msamples_per_s": samples_per_frame * a.steps / (t3 - t2) / 1e6,
+                    "ms_per_step": (t3 - t2) * 1e3 / a.steps,
+                    "note": "each frame's image copied to host memory (x-major float32), as Renderer::render hands it"},
         "scene_build_s": t_build,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
@@ -338,19 +449,36 @@ def main(argv=None):
     }
     if a.workload in EXTEND_ISSUE:  # profiled, not measured by this run
         line["extend_issue_profiled"] = EXTEND_ISSUE[a.workload]
+    threads, host = host_cpus()
+    if a.cpu_threads:
+        threads = a.cpu_threads
     tw = None
     if not a.no_cpu_baseline:
         import oracle
         tw = oracle.TwinScene(scene)
+        _progress("oracle scene built")
         r.clear_accumulation()
-        line["rmse"] = rmse_leg(r, tw, cam, opts, a.cpu_threads)
+        line["rmse"] = rmse_leg(r, tw, cam, opts, threads)
+        _progress("rmse leg done")
     r.close()
+    if not a.no_workloads:
+        line["workloads"] = {}
+        for w in ("random_spheres", "sdf_volume", "triangle_mesh"):
+            if w != a.workload:
+                line["workloads"][w] = frame_leg(w)
+                _progress(f"workload {w} done")
     if not a.no_adjoint:
         line["adjoint"] = adjoint_leg(max(3, a.steps))
+        _progress("adjoint leg done")
+    if not a.no_inverse:
+        line["inverse_rendering"] = inverse_leg()
+        _progress("inverse rendering done")
     if not a.no_batch:
         line["batch"] = batch_leg(tune=not a.no_tune)
+        _progress("batch leg done")
     if not a.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(tw, cam, opts, a.cpu_seconds, a.cpu_threads)
+        line["cpu_baseline"] = cpu_baseline(tw, cam, opts, a.cpu_seconds, threads, host)
+        _progress("cpu baseline done")
     print(json.dumps(line), flush=True)
     return 0
 

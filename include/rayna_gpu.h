@@ -38,10 +38,10 @@ typedef enum rayna_status {
  * `colour` (SolidTexture, rayna_engine/src/texture/solid.rs:29) when `texture` is 0, else the texture
  * textures[texture - 1] of the scene.
  *   LAMBERTIAN  lambertian.rs:28-48   colour = albedo
- *   METAL       metal.rs:65-95        colour = albedo, fuzz
- *   DIELECTRIC  dielectric.rs:113-175 colour = albedo, refractive_index, density
- *   ISOTROPIC   isotropic.rs:203-228  colour = albedo, density
- *   LIGHT       light.rs:244-260      colour = emissive
+ *   METAL       metal.rs:17-45         colour = albedo, fuzz
+ *   DIELECTRIC  dielectric.rs:19-79   colour = albedo, refractive_index, density
+ *   ISOTROPIC   isotropic.rs:29-52    colour = albedo, density
+ *   LIGHT       light.rs:17-32        colour = emissive
  */
 enum {
     RAYNA_MAT_LAMBERTIAN = 0,
@@ -62,10 +62,10 @@ typedef struct rayna_material {
 
 /* ---------------------------------------------------------------- meshes
  * rayna_engine/src/mesh/mod.rs:68-85 (MeshInstance).
- *   SPHERE         SphereMesh::new(centre, radius)              primitive/sphere.rs:122-133
- *   TRIANGLE       one Triangle::new(vertices, normals)         primitive/triangle.rs:240-256
- *   TRIANGLE_LIST  MeshList of Triangles (its own SAH BVH)      advanced/list.rs:161-192
- *   PARALLELOGRAM  ParallelogramMesh::new(Planar::new(p,u,v))   planar/parallelogram.rs:444-458
+ *   SPHERE         SphereMesh::new(centre, radius)              primitive/sphere.rs:27-36
+ *   TRIANGLE       one Triangle::new(vertices, normals)         primitive/triangle.rs:22-35
+ *   TRIANGLE_LIST  MeshList of Triangles (its own SAH BVH)      advanced/list.rs:30-59
+ *   PARALLELOGRAM  ParallelogramMesh::new(Planar::new(p,u,v))   planar/parallelogram.rs:25-37
  *   AXIS_BOX       AxisBoxMesh::new(a, b)                       primitive/axis_box.rs:28-37
  *   SDF            RaymarchedIsosurfaceMesh::new_custom         isosurface/raymarched.rs:51-57
  *   BVH            BvhMesh::new(meshes) (its own SAH BVH)        advanced/bvh.rs:28-45
@@ -121,7 +121,7 @@ typedef struct rayna_sdf {
 
 /* ---------------------------------------------------------------- objects
  * rayna_engine/src/object/mod.rs:47-52 (ObjectInstance).
- *   SIMPLE      SimpleObject::new_uncorrected(mesh, material, transform)   object/simple.rs:175-200
+ *   SIMPLE      SimpleObject::new_uncorrected(mesh, material, transform)   object/simple.rs:78-92
  *   VOLUMETRIC  VolumetricObject::new_uncorrected(mesh, material, density, transform)
  *                                                                          object/volumetric.rs:42-72
  * `transform` selects the object's ObjectTransform (object/transform.rs): 0 is the identity, k > 0 the matrix
@@ -190,7 +190,7 @@ typedef struct rayna_image {
     const float *rgb;
 } rayna_image;
 
-/* ---------------------------------------------------------------- skybox (skybox/mod.rs:196-204) */
+/* ---------------------------------------------------------------- skybox (skybox/mod.rs:26-43) */
 enum {
     RAYNA_SKY_NONE = 0,   /* NoSkybox       skybox/none.rs:8-13 */
     RAYNA_SKY_SIMPLE = 1, /* SimpleSkybox   skybox/simple.rs:12-22 */
@@ -314,7 +314,7 @@ rayna_status rayna_render_srgb8(rayna_renderer *r, uint8_t *out_rgba, uint32_t o
 /* Replaces the colour of materials [0, n) of the current scene: the albedo of lambertian, metal,
  * dielectric and isotropic materials, the emission of lights (3 floats each, in rayna_scene_desc.materials
  * order).  Clears accumulation.  No engine counterpart: the engine fixes a material's texture when the scene
- * is built (material/lambertian.rs:17-20 etc.), so a caller there rebuilds the Scene and calls set_scene
+ * is built (material/lambertian.rs:12-14 etc.: the albedo is a texture held by the material), so a caller there rebuilds the Scene and calls set_scene
  * (render/renderer.rs:158-162); this keeps the BVH and the device scene and rewrites only the colours. */
 rayna_status rayna_set_material_colours(rayna_renderer *r, const float *rgb, uint32_t n);
 
@@ -355,17 +355,6 @@ typedef struct rayna_tuning {
 
 /* Applies a tuning to a renderer.  A new chunk_slots re-plans the path buffers and clears accumulation. */
 rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t);
-
-/* rayna_adjoint without allocating: no counterpart in the engine; for callers that cannot allocate device
- * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  d_image_dev
- * (height x width x 3 floats, row-major) and grad_dev (3 doubles per material, overwritten) are device
- * memory on the renderer's device.  The gradient copies live in a static workspace of 4 MiB per GPU (the
- * gradients of at most 174762 materials, else RAYNA_INVALID_ARGUMENT), held until the process ends; the
- * renderer's own buffers must be in place (a frame rendered since the last set_scene / set_options /
- * set_tuning, else RAYNA_INVALID_ARGUMENT).  Calls on one GPU share the workspace: calls that overlap on one
- * GPU race.  Synchronous; no stats. */
-rayna_status rayna_adjoint_static(rayna_renderer *r, uint32_t frame, const float *d_image_dev, double *grad_dev,
-                                  uint32_t n_materials);
 
 /* ---------------------------------------------------------------- debug validation (shared/validate.rs:13-198)
  * The engine asserts these invariants in debug builds along every path (render/renderer.rs:395-561).  With

@@ -19,7 +19,7 @@
 //      (shared/generic_bvh.rs:67-86), which makes the tree a pure function of the input;
 //  (3) SDFs are an enumerated set instead of closures (mesh/isosurface/mod.rs:7-9);
 //  (4) "tight" box mode: the BVH topology is the engine's, but boxes are refitted without the origin
-//      that Aabb::encompass_iter folds in (shared/aabb.rs:117-120) and tested conservatively (relative
+//      that Aabb::encompass_iter folds in (shared/aabb.rs:64-67) and tested conservatively (relative
 //      slack 2^-16).  Culling then never removes the closest primitive, and the closest hit with ties
 //      going to the later primitive in depth-first order -- what the engine's shrinking inclusive
 //      interval computes (mesh/advanced/bvh.rs:77-91) -- no longer depends on the traversal order;
@@ -154,7 +154,7 @@ inline Aabb aabb_new(const double a[3], const double b[3]) {
 }
 // Aabb::default(): both corners at the origin.
 inline Aabb aabb_origin() { return Aabb{{0, 0, 0}, {0, 0, 0}}; }
-// Aabb::encompass (aabb.rs:109-114)
+// Aabb::encompass (aabb.rs:56-61)
 inline Aabb encompass(const Aabb &a, const Aabb &b) {
     double mn[3], mx[3];
     for (int i = 0; i < 3; ++i) {
@@ -163,7 +163,7 @@ inline Aabb encompass(const Aabb &a, const Aabb &b) {
     }
     return aabb_new(mn, mx);
 }
-// Aabb::encompass_points (aabb.rs:123-132)
+// Aabb::encompass_points (aabb.rs:70-79)
 inline Aabb encompass_points(const double (*pts)[3], int n) {
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int k = 0; k < n; ++k)
@@ -173,7 +173,7 @@ inline Aabb encompass_points(const double (*pts)[3], int n) {
         }
     return aabb_new(mn, mx);
 }
-// Aabb::min_padded (aabb.rs:137-142)
+// Aabb::min_padded (aabb.rs:84-89)
 inline Aabb min_padded(const Aabb &b, double thresh) {
     double dims[3], centre[3], mn[3], mx[3];
     for (int i = 0; i < 3; ++i) {
@@ -340,7 +340,7 @@ inline void hash_bvh(const Bvh &b, int node, uint64_t &h) {
 }
 
 // ================================================================================================
-// The scene: an ObjectList of the input objects (object/list.rs:297-316).
+// The scene: an ObjectList of the input objects (object/list.rs:59-78).
 // ================================================================================================
 
 enum BoxMode { BOX_REFERENCE = 0, BOX_TIGHT = 1 };
@@ -442,7 +442,7 @@ inline void noise_permutation(uint32_t seed, int32_t *out) {
 inline bool mesh_aabb(const Scene &s, int mesh, Aabb *ref, Aabb *tight) {
     const rayna_mesh &m = s.meshes[mesh];
     switch (m.kind) {
-        case RAYNA_MESH_SPHERE: {  // sphere.rs:123-132
+        case RAYNA_MESH_SPHERE: {  // sphere.rs:27-36
             const double *sp = &s.spheres[4 * m.index];
             double a[3], b[3];
             for (int i = 0; i < 3; ++i) {
@@ -452,11 +452,11 @@ inline bool mesh_aabb(const Scene &s, int mesh, Aabb *ref, Aabb *tight) {
             *ref = *tight = aabb_new(a, b);
             return true;
         }
-        case RAYNA_MESH_TRIANGLE: {  // triangle.rs:253
+        case RAYNA_MESH_TRIANGLE: {  // triangle.rs:33
             *ref = *tight = encompass_points((const double(*)[3]) & s.tri_v[9 * m.index], 3);
             return true;
         }
-        case RAYNA_MESH_TRIANGLE_LIST: {  // list.rs:176-181: encompass_iter over the triangles
+        case RAYNA_MESH_TRIANGLE_LIST: {  // mesh/advanced/list.rs:44-49: encompass_iter over the triangles
             if (m.count <= 0) return false;
             Aabb r = aabb_origin();
             Aabb t = encompass_points((const double(*)[3]) & s.tri_v[9 * m.index], 3);
@@ -469,7 +469,7 @@ inline bool mesh_aabb(const Scene &s, int mesh, Aabb *ref, Aabb *tight) {
             *tight = t;
             return true;
         }
-        case RAYNA_MESH_PARALLELOGRAM: {  // parallelogram.rs:445-457
+        case RAYNA_MESH_PARALLELOGRAM: {  // parallelogram.rs:25-37
             const double *pp = &s.paras[9 * m.index];
             double pts[4][3];
             for (int i = 0; i < 3; ++i) {
@@ -619,7 +619,7 @@ inline void build_scene(Scene &s, const rayna_scene_desc &d, bool literal) {
         }
     }
 
-    // BLAS per triangle-list mesh (MeshList -> BvhMesh -> GenericBvh over the triangles, list.rs:162-190) and per
+    // BLAS per triangle-list mesh (MeshList -> BvhMesh -> GenericBvh over the triangles, mesh/advanced/list.rs:30-59) and per
     // BVH mesh (BvhMesh::new -> GenericBvh over the meshes, bvh.rs:35-45)
     s.mesh_blas.assign(s.meshes.size(), -1);
     for (size_t m = 0; m < s.meshes.size(); ++m) {
@@ -670,7 +670,7 @@ inline void build_scene(Scene &s, const rayna_scene_desc &d, bool literal) {
         s.blas_dfs.push_back(std::move(dfs));
     }
 
-    // TLAS (BvhObject::new_uncorrected over the bounded objects, object/bvh.rs:422-431)
+    // TLAS (BvhObject::new_uncorrected over the bounded objects, object/bvh.rs:49-58)
     {
         std::vector<Aabb> ob(s.bounded.size());
         for (size_t i = 0; i < s.bounded.size(); ++i) ob[i] = s.obj_ref_box[s.bounded[i]];
@@ -756,7 +756,7 @@ struct Ctx {
     uint64_t *prim_tests;
 };
 
-// Aabb::hit (aabb.rs:170-206) in the engine's form, or the conservative tight form (deviation 4).
+// Aabb::hit (aabb.rs:117-153) in the engine's form, or the conservative tight form (deviation 4).
 template <class R>
 inline bool box_hit(const Ctx<R> &c, const NodeBoxes &nb, const Ray<R> &ray, const Interval<R> &iv) {
     const bool tight = c.box_mode == BOX_TIGHT;
@@ -774,7 +774,7 @@ inline bool box_hit(const Ctx<R> &c, const NodeBoxes &nb, const Ray<R> &ray, con
     tmin = Math<R>::fmax(tmin, Math<R>::fmin(tz1, tz2));
     tmax = Math<R>::fmin(tmax, Math<R>::fmax(tz1, tz2));
     if (!tight) {
-        // Interval::range_overlaps (interval.rs:298-324)
+        // Interval::range_overlaps (interval.rs:67-93)
         R low = tmin, high = tmax;
         if (iv.has_start) low = (tmin > iv.start) ? tmin : iv.start;
         if (iv.has_end) high = (tmax < iv.end) ? tmax : iv.end;
@@ -835,7 +835,7 @@ inline bool sphere_intersect(const double *sp, const Ray<R> &ray, const Interval
     return true;
 }
 
-// Triangle::intersect (triangle.rs:272-328), Moller-Trumbore.
+// Triangle::intersect (triangle.rs:53-109), Moller-Trumbore.
 template <class R>
 inline bool triangle_intersect(const double *tv, const double *tn, const Ray<R> &ray, const Interval<R> &iv,
                                Hit<R> &h) {
@@ -853,7 +853,7 @@ inline bool triangle_intersect(const double *tv, const double *tn, const Ray<R> 
     if (v < R(0) || u + v > R(1)) return false;
     const R t = dot(e2, q) * inv_det;
     if (!iv.contains(t)) return false;
-    // interpolate_normals (triangle.rs:333-338): fold from zero, try_normalize
+    // interpolate_normals (triangle.rs:114-119): fold from zero, try_normalize
     const R b0 = (R(1) - u) - v;
     const V3<R> n0 = cast3<R>(tn), n1 = cast3<R>(tn + 3), n2 = cast3<R>(tn + 6);
     const V3<R> zero{R(0), R(0), R(0)};
@@ -872,7 +872,7 @@ inline bool triangle_intersect(const double *tv, const double *tn, const Ray<R> 
     return true;
 }
 
-// Planar::new (planar/mod.rs:303-314) + ParallelogramMesh::intersect (parallelogram.rs:469-477).
+// Planar::new (planar/mod.rs:79-90) + ParallelogramMesh::intersect (parallelogram.rs:49-57).
 template <class R>
 inline bool parallelogram_intersect(const double *pp, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
     // plane constants in double (computed once at construction in the engine), then cast
@@ -1053,7 +1053,7 @@ inline R sdf_eval(const Scene &s, const rayna_sdf &f, const V3<R> &p) {
     }
 }
 
-// RaymarchedIsosurfaceMesh::intersect (raymarched.rs:228-277)
+// RaymarchedIsosurfaceMesh::intersect (raymarched.rs:73-122)
 template <class R>
 inline bool sdf_intersect(const Scene &s, const rayna_sdf &f, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
     const R eps = (R)f.epsilon;
@@ -1132,7 +1132,7 @@ inline bool blas_node(const Ctx<R> &c, int bi, const rayna_mesh &m, int node, in
     return found;
 }
 
-// Object::full_intersect of one object of the list in its own space (SimpleObject simple.rs:215-225,
+// Object::full_intersect of one object of the list in its own space (SimpleObject object/simple.rs:107-117,
 // VolumetricObject volumetric.rs:86-160).
 template <class R>
 inline bool object_intersect_local(const Ctx<R> &c, int o, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &h) {
@@ -1222,7 +1222,7 @@ inline bool object_intersect(const Ctx<R> &c, int o, const Ray<R> &ray, const In
     return !iv.has_end || h.t <= iv.end;
 }
 
-// BvhObject::bvh_node_intersect (object/bvh.rs:443-481)
+// BvhObject::bvh_node_intersect (object/bvh.rs:70-108)
 template <class R>
 inline bool tlas_node(const Ctx<R> &c, int node, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &out) {
     const Scene &s = *c.s;
@@ -1243,7 +1243,7 @@ inline bool tlas_node(const Ctx<R> &c, int node, const Ray<R> &ray, const Interv
     return found;
 }
 
-// ObjectList::full_intersect (object/list.rs:350-367): the BVH result, then the unbounded objects with the
+// ObjectList::full_intersect (object/list.rs:112-129): the BVH result, then the unbounded objects with the
 // caller's interval; Iterator::min keeps the first of equal minima.
 template <class R>
 inline bool scene_intersect(const Ctx<R> &c, const Ray<R> &ray, const Interval<R> &iv, Hit<R> &out) {
@@ -1275,7 +1275,7 @@ inline V3<R> unit_sphere(R r1, R r2) {  // rng::normal_on_unit_sphere (shared/rn
     return V3<R>{(cs * R(2)) * q, (sn * R(2)) * q, R(1) - R(2) * r2};
 }
 
-// exp of a colour channel: the engine calls f32::exp (dielectric.rs:164, isotropic.rs:226).
+// exp of a colour channel: the engine calls f32::exp (dielectric.rs:69, isotropic.rs:51).
 template <class R>
 inline float chan_exp(float x) {
     if (sizeof(R) == 4) return exp_f32(x);
@@ -1292,7 +1292,7 @@ inline bool scatter(const rayna_material &m, const Ray<R> &ray, const Hit<R> &h,
             if (!try_normalize(v, dir)) *dir = h.ray_normal;
             return true;
         }
-        case RAYNA_MAT_METAL: {  // metal.rs:66-82
+        case RAYNA_MAT_METAL: {  // metal.rs:17-33
             const V3<R> refl = reflect(ray.dir, h.ray_normal);
             const V3<R> rnd = unit_sphere(u[0], u[1]);
             const V3<R> v = refl + rnd * (R)m.fuzz;
@@ -1302,7 +1302,7 @@ inline bool scatter(const rayna_material &m, const Ray<R> &ray, const Hit<R> &h,
             }
             return false;
         }
-        case RAYNA_MAT_DIELECTRIC: {  // dielectric.rs:114-134, reflectance :169-174
+        case RAYNA_MAT_DIELECTRIC: {  // dielectric.rs:19-39, reflectance :74-79
             const R ior = (R)m.refractive_index;
             const R ratio = h.front_face ? R(1) / ior : ior;
             const R cos_t = Math<R>::fmin(dot(-ray.dir, h.ray_normal), R(1));
@@ -1315,15 +1315,15 @@ inline bool scatter(const rayna_material &m, const Ray<R> &ray, const Hit<R> &h,
             *dir = (tir || schlick) ? reflect(ray.dir, h.ray_normal) : refract(ray.dir, h.ray_normal, ratio);
             return true;
         }
-        case RAYNA_MAT_ISOTROPIC:  // isotropic.rs:204-206
+        case RAYNA_MAT_ISOTROPIC:  // isotropic.rs:29-31
             *dir = unit_sphere(u[0], u[1]);
             return true;
-        default:  // LIGHT (light.rs:245)
+        default:  // LIGHT (light.rs:17)
             return false;
     }
 }
 
-// Material::emitted_light (light.rs:247-249; default black mod.rs:100-103), with the material's colour at the hit.
+// Material::emitted_light (light.rs:19-21; default black mod.rs:100-103), with the material's colour at the hit.
 inline Col emitted(const rayna_material &m, const Col &colour) {
     if (m.kind == RAYNA_MAT_LIGHT) return colour;
     return Col{0, 0, 0};
@@ -1356,7 +1356,7 @@ template <class R>
 inline Col sky(int kind, const Ray<R> &ray) {
     switch (kind) {
         case RAYNA_SKY_WHITE: return Col{1, 1, 1};
-        case RAYNA_SKY_SIMPLE: {  // simple.rs:140-148: lerp(white, (0.5, 0.7, 1), a)
+        case RAYNA_SKY_SIMPLE: {  // skybox/simple.rs:13-21: lerp(white, (0.5, 0.7, 1), a)
             const float a = (float)(R(0.5) * ray.dir.y + R(0.5));
             return Col{1.0f + (0.5f - 1.0f) * a, 1.0f + (0.7f - 1.0f) * a, 1.0f + (1.0f - 1.0f) * a};
         }
@@ -1505,7 +1505,7 @@ inline Col sky_colour(const Scene &s, const Ray<R> &ray) {
 }
 
 // ================================================================================================
-// Camera (scene/camera.rs:285-384)
+// Camera (scene/camera.rs:114-213)
 // ================================================================================================
 
 struct Viewport {
@@ -1549,7 +1549,7 @@ inline Viewport calculate_viewport(const rayna_camera &cam) {
 }
 
 // The camera ray of sample k of pixel (x, y): render_px_msaa's stratified coordinate (renderer.rs:370-389)
-// and Viewport::calc_ray (camera.rs:362-384).
+// and Viewport::calc_ray (camera.rs:191-213).
 template <class R>
 inline Ray<R> camera_ray(const Viewport &vp, const rayna_render_opts &o, const RngKey &key, uint32_t x, uint32_t y,
                          uint32_t k) {

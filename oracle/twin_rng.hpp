@@ -20,7 +20,7 @@ namespace twin {
 
 // Draw-site slots (the fourth counter word).
 enum : uint32_t {
-    SLOT_CAMERA = 0,        // lanes 0,1: pixel jitter (renderer.rs:377,388); lanes 2,3: lens disk (camera.rs:379)
+    SLOT_CAMERA = 0,        // lanes 0,1: pixel jitter (renderer.rs:377,388); lanes 2,3: lens disk (camera.rs:208)
     SLOT_SCATTER = 1,       // the material's scatter draws at this bounce (material/*.rs)
     SLOT_MEDIUM_AUX = 2,    // the volume hit's arbitrary normals (object/volumetric.rs:151-153)
     SLOT_MEDIUM_UV = 3,     // lanes 0,1: the volume hit's arbitrary uv (object/volumetric.rs:153)

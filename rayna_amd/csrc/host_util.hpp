@@ -9,7 +9,7 @@
 
 namespace rayna {
 
-// Camera::calculate_viewport (scene/camera.rs:285-338) in double precision, rounded to float.
+// Camera::calculate_viewport (scene/camera.rs:114-167) in double precision, rounded to float.
 inline bool viewport(const rayna_camera &c, GViewport &vp) {
     const double focal = c.focus_dist;
     if (c.v_fov == 0.0 || focal == 0.0) return false;

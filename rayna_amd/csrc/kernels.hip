@@ -5,7 +5,6 @@
 #include <float.h>
 
 #include <algorithm>
-#include <mutex>
 #include <stdint.h>
 
 #include "../../include/rayna_gpu.h"
@@ -54,7 +53,7 @@ __device__ __forceinline__ bool try_normalize(f3 a, f3 *out) {  // glam try_norm
     return false;
 }
 __device__ __forceinline__ f3 reflect(f3 d, f3 n) { return d - n * (2.0f * dot(d, n)); }  // shared/math.rs:26
-__device__ __forceinline__ f3 refract(f3 v, f3 n, float ratio) {                        // shared/math.rs:28-33
+__device__ __forceinline__ f3 refract(f3 v, f3 n, float ratio) {                        // shared/math.rs:20-25
     const float cos_theta = fminf(dot(neg(v), n), 1.0f);
     const f3 perp = (v + n * cos_theta) * ratio;
     const f3 par = n * -sqrtf(fabsf(1.0f - dot(perp, perp)));
@@ -136,7 +135,7 @@ __device__ __forceinline__ bool sphere_t(const GSphere &s, const DRay &r, float 
     return true;
 }
 
-// Triangle::intersect (triangle.rs:272-328), Moller-Trumbore; e1, e2 precomputed in float.
+// Triangle::intersect (triangle.rs:53-109), Moller-Trumbore; e1, e2 precomputed in float.
 __device__ __forceinline__ bool tri_t(const GTri &g, const DRay &r, float t0, float t1, float *t, float *uu, float *vv,
                                       float *det_out) {
     const f3 v0{g.v0x, g.v0y, g.v0z}, e1{g.e1x, g.e1y, g.e1z}, e2{g.e2x, g.e2y, g.e2z};
@@ -159,7 +158,7 @@ __device__ __forceinline__ bool tri_t(const GTri &g, const DRay &r, float t0, fl
     return true;
 }
 
-// Triangle::interpolate_normals (triangle.rs:333-338): fold from zero, then try_normalize.
+// Triangle::interpolate_normals (triangle.rs:114-119): fold from zero, then try_normalize.
 __device__ __forceinline__ bool tri_normal(const GTriNormals &n, float u, float v, f3 *out) {
     const float b0 = (1.0f - u) - v;
     const f3 s{((0.0f + n.n0x * b0) + n.n1x * u) + n.n2x * v, ((0.0f + n.n0y * b0) + n.n1y * u) + n.n2y * v,
@@ -167,7 +166,7 @@ __device__ __forceinline__ bool tri_normal(const GTriNormals &n, float u, float 
     return try_normalize(s, out);
 }
 
-// Planar::intersect_bounded + ParallelogramMesh::intersect (planar/mod.rs:385-418, parallelogram.rs:469-477)
+// Planar::intersect_bounded + ParallelogramMesh::intersect (planar/mod.rs:161-194, parallelogram.rs:49-57)
 __device__ __forceinline__ bool para_t(const GPara &g, const DRay &r, float t0, float t1, float *t,
                                        float *uv_out = nullptr) {
     const f3 n{g.nx, g.ny, g.nz};
@@ -641,7 +640,7 @@ __device__ __forceinline__ bool scatter(const GMaterial &m, const DRay &ray, con
             if (!try_normalize(v, dir)) *dir = h.ray_normal;
             return true;
         }
-        case RAYNA_MAT_METAL: {  // metal.rs:66-82
+        case RAYNA_MAT_METAL: {  // metal.rs:17-33
             const f3 refl = reflect(ray.d, h.ray_normal);
             const f3 v = refl + unit_sphere(u[0], u[1]) * m.fuzz;
             if (dot(v, h.ray_normal) > 0.0f) {
@@ -650,7 +649,7 @@ __device__ __forceinline__ bool scatter(const GMaterial &m, const DRay &ray, con
             }
             return false;
         }
-        case RAYNA_MAT_DIELECTRIC: {  // dielectric.rs:114-134, 169-174
+        case RAYNA_MAT_DIELECTRIC: {  // dielectric.rs:19-39, 74-79
             const float ratio = h.front ? 1.0f / m.ior : m.ior;
             const float cos_t = fminf(dot(neg(ray.d), h.ray_normal), 1.0f);
             const float sin_t = sqrtf(1.0f - cos_t * cos_t);
@@ -662,7 +661,7 @@ __device__ __forceinline__ bool scatter(const GMaterial &m, const DRay &ray, con
             *dir = (tir || schlick) ? reflect(ray.d, h.ray_normal) : refract(ray.d, h.ray_normal, ratio);
             return true;
         }
-        case RAYNA_MAT_ISOTROPIC:  // isotropic.rs:204-206
+        case RAYNA_MAT_ISOTROPIC:  // isotropic.rs:29-31
             *dir = unit_sphere(u[0], u[1]);
             return true;
         default:
@@ -737,7 +736,7 @@ __device__ __forceinline__ void validate_hit(uint32_t *checks, const DRay &ray, 
 // ------------------------------------------------------------------------------------------ kernels
 
 // The camera ray of sample k of pixel `pixel`: render_px_msaa's stratified jitter (renderer.rs:370-389) and
-// Viewport::calc_ray (camera.rs:362-384).  k_generate stores it; k_shade recomputes it at the first bounce
+// Viewport::calc_ray (camera.rs:191-213).  k_generate stores it; k_shade recomputes it at the first bounce
 // instead of loading it (the shade kernel waits on memory, not on arithmetic).
 __device__ __forceinline__ DRay camera_ray(const GViewport &vp, uint32_t width, uint32_t height, uint32_t sdim,
                                            dm::Key key, uint32_t pixel, uint32_t k) {
@@ -1328,33 +1327,6 @@ void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s) {
 void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
                            unsigned long long *segments, hipStream_t s) {
     hipLaunchKernelGGL(k_count_segments, dim3(1), dim3(64), 0, s, queue_count, n, first_count, segments);
-}
-
-// ------------------------------------------------------------------------------------------ static workspace
-
-// The gradient copies of rayna_adjoint_static live in this file-scope array instead of memory the library
-// allocates, so that entry point allocates nothing.  Every GPU the library runs on holds its own copy until the
-// process ends.  In host code `scratch` names a host-side stand-in, not device memory: the device address of
-// each GPU's copy comes from hipGetSymbolAddress, taken once per device with that device current and kept.
-constexpr size_t ADJOINT_SCRATCH_DOUBLES = size_t(1) << 19;  // 4 MiB
-static __device__ __align__(256) double scratch[ADJOINT_SCRATCH_DOUBLES];
-
-size_t adjoint_scratch_doubles() { return ADJOINT_SCRATCH_DOUBLES; }
-
-hipError_t adjoint_scratch(int device, double **ptr) {
-    constexpr int MAX_DEVICES = 64;
-    static std::mutex mu;
-    static double *addr[MAX_DEVICES] = {};
-    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
-    std::lock_guard<std::mutex> lock(mu);
-    if (!addr[device]) {
-        void *p = nullptr;
-        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
-        if (e != hipSuccess) return e;
-        addr[device] = static_cast<double *>(p);
-    }
-    *ptr = addr[device];
-    return hipSuccess;
 }
 
 }  // namespace rayna

@@ -51,7 +51,7 @@ static_assert(sizeof(GNode) == 128, "GNode must be 128 bytes");
 
 // Triangle record, 48 bytes: v0 + rank, e1 = v1 - v0 + material, e2 = v2 - v0 + flags, all float32.
 // `flags` bit 0: the interpolated vertex normal may fail to normalise somewhere on the triangle, so a hit must
-// check it (Triangle::intersect returns None then, triangle.rs:316).
+// check it (Triangle::intersect returns None then, triangle.rs:97).
 struct alignas(16) GTri {
     float v0x, v0y, v0z;
     int32_t rank;
@@ -93,7 +93,7 @@ struct alignas(16) GSphere {
     float cx, cy, cz, radius;
 };
 
-// Planar constants computed in double on the host and rounded (planar/mod.rs:303-314).
+// Planar constants computed in double on the host and rounded (planar/mod.rs:79-90).
 struct alignas(16) GPara {
     float nx, ny, nz, d;
     float wx, wy, wz, pad0;
@@ -180,7 +180,7 @@ enum : uint32_t { HIT_KIND_SHIFT = 28, HIT_INDEX_MASK = 0x0fffffffu };
 // distance along the object-space ray.
 enum : uint32_t { HIT_AUX_BACK = 1u, HIT_AUX_SIDE_SHIFT = 8u };
 
-// Camera constants in float, from Camera::calculate_viewport evaluated in double (camera.rs:285-338).
+// Camera constants in float, from Camera::calculate_viewport evaluated in double (camera.rs:114-167).
 struct GViewport {
     float pos[3], pixel_center[3], vu[3], vv[3], du[3], dv[3];
 };

@@ -8,7 +8,7 @@
 // still iterate), so every lane of a wave executes the same iteration body whatever its march length or
 // iteration count.  A lane whose march ends takes the next item of its wave's claimed batch in
 // the next pass.  The operations of each march are exactly those of RaymarchedIsosurfaceMesh::intersect
-// (raymarched.rs:228-277) and sdf_eval, in the same order, so the outcome is the same bit for bit: the march
+// (raymarched.rs:73-122) and sdf_eval, in the same order, so the outcome is the same bit for bit: the march
 // starts at total = T_MIN; each step evaluates the estimate at the ray point of `total`, advances total by its
 // magnitude and hits when that magnitude is below epsilon, giving up after max_iterations + 2 estimates.  Two
 // exits end a march early with the same outcome as the full march: the march distance never decreases, so once
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
 }
 
 // The closest hit of each path over its bounded hit and its unbounded objects' marches, in list order with
-// strict < (object/list.rs:358-363).  The paths whose closest hit becomes an unbounded object are appended to
+// strict < (object/list.rs:120-125).  The paths whose closest hit becomes an unbounded object are appended to
 // sdf_queue for k_sdf_normal: each wave takes strips of MERGE_STRIP * 64 queue positions, stages those slots in
 // its part of LDS and appends them with one atomic per strip.
 constexpr uint32_t MERGE_STRIP = 16;

@@ -1,4 +1,4 @@
-// Host-side scene preparation: the engine's object graph (object/list.rs:297-316), its SAH trees
+// Host-side scene preparation: the engine's object graph (object/list.rs:59-78), its SAH trees
 // (shared/generic_bvh.rs:45-290), depth-first primitive ranks, tight refit and the BVH4 device layout.
 #include "scene_build.hpp"
 
@@ -25,7 +25,7 @@ Aabb64 box_new(const double a[3], const double b[3]) {  // Aabb::new (shared/aab
 
 Aabb64 origin_box() { return Aabb64{{0, 0, 0}, {0, 0, 0}}; }  // Aabb::default()
 
-void grow(Aabb64 &acc, const Aabb64 &b) {  // Aabb::encompass (aabb.rs:109-114)
+void grow(Aabb64 &acc, const Aabb64 &b) {  // Aabb::encompass (aabb.rs:56-61)
     for (int i = 0; i < 3; ++i) {
         acc.mn[i] = std::fmin(acc.mn[i], b.mn[i]);
         acc.mx[i] = std::fmax(acc.mx[i], b.mx[i]);
@@ -37,7 +37,7 @@ double area(const Aabb64 &b) {  // aabb.rs:90
     return ((sx * sy) + (sy * sz) + (sz * sx)) * 2.0;
 }
 
-Aabb64 points_box(const double *p, int n) {  // Aabb::encompass_points (aabb.rs:123-132)
+Aabb64 points_box(const double *p, int n) {  // Aabb::encompass_points (aabb.rs:70-79)
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int k = 0; k < n; ++k)
         for (int i = 0; i < 3; ++i) {
@@ -47,7 +47,7 @@ Aabb64 points_box(const double *p, int n) {  // Aabb::encompass_points (aabb.rs:
     return box_new(mn, mx);
 }
 
-Aabb64 min_padded(const Aabb64 &b, double thresh) {  // Aabb::min_padded (aabb.rs:137-142)
+Aabb64 min_padded(const Aabb64 &b, double thresh) {  // Aabb::min_padded (aabb.rs:84-89)
     double dims[3], centre[3], mn[3], mx[3];
     for (int i = 0; i < 3; ++i) {
         dims[i] = b.mx[i] - b.mn[i];
@@ -401,7 +401,7 @@ struct Packer {
 
     Aabb64 tri_box(int tri) const { return points_box(&d.tri_vertices[9 * (size_t)tri], 3); }
 
-    // The box of a primitive mesh: sphere.rs:123-132, triangle.rs:253, parallelogram.rs:445-457, axis_box.rs:29-37.
+    // The box of a primitive mesh: sphere.rs:27-36, triangle.rs:33, parallelogram.rs:25-37, axis_box.rs:29-37.
     Aabb64 prim_box(int mesh) const {
         const rayna_mesh &m = d.meshes[mesh];
         switch (m.kind) {
@@ -529,7 +529,7 @@ struct Packer {
         return (int32_t)P.tris.size() - 1;
     }
 
-    int32_t add_para(int idx) {  // Planar::new (planar/mod.rs:303-314) in double, then rounded
+    int32_t add_para(int idx) {  // Planar::new (planar/mod.rs:79-90) in double, then rounded
         const double *p = &d.parallelograms[9 * (size_t)idx];
         const double u[3] = {p[3], p[4], p[5]}, v[3] = {p[6], p[7], p[8]};
         const double nr[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
@@ -1278,7 +1278,7 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
         K.xf_fwd.insert(K.xf_fwd.end(), m, m + 12);
         K.xf_inv.insert(K.xf_inv.end(), inv, inv + 12);
     }
-    // ---- object bounds (reference boxes carry the origin for lists: list.rs:176-181) ----
+    // ---- object bounds (reference boxes carry the origin for lists: mesh/advanced/list.rs:44-49) ----
     std::vector<Aabb64> obj_ref(d.n_objects);
     K.obj_tight.resize(d.n_objects);
     std::vector<int32_t> unbounded;
@@ -1288,7 +1288,7 @@ rayna_status pack_scene(const rayna_scene_desc &d, PackedScene &P, std::string &
         switch (m.kind) {
             case RAYNA_MESH_TRIANGLE_LIST:
             case RAYNA_MESH_BVH: {
-                // MeshList (list.rs:176-181) folds its triangles' boxes from the origin; BvhMesh takes its root's
+                // MeshList (mesh/advanced/list.rs:44-49) folds its triangles' boxes from the origin; BvhMesh takes its root's
                 // box (bvh.rs:116-129), the origin-folded union, or the one mesh's own box
                 if (m.count <= 0) {
                     bnd = false;

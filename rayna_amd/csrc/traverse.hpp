@@ -2,7 +2,7 @@
 // primitive tests).
 //
 // Result: the closest hit over the BVH with ties to the higher depth-first rank -- what the engine's
-// shrinking inclusive interval computes (mesh/advanced/bvh.rs:77-91, object/bvh.rs:443-481).  The unbounded
+// shrinking inclusive interval computes (mesh/advanced/bvh.rs:77-91, object/bvh.rs:70-108).  The unbounded
 // objects follow in march.hpp.  Culling is conservative, so the result does not depend on the order nodes are
 // visited in.
 //

@@ -161,7 +161,7 @@ def icosphere(subdiv: int, seed: int, amplitude: float = 0.1, frequency: float =
         np.add.at(vn, tris[:, m], fn)
     vn /= np.linalg.norm(vn, axis=1, keepdims=True)
     # Wind clockwise as seen from outside: the engine's Triangle::intersect treats the side that
-    # (v1 - v0) x (v2 - v0) faces as the back face (front_face = det < 0, triangle.rs:321), so outward-facing
+    # (v1 - v0) x (v2 - v0) faces as the back face (front_face = det < 0, triangle.rs:102), so outward-facing
     # hits need the opposite winding; the vertex normals stay outward.
     tris = tris[:, [0, 2, 1]]
     return verts[tris], vn[tris], np.concatenate(face_of)

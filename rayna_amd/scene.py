@@ -187,7 +187,7 @@ class SphereMesh:
 
 @dataclass
 class TriangleMesh:
-    """One `Triangle::new(vertices, normals)`; normals must be unit length (triangle.rs:246-249)."""
+    """One `Triangle::new(vertices, normals)`; normals must be unit length (triangle.rs:26-29)."""
     vertices: Sequence[Sequence[float]]
     normals: Sequence[Sequence[float]]
 
@@ -203,7 +203,7 @@ class TriangleListMesh:
 
 @dataclass
 class ParallelogramMesh:
-    """`ParallelogramMesh::new(Planar::new(p, u, v))` (planar/mod.rs:303-314)."""
+    """`ParallelogramMesh::new(Planar::new(p, u, v))` (planar/mod.rs:79-90)."""
     p: Sequence[float]
     u: Sequence[float]
     v: Sequence[float]

@@ -135,7 +135,7 @@ def test_triangle_closed_form_and_edges():
     v = ((0, 0, 0), (2, 0, 0), (0, 2, 0))
     n = ((0, 0, 1),) * 3
     t = _one(SimpleObject(TriangleMesh(v, n), LambertianMaterial()))
-    # The engine sets front_face = det < 0 and ray_normal = n * -sign(det) (triangle.rs:321,325); with
+    # The engine sets front_face = det < 0 and ray_normal = n * -sign(det) (triangle.rs:102,106); with
     # det = -d . (e1 x e2), the side that e1 x e2 faces (+z here) is the engine's *back* face and its
     # ray_normal points along the ray.  Meshes for the engine are therefore wound clockwise seen from the front.
     h = t.trace([0.5, 0.5, 3], [0, 0, -1])
@@ -149,7 +149,7 @@ def test_triangle_closed_form_and_edges():
 
 
 def test_triangle_opposing_normals_are_not_a_hit():
-    # normals that cancel at a barycentric point: Triangle::intersect returns None there (triangle.rs:316)
+    # normals that cancel at a barycentric point: Triangle::intersect returns None there (triangle.rs:97)
     v = ((0, 0, 0), (2, 0, 0), (0, 2, 0))
     n = ((0, 0, 1), (0, 0, -1), (0, 0, 1))
     t = _one(SimpleObject(TriangleMesh(v, n), LambertianMaterial()))
@@ -296,7 +296,7 @@ def test_metal_mirror_and_absorption():
     ok, d, att = oracle.scatter_f64(m, ray, _hit((1, 0, 0), (0, 1, 0), (0, 1, 0), True), [0.3, 0.6, 0.1, 0.9])
     s2 = math.sqrt(0.5)
     assert ok and np.allclose(d, [s2, s2, 0], atol=1e-15) and np.allclose(att, [0.6, 0.7, 0.8])
-    # fuzz pushing the direction below the surface: absorbed (metal.rs:75-81)
+    # fuzz pushing the direction below the surface: absorbed (metal.rs:23-31)
     m = MetalMaterial((1, 1, 1), 1.0)
     ray = np.array([0, 0.001, 0, 1, -0.001, 0])
     # (r1, r2) = (0.75, 0.5) draws the unit vector (0, -1, 0)
