@@ -356,6 +356,17 @@ typedef struct rayna_tuning {
 /* Applies a tuning to a renderer.  A new chunk_slots re-plans the path buffers and clears accumulation. */
 rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t);
 
+/* rayna_adjoint without allocating: no counterpart in the engine; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  d_image_dev
+ * (height x width x 3 floats, row-major) and grad_dev (3 doubles per material, overwritten) are device
+ * memory on the renderer's device.  The gradient copies live in a static workspace of 4 MiB per GPU (the
+ * gradients of at most 174762 materials, else RAYNA_INVALID_ARGUMENT), held until the process ends; the
+ * renderer's own buffers must be in place (a frame rendered since the last set_scene / set_options /
+ * set_tuning, else RAYNA_INVALID_ARGUMENT).  Calls on one GPU share the workspace: calls that overlap on one
+ * GPU race.  Synchronous; no stats. */
+rayna_status rayna_adjoint_static(rayna_renderer *r, uint32_t frame, const float *d_image_dev, double *grad_dev,
+                                  uint32_t n_materials);
+
 /* ---------------------------------------------------------------- debug validation (shared/validate.rs:13-198)
  * The engine asserts these invariants in debug builds along every path (render/renderer.rs:395-561).  With
  * validation on, the PBR frames of rayna_render count, per check, the rays, hits and colours that break them,

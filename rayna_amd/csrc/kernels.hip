@@ -5,6 +5,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdint.h>
 
 #include "../../include/rayna_gpu.h"
@@ -1327,6 +1328,33 @@ void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s) {
 void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
                            unsigned long long *segments, hipStream_t s) {
     hipLaunchKernelGGL(k_count_segments, dim3(1), dim3(64), 0, s, queue_count, n, first_count, segments);
+}
+
+// ------------------------------------------------------------------------------------------ static workspace
+
+// The gradient copies of rayna_adjoint_static live in this file-scope array instead of memory the library
+// allocates, so that entry point allocates nothing.  Every GPU the library runs on holds its own copy until the
+// process ends.  In host code `scratch` names a host-side stand-in, not device memory: the device address of
+// each GPU's copy comes from hipGetSymbolAddress, taken once per device with that device current and kept.
+constexpr size_t ADJOINT_SCRATCH_DOUBLES = size_t(1) << 19;  // 4 MiB
+static __device__ __align__(256) double scratch[ADJOINT_SCRATCH_DOUBLES];
+
+size_t adjoint_scratch_doubles() { return ADJOINT_SCRATCH_DOUBLES; }
+
+hipError_t adjoint_scratch(int device, double **ptr) {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static double *addr[MAX_DEVICES] = {};
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = static_cast<double *>(p);
+    }
+    *ptr = addr[device];
+    return hipSuccess;
 }
 
 }  // namespace rayna

@@ -139,6 +139,11 @@ void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s);
 void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
                            unsigned long long *segments, hipStream_t s);
 
+// The static workspace of rayna_adjoint_static (a file-scope device array of adjoint_scratch_doubles()
+// doubles): its address on `device`, which must be the current device (taken on the first call per device).
+hipError_t adjoint_scratch(int device, double **ptr);
+size_t adjoint_scratch_doubles();
+
 int extend_lds_stack();
 int extend_blocks_per_cu();  // resident extend blocks per CU (the persistent grid is this times the CU count)
 int extend_block();

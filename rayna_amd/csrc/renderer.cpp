@@ -747,6 +747,49 @@ rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_ima
     return RAYNA_OK;
 }
 
+rayna_status rayna_adjoint_static(rayna_renderer *r, uint32_t frame, const float *d_image_dev, double *grad_dev,
+                                  uint32_t n_materials) {
+    if (!r || !d_image_dev || !grad_dev) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    if (n_materials != r->packed.materials.size())
+        return fail(r, RAYNA_INVALID_ARGUMENT, "n_materials is not the scene's material count");
+    if (!r->cam_valid) return fail(r, RAYNA_INVALID_ARGUMENT, "camera is invalid (the frame is the checkerboard)");
+    if (r->opts.mode != RAYNA_MODE_PBR) return fail(r, RAYNA_UNSUPPORTED, "the adjoint is of RenderMode::PBR frames");
+    if (n_materials == 0) return RAYNA_OK;
+    const size_t cap = adjoint_scratch_doubles();
+    const uint64_t per_copy = 3ull * n_materials;  // doubles
+    if (per_copy > cap)
+        return fail(r, RAYNA_INVALID_ARGUMENT,
+                    "the static workspace holds the gradients of at most " + std::to_string(cap / 3) + " materials");
+    // the handle's march buffers must already be in place (rayna_render or rayna_adjoint sets them up)
+    const uint64_t mt_need = r->slot_capacity * (uint64_t)std::max(0, r->dev.view.n_sdfs);
+    if (mt_need > r->mt_cap || (r->dev.view.n_sdfs > 0 && r->slot_capacity > r->sdf_cap))
+        return fail(r, RAYNA_INVALID_ARGUMENT, "render a frame first: the renderer's march buffers are not in place");
+    HIP_TRY(r, hipSetDevice(r->device));
+    double *ws = nullptr;
+    HIP_TRY(r, adjoint_scratch(r->device, &ws));
+    const uint32_t copies = (uint32_t)std::min<uint64_t>(64, cap / per_copy);
+    // the workspace keeps the last call's contents: clear the copies (a fill kernel, no memset) and the counter
+    launch_fill_u32(reinterpret_cast<uint32_t *>(ws), 0u, (uint32_t)(2 * per_copy * copies), r->stream);
+    launch_fill_u32(reinterpret_cast<uint32_t *>(r->segments), 0u, 2, r->stream);
+    const uint32_t spp = r->opts.samples;
+    const uint32_t n_chunks = (spp + r->chunk_samples - 1) / r->chunk_samples;
+    const bool reuse = n_chunks == 1 && r->primal_frame == (int64_t)frame;
+    FrameParams fp = frame_params(r, frame);
+    AdjointParams ap{d_image_dev, 1.0 / (double)spp, ws, n_materials, copies};
+    uint64_t launches = 0;
+    for (uint32_t c = 0; c < n_chunks; ++c) {
+        const uint32_t s0 = c * r->chunk_samples;
+        const uint32_t n = std::min(r->chunk_samples, spp - s0);
+        if (!reuse) trace_chunk(r, fp, s0, n, nullptr, nullptr, &launches);
+        trace_chunk(r, fp, s0, n, nullptr, nullptr, &launches, &ap);
+    }
+    launch_sum_grad(ws, 3 * n_materials, copies, grad_dev, r->stream);
+    HIP_TRY(r, hipGetLastError());
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    r->primal_frame = -1;
+    return RAYNA_OK;
+}
+
 rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_t *prim_rank, float *t) {
     if (!r || !prim_rank || !t) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
     if (!r->cam_valid) return fail(r, RAYNA_INVALID_ARGUMENT, "camera is invalid");

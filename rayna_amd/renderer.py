@@ -28,7 +28,7 @@ _lib: Optional[C.CDLL] = None
 # every symbol declared in include/rayna_gpu.h
 EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_options", "rayna_clear_accumulation",
            "rayna_render", "rayna_render_srgb8", "rayna_set_material_colours", "rayna_adjoint", "rayna_set_tuning",
-           "rayna_set_validate", "rayna_validate_counts", "rayna_batch_create",
+           "rayna_set_validate", "rayna_validate_counts", "rayna_adjoint_static", "rayna_batch_create",
            "rayna_batch_set_tuning", "rayna_render_batch", "rayna_batch_destroy", "rayna_batch_last_error",
            "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare", "rayna_scene_tree", "rayna_destroy",
            "rayna_last_error")
@@ -75,6 +75,8 @@ def lib() -> C.CDLL:
                                     C.c_uint32, C.POINTER(A.RenderStats)]
         L.rayna_set_tuning.restype = C.c_int
         L.rayna_set_tuning.argtypes = [vp, C.POINTER(A.Tuning)]
+        L.rayna_adjoint_static.restype = C.c_int
+        L.rayna_adjoint_static.argtypes = [vp, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32]
         L.rayna_set_validate.restype = C.c_int
         L.rayna_set_validate.argtypes = [vp, C.c_int32]
         L.rayna_validate_counts.restype = C.c_int
