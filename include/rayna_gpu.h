@@ -349,7 +349,7 @@ typedef struct rayna_tuning {
                                     * depend on it */
     int32_t packet_depths;         /* extend: bounces traced as wave packets when the chunk's samples of a pixel
                                     * are neighbouring slots (at least 16 per pixel, or a batch): 1 = the camera
-                                    * rays only, d = the bounces before depth d, -1 = none, 0 = the default (2);
+                                    * rays only, d = the bounces before depth d, -1 = none, 0 = the default (1);
                                     * up to 16; the images do not depend on it */
 } rayna_tuning;
 

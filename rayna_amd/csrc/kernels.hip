@@ -78,19 +78,6 @@ __device__ __forceinline__ DRay make_ray(f3 o, f3 d) {
     r.inv = f3{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
     return r;
 }
-// The absolute slack of the slab test of a ray (expand and the packets): a slab distance is computed as
-// fma(bound, inv, -o inv), where o inv rounds once (an error up to 2^-24 |o_a inv_a| on that axis) and the fma
-// once more (2^-24 relative, which BOX_SLACK covers), so a box is kept when
-// max(t0, T_MIN) <= min(t1, best) BOX_SLACK + slack with slack = 2^-22 max_a |o_a inv_a|: the entry and exit
-// distances may each be off by half of that, and culling stays conservative.  A direction component below
-// 2^-60 in magnitude (its reciprocal 2^60 or infinite: its slab distances are NaN, which the min / max ignore,
-// or could overflow) turns culling off for the ray: slack +infinity.
-__device__ __forceinline__ float slab_slack(const DRay &r) {
-    const float m = fmaxf(fmaxf(fabsf(r.o.x * r.inv.x), fabsf(r.o.y * r.inv.y)), fabsf(r.o.z * r.inv.z));
-    const float dmin = fminf(fminf(fabsf(r.d.x), fabsf(r.d.y)), fabsf(r.d.z));
-    return dmin < 0x1.0p-60f ? __builtin_inff() : m * 0x1.0p-22f;
-}
-
 __device__ __forceinline__ DRay ray_from_dir(f3 o, f3 unit_d) {
     DRay r;
     r.o = o;

@@ -76,7 +76,7 @@ struct AdjointParams {
 struct ExtendTuning {
     uint32_t refill_lanes = 32;
     uint32_t wave_batch = 128;
-    uint32_t packet_depths = 2;
+    uint32_t packet_depths = 1;
 };
 
 // Shade grid: blocks of 256 threads per compute unit (grid-stride over the queue).

@@ -83,11 +83,9 @@ struct LaneStack {
     }
 };
 
-// Tests the four child boxes of inner node `node` against [T_MIN, best] (conservatively: slab distances by fma,
-// slab_slack) and pushes the hit children -- inner nodes and leaf ranges alike -- far to near, so the nearest is
-// popped first.
-__device__ __forceinline__ void expand(const SceneView &sv, const DRay &ray, float slack, float best, int32_t node,
-                                       LaneStack &st) {
+// Tests the four child boxes of inner node `node` against [T_MIN, best] (conservatively) and pushes the hit
+// children -- inner nodes and leaf ranges alike -- far to near, so the nearest is popped first.
+__device__ __forceinline__ void expand(const SceneView &sv, const DRay &ray, float best, int32_t node, LaneStack &st) {
     const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
     const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
     const int4 cc = reinterpret_cast<const int4 *>(np)[7];  // the slots' entry codes (device node copy)
@@ -97,19 +95,18 @@ __device__ __forceinline__ void expand(const SceneView &sv, const DRay &ray, flo
                 hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
     int32_t code[4] = {cc.x, cc.y, cc.z, cc.w};
     float key[4];
-    const f3 oi{ray.o.x * ray.inv.x, ray.o.y * ray.inv.y, ray.o.z * ray.inv.z};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const float tx1 = fmaf(lo_x[k], ray.inv.x, -oi.x), tx2 = fmaf(hi_x[k], ray.inv.x, -oi.x);
+        const float tx1 = (lo_x[k] - ray.o.x) * ray.inv.x, tx2 = (hi_x[k] - ray.o.x) * ray.inv.x;
         float t0 = fminf(tx1, tx2), t1 = fmaxf(tx1, tx2);
-        const float ty1 = fmaf(lo_y[k], ray.inv.y, -oi.y), ty2 = fmaf(hi_y[k], ray.inv.y, -oi.y);
+        const float ty1 = (lo_y[k] - ray.o.y) * ray.inv.y, ty2 = (hi_y[k] - ray.o.y) * ray.inv.y;
         t0 = fmaxf(t0, fminf(ty1, ty2));
         t1 = fminf(t1, fmaxf(ty1, ty2));
-        const float tz1 = fmaf(lo_z[k], ray.inv.z, -oi.z), tz2 = fmaf(hi_z[k], ray.inv.z, -oi.z);
+        const float tz1 = (lo_z[k] - ray.o.z) * ray.inv.z, tz2 = (hi_z[k] - ray.o.z) * ray.inv.z;
         t0 = fmaxf(t0, fminf(tz1, tz2));
         t1 = fminf(t1, fmaxf(tz1, tz2));
         const float lo = fmaxf(t0, T_MIN), hi = fminf(t1, best);
-        const bool hit = code[k] != ENTRY_NONE && lo <= fmaf(hi, BOX_SLACK, slack);
+        const bool hit = code[k] != ENTRY_NONE && lo <= hi * BOX_SLACK;
         key[k] = hit ? t0 : -__builtin_inff();
         if (!hit) code[k] = ENTRY_NONE;  // not pushed
     }
@@ -283,7 +280,6 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
     bool active = false, exhausted = false;
     uint32_t slot = 0, pixel = 0, sample = 0;
     DRay ray{};
-    float slack = 0.0f;  // slab_slack of the lane's current ray (world or, inside an instance, object space)
     Best b{FLT_MAX, -1, 0u, 0u};
     int32_t leaf = 0;
     int leaf_i = 0, leaf_n = 0;
@@ -341,7 +337,6 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
                     slot = queue ? (uint32_t)queue[q] : q;
                     ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
                                        f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
-                    slack = slab_slack(ray);
                     int32_t root = sv.root;
                     if (BATCH) {
                         const RequestParams &R = bv.reqs[bv.slot_req[slot]];
@@ -379,14 +374,13 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
                 float tent;
                 st.pop(&code, &tent);
                 RAYNA_STAT(pops, 1);
-                if (!(tent <= fmaf(b.t, BOX_SLACK, slack))) {  // culled, or an instance's marker (a NaN distance)
+                if (!(tent <= b.t * BOX_SLACK)) {  // culled, or an instance's marker (a NaN distance)
                     if (INST && tent != tent) {
                         if (leaf != 0) {
                             st.push(code, tent);
                             stalled = true;
                         } else {
                             finish_instance(sv, pb, slot, code, ray, b, st);
-                            slack = slab_slack(ray);
                         }
                     }
                     continue;
@@ -404,16 +398,13 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
                     continue;
                 }
                 RAYNA_STAT(expands, 1);
-                expand(sv, ray, slack, b.t, code, st);
+                expand(sv, ray, b.t, code, st);
             }
             while (leaf != 0) {  // the leaf's primitives, one per iteration
                 RAYNA_STAT(prims, 1);
-                if (leaf_test<INST, SPH>(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth)) {
-                    leaf = 0;  // an instance entered: the lane's ray is now in object space
-                    slack = slab_slack(ray);
-                } else if (++leaf_i == leaf_n) {
+                if (leaf_test<INST, SPH>(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth) ||
+                    ++leaf_i == leaf_n)
                     leaf = 0;
-                }
             }
             if (st.sp == 0) {
                 pb.hit_t[slot] = b.t;
@@ -431,7 +422,9 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
 // Rays traced as wave packets: at depth 0 of a single-request frame the 64 lanes of a wave hold neighbouring
 // samples of one pixel (slot = pixel * n_spp + sample), whose rays start together and nearly agree in direction;
 // at the next depths (queue: the paths k_shade kept, in slot order within each strip) a wave's 64 queue positions
-// hold the surviving samples of one or two pixels, whose first-bounce rays start from nearly one point.  The wave walks the tree once, as a unit: it keeps one stack in LDS, whose entries hold an entry
+// hold the surviving samples of one or two pixels, whose first-bounce rays start from nearly one point (traced
+// as packets only on request: packet_depths).  The wave walks the tree once, as a unit: it keeps one stack in
+// LDS, whose entries hold an entry
 // code, the lanes that take it and the least entry distance among those lanes; node and primitive addresses are
 // wave-uniform.  Each lane still tests every box against its own ray and its own closest hit, and every
 // primitive of a visited leaf range against its own ray (leaf_test), so a lane visits a superset of the nodes
@@ -472,8 +465,6 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
     const uint32_t slot = valid ? (queue ? (uint32_t)queue[q] : q) : 0u;
     const uint32_t ls = slot;
     DRay ray = ray_from_dir(f3{pb.ox[ls], pb.oy[ls], pb.oz[ls]}, f3{pb.dx[ls], pb.dy[ls], pb.dz[ls]});
-    const float slack = slab_slack(ray);
-    const f3 oi{ray.o.x * ray.inv.x, ray.o.y * ray.inv.y, ray.o.z * ray.inv.z};
     uint32_t pixel, sample;
     dm::Key key{fp.key0, fp.key1};
     int32_t root = sv.root;
@@ -513,7 +504,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
             const int32_t code = __builtin_amdgcn_readfirstlane(s_code[w][sp]);
             const float kt = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s_key[w][sp])));
             const unsigned long long m = s_mask[w][sp];
-            const bool take = ((m >> lane) & 1ull) != 0ull && kt <= fmaf(b.t, BOX_SLACK, slack);
+            const bool take = ((m >> lane) & 1ull) != 0ull && kt <= b.t * BOX_SLACK;
             const unsigned long long tm = __ballot(take);
             if (tm == 0ull) continue;
             if (code >= 0) {  // inner node: each child the lanes' boxes hit, far to near
@@ -529,16 +520,16 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
                 unsigned long long cmask[4];
     #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    const float tx1 = fmaf(lo_x[k], ray.inv.x, -oi.x), tx2 = fmaf(hi_x[k], ray.inv.x, -oi.x);
+                    const float tx1 = (lo_x[k] - ray.o.x) * ray.inv.x, tx2 = (hi_x[k] - ray.o.x) * ray.inv.x;
                     float t0 = fminf(tx1, tx2), t1 = fmaxf(tx1, tx2);
-                    const float ty1 = fmaf(lo_y[k], ray.inv.y, -oi.y), ty2 = fmaf(hi_y[k], ray.inv.y, -oi.y);
+                    const float ty1 = (lo_y[k] - ray.o.y) * ray.inv.y, ty2 = (hi_y[k] - ray.o.y) * ray.inv.y;
                     t0 = fmaxf(t0, fminf(ty1, ty2));
                     t1 = fminf(t1, fmaxf(ty1, ty2));
-                    const float tz1 = fmaf(lo_z[k], ray.inv.z, -oi.z), tz2 = fmaf(hi_z[k], ray.inv.z, -oi.z);
+                    const float tz1 = (lo_z[k] - ray.o.z) * ray.inv.z, tz2 = (hi_z[k] - ray.o.z) * ray.inv.z;
                     t0 = fmaxf(t0, fminf(tz1, tz2));
                     t1 = fminf(t1, fmaxf(tz1, tz2));
                     const float lo = fmaxf(t0, T_MIN), hi = fminf(t1, b.t);
-                    const bool hit = take && ccode[k] != ENTRY_NONE && lo <= fmaf(hi, BOX_SLACK, slack);
+                    const bool hit = take && ccode[k] != ENTRY_NONE && lo <= hi * BOX_SLACK;
                     cmask[k] = __ballot(hit);
                     ckey[k] = cmask[k] != 0ull ? wave_min(hit ? t0 : __builtin_inff()) : -__builtin_inff();
                 }
