@@ -1895,7 +1895,8 @@ inline uint64_t render_rows(const RenderArgs &a, float *out, uint64_t *prim_test
         tests += my_tests;
         for (int i = 0; i < N_CHECKS; ++i) checks[i] += my_checks[i];
     };
-    int nt = a.threads > 0 ? a.threads : (int)std::thread::hardware_concurrency();
+    // default: the host threads, at most 16 (one GPU's share of a host)
+    int nt = a.threads > 0 ? a.threads : std::min(16, (int)std::thread::hardware_concurrency());
     if (nt < 1) nt = 1;
     std::vector<std::thread> pool;
     for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
@@ -1942,7 +1943,8 @@ inline uint64_t adjoint_frame(const RenderArgs &a, const float *d_image, double 
     const size_t n_grad = 3 * a.scene->materials.size();
     std::atomic<uint32_t> next{0};
     std::atomic<uint64_t> segs{0};
-    int nt = a.threads > 0 ? a.threads : (int)std::thread::hardware_concurrency();
+    // default: the host threads, at most 16 (one GPU's share of a host)
+    int nt = a.threads > 0 ? a.threads : std::min(16, (int)std::thread::hardware_concurrency());
     if (nt < 1) nt = 1;
     std::vector<std::vector<double>> part(nt, std::vector<double>(n_grad, 0.0));
     auto worker = [&](int id) {
