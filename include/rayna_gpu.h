@@ -30,7 +30,11 @@ typedef enum rayna_status {
     RAYNA_INVALID_ARGUMENT = 1, /* a bad index, size or null pointer in the request */
     RAYNA_UNSUPPORTED = 2,      /* valid for the engine but not for this renderer (e.g. ray_branching > 1) */
     RAYNA_DEVICE_ERROR = 3,     /* a HIP call failed; rayna_last_error() names it */
-    RAYNA_OUT_OF_MEMORY = 4
+    RAYNA_OUT_OF_MEMORY = 4,
+    RAYNA_RENDERER_LOST = 5     /* a launch on this renderer failed earlier (that call returned RAYNA_DEVICE_ERROR):
+                                   its device state is unknown, so every later call is refused; destroy it and create
+                                   a new one, as the engine's caller restarts a dead render worker
+                                   (rayna_ui/src/app.rs:483-495, integration/mod.rs:100-125) */
 } rayna_status;
 
 /* ---------------------------------------------------------------- materials
@@ -465,6 +469,16 @@ rayna_status rayna_scene_tree(const rayna_scene_desc *scene, int32_t *node_words
 
 /* Renderer drop. */
 void rayna_destroy(rayna_renderer *r);
+
+/* Debug fault injection.  No counterpart in the engine, whose render cannot fail (render/renderer.rs:224-264);
+ * it is for the callers' recovery tests.  countdown >= 0 makes the launch check that many checks from now (0: the
+ * next one) of any renderer in the process report a launch failure: that call returns RAYNA_DEVICE_ERROR with
+ * "(injected)" in rayna_last_error and leaves its renderer lost (RAYNA_RENDERER_LOST from then on).  The kernels
+ * themselves run as usual and no GPU fault is made.  countdown < 0 switches injection off.  rayna_render,
+ * rayna_adjoint, rayna_adjoint_static and rayna_primary_hits make one launch check per call (rayna_render none for
+ * an invalid camera's checkerboard); rayna_render_srgb8 makes its rayna_render's and one of its own.
+ * Makes no HIP call; returns the previous setting (-1 when off). */
+int32_t rayna_debug_fail_launches(int32_t countdown);
 
 /* Text of the last error on this handle (or of the last failed rayna_create when r is NULL). */
 const char *rayna_last_error(const rayna_renderer *r);

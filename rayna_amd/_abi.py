@@ -6,7 +6,9 @@ RAYNA_INVALID_ARGUMENT = 1
 RAYNA_UNSUPPORTED = 2
 RAYNA_DEVICE_ERROR = 3
 RAYNA_OUT_OF_MEMORY = 4
-STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "UNSUPPORTED", 3: "DEVICE_ERROR", 4: "OUT_OF_MEMORY"}
+RAYNA_RENDERER_LOST = 5
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "UNSUPPORTED", 3: "DEVICE_ERROR", 4: "OUT_OF_MEMORY",
+                5: "RENDERER_LOST"}
 
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_LIGHT = 0, 1, 2, 3, 4
 MESH_SPHERE, MESH_TRIANGLE, MESH_TRIANGLE_LIST, MESH_PARALLELOGRAM, MESH_AXIS_BOX, MESH_SDF, MESH_BVH = 0, 1, 2, 3, 4, 5, 6

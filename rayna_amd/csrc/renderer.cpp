@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -20,6 +21,9 @@ using namespace rayna;
 namespace {
 
 std::string g_create_error;
+
+// rayna_debug_fail_launches: < 0 off, else the number of launch checks still to pass before one fails
+std::atomic<int32_t> g_fail_launches{-1};
 
 // Paths in flight per chunk: 2^27 (a 1080p frame of 64 samples in one chunk; ~10 GB of path state, sized for
 // the 288 GB of HBM of one MI355X).  Whole-frame chunks keep every depth's launch full: at 2^23 paths the
@@ -82,6 +86,8 @@ struct rayna_renderer {
     std::vector<hipEvent_t> events;
     // debug validation (shared/validate.rs): RAYNA_CHECKS counters, counted in PBR frames while `validate`
     uint32_t *checks = nullptr;
+    // non-empty once a launch on this renderer failed (the failure's text): every later call is refused
+    std::string lost;
     bool validate = false;
     std::string error;
 };
@@ -101,6 +107,39 @@ rayna_status fail(rayna_renderer *r, rayna_status st, const std::string &msg) {
             return fail((r), e_ == hipErrorOutOfMemory ? RAYNA_OUT_OF_MEMORY : RAYNA_DEVICE_ERROR,           \
                         std::string(#expr) + ": " + hipGetErrorString(e_));                                   \
     } while (0)
+
+// HIP_TRY for the calls that wait on a renderer's launches: a failure there leaves the renderer lost.
+#define HIP_TRY_LAUNCHED(r, expr)                                                                             \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess) {                                                                               \
+            (r)->lost = std::string(#expr) + ": " + hipGetErrorString(e_);                                    \
+            return fail((r), RAYNA_DEVICE_ERROR, (r)->lost);                                                  \
+        }                                                                                                     \
+    } while (0)
+
+// A lost renderer's stream and buffers are in an unknown state: it answers every call with
+// RAYNA_RENDERER_LOST until the caller destroys it and creates a new one.
+#define REQUIRE_LIVE(r)                                                                                       \
+    do {                                                                                                      \
+        if (!(r)->lost.empty())                                                                               \
+            return fail((r), RAYNA_RENDERER_LOST,                                                             \
+                        "renderer lost after \"" + (r)->lost + "\": destroy it and create a new one");        \
+    } while (0)
+
+// The status of the launches just issued on r's stream (hipGetLastError), or the launch failure that
+// rayna_debug_fail_launches planted for this check.  Either failure leaves r lost.
+rayna_status launch_status(rayna_renderer *r, const char *where) {
+    hipError_t e = hipGetLastError();
+    int32_t n = g_fail_launches.load();
+    while (n >= 0 && !g_fail_launches.compare_exchange_weak(n, n - 1)) {
+    }
+    const bool injected = n == 0;
+    if (injected) e = hipErrorLaunchFailure;
+    if (e == hipSuccess) return RAYNA_OK;
+    r->lost = std::string(where) + ": " + hipGetErrorString(e) + (injected ? " (injected)" : "");
+    return fail(r, RAYNA_DEVICE_ERROR, r->lost);
+}
 
 template <class T>
 rayna_status upload(rayna_renderer *r, DevScene &ds, const std::vector<T> &v, const T **out) {
@@ -442,6 +481,7 @@ rayna_status rayna_create(const rayna_scene_desc *scene, const rayna_camera *cam
 
 rayna_status rayna_set_scene(rayna_renderer *r, const rayna_scene_desc *scene) {
     if (!r || !scene) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     (void)hipSetDevice(r->device);
     rayna_status st = load_scene(r, scene);
     if (st != RAYNA_OK) return st;
@@ -452,6 +492,7 @@ rayna_status rayna_set_scene(rayna_renderer *r, const rayna_scene_desc *scene) {
 
 rayna_status rayna_set_camera(rayna_renderer *r, const rayna_camera *camera) {
     if (!r || !camera) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     r->cam = *camera;
     r->cam_valid = viewport(r->cam, r->vp);
     r->primal_frame = -1;
@@ -460,6 +501,7 @@ rayna_status rayna_set_camera(rayna_renderer *r, const rayna_camera *camera) {
 
 rayna_status rayna_set_options(rayna_renderer *r, const rayna_render_opts *opts) {
     if (!r) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     rayna_status st = check_opts(r, opts);
     if (st != RAYNA_OK) return st;
     (void)hipSetDevice(r->device);
@@ -473,6 +515,7 @@ rayna_status rayna_set_options(rayna_renderer *r, const rayna_render_opts *opts)
 
 rayna_status rayna_clear_accumulation(rayna_renderer *r) {
     if (!r) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     r->frames = 0;
     r->acc_dirty = true;
     return RAYNA_OK;
@@ -481,6 +524,7 @@ rayna_status rayna_clear_accumulation(rayna_renderer *r) {
 rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uint32_t out_dims[2],
                           rayna_render_stats *stats) {
     if (!r) return fail(r, RAYNA_INVALID_ARGUMENT, "null renderer");
+    REQUIRE_LIVE(r);
     if (layout != RAYNA_LAYOUT_XMAJOR && layout != RAYNA_LAYOUT_ROWMAJOR)
         return fail(r, RAYNA_INVALID_ARGUMENT, "unknown layout");
     const auto t0 = std::chrono::steady_clock::now();
@@ -536,12 +580,16 @@ rayna_status rayna_render(rayna_renderer *r, float *out_rgb, int32_t layout, uin
         launch_finalize(fp, r->sum, r->acc, (double)r->frames, r->out_dev, layout, r->stream);
         r->primal_frame = n_chunks == 1 ? frame_index : -1;
         if (ev) HIP_TRY(r, hipEventRecord((*ev)[evi++], r->stream));
-        HIP_TRY(r, hipGetLastError());
+        {
+            const rayna_status ls = launch_status(r, "render");
+            if (ls != RAYNA_OK) return ls;
+        }
         if (out_rgb)
-            HIP_TRY(r, hipMemcpyAsync(out_rgb, r->out_dev, 3 * P * sizeof(float), hipMemcpyDeviceToHost, r->stream));
+            HIP_TRY_LAUNCHED(
+                r, hipMemcpyAsync(out_rgb, r->out_dev, 3 * P * sizeof(float), hipMemcpyDeviceToHost, r->stream));
         unsigned long long segs = 0;
-        HIP_TRY(r, hipMemcpyAsync(&segs, r->segments, sizeof(segs), hipMemcpyDeviceToHost, r->stream));
-        HIP_TRY(r, hipStreamSynchronize(r->stream));
+        HIP_TRY_LAUNCHED(r, hipMemcpyAsync(&segs, r->segments, sizeof(segs), hipMemcpyDeviceToHost, r->stream));
+        HIP_TRY_LAUNCHED(r, hipStreamSynchronize(r->stream));
         st.accum_frames = r->frames;
         st.samples = P * spp;
         st.segments = segs;
@@ -583,14 +631,18 @@ rayna_status rayna_render_srgb8(rayna_renderer *r, uint8_t *out_rgba, uint32_t o
         r->rgba_cap = 4 * P;
     }
     launch_to_srgb8(r->out_dev, (uint32_t)P, r->rgba_dev, r->stream);
-    HIP_TRY(r, hipGetLastError());
-    HIP_TRY(r, hipMemcpyAsync(out_rgba, r->rgba_dev, 4 * P, hipMemcpyDeviceToHost, r->stream));
-    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    {
+        const rayna_status ls = launch_status(r, "render_srgb8");
+        if (ls != RAYNA_OK) return ls;
+    }
+    HIP_TRY_LAUNCHED(r, hipMemcpyAsync(out_rgba, r->rgba_dev, 4 * P, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY_LAUNCHED(r, hipStreamSynchronize(r->stream));
     return RAYNA_OK;
 }
 
 rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t) {
     if (!r || !t) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     if (t->refill_lanes > 64 || t->wave_batch > 4096 || t->shade_blocks_per_cu > 64)
         return fail(r, RAYNA_INVALID_ARGUMENT, "tuning out of range");
     r->tune.refill_lanes = t->refill_lanes ? t->refill_lanes : ExtendTuning{}.refill_lanes;
@@ -610,6 +662,7 @@ rayna_status rayna_set_tuning(rayna_renderer *r, const rayna_tuning *t) {
 
 rayna_status rayna_set_validate(rayna_renderer *r, int32_t enable) {
     if (!r) return fail(r, RAYNA_INVALID_ARGUMENT, "null renderer");
+    REQUIRE_LIVE(r);
     HIP_TRY(r, hipSetDevice(r->device));
     if (enable && !r->checks) {
         void *p = nullptr;
@@ -626,6 +679,7 @@ rayna_status rayna_set_validate(rayna_renderer *r, int32_t enable) {
 
 rayna_status rayna_validate_counts(rayna_renderer *r, uint64_t *counts, int32_t reset) {
     if (!r || !counts) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     uint32_t c[RAYNA_CHECKS] = {};
     if (r->checks) {
         HIP_TRY(r, hipSetDevice(r->device));
@@ -639,6 +693,7 @@ rayna_status rayna_validate_counts(rayna_renderer *r, uint64_t *counts, int32_t 
 
 rayna_status rayna_set_material_colours(rayna_renderer *r, const float *rgb, uint32_t n) {
     if (!r || (n && !rgb)) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     if (n > r->packed.materials.size()) return fail(r, RAYNA_INVALID_ARGUMENT, "more colours than materials");
     for (uint32_t i = 0; i < 3 * n; ++i)
         if (!std::isfinite(rgb[i])) return fail(r, RAYNA_INVALID_ARGUMENT, "non-finite material colour");
@@ -662,6 +717,7 @@ rayna_status rayna_set_material_colours(rayna_renderer *r, const float *rgb, uin
 rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_image, int32_t layout, double *grad_rgb,
                            uint32_t n_materials, rayna_render_stats *stats) {
     if (!r || !d_image || !grad_rgb) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     if (layout != RAYNA_LAYOUT_XMAJOR && layout != RAYNA_LAYOUT_ROWMAJOR)
         return fail(r, RAYNA_INVALID_ARGUMENT, "unknown layout");
     if (n_materials != r->packed.materials.size())
@@ -726,11 +782,14 @@ rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_ima
     }
     launch_sum_grad(r->grad, 3 * n_materials, copies, r->grad_sum, r->stream);
     if (ev) HIP_TRY(r, hipEventRecord((*ev)[evi++], r->stream));
-    HIP_TRY(r, hipGetLastError());
-    HIP_TRY(r, hipMemcpyAsync(grad_rgb, r->grad_sum, per_copy, hipMemcpyDeviceToHost, r->stream));
+    {
+        const rayna_status ls = launch_status(r, "adjoint");
+        if (ls != RAYNA_OK) return ls;
+    }
+    HIP_TRY_LAUNCHED(r, hipMemcpyAsync(grad_rgb, r->grad_sum, per_copy, hipMemcpyDeviceToHost, r->stream));
     unsigned long long segs = 0;
-    HIP_TRY(r, hipMemcpyAsync(&segs, r->segments, sizeof(segs), hipMemcpyDeviceToHost, r->stream));
-    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    HIP_TRY_LAUNCHED(r, hipMemcpyAsync(&segs, r->segments, sizeof(segs), hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY_LAUNCHED(r, hipStreamSynchronize(r->stream));
     // the radiance field now holds what the replay left of it
     r->primal_frame = -1;
     if (stats) {
@@ -750,6 +809,7 @@ rayna_status rayna_adjoint(rayna_renderer *r, uint32_t frame, const float *d_ima
 rayna_status rayna_adjoint_static(rayna_renderer *r, uint32_t frame, const float *d_image_dev, double *grad_dev,
                                   uint32_t n_materials) {
     if (!r || !d_image_dev || !grad_dev) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     if (n_materials != r->packed.materials.size())
         return fail(r, RAYNA_INVALID_ARGUMENT, "n_materials is not the scene's material count");
     if (!r->cam_valid) return fail(r, RAYNA_INVALID_ARGUMENT, "camera is invalid (the frame is the checkerboard)");
@@ -784,14 +844,18 @@ rayna_status rayna_adjoint_static(rayna_renderer *r, uint32_t frame, const float
         trace_chunk(r, fp, s0, n, nullptr, nullptr, &launches, &ap);
     }
     launch_sum_grad(ws, 3 * n_materials, copies, grad_dev, r->stream);
-    HIP_TRY(r, hipGetLastError());
-    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    {
+        const rayna_status ls = launch_status(r, "adjoint_static");
+        if (ls != RAYNA_OK) return ls;
+    }
+    HIP_TRY_LAUNCHED(r, hipStreamSynchronize(r->stream));
     r->primal_frame = -1;
     return RAYNA_OK;
 }
 
 rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_t *prim_rank, float *t) {
     if (!r || !prim_rank || !t) return fail(r, RAYNA_INVALID_ARGUMENT, "null argument");
+    REQUIRE_LIVE(r);
     if (!r->cam_valid) return fail(r, RAYNA_INVALID_ARGUMENT, "camera is invalid");
     HIP_TRY(r, hipSetDevice(r->device));
     {
@@ -817,12 +881,20 @@ rayna_status rayna_primary_hits(rayna_renderer *r, uint32_t sample_index, int32_
                   BatchView{}, r->stream);
     launch_march(r->dev.view, fp, pb, nullptr, nullptr, (uint32_t)r->dev.view.n_sdfs, r->mt, r->mt_cap, work + 1,
                  r->grid_march, r->grid_shade, r->tune, BatchView{}, r->sdf_queue, r->sdf_count, r->stream);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(prim_rank, rank_dev, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(t, pb.hit_t, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    rayna_status ls = launch_status(r, "primary_hits");
+    hipError_t e = hipSuccess;
+    if (ls == RAYNA_OK) e = hipMemcpyAsync(prim_rank, rank_dev, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
+    if (ls == RAYNA_OK && e == hipSuccess)
+        e = hipMemcpyAsync(t, pb.hit_t, (size_t)P * 4, hipMemcpyDeviceToHost, r->stream);
+    // the stream has to drain before rank_dev is freed, whatever the launches reported
+    const hipError_t se = hipStreamSynchronize(r->stream);
+    if (e == hipSuccess) e = se;
     (void)hipFree(rank_dev);
-    if (e != hipSuccess) return fail(r, RAYNA_DEVICE_ERROR, std::string("primary hits: ") + hipGetErrorString(e));
+    if (ls != RAYNA_OK) return ls;
+    if (e != hipSuccess) {
+        r->lost = std::string("primary hits: ") + hipGetErrorString(e);
+        return fail(r, RAYNA_DEVICE_ERROR, r->lost);
+    }
     return RAYNA_OK;
 }
 
@@ -915,6 +987,8 @@ void rayna_destroy(rayna_renderer *r) {
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
 }
+
+int32_t rayna_debug_fail_launches(int32_t countdown) { return g_fail_launches.exchange(countdown < 0 ? -1 : countdown); }
 
 const char *rayna_last_error(const rayna_renderer *r) { return r ? r->error.c_str() : g_create_error.c_str(); }
 

@@ -31,7 +31,7 @@ EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_opt
            "rayna_set_validate", "rayna_validate_counts", "rayna_adjoint_static", "rayna_batch_create",
            "rayna_batch_set_tuning", "rayna_render_batch", "rayna_batch_destroy", "rayna_batch_last_error",
            "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare", "rayna_scene_tree", "rayna_destroy",
-           "rayna_last_error")
+           "rayna_debug_fail_launches", "rayna_last_error")
 
 
 # RAYNA_CHECK_* of include/rayna_gpu.h, in order: the invariants of the engine's shared/validate.rs
@@ -109,10 +109,20 @@ def lib() -> C.CDLL:
                                        C.POINTER(C.c_int32)]
         L.rayna_destroy.restype = None
         L.rayna_destroy.argtypes = [vp]
+        L.rayna_debug_fail_launches.restype = C.c_int32
+        L.rayna_debug_fail_launches.argtypes = [C.c_int32]
         L.rayna_last_error.restype = C.c_char_p
         L.rayna_last_error.argtypes = [vp]
         _lib = L
     return _lib
+
+
+def debug_fail_launches(countdown: int) -> int:
+    """Fault injection for recovery tests: the launch check `countdown` checks from now (0: the next) of any
+    renderer reports a launch failure (RayError with status DEVICE_ERROR), after which that renderer raises
+    RENDERER_LOST until it is replaced (Renderer.recreated).  No GPU fault is made.  countdown < 0 switches it off.
+    Returns the previous setting (-1 when off)."""
+    return int(lib().rayna_debug_fail_launches(int(countdown)))
 
 
 def _check(status: int, handle=None):
@@ -183,6 +193,13 @@ class Renderer:
     @classmethod
     def new_from(cls, scene, camera, options, num_threads: int = 0):
         return cls(scene, camera, options, num_threads)
+
+    def recreated(self) -> "Renderer":
+        """A new renderer of this one's scene, camera and options, after this one is closed: the recovery from
+        RENDERER_LOST, as the engine's UI restarts its render worker from the scene, camera and options it
+        keeps (rayna_ui/src/app.rs:483-495).  Tuning and validation go back to their defaults."""
+        self.close()
+        return Renderer.new_from(self._scene, self._camera, self._options, self.num_threads)
 
     def close(self):
         if self._h:

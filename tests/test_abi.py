@@ -62,3 +62,22 @@ def test_struct_layouts_match_a_compiled_probe(tmp_path):
         assert got[f"{cname} size"] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[f"{cname} {f}"] == getattr(py, f).offset, (cname, f)
+
+
+def test_status_names_cover_the_header():
+    text = open(HEADER).read()
+    codes = dict((int(v), n) for n, v in re.findall(r"RAYNA_([A-Z_]+) = (\d+)", text.split("} rayna_status;")[0]))
+    assert codes == A.STATUS_NAMES
+    assert A.RAYNA_RENDERER_LOST == 5
+
+
+def test_fault_injection_flag_is_host_state():
+    """rayna_debug_fail_launches only sets a countdown (no HIP call, so it runs without a GPU): it returns the
+    previous setting, and any negative value means off."""
+    L = renderer.lib()
+    assert L.rayna_debug_fail_launches(-1) == -1
+    assert renderer.debug_fail_launches(3) == -1
+    assert renderer.debug_fail_launches(-7) == 3
+    assert renderer.debug_fail_launches(0) == -1
+    assert renderer.debug_fail_launches(-1) == 0
+    assert renderer.debug_fail_launches(-1) == -1
