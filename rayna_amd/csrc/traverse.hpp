@@ -39,11 +39,18 @@ __device__ unsigned long long g_trav_stats[4];
 
 
 // Traversal stack of one lane: the top EXT_LDS_STACK entries in LDS, each (code, distance) one 64-bit word (one
-// column per lane, conflict-free),
-// deeper entries in this thread's column of a global overflow array (`stride` = threads in the grid).  The
-// host sizes the overflow for the deepest stack the tree can need, so no push is ever dropped.
+// column per lane, conflict-free), deeper entries in this thread's column of a global overflow array (`stride` =
+// threads in the grid).  The host sizes the overflow for the deepest stack the tree can need, so no push is ever
+// dropped.  The LDS column is addressed through an LDS pointer and each entry moves as one 64-bit word: through
+// generic pointers the compiler merged the LDS and global halves of a push into one flat store.
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
+__device__ __forceinline__ unsigned long long stack_word(int32_t n, float t) {
+    return (unsigned long long)(uint32_t)n | ((unsigned long long)__float_as_uint(t) << 32);
+}
+
 struct LaneStack {
-    int2 *s_entry;
+    lds_u64 *s_entry;
     int2 *g;
     uint32_t stride;
     int sp;
@@ -53,7 +60,7 @@ struct LaneStack {
     float top_t = 0.0f;
     bool top_valid = false;
     __device__ __forceinline__ void store(int i, int32_t n, float t) {
-        if (i < EXT_LDS_STACK) s_entry[i * EXT_BLOCK + threadIdx.x] = make_int2(n, __float_as_int(t));
+        if (i < EXT_LDS_STACK) s_entry[i * EXT_BLOCK + threadIdx.x] = stack_word(n, t);
         else g[(size_t)(i - EXT_LDS_STACK) * stride] = make_int2(n, __float_as_int(t));
     }
     __device__ __forceinline__ void push(int32_t n, float t) {
@@ -72,9 +79,9 @@ struct LaneStack {
             return;
         }
         if (sp < EXT_LDS_STACK) {
-            const int2 e = s_entry[sp * EXT_BLOCK + threadIdx.x];
-            *n = e.x;
-            *t = __int_as_float(e.y);
+            const unsigned long long e = s_entry[sp * EXT_BLOCK + threadIdx.x];
+            *n = (int32_t)(uint32_t)e;
+            *t = __uint_as_float((uint32_t)(e >> 32));
         } else {
             const int2 e = g[(size_t)(sp - EXT_LDS_STACK) * stride];
             *n = e.x;
@@ -270,13 +277,13 @@ template <bool BATCH, bool INST, bool SPH>
 __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_EXT_WAVES))) void k_extend(
     SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue, const uint32_t *count_ptr, uint32_t depth,
     int2 *overflow, uint32_t *work, ExtendTuning tune, BatchView bv) {
-    __shared__ int2 s_entry[EXT_LDS_STACK * EXT_BLOCK];
+    __shared__ unsigned long long s_entry[EXT_LDS_STACK * EXT_BLOCK];
     const uint32_t count = queue ? *count_ptr : fp.n_slots;
     const uint32_t refill_lanes = tune.refill_lanes, wave_batch = tune.wave_batch;
     dm::Key key{fp.key0, fp.key1};
     const uint32_t gid = blockIdx.x * EXT_BLOCK + threadIdx.x;
     const uint32_t lane = __lane_id();
-    LaneStack st{s_entry, overflow + gid, gridDim.x * EXT_BLOCK, 0};
+    LaneStack st{(lds_u64 *)s_entry, overflow + gid, gridDim.x * EXT_BLOCK, 0};
     bool active = false, exhausted = false;
     uint32_t slot = 0, pixel = 0, sample = 0;
     DRay ray{};
