@@ -30,17 +30,24 @@ EXTEND_BYTES_PER_SEGMENT = 40
 # reads 12 B.
 FRAME_BYTES_PER_SEGMENT = 40 + 116
 FRAME_BYTES_PER_SAMPLE = 24 + 12
-# HBM traffic of the extend stage per closest-hit query on the 1080p triangle-mesh frame, from rocprofv3
-# counters over one frame (9 launches -- k_extend_packet for the camera rays, k_extend for depths 1-8 --
-# 263,044,862 queries): FETCH_SIZE 2.0200e7 KiB read as 64-byte units (x2.0: a 256 MiB copy reads 131,088 KiB)
-# plus WRITE_SIZE 6.4708e6 KiB (x1.0: the same copy writes 262,144 KiB).  Node and triangle fetches that miss L2
-# make up the difference to the 40 algorithmic bytes.
-EXTEND_TRAFFIC_BYTES_PER_SEGMENT = {"triangle_mesh": (2.0200e7 * 2.0 + 6.4708e6) * 1024 / 263044862}
-# What bounds the extend stage instead, from the same frame's counters (VALUBusy and VALUUtilization passes,
-# time-weighted over the 9 launches, 49.997 ms in all): the VALU issues every cycle and 41.8% of the lanes it
-# issues for are active (camera-ray packets 89.7%, the later depths 30.5%).
-EXTEND_ISSUE = {"triangle_mesh": {"valu_busy_pct": 102.7, "valu_lanes_active_pct": 41.8,
-                                  "achieved_hbm_gbps": (2.0200e7 * 2.0 + 6.4708e6) * 1024 / 9 / 5.5552e-3 / 1e9}}
+# Counter-derived figures of the extend stage on the 1080p triangle-mesh frame, from rocprofv3 passes over two
+# frames of 263,040,107 closest-hit queries each (132,710,400 of them camera rays) at the commit that added them;
+# the times they are divided by are measured live.  Traffic (both extend kernels): what leaves L2 for the
+# fabric, FETCH_SIZE 4.0444e7 KiB read as 64-byte units (x2.0: a 256 MiB copy reads 131,087 KiB) plus WRITE_SIZE
+# 1.2944e7 KiB (x1.0: the copy writes 262,143 KiB); it counts hits in the Infinity Cache as well as HBM, so it
+# bounds the HBM bytes from above.  VALU (k_extend, depths 1-8): SQ_INSTS_VALU 4.8190e10 wave64 instructions over
+# 2 x 130,329,707 queries; lanes active SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU) = 9.5818e11 / (64 x
+# 4.8452e10); 0.990 instructions per CU per cycle (GRBM_GUI_ACTIVE 1.5204e9 over 8 XCDs, 256 CUs).
+EXTEND_COUNTER_FRAMES = 2 * 263040107
+EXTEND_FABRIC_BYTES_PER_SEGMENT = {"triangle_mesh": (4.0444e7 * 2.0 + 1.2944e7) * 1024 / EXTEND_COUNTER_FRAMES}
+K_EXTEND_VALU_PER_QUERY = {"triangle_mesh": 4.8190e10 / (2 * 130329707)}
+K_EXTEND_LANES_ACTIVE = {"triangle_mesh": 9.5818e11 / (64 * 4.8452e10)}
+K_EXTEND_VALU_PER_CU_CYCLE = {"triangle_mesh": 4.8190e10 / (1.5204e9 / 8 * 256)}
+# VALU issue peak: one wave64 instruction per CU per cycle -- the rate the calibration kernel
+# (rayna_measure_valu_peak: independent fp32 multiply-add chains, no packed or fused forms) reaches per active
+# cycle (0.977 by the same counters) -- at MI355X's published 256 CUs and 2.4 GHz.
+MI355X_CUS, MI355X_GHZ = 256, 2.4
+VALU_PEAK_G = MI355X_CUS * MI355X_GHZ * 1.0
 
 
 _T0 = time.perf_counter()
@@ -49,6 +56,27 @@ _T0 = time.perf_counter()
 def _progress(msg: str):
     """A progress line on stderr (stdout carries only the JSON line)."""
     print(f"bench [{time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
+def valu_roofline(workload: str, camera_rays: int, segs: int, ext_ms: float, cam_ms: float):
+    """k_extend (depths 1-8) against the VALU issue peak: its wave64 VALU instructions per closest-hit query
+    (counted by rocprofv3 on this workload) times this run's queries at those depths, over this run's k_extend
+    time (the extend launches' HIP events less the camera rays')."""
+    per_q = K_EXTEND_VALU_PER_QUERY.get(workload)
+    t = (ext_ms - cam_ms) * 1e-3
+    if not per_q or t <= 0:
+        return None
+    from rayna_amd.renderer import measure_valu_peak
+    calib, _ = measure_valu_peak()
+    achieved = per_q * (segs - camera_rays) / t / 1e9
+    lanes = K_EXTEND_LANES_ACTIVE[workload]
+    return {"kernel": "k_extend (depths 1-8)", "achieved": achieved, "peak": VALU_PEAK_G,
+            "unit": "G wave64 VALU instructions/s", "frac": achieved / VALU_PEAK_G, "insts_per_query": per_q,
+            "lanes_active": lanes, "useful_frac": achieved / VALU_PEAK_G * lanes,
+            "insts_per_cu_cycle_profiled": K_EXTEND_VALU_PER_CU_CYCLE[workload],
+            "k_extend_ms_per_frame": None, "calibration_g_per_s": calib / 1e9,
+            "peak_basis": "1 wave64 VALU instruction per CU per cycle (the calibration kernel's rate per active "
+                          "cycle) x 256 CUs x 2.4 GHz"}
 
 
 class _NoGc:
@@ -349,6 +377,8 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true", help="batched leg with the default launch shape")
     ap.add_argument("--no-workloads", action="store_true", help="skip the other workloads' frames")
     ap.add_argument("--no-inverse", action="store_true", help="skip the inverse-rendering run")
+    ap.add_argument("--valu-peak", action="store_true",
+                    help="run only the VALU issue-rate calibration kernel (for counter runs) and print its rate")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB three times (the counter calibration kernel)")
     a = ap.parse_args(argv)
@@ -367,6 +397,11 @@ def main(argv=None):
         torch.cuda.synchronize()
         print(f"copied {a.copy_mib} MiB x3", flush=True)
         return 0
+    if a.valu_peak:
+        from rayna_amd.renderer import measure_valu_peak
+        rate, secs = measure_valu_peak()
+        print(f"VALU calibration: {rate / 1e9:.1f} G wave64 instructions/s ({secs * 1e3:.3f} ms)", flush=True)
+        return 0
     from rayna_amd import Renderer
 
     if torch.cuda.is_available():  # the framework's device context first, so that its setup is not timed
@@ -380,7 +415,7 @@ def main(argv=None):
     for _ in range(a.warmup):
         r.render(to_host=False)
     torch.cuda.synchronize() if torch.cuda.is_available() else None
-    ext_ms = sh_ms = kern_ms = call_ms = 0.0
+    ext_ms = sh_ms = kern_ms = call_ms = cam_ms = 0.0
     segs = 0
     launches = 0
     step_ms = []
@@ -392,6 +427,7 @@ def main(argv=None):
         st = r.render(to_host=False).stats
         step_ms.append((time.perf_counter() - ts) * 1e3)
         ext_ms += st.extend_ms
+        cam_ms += st.extend_camera_ms
         sh_ms += st.shade_ms
         kern_ms += st.kernel_ms
         call_ms += st.duration * 1e3
@@ -411,7 +447,7 @@ def main(argv=None):
     value = samples_per_frame * a.steps / secs / 1e6
     ext_bytes = EXTEND_BYTES_PER_SEGMENT * segs
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
-    per_seg = EXTEND_TRAFFIC_BYTES_PER_SEGMENT.get(a.workload)
+    per_seg = EXTEND_FABRIC_BYTES_PER_SEGMENT.get(a.workload)
     traffic = per_seg * segs / launches if per_seg and launches else None
     frame_bytes = FRAME_BYTES_PER_SEGMENT * segs + FRAME_BYTES_PER_SAMPLE * samples_per_frame * a.steps
     line = {
@@ -432,6 +468,7 @@ def main(argv=None):
         "mrays_per_s": segs / secs / 1e6,
         "segments_per_frame": segs // max(1, a.steps),
         "extend_ms_per_frame": ext_ms / a.steps,
+        "extend_camera_ms_per_frame": cam_ms / a.steps,
         "shade_ms_per_frame": sh_ms / a.steps,
         "kernel_ms_per_frame": kern_ms / a.steps,
         "render_call_ms_per_frame": call_ms / a.steps,
@@ -444,11 +481,15 @@ def main(argv=None):
         "scene_build_s": t_build,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "traffic_kind": "L2-miss (fabric) bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                     "calibrated on a 256 MiB copy; Infinity Cache hits included",
                      "kernel": "extend: k_extend_packet (camera rays) + k_extend (later depths)",
                      "bytes_per_segment": EXTEND_BYTES_PER_SEGMENT},
     }
-    if a.workload in EXTEND_ISSUE:  # profiled, not measured by this run
-        line["extend_issue_profiled"] = EXTEND_ISSUE[a.workload]
+    valu = valu_roofline(a.workload, samples_per_frame * a.steps, segs, ext_ms, cam_ms)
+    if valu:
+        valu["k_extend_ms_per_frame"] = (ext_ms - cam_ms) / a.steps
+        line["roofline"]["valu"] = valu
     threads, host = host_cpus()
     if a.cpu_threads:
         threads = a.cpu_threads

@@ -278,6 +278,7 @@ typedef struct rayna_render_stats {
     uint64_t samples;        /* camera samples traced this frame (width * height * samples) */
     uint64_t segments;       /* closest-hit queries this frame (one per path vertex) */
     uint64_t extend_launches;
+    double extend_camera_ms; /* of extend_ms: the launches for the camera rays (depth 0) */
 } rayna_render_stats;
 
 /* Output layouts of rayna_render. */
@@ -469,6 +470,14 @@ rayna_status rayna_scene_tree(const rayna_scene_desc *scene, int32_t *node_words
 
 /* Renderer drop. */
 void rayna_destroy(rayna_renderer *r);
+
+/* Measurement hook (no engine counterpart): the fp32 VALU issue rate the current GPU sustains, the peak the
+ * traversal kernels' VALU roofline is taken against.  A calibration kernel runs eight independent multiply-add
+ * chains per lane (one v_mul_f32 and one v_add_f32 per link: no packed or fused forms), eight waves on every
+ * SIMD, `iters` links per chain (1..2^20).  *wave_insts_per_s: wave64 VALU instructions per second, counted by
+ * construction over one timed launch after a warm-up; *seconds (may be NULL): that launch's duration.  Errors
+ * are reported through rayna_last_error(NULL). */
+rayna_status rayna_measure_valu_peak(uint32_t iters, double *wave_insts_per_s, double *seconds);
 
 /* Debug fault injection.  No counterpart in the engine, whose render cannot fail (render/renderer.rs:224-264);
  * it is for the callers' recovery tests.  countdown >= 0 makes the launch check that many checks from now (0: the

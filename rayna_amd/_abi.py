@@ -83,7 +83,7 @@ class RenderOptsDesc(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("duration_s", C.c_double), ("kernel_ms", C.c_double), ("extend_ms", C.c_double),
                 ("shade_ms", C.c_double), ("accum_frames", C.c_uint64), ("samples", C.c_uint64),
-                ("segments", C.c_uint64), ("extend_launches", C.c_uint64)]
+                ("segments", C.c_uint64), ("extend_launches", C.c_uint64), ("extend_camera_ms", C.c_double)]
 
 
 class Tuning(C.Structure):

@@ -1174,6 +1174,25 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t value, uint32_t n) {
     if (i < n) p[i] = value;
 }
 
+// VALU issue-rate calibration: eight independent multiply-add chains per lane, each link one v_mul_f32 and one
+// v_add_f32 (no packed or fused forms are generated), so a lane issues 16 fp32 VALU instructions per iteration
+// and enough waves fill every SIMD that the dependency latency is hidden.
+constexpr uint32_t VALU_PEAK_CHAINS = 8;
+__global__ __launch_bounds__(BLOCK) void k_valu_peak(float *out, uint32_t iters, float m, float c) {
+    const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
+    float a[VALU_PEAK_CHAINS];
+#pragma unroll
+    for (uint32_t k = 0; k < VALU_PEAK_CHAINS; ++k) a[k] = (float)(gid + k) * 1e-6f;
+    for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+        for (uint32_t k = 0; k < VALU_PEAK_CHAINS; ++k) a[k] = a[k] * m + c;
+    }
+    float sum = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < VALU_PEAK_CHAINS; ++k) sum += a[k];
+    out[gid] = sum;
+}
+
 __global__ void k_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
                                  unsigned long long *segments) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -1310,6 +1329,11 @@ void launch_finalize(const FrameParams &fp, const float *sum, float *acc, double
 
 void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, BLOCK)), dim3(BLOCK), 0, s, p, value, n);
+}
+
+uint64_t launch_valu_peak(float *out, uint32_t blocks, uint32_t iters, hipStream_t s) {
+    hipLaunchKernelGGL(k_valu_peak, dim3(blocks), dim3(BLOCK), 0, s, out, iters, 0.999f, 1e-3f);
+    return (uint64_t)blocks * (BLOCK / 64) * iters * (2 * VALU_PEAK_CHAINS);
 }
 
 void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,

@@ -136,6 +136,9 @@ void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, ui
 void launch_finalize(const FrameParams &fp, const float *sum, float *acc, double frames, float *out, int layout,
                      hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s);
+// The VALU issue-rate calibration kernel over `blocks` blocks of 256 lanes (out: one float per lane); returns
+// the wave64 VALU instructions its chains issue (loop and set-up instructions not counted).
+uint64_t launch_valu_peak(float *out, uint32_t blocks, uint32_t iters, hipStream_t s);
 void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
                            unsigned long long *segments, hipStream_t s);
 

@@ -409,21 +409,25 @@ rayna_status ensure_events(rayna_renderer *r, size_t n) {
 }
 
 // Device times from the events of one call: events[0] before the first kernel, events[evi - 1] after the
-// last, and per trace_chunk depth the triple [before extend, after extend, after shade] in between.
+// last, and per trace_chunk depth the triple [before extend, after extend, after shade] in between (each pass
+// over a chunk records one triple per depth, ray_depth + 1 of them; one for the non-PBR modes).
 rayna_status event_times(rayna_renderer *r, size_t evi, rayna_render_stats *st) {
     float ms = 0;
     HIP_TRY(r, hipEventElapsedTime(&ms, r->events[0], r->events[evi - 1]));
     st->kernel_ms = ms;
-    double ext = 0, sh = 0;
-    for (size_t i = 1; i + 2 < evi; i += 3) {
+    const size_t per_pass = r->opts.mode == RAYNA_MODE_PBR ? (size_t)r->opts.ray_depth + 1 : 1;
+    double ext = 0, sh = 0, ext0 = 0;
+    for (size_t i = 1, k = 0; i + 2 < evi; i += 3, ++k) {
         float a = 0, b = 0;
         HIP_TRY(r, hipEventElapsedTime(&a, r->events[i], r->events[i + 1]));
         HIP_TRY(r, hipEventElapsedTime(&b, r->events[i + 1], r->events[i + 2]));
         ext += a;
         sh += b;
+        if (k % per_pass == 0) ext0 += a;
     }
     st->extend_ms = ext;
     st->shade_ms = sh;
+    st->extend_camera_ms = ext0;
     return RAYNA_OK;
 }
 
@@ -986,6 +990,43 @@ void rayna_destroy(rayna_renderer *r) {
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
+}
+
+rayna_status rayna_measure_valu_peak(uint32_t iters, double *wave_insts_per_s, double *seconds) {
+    g_create_error.clear();
+    if (!wave_insts_per_s || iters == 0 || iters > (1u << 20))
+        return fail(nullptr, RAYNA_INVALID_ARGUMENT, "null output or iterations outside 1..2^20");
+    int device = 0;
+    HIP_TRY(nullptr, hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    HIP_TRY(nullptr, hipGetDeviceProperties(&prop, device));
+    // 32 blocks of four waves per CU: eight waves on each SIMD, four rounds of them
+    const uint32_t blocks = 32u * (uint32_t)std::max(1, prop.multiProcessorCount);
+    float *out = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipMalloc((void **)&out, (size_t)blocks * 256 * sizeof(float));
+    if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    uint64_t insts = 0;
+    float ms = 0.0f;
+    if (e == hipSuccess) {
+        launch_valu_peak(out, blocks, iters, nullptr);  // warm-up (clocks, code load)
+        e = hipEventRecord(ev[0], nullptr);
+    }
+    if (e == hipSuccess) {
+        insts = launch_valu_peak(out, blocks, iters, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev[1], nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (out) (void)hipFree(out);
+    if (e != hipSuccess) return fail(nullptr, RAYNA_DEVICE_ERROR, std::string("VALU calibration: ") + hipGetErrorString(e));
+    *wave_insts_per_s = ms > 0.0f ? (double)insts / (ms * 1e-3) : 0.0;
+    if (seconds) *seconds = ms * 1e-3;
+    return RAYNA_OK;
 }
 
 int32_t rayna_debug_fail_launches(int32_t countdown) { return g_fail_launches.exchange(countdown < 0 ? -1 : countdown); }

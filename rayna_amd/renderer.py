@@ -31,7 +31,7 @@ EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_opt
            "rayna_set_validate", "rayna_validate_counts", "rayna_adjoint_static", "rayna_batch_create",
            "rayna_batch_set_tuning", "rayna_render_batch", "rayna_batch_destroy", "rayna_batch_last_error",
            "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare", "rayna_scene_tree", "rayna_destroy",
-           "rayna_debug_fail_launches", "rayna_last_error")
+           "rayna_measure_valu_peak", "rayna_debug_fail_launches", "rayna_last_error")
 
 
 # RAYNA_CHECK_* of include/rayna_gpu.h, in order: the invariants of the engine's shared/validate.rs
@@ -109,12 +109,22 @@ def lib() -> C.CDLL:
                                        C.POINTER(C.c_int32)]
         L.rayna_destroy.restype = None
         L.rayna_destroy.argtypes = [vp]
+        L.rayna_measure_valu_peak.restype = C.c_int
+        L.rayna_measure_valu_peak.argtypes = [C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.rayna_debug_fail_launches.restype = C.c_int32
         L.rayna_debug_fail_launches.argtypes = [C.c_int32]
         L.rayna_last_error.restype = C.c_char_p
         L.rayna_last_error.argtypes = [vp]
         _lib = L
     return _lib
+
+
+def measure_valu_peak(iters: int = 2048):
+    """(wave64 VALU instructions per second, seconds) of the library's issue-rate calibration kernel on the
+    current GPU: eight independent fp32 multiply-add chains per lane, no packed or fused forms."""
+    rate, secs = C.c_double(0.0), C.c_double(0.0)
+    _check(lib().rayna_measure_valu_peak(int(iters), C.byref(rate), C.byref(secs)))
+    return rate.value, secs.value
 
 
 def debug_fail_launches(countdown: int) -> int:
@@ -142,6 +152,7 @@ class RenderStats:
     extend_ms: float
     shade_ms: float
     extend_launches: int
+    extend_camera_ms: float = 0.0  # of extend_ms: the camera rays' (depth 0) launches
 
 
 @dataclass
@@ -284,7 +295,7 @@ class Renderer:
             img = img[: w * h * 3]
             img = img.reshape(w, h, 3) if lay == A.LAYOUT_XMAJOR else img.reshape(h, w, 3)
         stats = RenderStats(st.duration_s, int(st.accum_frames), int(st.samples), int(st.segments), st.kernel_ms,
-                            st.extend_ms, st.shade_ms, int(st.extend_launches))
+                            st.extend_ms, st.shade_ms, int(st.extend_launches), st.extend_camera_ms)
         return Render(img, stats)
 
     def render_srgb8(self) -> Render:
@@ -298,7 +309,7 @@ class Renderer:
                self._h)
         w, h = int(dims[0]), int(dims[1])
         stats = RenderStats(st.duration_s, int(st.accum_frames), int(st.samples), int(st.segments), st.kernel_ms,
-                            st.extend_ms, st.shade_ms, int(st.extend_launches))
+                            st.extend_ms, st.shade_ms, int(st.extend_launches), st.extend_camera_ms)
         return Render(img[: w * h * 4].reshape(h, w, 4), stats)
 
     # ---- differentiation ----
@@ -323,7 +334,7 @@ class Renderer:
         _check(lib().rayna_adjoint(self._h, frame, d.ctypes.data_as(C.POINTER(C.c_float)), lay,
                                    grad.ctypes.data_as(C.POINTER(C.c_double)), n, C.byref(st)), self._h)
         stats = RenderStats(st.duration_s, int(st.accum_frames), int(st.samples), int(st.segments), st.kernel_ms,
-                            st.extend_ms, st.shade_ms, int(st.extend_launches))
+                            st.extend_ms, st.shade_ms, int(st.extend_launches), st.extend_camera_ms)
         return grad, stats
 
     # ---- parity hooks ----
