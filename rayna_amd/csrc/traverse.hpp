@@ -275,6 +275,12 @@ __device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_
 // belonged to lanes whose traversal had ended and a fifth to lanes stalled on a second range).  Measured there:
 // 0 (wait for every lane) 40.0 ms of k_extend per frame, 4: 34.8, 6: 34.4, 8: 34.1, 10: 34.2, 12: 34.5, 16: 35.3.
 constexpr uint32_t EXTEND_INNER_EXIT = 8;
+// Likewise the primitive loop ends once this many lanes or fewer still test primitives (ranges hold 1 to 8 of
+// them, and lanes with short ones waited for the longest).  Measured there with EXTEND_INNER_EXIT 8: 0 (every
+// range tested to its end) 34.4-34.6 ms, 4: 33.5, 8: 32.8, 16: 32.2, 24: 32.5, 32: 33.0, 64 (one primitive per
+// pass) 33.6; with EXTEND_INNER_EXIT 4 or 16 and this at 16: 32.9 and 33.1.  Holding part-tested ranges of items
+// back from speculation (below) left it at 32.6.
+constexpr uint32_t EXTEND_PRIM_EXIT = 16;
 
 template <bool BATCH, bool INST, bool SPH>
 #ifndef RAYNA_EXT_WAVES
@@ -383,7 +389,7 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
             // end, goes back on the stack and the lane waits.  The lanes still searching when the loop ends keep
             // their stacks and go on in the next pass, after the ranges are tested; every pass tests a range or
             // pops an entry, so the loop ends.
-            bool stalled = false;
+            bool stalled = leaf != 0 && ((uint32_t)leaf & LEAF_CODE_ITEMS) && leaf_i > 0;
             for (;;) {
                 const unsigned long long searching = __ballot(leaf == 0 && st.sp > 0);
                 if (searching == 0ull) break;
@@ -419,13 +425,22 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
                 RAYNA_STAT(expands, 1);
                 expand(sv, ray, b.t, code, st);
             }
-            while (leaf != 0) {  // the leaf's primitives, one per iteration
-                RAYNA_STAT(prims, 1);
-                if (leaf_test<INST, SPH>(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth) ||
-                    ++leaf_i == leaf_n)
-                    leaf = 0;
-            }
-            if (st.sp == 0) {
+            // The held ranges' primitives, one per pass, until at most EXTEND_PRIM_EXIT lanes still test.  A lane
+            // left holding part of a range keeps it (its next entry in leaf_i) through the next box loop: with a
+            // range of triangles, which push nothing, it speculates there as a lane holding a range does; with a
+            // range of items it waits (stalled), because the links and instance entries its items push count
+            // towards the stack bound only as a block on top of a depth-first state, not interleaved with
+            // speculative expansions of the links pushed so far.  Every pass tests a primitive of each holder, so
+            // the loop ends.
+            do {
+                if (leaf != 0) {
+                    RAYNA_STAT(prims, 1);
+                    if (leaf_test<INST, SPH>(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth) ||
+                        ++leaf_i == leaf_n)
+                        leaf = 0;
+                }
+            } while ((uint32_t)__popcll(__ballot(leaf != 0)) > EXTEND_PRIM_EXIT);
+            if (st.sp == 0 && leaf == 0) {
                 pb.hit_t[slot] = b.t;
                 pb.hit_ref[slot] = b.ref;
                 pb.hit_aux[slot] = b.aux;

@@ -6,7 +6,9 @@ packer's `packet_need` fits.  These tests replay both disciplines on the CPU ove
 packer emits (rayna_scene_tree, no GPU): random culling, random child order, and -- for k_extend -- a
 speculation that a lane holding a leaf range carries on for as long as the rest of its wave keeps searching
 (any length, up to a second range or an empty stack), after which the held range's mesh-BVH links and
-instance entries go on top.  The peak must stay within the reported bound on scenes built to be adversarial
+instance entries go on top; a held triangle range (which pushes nothing) may also be kept, part-tested, through
+further speculation, as the primitive loop's early end leaves it (a part-tested range of items keeps its lane from
+speculating until it is done, so its entries still go on top as one block, as modelled).  The peak must stay within the reported bound on scenes built to be adversarial
 (a mesh-BVH leaf next to a wide subtree, instances, nested BVH meshes) and on the presets.
 """
 import numpy as np
@@ -83,9 +85,14 @@ def lane_peak(tree, rng, p_hit, p_go_on):
                 return peak
             continue
         _, start, count, kind = held
-        held = None
         if kind != LEAF_ITEMS:
+            # a triangle range may be left part-tested when the primitive loop ends early; the lane keeps it
+            # and speculates again
+            if rng.random() < 0.5:
+                continue
+            held = None
             continue
+        held = None
         for i in range(count):
             ik, idx = (int(v) for v in tree.items[start + i])
             if ik == ITEM_LINK:
