@@ -394,6 +394,10 @@ __device__ __forceinline__ uint32_t make_ref(int kind, int32_t index) {
     return ((uint32_t)kind << HIT_KIND_SHIFT) | ((uint32_t)index & HIT_INDEX_MASK);
 }
 
+// defined with the generate kernel below; k_extend_packet computes the camera rays at depth 0
+__device__ __forceinline__ DRay camera_ray(const GViewport &vp, uint32_t width, uint32_t height, uint32_t sdim,
+                                           dm::Key key, uint32_t pixel, uint32_t k);
+
 #include "traverse.hpp"
 #include "march.hpp"
 
@@ -724,8 +728,8 @@ __device__ __forceinline__ void validate_hit(uint32_t *checks, const DRay &ray, 
 // ------------------------------------------------------------------------------------------ kernels
 
 // The camera ray of sample k of pixel `pixel`: render_px_msaa's stratified jitter (renderer.rs:370-389) and
-// Viewport::calc_ray (camera.rs:191-213).  k_generate stores it; k_shade recomputes it at the first bounce
-// instead of loading it (the shade kernel waits on memory, not on arithmetic).
+// Viewport::calc_ray (camera.rs:191-213).  k_generate stores it where a kernel loads it; k_extend_packet and the
+// first k_shade compute it instead of loading it (they wait on memory more than on arithmetic).
 __device__ __forceinline__ DRay camera_ray(const GViewport &vp, uint32_t width, uint32_t height, uint32_t sdim,
                                            dm::Key key, uint32_t pixel, uint32_t k) {
     const uint32_t x = pixel % width, y = pixel / width;
@@ -828,11 +832,19 @@ __device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy,
 #define RAYNA_SHADE_STRIP 16
 #endif
 constexpr uint32_t SHADE_STRIP = RAYNA_SHADE_STRIP;
-template <bool ADJOINT, bool BATCH, bool TEX, bool VALIDATE = false>
+// Waves per SIMD the register budget of k_shade allows.  The forward shade without textures runs at eight: it
+// then spills about 112 bytes per lane, but its loads (queue entry, hit, triangle, material) are a chain of
+// dependent round trips, and more waves hide more of them (the 1080p mesh frame's shading 8.7 -> 8.0 ms; six
+// waves 8.3).  The adjoint, textured and validating forms keep five: their registers spill more.
 #ifndef RAYNA_SHADE_WAVES
 #define RAYNA_SHADE_WAVES 5
 #endif
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_SHADE_WAVES))) void k_shade(
+#ifndef RAYNA_SHADE_WAVES_PLAIN
+#define RAYNA_SHADE_WAVES_PLAIN 8
+#endif
+template <bool ADJOINT, bool BATCH, bool TEX, bool VALIDATE = false>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
+    (ADJOINT || TEX || VALIDATE) ? RAYNA_SHADE_WAVES : RAYNA_SHADE_WAVES_PLAIN))) void k_shade(
     SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue_in, const uint32_t *count_in,
     int32_t *queue_out, uint32_t *count_out, uint32_t depth, AdjointParams ap, BatchView bv) {
     const uint32_t count = queue_in ? *count_in : fp.n_slots;
@@ -1238,11 +1250,17 @@ void launch_generate(const FrameParams &fp, const PathBuffers &pb, const BatchVi
         hipLaunchKernelGGL(k_generate<false>, dim3(grid_for(fp.n_slots, BLOCK)), dim3(BLOCK), 0, s, fp, pb, bv);
 }
 
+bool extend_packets(const SceneView &sv, const FrameParams &fp, const ExtendTuning &tune, const BatchView &bv,
+                    uint32_t depth) {
+    // a pixel's samples in neighbouring slots: wave packets
+    return depth < tune.packet_depths && !sv.instances && sv.packet_need <= PACKET_STACK &&
+           (bv.reqs || fp.n_spp >= PACKET_MIN_SPP);
+}
+
 void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue,
                    const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work,
                    const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
-    if (depth < tune.packet_depths && !sv.instances && sv.packet_need <= PACKET_STACK &&
-        (bv.reqs || fp.n_spp >= PACKET_MIN_SPP)) {  // a pixel's samples in neighbouring slots: wave packets
+    if (extend_packets(sv, fp, tune, bv, depth)) {
         auto packet = bv.reqs ? (sv.sphere_tests ? k_extend_packet<true, true> : k_extend_packet<true, false>)
                               : (sv.sphere_tests ? k_extend_packet<false, true> : k_extend_packet<false, false>);
         hipLaunchKernelGGL(packet, dim3(grid_for(fp.n_slots, EXT_BLOCK)), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue,

@@ -136,6 +136,10 @@ void launch_resolve(const FrameParams &fp, const PathBuffers &pb, float *sum, ui
 void launch_finalize(const FrameParams &fp, const float *sum, float *acc, double frames, float *out, int layout,
                      hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s);
+// Whether launch_extend traces this depth as wave packets (k_extend_packet).  At depth 0 of a single request the
+// packets compute the camera rays themselves, so only the march and the non-PBR modes need k_generate's rays.
+bool extend_packets(const SceneView &sv, const FrameParams &fp, const ExtendTuning &tune, const BatchView &bv,
+                    uint32_t depth);
 // The VALU issue-rate calibration kernel over `blocks` blocks of 256 lanes (out: one float per lane); returns
 // the wave64 VALU instructions its chains issue (loop and set-up instructions not counted).
 uint64_t launch_valu_peak(float *out, uint32_t blocks, uint32_t iters, hipStream_t s);

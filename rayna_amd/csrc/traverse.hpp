@@ -498,10 +498,10 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
     const bool valid = q < count;
     const uint32_t slot = valid ? (queue ? (uint32_t)queue[q] : q) : 0u;
     const uint32_t ls = slot;
-    DRay ray = ray_from_dir(f3{pb.ox[ls], pb.oy[ls], pb.oz[ls]}, f3{pb.dx[ls], pb.dy[ls], pb.dz[ls]});
     uint32_t pixel, sample;
     dm::Key key{fp.key0, fp.key1};
     int32_t root = sv.root;
+    DRay ray;
     if (BATCH) {
         const RequestParams &R = bv.reqs[bv.slot_req[ls]];
         const uint32_t l = ls - R.slot_base;
@@ -512,6 +512,12 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
     } else {
         pixel = ls / fp.n_spp;
         sample = fp.s0 + ls % fp.n_spp;
+    }
+    if (!BATCH && !queue) {  // the camera ray, computed as k_generate computes it (which then need not run)
+        const DRay c = camera_ray(fp.vp, fp.width, fp.height, fp.sdim, key, pixel, sample);
+        ray = ray_from_dir(c.o, c.d);
+    } else {
+        ray = ray_from_dir(f3{pb.ox[ls], pb.oy[ls], pb.oz[ls]}, f3{pb.dx[ls], pb.dy[ls], pb.dz[ls]});
     }
     Best b{FLT_MAX, -1, 0u, 0u, 0u};
     LaneStack unused{nullptr, nullptr, 0, 0};  // leaf_test<false, SPH> pushes only for mesh-BVH links, handled here
