@@ -472,6 +472,9 @@ __device__ __forceinline__ HitPrim hit_prim(const SceneView &sv, const DRay &ray
 // Re-derives the full intersection record of a hit (the extend kernel keeps only distance and reference).  A
 // transformed object's record is derived in object space and mapped out as outgoing_intersection does
 // (object/transform.rs): the position by the object-to-world map, both normals by its linear part, normalised.
+// SDF: the scene may hold SDF hits whose normal was not given (without, that case is compiled out: for scenes
+// without SDFs, whose shading then keeps fewer registers live).
+template <bool SDF = true>
 __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint32_t ref, uint32_t aux,
                              const uint32_t *sub, const dm::Key &key, uint32_t pixel, uint32_t sample,
                              uint32_t bounce, SHit *h, bool have_sdf_n = false, f3 sdf_n = f3{0.0f, 0.0f, 0.0f}) {
@@ -532,7 +535,7 @@ __device__ void hit_geometry(const SceneView &sv, const DRay &ray, float t, uint
             break;
         }
         default: {  // ITEM_SDF: computed by k_sdf_normal when given
-            const f3 n = have_sdf_n ? sdf_n : sdf_normal(sv, sv.sdfs[p.prim], h->pos);
+            const f3 n = (have_sdf_n || !SDF) ? sdf_n : sdf_normal(sv, sv.sdfs[p.prim], h->pos);
             h->normal = n;
             h->ray_normal = n;
             h->front = !(aux & HIT_AUX_BACK);
@@ -844,19 +847,20 @@ __device__ __forceinline__ void grad_add(const AdjointParams &ap, uint32_t copy,
 #define RAYNA_SHADE_STRIP 16
 #endif
 constexpr uint32_t SHADE_STRIP = RAYNA_SHADE_STRIP;
-// Waves per SIMD the register budget of k_shade allows.  The forward shade without textures runs at eight: it
-// then spills about 112 bytes per lane, but its loads (queue entry, hit, triangle, material) are a chain of
-// dependent round trips, and more waves hide more of them (the 1080p mesh frame's shading 8.7 -> 8.0 ms; six
-// waves 8.3).  The adjoint, textured and validating forms keep five: their registers spill more.
+// Waves per SIMD the register budget of k_shade allows.  WIDE (the forward shade without textures, for scenes
+// without SDFs: the SDF normal code is compiled out) runs at eight: it then spills some registers, but its loads
+// (queue entry, hit, triangle, material) are a chain of dependent round trips, and more waves hide more of them
+// (the 1080p mesh frame's shading 8.7 -> 8.0 ms; six waves 8.3).  Everything else keeps five: the SDF frame's
+// shading went from 4.8 to 6.2 ms at eight, and the adjoint, textured and validating forms spill more.
 #ifndef RAYNA_SHADE_WAVES
 #define RAYNA_SHADE_WAVES 5
 #endif
-#ifndef RAYNA_SHADE_WAVES_PLAIN
-#define RAYNA_SHADE_WAVES_PLAIN 8
+#ifndef RAYNA_SHADE_WAVES_WIDE
+#define RAYNA_SHADE_WAVES_WIDE 8
 #endif
-template <bool ADJOINT, bool BATCH, bool TEX, bool VALIDATE = false>
+template <bool ADJOINT, bool BATCH, bool TEX, bool VALIDATE = false, bool WIDE = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
-    (ADJOINT || TEX || VALIDATE) ? RAYNA_SHADE_WAVES : RAYNA_SHADE_WAVES_PLAIN))) void k_shade(
+    (WIDE && !ADJOINT && !TEX && !VALIDATE) ? RAYNA_SHADE_WAVES_WIDE : RAYNA_SHADE_WAVES))) void k_shade(
     SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue_in, const uint32_t *count_in,
     int32_t *queue_out, uint32_t *count_out, uint32_t depth, AdjointParams ap, BatchView bv) {
     const uint32_t count = queue_in ? *count_in : fp.n_slots;
@@ -923,7 +927,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
                 f3 sdf_n{0.0f, 0.0f, 0.0f};
                 const bool sdf_hit = (ref >> HIT_KIND_SHIFT) == (uint32_t)ITEM_SDF && pb.sdf_nx;
                 if (sdf_hit) sdf_n = f3{pb.sdf_nx[slot], pb.sdf_ny[slot], pb.sdf_nz[slot]};
-                hit_geometry(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, &h, sdf_hit, sdf_n);
+                hit_geometry<!WIDE>(sv, ray, t, ref, aux, pb.hit_sub + slot, key, pixel, sample, depth, &h, sdf_hit,
+                                    sdf_n);
                 if (VALIDATE) {
                     float uv[2];
                     int side;
@@ -1293,10 +1298,13 @@ void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers 
                   const AdjointParams *adjoint, const BatchView &bv, hipStream_t s) {
     const bool tex = sv.textured != 0;
     const bool validate = fp.checks && !bv.reqs && !adjoint;
-    auto kernel = bv.reqs    ? (tex ? k_shade<false, true, true> : k_shade<false, true, false>)  // batches: forward
+    const bool wide = sv.n_sdfs == 0;
+    auto kernel = bv.reqs    ? (tex ? k_shade<false, true, true>
+                                    : (wide ? k_shade<false, true, false, false, true> : k_shade<false, true, false>))
                   : adjoint  ? (tex ? k_shade<true, false, true> : k_shade<true, false, false>)
                   : validate ? (tex ? k_shade<false, false, true, true> : k_shade<false, false, false, true>)
-                             : (tex ? k_shade<false, false, true> : k_shade<false, false, false>);
+                             : (tex ? k_shade<false, false, true>
+                                    : (wide ? k_shade<false, false, false, false, true> : k_shade<false, false, false>));
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, s, sv, fp, pb, queue_in, count_in, queue_out, count_out,
                        depth, adjoint ? *adjoint : AdjointParams{}, bv);
 }

@@ -37,6 +37,7 @@ struct FrameParams {
     uint32_t n_slots;        // slots in the chunk: n_pixels * n_spp
     int32_t mode;            // RAYNA_MODE_*
     uint32_t *checks = nullptr;  // validation on: RAYNA_CHECKS violation counters (RAYNA_CHECK_*), else null
+    uint32_t rays_stored = 0;    // k_generate stored this chunk's camera rays (else the first kernels compute them)
 };
 
 // One request of a batch, as seen by one chunk of paths: its slots are [slot_base, slot_base + n_slots), sample

@@ -372,8 +372,9 @@ void trace_chunk(rayna_renderer *r, FrameParams &fp, uint32_t s0, uint32_t n, st
     launch_fill_u32(r->claims, 0u, D2 * EXTEND_CLAIM_WORDS, r->stream);
     // the camera rays are stored only for the kernels that load them: k_extend and k_march at depth 0 and
     // k_shade_mode (k_extend_packet and the first k_shade compute them)
-    if (fp.mode != RAYNA_MODE_PBR || r->dev.view.n_sdfs > 0 || !extend_packets(r->dev.view, fp, r->tune, BatchView{}, 0))
-        launch_generate(fp, r->pb, BatchView{}, r->stream);
+    fp.rays_stored = fp.mode != RAYNA_MODE_PBR || r->dev.view.n_sdfs > 0 ||
+                     !extend_packets(r->dev.view, fp, r->tune, BatchView{}, 0);
+    if (fp.rays_stored) launch_generate(fp, r->pb, BatchView{}, r->stream);
     for (uint32_t depth = 0; depth <= r->opts.ray_depth; ++depth) {
         const int32_t *qin = depth == 0 ? nullptr : r->queue[depth & 1];
         const uint32_t *cin = depth == 0 ? nullptr : r->queue_count + depth;

@@ -513,7 +513,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
         pixel = ls / fp.n_spp;
         sample = fp.s0 + ls % fp.n_spp;
     }
-    if (!BATCH && !queue) {  // the camera ray, computed as k_generate computes it (which then need not run)
+    if (!BATCH && !queue && !fp.rays_stored) {  // the camera ray, computed as k_generate computes it
         const DRay c = camera_ray(fp.vp, fp.width, fp.height, fp.sdim, key, pixel, sample);
         ray = ray_from_dir(c.o, c.d);
     } else {
