@@ -562,6 +562,7 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     v.packet_need = merged.packet_need;
     v.instances = has_instances(merged) ? 1 : 0;
     v.sphere_tests = has_spheres(merged) ? 1 : 0;
+    v.media = merged.volumes.empty() ? 0 : 1;
     v.textured = has_textures(merged) ? 1 : 0;
     for (const SceneSlot &sl : b->slots)
         if (sl.skybox == RAYNA_SKY_HDRI) v.textured = 1;

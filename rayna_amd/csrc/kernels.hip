@@ -23,7 +23,7 @@ constexpr int BLOCK = 256;
 #define RAYNA_EXT_BLOCK 128
 #endif
 #ifndef RAYNA_EXT_LDS_STACK
-#define RAYNA_EXT_LDS_STACK 12
+#define RAYNA_EXT_LDS_STACK 11
 #endif
 constexpr int EXT_BLOCK = RAYNA_EXT_BLOCK;
 constexpr int EXT_LDS_STACK = RAYNA_EXT_LDS_STACK;  // traversal stack entries per lane held in LDS
@@ -1235,18 +1235,48 @@ inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block -
 
 }  // namespace
 
+using ExtendKernel = void (*)(SceneView, FrameParams, PathBuffers, const int32_t *, const uint32_t *, uint32_t,
+                              int2 *, uint32_t *, ExtendTuning, BatchView);
+// The k_extend instantiations, [batch][instances][form]: form 0 neither spheres nor media, 1 spheres, 2 media
+// (with the sphere tests, which sphere-bounded media need).
+const ExtendKernel EXTEND_KERNELS[2][2][3] = {
+    {{k_extend<false, false, false, false>, k_extend<false, false, true, false>, k_extend<false, false, true, true>},
+     {k_extend<false, true, false, false>, k_extend<false, true, true, false>, k_extend<false, true, true, true>}},
+    {{k_extend<true, false, false, false>, k_extend<true, false, true, false>, k_extend<true, false, true, true>},
+     {k_extend<true, true, false, false>, k_extend<true, true, true, false>, k_extend<true, true, true, true>}}};
+
+int extend_form(const SceneView &sv) { return sv.media ? 2 : (sv.sphere_tests ? 1 : 0); }
+
+// Resident 128-lane blocks per CU of each instantiation, taken once per process.
+struct ExtendOccupancy {
+    int n[2][2][3];
+};
+const ExtendOccupancy &extend_occupancy() {
+    static ExtendOccupancy o;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (int b = 0; b < 2; ++b)
+            for (int i = 0; i < 2; ++i)
+                for (int f = 0; f < 3; ++f) {
+                    int n = 0;
+                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, EXTEND_KERNELS[b][i][f], EXT_BLOCK, 0) !=
+                            hipSuccess ||
+                        n < 1)
+                        n = 4;
+                    o.n[b][i][f] = n;
+                }
+    });
+    return o;
+}
+
 int extend_lds_stack() { return EXT_LDS_STACK; }
 
 int extend_blocks_per_cu() {
-    int n = 0, nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend<false, false, false>, EXT_BLOCK, 0) != hipSuccess ||
-        n < 1)
-        n = 4;
-    for (auto k : {k_extend<true, false, false>, k_extend<false, true, false>, k_extend<true, true, false>,
-                   k_extend<false, false, true>, k_extend<true, false, true>, k_extend<false, true, true>,
-                   k_extend<true, true, true>})
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, EXT_BLOCK, 0) == hipSuccess && nb >= 1)
-            n = std::min(n, nb);
+    const ExtendOccupancy &o = extend_occupancy();
+    int n = 1;
+    for (int b = 0; b < 2; ++b)
+        for (int i = 0; i < 2; ++i)
+            for (int f = 0; f < 3; ++f) n = std::max(n, o.n[b][i][f]);
     return n;
 }
 int extend_block() { return EXT_BLOCK; }
@@ -1284,13 +1314,12 @@ void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers
                            count, depth, bv);
         return;
     }
-    auto kernel = sv.sphere_tests
-                      ? (bv.reqs ? (sv.instances ? k_extend<true, true, true> : k_extend<true, false, true>)
-                                 : (sv.instances ? k_extend<false, true, true> : k_extend<false, false, true>))
-                      : (bv.reqs ? (sv.instances ? k_extend<true, true, false> : k_extend<true, false, false>)
-                                 : (sv.instances ? k_extend<false, true, false> : k_extend<false, false, false>));
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth, overflow, work,
-                       tune, bv);
+    const int b = bv.reqs ? 1 : 0, i = sv.instances ? 1 : 0, f = extend_form(sv);
+    // `grid` is the CU count times the largest residency of any form, which the overflow columns are sized
+    // for; each form launches as many blocks as stay resident (never more than `grid`)
+    const int g = std::max(1, std::min(grid, std::max(1, grid / extend_blocks_per_cu()) * extend_occupancy().n[b][i][f]));
+    hipLaunchKernelGGL(EXTEND_KERNELS[b][i][f], dim3(g), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue, count, depth,
+                       overflow, work, tune, bv);
 }
 
 void launch_shade(const SceneView &sv, const FrameParams &fp, const PathBuffers &pb, const int32_t *queue_in,

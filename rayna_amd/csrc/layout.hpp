@@ -209,6 +209,7 @@ struct SceneView {
     int32_t packet_need;  // deepest wave stack k_extend_packet can need (depth-first, no speculation)
     int32_t instances;   // 1 if some object is a transformed mesh BVH (an instance)
     int32_t sphere_tests;  // 1 if traversal intersects spheres (sphere items, sphere-bounded media)
+    int32_t media;       // 1 if the scene holds constant-density media (volume items)
     int32_t textured;    // 1 if some material has a texture or the sky is an image
     int32_t sky_image;   // RAYNA_SKY_HDRI: the sky's image
 };

@@ -195,6 +195,7 @@ rayna_status load_scene(rayna_renderer *r, const rayna_scene_desc *d) {
     v.packet_need = ps.packet_need;
     v.instances = has_instances(ps) ? 1 : 0;
     v.sphere_tests = has_spheres(ps) ? 1 : 0;
+    v.media = ps.volumes.empty() ? 0 : 1;
     v.textured = has_textures(ps) ? 1 : 0;
     v.sky_image = ps.sky_image;
     // traversal stack overflow beyond the LDS entries, one column per extend thread

@@ -181,8 +181,9 @@ __device__ __forceinline__ void finish_instance(const SceneView &sv, const PathB
 // Tests entry `i` of leaf range `code` and keeps it if closer, or equally close with a higher rank.  Returns true
 // when the entry is an instance, which ends the pass over the range.  INST: the scene holds instances (without,
 // the instance code is compiled out).  SPH: the scene intersects spheres (without, the double-precision sphere
-// test is compiled out, which keeps the traversal loop's registers free; no sphere item then occurs).
-template <bool INST, bool SPH>
+// test is compiled out, which keeps the traversal loop's registers free; no sphere item then occurs).  VOL: the
+// scene holds media (without, the free-flight sampling is compiled out; no volume item then occurs).
+template <bool INST, bool SPH, bool VOL>
 __device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_t code, int i, float leaf_t,
                                           Best &b, LaneStack &st, const dm::Key &key, uint32_t pixel,
                                           uint32_t sample, uint32_t bounce) {
@@ -248,7 +249,7 @@ __device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_
             ok = box_t(sv.boxes[j], r, T_MIN, t_end, &t, &axis, &w, &sg);
             break;
         }
-        case ITEM_VOLUME: ok = volume_t<SPH>(sv, sv.volumes[j], r, t_end, key, pixel, sample, bounce, &t); break;
+        case ITEM_VOLUME: ok = VOL && volume_t<SPH>(sv, sv.volumes[j], r, t_end, key, pixel, sample, bounce, &t); break;
         default: break;
     }
     if (!ok) return false;
@@ -267,9 +268,12 @@ __device__ __forceinline__ bool leaf_test(const SceneView &sv, DRay &ray, int32_
 
 // Closest bounded hit of every live path (Renderer::calculate_intersection, renderer.rs:494-501); the
 // unbounded objects are marched afterwards by k_march.  BATCH: the slots belong to the requests of bv, each
-// with its own scene root and RNG key.  INST: the scene holds instances.  SPH: it intersects spheres.  Occupancy: a 128-lane block holds
-// 12 KiB of LDS stack, so 13 blocks fit a CU; the register budget of 80 VGPRs keeps six waves per SIMD (measured:
-// 8.5% faster than five waves with a 16-entry stack; seven waves spill).
+// with its own scene root and RNG key.  INST: the scene holds instances.  SPH: it intersects spheres.  VOL: it
+// holds media.  Occupancy: a 128-lane block holds 11 KiB of LDS stack, so 14 blocks fit a CU.  Without
+// instances, spheres or media the kernel runs at seven waves per SIMD (72 VGPRs, a few bytes of spills): on the
+// 1080p mesh frame 32.8 ms of k_extend with everything compiled in at six waves, 31.7 without the media code,
+// 31.1 at seven waves with 11 LDS entries (31.4 with 10).  The other forms keep six waves (80 VGPRs; at six
+// waves with a 16-entry stack against five: 8.5% faster).
 // The box loop of k_extend ends for the wave once this many lanes or fewer still search: waiting for the last
 // few searching lanes left most of the wave idle (on the 1080p mesh frame a quarter of the loop's lane slots
 // belonged to lanes whose traversal had ended and a fifth to lanes stalled on a second range).  Measured there:
@@ -282,11 +286,15 @@ constexpr uint32_t EXTEND_INNER_EXIT = 8;
 // back from speculation (below) left it at 32.6.
 constexpr uint32_t EXTEND_PRIM_EXIT = 16;
 
-template <bool BATCH, bool INST, bool SPH>
 #ifndef RAYNA_EXT_WAVES
 #define RAYNA_EXT_WAVES 6
 #endif
-__global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA_EXT_WAVES))) void k_extend(
+#ifndef RAYNA_EXT_WAVES_LEAN
+#define RAYNA_EXT_WAVES_LEAN 7
+#endif
+template <bool BATCH, bool INST, bool SPH, bool VOL>
+__global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(
+    (INST || SPH || VOL) ? RAYNA_EXT_WAVES : RAYNA_EXT_WAVES_LEAN))) void k_extend(
     SceneView sv, FrameParams fp, PathBuffers pb, const int32_t *queue, const uint32_t *count_ptr, uint32_t depth,
     int2 *overflow, uint32_t *work, ExtendTuning tune, BatchView bv) {
     __shared__ unsigned long long s_entry[EXT_LDS_STACK * EXT_BLOCK];
@@ -435,7 +443,7 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(RAYNA
             do {
                 if (leaf != 0) {
                     RAYNA_STAT(prims, 1);
-                    if (leaf_test<INST, SPH>(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth) ||
+                    if (leaf_test<INST, SPH, VOL>(sv, ray, leaf, leaf_i, leaf_t, b, st, key, pixel, sample, depth) ||
                         ++leaf_i == leaf_n)
                         leaf = 0;
                 }
@@ -606,7 +614,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
                     push(sv.items[start + i].index, kt, tm);
                     continue;
                 }
-                if (take) (void)leaf_test<false, SPH>(sv, ray, code, i, kt, b, unused, key, pixel, sample, depth);
+                if (take) (void)leaf_test<false, SPH, true>(sv, ray, code, i, kt, b, unused, key, pixel, sample, depth);
             }
         }
     }
