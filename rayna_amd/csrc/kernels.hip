@@ -1308,8 +1308,13 @@ void launch_extend(const SceneView &sv, const FrameParams &fp, const PathBuffers
                    const uint32_t *count, uint32_t depth, int grid, int2 *overflow, uint32_t *work,
                    const ExtendTuning &tune, const BatchView &bv, hipStream_t s) {
     if (extend_packets(sv, fp, tune, bv, depth)) {
-        auto packet = bv.reqs ? (sv.sphere_tests ? k_extend_packet<true, true> : k_extend_packet<true, false>)
-                              : (sv.sphere_tests ? k_extend_packet<false, true> : k_extend_packet<false, false>);
+        const int f = extend_form(sv);
+        auto packet = bv.reqs ? (f == 2   ? k_extend_packet<true, true, true>
+                                 : f == 1 ? k_extend_packet<true, true, false>
+                                          : k_extend_packet<true, false, false>)
+                              : (f == 2   ? k_extend_packet<false, true, true>
+                                 : f == 1 ? k_extend_packet<false, true, false>
+                                          : k_extend_packet<false, false, false>);
         hipLaunchKernelGGL(packet, dim3(grid_for(fp.n_slots, EXT_BLOCK)), dim3(EXT_BLOCK), 0, s, sv, fp, pb, queue,
                            count, depth, bv);
         return;

@@ -492,7 +492,8 @@ __device__ __forceinline__ float wave_min(float v) {
     return fminf(fminf(r0, r1), fminf(r2, r3));
 }
 
-template <bool BATCH, bool SPH>
+// SPH and VOL as for k_extend: the sphere test and the media code are compiled in only for scenes that need them.
+template <bool BATCH, bool SPH, bool VOL>
 __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, FrameParams fp, PathBuffers pb,
                                                              const int32_t *queue, const uint32_t *count_ptr,
                                                              uint32_t depth, BatchView bv) {
@@ -528,7 +529,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
         ray = ray_from_dir(f3{pb.ox[ls], pb.oy[ls], pb.oz[ls]}, f3{pb.dx[ls], pb.dy[ls], pb.dz[ls]});
     }
     Best b{FLT_MAX, -1, 0u, 0u, 0u};
-    LaneStack unused{nullptr, nullptr, 0, 0};  // leaf_test<false, SPH> pushes only for mesh-BVH links, handled here
+    LaneStack unused{nullptr, nullptr, 0, 0};  // leaf_test<false, ...> pushes only for mesh-BVH links, handled here
     int sp = 0;
     auto push = [&](int32_t code, float k, unsigned long long m) {
         s_code[w][sp] = code;
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
                     push(sv.items[start + i].index, kt, tm);
                     continue;
                 }
-                if (take) (void)leaf_test<false, SPH, true>(sv, ray, code, i, kt, b, unused, key, pixel, sample, depth);
+                if (take) (void)leaf_test<false, SPH, VOL>(sv, ray, code, i, kt, b, unused, key, pixel, sample, depth);
             }
         }
     }
