@@ -1,6 +1,6 @@
 """Benchmark of the render hot path on one MI355X: frames of the 1080p triangle-mesh workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload triangle_mesh|random_spheres|sdf_volume]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload triangle_mesh|random_spheres|sdf_volume|rttnw|cornell]
 
 A step is one `Renderer::render()` frame: every sample of every pixel traced through the wavefront kernels, with
 the scene, BVH and path buffers already resident in HBM (the image stays on the device).  Prints one JSON line:
@@ -99,6 +99,10 @@ def workload(name: str):
         return presets.random_spheres()
     if name == "sdf_volume":
         return presets.sdf_volume()
+    if name == "rttnw":  # the engine's RTTNW_DEMO: scattering sphere medium, instances, noise texture
+        return presets.rttnw()
+    if name == "cornell":  # the engine's CORNELL: rotated boxes (transforms), one light
+        return presets.cornell()
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -504,7 +508,7 @@ def main(argv=None):
     r.close()
     if not a.no_workloads:
         line["workloads"] = {}
-        for w in ("random_spheres", "sdf_volume", "triangle_mesh"):
+        for w in ("random_spheres", "sdf_volume", "triangle_mesh", "rttnw", "cornell"):
             if w != a.workload:
                 line["workloads"][w] = frame_leg(w)
                 _progress(f"workload {w} done")
