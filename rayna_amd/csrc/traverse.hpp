@@ -480,16 +480,19 @@ constexpr int PACKET_STACK = 64;
 #endif
 constexpr uint32_t PACKET_MIN_SPP = RAYNA_PACKET_MIN_SPP;  // fewer samples per pixel spread a wave over pixels
 
-// Least value over the 64 lanes (every lane active): within rows of 16 by DPP, then over the four rows.
-__device__ __forceinline__ float wave_min(float v) {
-    int x = __float_as_int(v);
-    x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0xB1, 0xf, 0xf, false))));
-    x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x4E, 0xf, 0xf, false))));
-    x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x141, 0xf, 0xf, false))));
-    x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x140, 0xf, 0xf, false))));
-    const float r0 = __int_as_float(__builtin_amdgcn_readlane(x, 0)), r1 = __int_as_float(__builtin_amdgcn_readlane(x, 16));
-    const float r2 = __int_as_float(__builtin_amdgcn_readlane(x, 32)), r3 = __int_as_float(__builtin_amdgcn_readlane(x, 48));
-    return fminf(fminf(r0, r1), fminf(r2, r3));
+// Least entry distance over the 64 lanes (every lane active), as a key: negative distances (the origin inside
+// the box) count as 0, which orders and culls like them (every closest hit lies beyond T_MIN > 0), so the keys
+// are non-negative floats, ordered as their bit patterns: an unsigned minimum within rows of 16 by DPP, then over
+// the four rows.  (A lane outside the exec mask would read as 0 through bound_ctrl: still a lower bound.)
+__device__ __forceinline__ float wave_min_key(float v) {
+    uint32_t x = (uint32_t)max(__float_as_int(v), 0);
+    x = min(x, (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xf, 0xf, true));
+    x = min(x, (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xf, 0xf, true));
+    x = min(x, (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xf, 0xf, true));
+    x = min(x, (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x140, 0xf, 0xf, true));
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)x, 0), r1 = (uint32_t)__builtin_amdgcn_readlane((int)x, 16);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)x, 32), r3 = (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
+    return __uint_as_float(min(min(r0, r1), min(r2, r3)));
 }
 
 // SPH and VOL as for k_extend: the sphere test and the media code are compiled in only for scenes that need them.
@@ -529,6 +532,19 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
         ray = ray_from_dir(f3{pb.ox[ls], pb.oy[ls], pb.oz[ls]}, f3{pb.dx[ls], pb.dy[ls], pb.dz[ls]});
     }
     Best b{FLT_MAX, -1, 0u, 0u, 0u};
+    // The slab distances as fused multiply-adds, fma(b, 1/d, -o/d): the rounding of -o/d adds an error of at most
+    // 2^-24 |o/d| per axis, not relative to t, so the entry distance is lowered and the exit distance raised by
+    // 2^-20 max |o/d|, which exceeds it and the roundings of that adjustment, keeping both within the bounds of
+    // the two-rounding form (b - o) * (1/d), conservative under the same BOX_SLACK.  A wave with a lane whose ray
+    // has a zero direction component (1/d infinite), or |o/d| beyond 2^126, keeps the two-rounding form: there a
+    // slab distance is +-inf, or NaN where the ray lies in a box's plane, and the engine's test culls by exactly
+    // those values.
+    const f3 noi{-(ray.o.x * ray.inv.x), -(ray.o.y * ray.inv.y), -(ray.o.z * ray.inv.z)};
+    const float oi_max = fmaxf(fmaxf(fabsf(noi.x), fabsf(noi.y)), fabsf(noi.z));
+    const float slack = oi_max * 0x1p-20f;
+    // per axis, since the maximum passes over a NaN (0 * inf where the origin lies on the plane o = 0)
+    const bool lane_fused = fabsf(noi.x) <= 0x1p126f && fabsf(noi.y) <= 0x1p126f && fabsf(noi.z) <= 0x1p126f;
+    const bool fused = !__any(valid && !lane_fused);
     LaneStack unused{nullptr, nullptr, 0, 0};  // leaf_test<false, ...> pushes only for mesh-BVH links, handled here
     int sp = 0;
     auto push = [&](int32_t code, float k, unsigned long long m) {
@@ -569,18 +585,27 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
                 unsigned long long cmask[4];
     #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    const float tx1 = (lo_x[k] - ray.o.x) * ray.inv.x, tx2 = (hi_x[k] - ray.o.x) * ray.inv.x;
-                    float t0 = fminf(tx1, tx2), t1 = fmaxf(tx1, tx2);
-                    const float ty1 = (lo_y[k] - ray.o.y) * ray.inv.y, ty2 = (hi_y[k] - ray.o.y) * ray.inv.y;
-                    t0 = fmaxf(t0, fminf(ty1, ty2));
-                    t1 = fminf(t1, fmaxf(ty1, ty2));
-                    const float tz1 = (lo_z[k] - ray.o.z) * ray.inv.z, tz2 = (hi_z[k] - ray.o.z) * ray.inv.z;
-                    t0 = fmaxf(t0, fminf(tz1, tz2));
-                    t1 = fminf(t1, fmaxf(tz1, tz2));
+                    float t0, t1;
+                    if (fused) {
+                        const float tx1 = fmaf(lo_x[k], ray.inv.x, noi.x), tx2 = fmaf(hi_x[k], ray.inv.x, noi.x);
+                        const float ty1 = fmaf(lo_y[k], ray.inv.y, noi.y), ty2 = fmaf(hi_y[k], ray.inv.y, noi.y);
+                        const float tz1 = fmaf(lo_z[k], ray.inv.z, noi.z), tz2 = fmaf(hi_z[k], ray.inv.z, noi.z);
+                        t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2)) - slack;
+                        t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) + slack;
+                    } else {
+                        const float tx1 = (lo_x[k] - ray.o.x) * ray.inv.x, tx2 = (hi_x[k] - ray.o.x) * ray.inv.x;
+                        t0 = fminf(tx1, tx2), t1 = fmaxf(tx1, tx2);
+                        const float ty1 = (lo_y[k] - ray.o.y) * ray.inv.y, ty2 = (hi_y[k] - ray.o.y) * ray.inv.y;
+                        t0 = fmaxf(t0, fminf(ty1, ty2));
+                        t1 = fminf(t1, fmaxf(ty1, ty2));
+                        const float tz1 = (lo_z[k] - ray.o.z) * ray.inv.z, tz2 = (hi_z[k] - ray.o.z) * ray.inv.z;
+                        t0 = fmaxf(t0, fminf(tz1, tz2));
+                        t1 = fminf(t1, fmaxf(tz1, tz2));
+                    }
                     const float lo = fmaxf(t0, T_MIN), hi = fminf(t1, b.t);
                     const bool hit = take && ccode[k] != ENTRY_NONE && lo <= hi * BOX_SLACK;
                     cmask[k] = __ballot(hit);
-                    ckey[k] = cmask[k] != 0ull ? wave_min(hit ? t0 : __builtin_inff()) : -__builtin_inff();
+                    ckey[k] = cmask[k] != 0ull ? wave_min_key(hit ? t0 : __builtin_inff()) : -__builtin_inff();
                 }
     #define RAYNA_PSWAP(a, b)                          \
         if (ckey[a] < ckey[b]) {                       \
