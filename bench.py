@@ -33,16 +33,16 @@ FRAME_BYTES_PER_SAMPLE = 24 + 12
 # Counter-derived figures of the extend stage on the 1080p triangle-mesh frame, from rocprofv3 passes over two
 # frames of 263,040,107 closest-hit queries each (132,710,400 of them camera rays) at the commit that added them;
 # the times they are divided by are measured live.  Traffic (both extend kernels): what leaves L2 for the
-# fabric, FETCH_SIZE 4.1534e7 KiB read as 64-byte units (x2.0: a 256 MiB copy reads 131,087 KiB) plus WRITE_SIZE
-# 1.4768e7 KiB (x1.0: the copy writes 262,143 KiB); it counts hits in the Infinity Cache as well as HBM, so it
-# bounds the HBM bytes from above.  VALU (k_extend, depths 1-8): SQ_INSTS_VALU 3.3128e10 wave64 instructions over
+# fabric, FETCH_SIZE 4.1536e7 KiB read as 64-byte units (x2.0: a 256 MiB copy reads 131,087 KiB) plus WRITE_SIZE
+# 1.4767e7 KiB (x1.0: the copy writes 262,143 KiB); it counts hits in the Infinity Cache as well as HBM, so it
+# bounds the HBM bytes from above.  VALU (k_extend, depths 1-8): SQ_INSTS_VALU 3.3124e10 wave64 instructions over
 # 2 x 130,329,707 queries; lanes active SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU) = 7.9352e11 / (64 x
-# 3.3372e10); 0.875 instructions per CU per cycle (GRBM_GUI_ACTIVE 1.1835e9 over 8 XCDs, 256 CUs).
+# 3.3368e10); 0.878 instructions per CU per cycle (GRBM_GUI_ACTIVE 1.1794e9 over 8 XCDs, 256 CUs).
 EXTEND_COUNTER_FRAMES = 2 * 263040107
-EXTEND_FABRIC_BYTES_PER_SEGMENT = {"triangle_mesh": (4.1534e7 * 2.0 + 1.4768e7) * 1024 / EXTEND_COUNTER_FRAMES}
-K_EXTEND_VALU_PER_QUERY = {"triangle_mesh": 3.3128e10 / (2 * 130329707)}
-K_EXTEND_LANES_ACTIVE = {"triangle_mesh": 7.9352e11 / (64 * 3.3372e10)}
-K_EXTEND_VALU_PER_CU_CYCLE = {"triangle_mesh": 3.3128e10 / (1.1835e9 / 8 * 256)}
+EXTEND_FABRIC_BYTES_PER_SEGMENT = {"triangle_mesh": (4.1536e7 * 2.0 + 1.4767e7) * 1024 / EXTEND_COUNTER_FRAMES}
+K_EXTEND_VALU_PER_QUERY = {"triangle_mesh": 3.3124e10 / (2 * 130329707)}
+K_EXTEND_LANES_ACTIVE = {"triangle_mesh": 7.9352e11 / (64 * 3.3368e10)}
+K_EXTEND_VALU_PER_CU_CYCLE = {"triangle_mesh": 3.3124e10 / (1.1794e9 / 8 * 256)}
 # VALU issue peak: one wave64 instruction per CU per cycle -- the rate the calibration kernel
 # (rayna_measure_valu_peak: independent fp32 multiply-add chains, no packed or fused forms) reaches per active
 # cycle (0.977 by the same counters) -- at MI355X's published 256 CUs and 2.4 GHz.
