@@ -28,6 +28,15 @@ constexpr uint64_t DEFAULT_CHUNK_SLOTS = 1ull << 29;
 constexpr int DEFAULT_PIPELINES = 3;
 constexpr int MAX_PIPELINES = 4;
 
+// Per-scene features that select kernel forms.
+enum : uint32_t {
+    SCENE_INSTANCES = 1u,
+    SCENE_SPHERES = 2u,
+    SCENE_MEDIA = 4u,
+    SCENE_SDFS = 8u,
+    SCENE_TEXTURED = 16u,
+};
+
 // One request's share of a chunk: samples [s0, s0 + n) of request `req`.
 struct Piece {
     uint32_t req, s0, n;
@@ -44,6 +53,7 @@ struct Plan {
     std::vector<uint32_t> resolve_blocks;      // per chunk: the most resolve blocks of one of its requests
     std::vector<uint32_t> depth;               // per chunk: deepest ray_depth of its requests
     std::vector<uint32_t> n_sdf;               // per chunk: most unbounded objects of its requests' scenes
+    std::vector<uint32_t> features;            // per chunk: SCENE_* bits of any of its requests' scenes
     std::vector<uint32_t> slots;               // per chunk
     std::vector<int32_t> dep;                  // per chunk: the last earlier chunk holding a piece of one of its
                                                // requests (its resolve must come first), or -1
@@ -82,6 +92,7 @@ struct rayna_batch {
     std::vector<void *> scene_allocs;
     SceneView view{};
     std::vector<SceneSlot> slots;
+    std::vector<uint32_t> scene_features;  // per scene: SCENE_* bits
     size_t overflow_entries = 0;  // per pipeline: extend grid * block * stack entries beyond the LDS part
     int cus = 1, grid_extend = 1024, grid_shade = 1024, grid_march = 1024;
     uint32_t n_sdf_max = 0;  // most unbounded objects of one scene
@@ -351,7 +362,7 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
     for (size_t c = 0; c < pl.chunks.size(); ++c) {
         pl.first_entry.push_back((uint32_t)pl.table.size());
         pl.first_final.push_back((uint32_t)pl.finals.size());
-        uint32_t slot = 0, maxp = 0, depth = 0, nsdf = 0, rblocks = 1;
+        uint32_t slot = 0, maxp = 0, depth = 0, nsdf = 0, rblocks = 1, feat = 0;
         for (size_t e = 0; e < pl.chunks[c].size(); ++e) {
             const Piece &pc = pl.chunks[c][e];
             const rayna_request &q = reqs[pc.req];
@@ -387,15 +398,31 @@ void make_plan(rayna_batch *b, const rayna_request *reqs, int n, const std::vect
             rblocks = std::max(rblocks, (P + per - 1) / per);
             depth = std::max(depth, q.opts.ray_depth);
             nsdf = std::max(nsdf, (uint32_t)sc.n_sdfs);
+            feat |= b->scene_features[q.scene];
         }
         pl.max_pixels.push_back(maxp);
         pl.resolve_blocks.push_back(rblocks);
         pl.depth.push_back(depth);
         pl.n_sdf.push_back(nsdf);
+        pl.features.push_back(feat);
         pl.slots.push_back(slot);
         pl.max_depth = std::max(pl.max_depth, depth);
     }
     pl.first_final.push_back((uint32_t)pl.finals.size());
+}
+
+// The merged scene set's view as one chunk sees it: the kernels compile out the sphere test, media, instance
+// and texture code and, without SDFs, the SDF normal code (k_shade's eight-wave form), so a chunk whose requests'
+// scenes hold none of these runs the forms a single render of those scenes would.  A request's rays only reach
+// its own scene's tree, so the other scenes' items never meet the compiled-out code.
+SceneView chunk_view(const SceneView &merged, uint32_t features) {
+    SceneView v = merged;
+    v.instances = (features & SCENE_INSTANCES) ? merged.instances : 0;
+    v.sphere_tests = (features & SCENE_SPHERES) ? merged.sphere_tests : 0;
+    v.media = (features & SCENE_MEDIA) ? merged.media : 0;
+    v.textured = (features & SCENE_TEXTURED) ? merged.textured : 0;
+    v.n_sdfs = (features & SCENE_SDFS) ? merged.n_sdfs : 0;
+    return v;
 }
 
 // Launches of a plan traced on `used` pipelines.
@@ -434,6 +461,7 @@ hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
         fp.n_slots = pl.slots[c];
         fp.ray_depth = pl.depth[c];
         const uint32_t D = pl.depth[c];
+        const SceneView cv = chunk_view(b->view, pl.features[c]);
         launch_fill_u32(P.queue_count, 0u, 3 * (D + 2), s);
         launch_fill_u32(P.claims, 0u, (D + 1) * EXTEND_CLAIM_WORDS, s);
         launch_generate(fp, P.pb, bv, s);
@@ -443,11 +471,11 @@ hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
             int32_t *qout = P.queue[(depth + 1) & 1];
             uint32_t *cout = P.queue_count + depth + 1;
             uint32_t *work = P.claims + (size_t)depth * EXTEND_CLAIM_WORDS;
-            launch_extend(b->view, fp, P.pb, qin, cin, depth, b->grid_extend, P.overflow, work, b->tune, bv, s);
-            launch_march(b->view, fp, P.pb, qin, cin, pl.n_sdf[c], P.mt, P.capacity * b->n_sdf_max,
+            launch_extend(cv, fp, P.pb, qin, cin, depth, b->grid_extend, P.overflow, work, b->tune, bv, s);
+            launch_march(cv, fp, P.pb, qin, cin, pl.n_sdf[c], P.mt, P.capacity * b->n_sdf_max,
                          P.queue_count + 2 * (D + 2) + depth, b->grid_march, b->grid_shade, b->tune, bv,
                          P.sdf_queue, P.sdf_count, s);
-            launch_shade(b->view, fp, P.pb, qin, cin, qout, cout, depth, b->grid_shade, nullptr, bv, s);
+            launch_shade(cv, fp, P.pb, qin, cin, qout, cout, depth, b->grid_shade, nullptr, bv, s);
         }
         launch_count_segments(P.queue_count, D + 1, fp.n_slots, P.segments, s);
         // (a dependency on the same pipeline is kept by its stream's order)
@@ -515,6 +543,7 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
     b->grid_march = b->cus * march_blocks_per_cu();
     b->grid_shade = b->cus * SHADE_BLOCKS_PER_CU;
     std::vector<PackedScene> packed(n_scenes);
+    b->scene_features.assign(n_scenes, 0u);
     for (int i = 0; i < n_scenes; ++i) {
         std::string err;
         const rayna_status st = pack_scene(scenes[i], packed[i], err);
@@ -522,6 +551,16 @@ rayna_status rayna_batch_create(const rayna_scene_desc *scenes, int32_t n_scenes
             b->error = "scene " + std::to_string(i) + ": " + err;
             return bail(st);
         }
+    }
+    for (int i = 0; i < n_scenes; ++i) {
+        const PackedScene &p = packed[i];
+        uint32_t f = 0;
+        if (has_instances(p)) f |= SCENE_INSTANCES;
+        if (has_spheres(p)) f |= SCENE_SPHERES;
+        if (!p.volumes.empty()) f |= SCENE_MEDIA;
+        if (!p.sdfs.empty()) f |= SCENE_SDFS;
+        if (has_textures(p) || p.skybox == RAYNA_SKY_HDRI) f |= SCENE_TEXTURED;
+        b->scene_features[i] = f;
     }
     PackedScene merged;
     if (!merge_scenes(packed, merged, b->slots)) {
