@@ -44,6 +44,23 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 __device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+
+// sqrtf(x), correctly rounded, without the pass-through of zeros and infinities that sqrtf's general sequence ends
+// with: inputs below 2^-96 are scaled by 2^32 (the root by 2^-16 back), the hardware square root is corrected by
+// the signs of the residuals x - y'y at the root's neighbours y' = y -+ 1 ulp, and for zeros and infinities that
+// correction already leaves the root as it is.  Equal to sqrtf bit for bit on every input (checked over all 2^32
+// bit patterns: rayna_debug_sqrt_mismatches).  For the SDF sphere distances, which the march evaluates sixteen at
+// a time.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const bool tiny = x < 0x1p-96f;
+    const float xs = tiny ? x * 0x1p32f : x;
+    const float y = __builtin_amdgcn_sqrtf(xs);
+    const float ym = __uint_as_float(__float_as_uint(y) - 1u), yp = __uint_as_float(__float_as_uint(y) + 1u);
+    float r = fmaf(-ym, y, xs) <= 0.0f ? ym : y;
+    r = fmaf(-yp, y, xs) > 0.0f ? yp : r;
+    return tiny ? r * 0x1p-16f : r;
+}
+__device__ __forceinline__ float length_rn(f3 a) { return sqrt_rn(dot(a, a)); }
 __device__ __forceinline__ f3 normalize(f3 a) { return a * (1.0f / length(a)); }
 __device__ __forceinline__ bool try_normalize(f3 a, f3 *out) {  // glam try_normalize
     const float rcp = 1.0f / length(a);
@@ -362,14 +379,14 @@ __device__ __forceinline__ float bulb_estimate(float r, float dr) {
 __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
     if (f.kind == RAYNA_SDF_SPHERE) {
         const GSphere s = sv.spheres[f.first_sphere];
-        return length(p - f3{s.cx, s.cy, s.cz}) - s.radius;
+        return length_rn(p - f3{s.cx, s.cy, s.cz}) - s.radius;
     }
     if (f.kind == RAYNA_SDF_SMIN_SPHERES) {
         const GSphere s0 = sv.spheres[f.first_sphere];
-        float d = length(p - f3{s0.cx, s0.cy, s0.cz}) - s0.radius;
+        float d = length_rn(p - f3{s0.cx, s0.cy, s0.cz}) - s0.radius;
         for (int i = 1; i < f.n_spheres; ++i) {
             const GSphere s = sv.spheres[f.first_sphere + i];
-            const float di = length(p - f3{s.cx, s.cy, s.cz}) - s.radius;
+            const float di = length_rn(p - f3{s.cx, s.cy, s.cz}) - s.radius;
             const float h = fmaxf(f.k - fabsf(d - di), 0.0f) * (1.0f / f.k);
             d = fminf(d, di) - h * h * f.k * 0.25f;
         }
@@ -1222,6 +1239,17 @@ __global__ __launch_bounds__(BLOCK) void k_valu_peak(float *out, uint32_t iters,
     out[gid] = sum;
 }
 
+// Counts the bit patterns [first, first + count) on which sqrt_rn and sqrtf differ (NaNs compare equal).
+__global__ __launch_bounds__(BLOCK) void k_sqrt_check(uint32_t first, uint32_t count, unsigned long long *mismatches) {
+    uint32_t bad = 0;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += gridDim.x * BLOCK) {
+        const float x = __uint_as_float(first + i);
+        const float a = sqrt_rn(x), b = sqrtf(x);
+        if (__float_as_uint(a) != __float_as_uint(b) && !(a != a && b != b)) ++bad;
+    }
+    if (bad) atomicAdd(mismatches, (unsigned long long)bad);
+}
+
 __global__ void k_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
                                  unsigned long long *segments) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -1401,6 +1429,10 @@ void launch_finalize(const FrameParams &fp, const float *sum, float *acc, double
 
 void launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, BLOCK)), dim3(BLOCK), 0, s, p, value, n);
+}
+
+void launch_sqrt_check(uint32_t first, uint32_t count, unsigned long long *mismatches, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_sqrt_check, dim3(blocks), dim3(BLOCK), 0, s, first, count, mismatches);
 }
 
 uint64_t launch_valu_peak(float *out, uint32_t blocks, uint32_t iters, hipStream_t s) {

@@ -145,6 +145,7 @@ bool extend_packets(const SceneView &sv, const FrameParams &fp, const ExtendTuni
 // The VALU issue-rate calibration kernel over `blocks` blocks of 256 lanes (out: one float per lane); returns
 // the wave64 VALU instructions its chains issue (loop and set-up instructions not counted).
 uint64_t launch_valu_peak(float *out, uint32_t blocks, uint32_t iters, hipStream_t s);
+void launch_sqrt_check(uint32_t first, uint32_t count, unsigned long long *mismatches, uint32_t blocks, hipStream_t s);
 void launch_count_segments(const uint32_t *queue_count, uint32_t n, uint32_t first_count,
                            unsigned long long *segments, hipStream_t s);
 

@@ -997,6 +997,34 @@ void rayna_destroy(rayna_renderer *r) {
     delete r;
 }
 
+rayna_status rayna_debug_sqrt_mismatches(uint32_t first, uint64_t count, uint64_t *mismatches) {
+    g_create_error.clear();
+    if (!mismatches || count > (1ull << 32) - first)
+        return fail(nullptr, RAYNA_INVALID_ARGUMENT, "null output or range beyond 2^32");
+    *mismatches = 0;
+    int device = 0;
+    HIP_TRY(nullptr, hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    HIP_TRY(nullptr, hipGetDeviceProperties(&prop, device));
+    const uint32_t blocks = 16u * (uint32_t)std::max(1, prop.multiProcessorCount);
+    unsigned long long *dev = nullptr;
+    HIP_TRY(nullptr, hipMalloc((void **)&dev, sizeof(unsigned long long)));
+    hipError_t e = hipMemset(dev, 0, sizeof(unsigned long long));
+    // pieces of at most 2^31 patterns (the kernel's count is 32-bit)
+    for (uint64_t done = 0; e == hipSuccess && done < count;) {
+        const uint32_t n = (uint32_t)std::min<uint64_t>(count - done, 1ull << 31);
+        launch_sqrt_check((uint32_t)(first + done), n, dev, blocks, nullptr);
+        e = hipGetLastError();
+        done += n;
+    }
+    unsigned long long host = 0;
+    if (e == hipSuccess) e = hipMemcpy(&host, dev, sizeof(host), hipMemcpyDeviceToHost);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return fail(nullptr, RAYNA_DEVICE_ERROR, std::string("sqrt check: ") + hipGetErrorString(e));
+    *mismatches = host;
+    return RAYNA_OK;
+}
+
 rayna_status rayna_measure_valu_peak(uint32_t iters, double *wave_insts_per_s, double *seconds) {
     g_create_error.clear();
     if (!wave_insts_per_s || iters == 0 || iters > (1u << 20))

@@ -31,7 +31,7 @@ EXPORTS = ("rayna_create", "rayna_set_scene", "rayna_set_camera", "rayna_set_opt
            "rayna_set_validate", "rayna_validate_counts", "rayna_adjoint_static", "rayna_batch_create",
            "rayna_batch_set_tuning", "rayna_render_batch", "rayna_batch_destroy", "rayna_batch_last_error",
            "rayna_primary_hits", "rayna_dfs_order", "rayna_tree_hash", "rayna_scene_prepare", "rayna_scene_tree", "rayna_destroy",
-           "rayna_measure_valu_peak", "rayna_debug_fail_launches", "rayna_last_error")
+           "rayna_measure_valu_peak", "rayna_debug_sqrt_mismatches", "rayna_debug_fail_launches", "rayna_last_error")
 
 
 # RAYNA_CHECK_* of include/rayna_gpu.h, in order: the invariants of the engine's shared/validate.rs
@@ -111,6 +111,8 @@ def lib() -> C.CDLL:
         L.rayna_destroy.argtypes = [vp]
         L.rayna_measure_valu_peak.restype = C.c_int
         L.rayna_measure_valu_peak.argtypes = [C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.rayna_debug_sqrt_mismatches.restype = C.c_int
+        L.rayna_debug_sqrt_mismatches.argtypes = [C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]
         L.rayna_debug_fail_launches.restype = C.c_int32
         L.rayna_debug_fail_launches.argtypes = [C.c_int32]
         L.rayna_last_error.restype = C.c_char_p
@@ -125,6 +127,14 @@ def measure_valu_peak(iters: int = 2048):
     rate, secs = C.c_double(0.0), C.c_double(0.0)
     _check(lib().rayna_measure_valu_peak(int(iters), C.byref(rate), C.byref(secs)))
     return rate.value, secs.value
+
+
+def debug_sqrt_mismatches(first: int = 0, count: int = 1 << 32) -> int:
+    """The number of float bit patterns in [first, first + count) on which the SDF distances' square root differs
+    from the correctly rounded sqrtf (NaNs compare equal), counted on the current GPU."""
+    out = C.c_uint64(0)
+    _check(lib().rayna_debug_sqrt_mismatches(int(first), int(count), C.byref(out)))
+    return int(out.value)
 
 
 def debug_fail_launches(countdown: int) -> int:
