@@ -464,6 +464,9 @@ hipError_t enqueue(rayna_batch *b, const Plan &pl, int used, bool timing) {
         const SceneView cv = chunk_view(b->view, pl.features[c]);
         launch_fill_u32(P.queue_count, 0u, 3 * (D + 2), s);
         launch_fill_u32(P.claims, 0u, (D + 1) * EXTEND_CLAIM_WORDS, s);
+        // the camera rays are stored only where a kernel loads them: k_extend and k_march at depth 0 (the camera-ray
+        // packets and the first k_shade compute them); k_generate records every slot's request either way
+        fp.rays_stored = (pl.features[c] & SCENE_SDFS) != 0u || !extend_packets(cv, fp, b->tune, bv, 0);
         launch_generate(fp, P.pb, bv, s);
         for (uint32_t depth = 0; depth <= D; ++depth) {
             const int32_t *qin = depth == 0 ? nullptr : P.queue[depth & 1];

@@ -798,7 +798,7 @@ __device__ __forceinline__ DRay camera_ray(const GViewport &vp, uint32_t width, 
 // Camera ray of every slot of the chunk.  Throughput and radiance get no initial values: the first k_shade takes
 // them as 1 and 0 (the adjoint replay reads the primal radiance the field still holds) and writes the radiance
 // of every slot, as k_shade_mode does.  BATCH: each slot finds its request (the last with slot_base <= slot) and
-// records it for the later kernels.
+// records it for the later kernels; the ray itself only where the chunk stores its camera rays.
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers pb, BatchView bv) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -812,6 +812,7 @@ __global__ __launch_bounds__(BLOCK) void k_generate(FrameParams fp, PathBuffers 
             else hi = mid - 1;
         }
         bv.slot_req[i] = lo;
+        if (!fp.rays_stored) return;
         const RequestParams &R = bv.reqs[lo];
         fp.vp = R.vp;
         fp.width = R.width;
@@ -912,17 +913,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
                 ray_depth = R.ray_depth;
                 skybox = R.skybox;
                 sky_img = R.sky_image;
+                if (!queue_in && !fp.rays_stored) ray = camera_ray(R.vp, R.width, R.height, R.sdim, key, pixel, sample);
             } else {
                 pixel = slot / fp.n_spp;
                 sample = fp.s0 + slot % fp.n_spp;
                 if (!queue_in) ray = camera_ray(fp.vp, fp.width, fp.height, fp.sdim, key, pixel, sample);
             }
             // The first bounce (identity queue) starts from the camera ray with throughput 1 and radiance 0 (the
-            // adjoint's radiance field holds the primal radiance); k_generate stores only the ray.
-            // (A batch chunk's first bounce loads the ray k_generate stored: recomputing it would read the
-            // request's viewport per lane.)
+            // adjoint's radiance field holds the primal radiance); k_generate stores only the ray.  A batch chunk
+            // whose camera rays k_generate stored (chunks with SDFs, whose march loads them) loads it; others
+            // recompute it from the request's viewport.
             Col T{1.0f, 1.0f, 1.0f}, L{0.0f, 0.0f, 0.0f};
-            if (queue_in || BATCH)
+            if (queue_in || (BATCH && fp.rays_stored))
                 ray = ray_lazy(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]}, f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
             else
                 ray = ray_lazy(ray.o, ray.d);

@@ -514,6 +514,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
     dm::Key key{fp.key0, fp.key1};
     int32_t root = sv.root;
     DRay ray;
+    const bool computed = !queue && !fp.rays_stored;  // the camera ray, computed as k_generate computes it
     if (BATCH) {
         const RequestParams &R = bv.reqs[bv.slot_req[ls]];
         const uint32_t l = ls - R.slot_base;
@@ -521,16 +522,19 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(SceneView sv, Frame
         sample = R.s0 + l % R.n_spp;
         key = dm::Key{R.key0, R.key1};
         root = R.root;
+        if (computed) {
+            const DRay c = camera_ray(R.vp, R.width, R.height, R.sdim, key, pixel, sample);
+            ray = ray_from_dir(c.o, c.d);
+        }
     } else {
         pixel = ls / fp.n_spp;
         sample = fp.s0 + ls % fp.n_spp;
+        if (computed) {
+            const DRay c = camera_ray(fp.vp, fp.width, fp.height, fp.sdim, key, pixel, sample);
+            ray = ray_from_dir(c.o, c.d);
+        }
     }
-    if (!BATCH && !queue && !fp.rays_stored) {  // the camera ray, computed as k_generate computes it
-        const DRay c = camera_ray(fp.vp, fp.width, fp.height, fp.sdim, key, pixel, sample);
-        ray = ray_from_dir(c.o, c.d);
-    } else {
-        ray = ray_from_dir(f3{pb.ox[ls], pb.oy[ls], pb.oz[ls]}, f3{pb.dx[ls], pb.dy[ls], pb.dz[ls]});
-    }
+    if (!computed) ray = ray_from_dir(f3{pb.ox[ls], pb.oy[ls], pb.oz[ls]}, f3{pb.dx[ls], pb.dy[ls], pb.dz[ls]});
     Best b{FLT_MAX, -1, 0u, 0u, 0u};
     // The slab distances as fused multiply-adds, fma(b, 1/d, -o/d): the rounding of -o/d adds an error of at most
     // 2^-24 |o/d| per axis, not relative to t, so the entry distance is lowered and the exit distance raised by
