@@ -383,6 +383,8 @@ def main(argv=None):
     ap.add_argument("--no-inverse", action="store_true", help="skip the inverse-rendering run")
     ap.add_argument("--valu-peak", action="store_true",
                     help="run only the VALU issue-rate calibration kernel (for counter runs) and print its rate")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched-rendering leg (for kernel traces) and print its JSON object")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB three times (the counter calibration kernel)")
     a = ap.parse_args(argv)
@@ -400,6 +402,9 @@ def main(argv=None):
             dst.copy_(src)
         torch.cuda.synchronize()
         print(f"copied {a.copy_mib} MiB x3", flush=True)
+        return 0
+    if a.batch_only:
+        print(json.dumps(batch_leg(tune=not a.no_tune)), flush=True)
         return 0
     if a.valu_peak:
         from rayna_amd.renderer import measure_valu_peak
