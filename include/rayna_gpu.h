@@ -480,9 +480,10 @@ void rayna_destroy(rayna_renderer *r);
 rayna_status rayna_measure_valu_peak(uint32_t iters, double *wave_insts_per_s, double *seconds);
 
 /* Verification hook (no engine counterpart): the number of float bit patterns in [first, first + count) on which
- * the SDF distances' square root (sqrtf's sequence without its final pass-through of zeros and infinities) differs
- * from the correctly rounded sqrtf, NaNs comparing equal; 0 means the two agree bit for bit there.  count may be up to 2^32 - first
- * (a uint64_t so that the whole range fits one call).  Errors are reported through rayna_last_error(NULL). */
+ * the SDF distances' square root (sqrtf's sequence without its final pass-through of zeros and infinities, in
+ * both its per-lane and its wave-scaled form) differs from the correctly rounded sqrtf, NaNs comparing equal; 0
+ * means they agree bit for bit there.  count may be up to 2^32 - first (a uint64_t so that the whole range fits
+ * one call).  Errors are reported through rayna_last_error(NULL). */
 rayna_status rayna_debug_sqrt_mismatches(uint32_t first, uint64_t count, uint64_t *mismatches);
 
 /* Debug fault injection.  No counterpart in the engine, whose render cannot fail (render/renderer.rs:224-264);

@@ -1015,11 +1015,12 @@ inline V3<R> bulb_step(const V3<R> &z, R rho2, R r, R zr, int power, const V3<R>
     return V3<R>{un.im * wn.re, un.im * wn.im, un.re} * zr + p0;
 }
 
-// Single precision (deviation 11): 1/sqrt(s) by three Newton steps y <- y (3/2 - (s/2) y^2) from the bit-level
-// first guess 0x5f3759df - (bits >> 1) (Lomont, "Fast Inverse Square Root", 2003), which reach fp32 rounding
-// (relative error of the guess <= 3.5%, then about 1.8e-3, 4.7e-6 and below 1e-7).  Finite for every finite
-// s >= 0: at 0 the guess is 2^63-scale and the steps multiply it by 1.5.  The device evaluates the same
-// operations in the same order.
+// Single precision (deviation 11): 1/sqrt(s) by three Newton steps y <- y (3/2 - (s/2) y^2), each evaluated as
+// y * fma(-(h y), y, 3/2) with h = s/2, from the bit-level first guess 0x5f3759df - (bits >> 1) (Lomont, "Fast
+// Inverse Square Root", 2003), which reach fp32 rounding (relative error of the guess <= 3.5%, then about 1.8e-3,
+// 4.7e-6 and below 1e-7).  Finite for every finite s >= 0: at 0 the guess is 2^63-scale and the steps multiply it
+// by 1.5.  The device evaluates the same operations in the same order (std::fma rounds once, as the device's
+// fma does).
 inline float inv_sqrt_nr(float s) {
     uint32_t b;
     std::memcpy(&b, &s, 4);
@@ -1027,14 +1028,35 @@ inline float inv_sqrt_nr(float s) {
     float y;
     std::memcpy(&y, &b, 4);
     const float h = 0.5f * s;
-    y = y * (1.5f - h * y * y);
-    y = y * (1.5f - h * y * y);
-    y = y * (1.5f - h * y * y);
+    y = y * std::fma(-(h * y), y, 1.5f);
+    y = y * std::fma(-(h * y), y, 1.5f);
+    y = y * std::fma(-(h * y), y, 1.5f);
     return y;
 }
 
+// Single precision: the powers of a unit complex number (|a| = 1 within rounding), a square as
+// (2 re^2 - 1, 2 re im) -- re^2 - im^2 = 2 re^2 - 1 on the unit circle -- and a product with fmas, in cpow's
+// binary powering order (for n = 8, three squarings).
+inline Cx<float> csq_unit(const Cx<float> &a) {
+    const float t = a.re + a.re;
+    return {std::fma(t, a.re, -1.0f), t * a.im};
+}
+inline Cx<float> cpow_unit(Cx<float> a, int n) {
+    Cx<float> r{1.0f, 0.0f};
+    bool have = false;
+    while (n > 0) {
+        if (n & 1) {
+            r = have ? Cx<float>{std::fma(r.re, a.re, -(r.im * a.im)), std::fma(r.re, a.im, r.im * a.re)} : a;
+            have = true;
+        }
+        n >>= 1;
+        if (n) a = csq_unit(a);
+    }
+    return r;
+}
+
 // The single-precision step: |z| and rho come from rsqrt products (s * (1/sqrt s)) instead of square roots and
-// divisions; `ir` is 1/|z|.
+// divisions, `ir` is 1/|z|; u^n and w^n are unit-circle powers and z^n + p0 is fused.
 inline V3<float> bulb_step_f(const V3<float> &z, float rho2, float r, float ir, float zr, int power,
                              const V3<float> &p0) {
     const float iq = inv_sqrt_nr(rho2);
@@ -1042,8 +1064,19 @@ inline V3<float> bulb_step_f(const V3<float> &z, float rho2, float r, float ir, 
     Cx<float> u{1.0f, 0.0f}, w{1.0f, 0.0f};
     if (r > 0.0f) u = Cx<float>{z.z * ir, rho * ir};
     if (rho > 0.0f) w = Cx<float>{z.x * iq, z.y * iq};
-    const Cx<float> un = cpow(u, power), wn = cpow(w, power);
-    return V3<float>{un.im * wn.re, un.im * wn.im, un.re} * zr + p0;
+    const Cx<float> un = cpow_unit(u, power), wn = cpow_unit(w, power);
+    return V3<float>{std::fma(un.im * wn.re, zr, p0.x), std::fma(un.im * wn.im, zr, p0.y),
+                     std::fma(un.re, zr, p0.z)};
+}
+
+// |a| as the SDF sphere distances take it: length() in double precision; in single precision (deviation 11) from
+// the fused sum of squares fma(z, z, fma(y, y, x x)), correctly rounded square root.
+template <class R>
+inline R sdf_length(const V3<R> &a) {
+    if constexpr (std::is_same<R, float>::value)
+        return std::sqrt(std::fma(a.z, a.z, std::fma(a.y, a.y, a.x * a.x)));
+    else
+        return length(a);
 }
 
 template <class R>
@@ -1051,20 +1084,24 @@ inline R sdf_eval(const Scene &s, const rayna_sdf &f, const V3<R> &p) {
     switch (f.kind) {
         case RAYNA_SDF_SPHERE: {
             const double *sp = &s.spheres[4 * f.first_sphere];
-            return length(p - cast3<R>(sp)) - (R)sp[3];
+            return sdf_length(p - cast3<R>(sp)) - (R)sp[3];
         }
         case RAYNA_SDF_SMIN_SPHERES: {
             const R k = (R)f.k;
             const double *sp = &s.spheres[4 * f.first_sphere];
-            R d = length(p - cast3<R>(sp)) - (R)sp[3];
+            R d = sdf_length(p - cast3<R>(sp)) - (R)sp[3];
             const R ik = R(1) / k;
             for (int i = 1; i < f.n_spheres; ++i) {
                 const double *q = &s.spheres[4 * (f.first_sphere + i)];
-                const R di = length(p - cast3<R>(q)) - (R)q[3];
-                // single precision multiplies by 1/k (deviation 11); double divides, as the engine's blend does
-                const R h = std::is_same<R, float>::value ? std::max(k - std::fabs(d - di), R(0)) * ik
-                                                          : std::max(k - std::fabs(d - di), R(0)) / k;
-                d = std::min(d, di) - h * h * k * R(0.25);
+                const R di = sdf_length(p - cast3<R>(q)) - (R)q[3];
+                if constexpr (std::is_same<R, float>::value) {
+                    // deviation 11: a multiply by 1/k, and h^2 (k / 4) fused into the subtraction
+                    const float h = std::max(k - std::fabs(d - di), 0.0f) * ik;
+                    d = std::fma(-(h * h), k * 0.25f, std::min(d, di));
+                } else {  // as the engine's blend
+                    const R h = std::max(k - std::fabs(d - di), R(0)) / k;
+                    d = std::min(d, di) - h * h * k * R(0.25);
+                }
             }
             return d;
         }
@@ -1076,15 +1113,16 @@ inline R sdf_eval(const Scene &s, const rayna_sdf &f, const V3<R> &p) {
             V3<R> z = p0;
             R dr = R(1), r = R(0);
             for (int i = 0; i < f.bulb_iterations; ++i) {
-                const R rho2 = z.x * z.x + z.y * z.y;
                 if constexpr (std::is_same<R, float>::value) {  // deviation 11
-                    const float s2 = rho2 + z.z * z.z;  // |z|^2, in length's summation order
+                    const float rho2 = std::fma(z.x, z.x, z.y * z.y);
+                    const float s2 = std::fma(z.z, z.z, rho2);  // |z|^2
                     const float ir = inv_sqrt_nr(s2);
                     r = s2 * ir;
                     if (r > bail) break;
-                    dr = ipow(r, power - 1) * (R)power * dr + R(1);
+                    dr = std::fma(ipow(r, power - 1) * (R)power, dr, R(1));
                     z = bulb_step_f(z, rho2, r, ir, ipow(r, power), power, p0);
                 } else {
+                    const R rho2 = z.x * z.x + z.y * z.y;
                     r = Math<R>::sqrt(rho2 + z.z * z.z);  // length(z), in its summation order
                     if (r > bail) break;
                     dr = ipow(r, power - 1) * (R)power * dr + R(1);

@@ -51,16 +51,28 @@ __device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
 // correction already leaves the root as it is.  Equal to sqrtf bit for bit on every input (checked over all 2^32
 // bit patterns: rayna_debug_sqrt_mismatches).  For the SDF sphere distances, which the march evaluates sixteen at
 // a time.
-__device__ __forceinline__ float sqrt_rn(float x) {
-    const bool tiny = x < 0x1p-96f;
-    const float xs = tiny ? x * 0x1p32f : x;
+__device__ __forceinline__ float sqrt_rn_corrected(float xs) {  // the hardware root of xs, corrected
     const float y = __builtin_amdgcn_sqrtf(xs);
     const float ym = __uint_as_float(__float_as_uint(y) - 1u), yp = __uint_as_float(__float_as_uint(y) + 1u);
     float r = fmaf(-ym, y, xs) <= 0.0f ? ym : y;
-    r = fmaf(-yp, y, xs) > 0.0f ? yp : r;
+    return fmaf(-yp, y, xs) > 0.0f ? yp : r;
+}
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const bool tiny = x < 0x1p-96f;
+    const float r = sqrt_rn_corrected(tiny ? x * 0x1p32f : x);
     return tiny ? r * 0x1p-16f : r;
 }
-__device__ __forceinline__ float length_rn(f3 a) { return sqrt_rn(dot(a, a)); }
+// sqrt_rn with the scaling taken only by waves that have an input below 2^-96 (a point within 2^-48 of a sphere's
+// centre): the same value on every input.
+__device__ __forceinline__ float sqrt_rn_wave(float x) {
+    if (__any(x < 0x1p-96f)) return sqrt_rn(x);
+    return sqrt_rn_corrected(x);
+}
+// |a| from the fused sum of squares fma(z, z, fma(y, y, x x)) and a correctly rounded square root: the SDF sphere
+// distances, as the oracle's single precision takes them.
+__device__ __forceinline__ float length_fma(f3 a) {
+    return sqrt_rn_wave(fmaf(a.z, a.z, fmaf(a.y, a.y, a.x * a.x)));
+}
 __device__ __forceinline__ f3 normalize(f3 a) { return a * (1.0f / length(a)); }
 __device__ __forceinline__ bool try_normalize(f3 a, f3 *out) {  // glam try_normalize
     const float rcp = 1.0f / length(a);
@@ -310,38 +322,43 @@ __device__ __forceinline__ void bulb_powers(float x, int n, float *x_nm1, float 
 struct cx {
     float re, im;
 };
-__device__ __forceinline__ cx cmul(cx a, cx b) { return cx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 
-// a^n (n >= 1) by binary powering, the lowest factor taken as is; for the usual n = 8 three squarings.
-__device__ __forceinline__ cx cpow(cx a, int n) {
-    if (n == 8) {
-        a = cmul(a, a);
-        a = cmul(a, a);
-        return cmul(a, a);
-    }
+// The Mandelbulb's powers of a unit complex number (|a| = 1 within rounding): a square as (2 re^2 - 1, 2 re im),
+// since re^2 - im^2 = 2 re^2 - 1 on the unit circle (three instructions instead of five), a product with fused
+// multiply-adds; binary powering (the lowest factor taken as is), so for n = 8 three squarings.  IEEE operations
+// only (an fma rounds once), so the oracle's single precision evaluates it bit for bit.
+__device__ __forceinline__ cx csq_unit(cx a) {
+    const float t = a.re + a.re;
+    return cx{fmaf(t, a.re, -1.0f), t * a.im};
+}
+__device__ __forceinline__ cx cmul_fma(cx a, cx b) {
+    return cx{fmaf(a.re, b.re, -(a.im * b.im)), fmaf(a.re, b.im, a.im * b.re)};
+}
+__device__ __forceinline__ cx cpow_unit(cx a, int n) {
+    if (n == 8) return csq_unit(csq_unit(csq_unit(a)));
     cx r{1.0f, 0.0f};
     bool have = false;
     while (n > 0) {
         if (n & 1) {
-            r = have ? cmul(r, a) : a;
+            r = have ? cmul_fma(r, a) : a;
             have = true;
         }
         n >>= 1;
-        if (n) a = cmul(a, a);
+        if (n) a = csq_unit(a);
     }
     return r;
 }
 
-// 1/sqrt(s) by three Newton steps y <- y (3/2 - (s/2) y^2) from the bit-level first guess 0x5f3759df - (bits >> 1)
-// (Lomont, "Fast Inverse Square Root", 2003): the guess is within 3.5%, the steps reach fp32 rounding (1.8e-3,
-// 4.7e-6, then below 1e-7).  Finite for every finite s >= 0.  Plain multiplies and adds, so the oracle's single
-// precision evaluates it bit for bit.
+// 1/sqrt(s) by three Newton steps y <- y (3/2 - (s/2) y^2), each as y * fma(-(h y), y, 3/2) with h = s/2, from the
+// bit-level first guess 0x5f3759df - (bits >> 1) (Lomont, "Fast Inverse Square Root", 2003): the guess is within
+// 3.5%, the steps reach fp32 rounding (1.8e-3, 4.7e-6, then below 1e-7).  Finite for every finite s >= 0.  IEEE
+// multiplies and fmas, so the oracle's single precision evaluates it bit for bit.
 __device__ __forceinline__ float inv_sqrt_nr(float s) {
     float y = __uint_as_float(0x5f3759dfu - (__float_as_uint(s) >> 1));
     const float h = 0.5f * s;
-    y = y * (1.5f - h * y * y);
-    y = y * (1.5f - h * y * y);
-    y = y * (1.5f - h * y * y);
+    y = y * fmaf(-(h * y), y, 1.5f);
+    y = y * fmaf(-(h * y), y, 1.5f);
+    y = y * fmaf(-(h * y), y, 1.5f);
     return y;
 }
 
@@ -351,23 +368,24 @@ __device__ __forceinline__ float inv_sqrt_nr(float s) {
 // e^{i phi}, z^n = r^n (Im u^n Re w^n, Im u^n Im w^n, Re u^n); w is 1 on the z axis and u is 1 at r = 0, where
 // the other factor vanishes.  Returns false when r exceeds the bail-out radius (z and dr unchanged).
 // |z| and rho are rsqrt products (s * (1/sqrt s)) and the unit vectors use those reciprocals: two Newton
-// reciprocal square roots instead of two correctly rounded square roots and two divisions per iteration.
+// reciprocal square roots instead of two correctly rounded square roots and two divisions per iteration.  The
+// sums of products (|z|^2, dr, z^n + p0) are fused multiply-adds and u^n, w^n unit-circle powers.
 __device__ __forceinline__ bool bulb_iterate(const GSdf &f, f3 p0, f3 &z, float &dr, float &r) {
-    const float rho2 = z.x * z.x + z.y * z.y;
-    const float s2 = rho2 + z.z * z.z;  // |z|^2, in length's summation order
+    const float rho2 = fmaf(z.x, z.x, z.y * z.y);
+    const float s2 = fmaf(z.z, z.z, rho2);  // |z|^2
     const float ir = inv_sqrt_nr(s2);
     r = s2 * ir;
     if (r > f.bailout) return false;
     float rp1, zr;
     bulb_powers(r, f.power, &rp1, &zr);
-    dr = rp1 * (float)f.power * dr + 1.0f;
+    dr = fmaf(rp1 * (float)f.power, dr, 1.0f);
     const float iq = inv_sqrt_nr(rho2);
     const float rho = rho2 * iq;
     cx u{1.0f, 0.0f}, w{1.0f, 0.0f};
     if (r > 0.0f) u = cx{z.z * ir, rho * ir};
     if (rho > 0.0f) w = cx{z.x * iq, z.y * iq};
-    const cx un = cpow(u, f.power), wn = cpow(w, f.power);
-    z = f3{un.im * wn.re, un.im * wn.im, un.re} * zr + p0;
+    const cx un = cpow_unit(u, f.power), wn = cpow_unit(w, f.power);
+    z = f3{fmaf(un.im * wn.re, zr, p0.x), fmaf(un.im * wn.im, zr, p0.y), fmaf(un.re, zr, p0.z)};
     return true;
 }
 
@@ -376,21 +394,32 @@ __device__ __forceinline__ float bulb_estimate(float r, float dr) {
     return r == 0.0f ? 0.0f : 0.5f * dm::log_f(r) * r / dr;
 }
 
+// The smooth minimum of n sphere distances in list order: d <- min(d, d_i) - h^2 k / 4 with
+// h = max(k - |d - d_i|, 0) / k, the division a multiply by 1/k and h^2 (k / 4) fused into the subtraction.
+__device__ __forceinline__ float smin_spheres(const GSphere *s, int n, float k, f3 p) {
+    float d = length_fma(p - f3{s[0].cx, s[0].cy, s[0].cz}) - s[0].radius;
+    const float ik = 1.0f / k, k4 = k * 0.25f;
+    for (int i = 1; i < n; ++i) {
+        const GSphere q = s[i];
+        const float di = length_fma(p - f3{q.cx, q.cy, q.cz}) - q.radius;
+        const float h = fmaxf(k - fabsf(d - di), 0.0f) * ik;
+        d = fmaf(-(h * h), k4, fminf(d, di));
+    }
+    return d;
+}
+
 __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
     if (f.kind == RAYNA_SDF_SPHERE) {
         const GSphere s = sv.spheres[f.first_sphere];
-        return length_rn(p - f3{s.cx, s.cy, s.cz}) - s.radius;
+        return length_fma(p - f3{s.cx, s.cy, s.cz}) - s.radius;
     }
     if (f.kind == RAYNA_SDF_SMIN_SPHERES) {
-        const GSphere s0 = sv.spheres[f.first_sphere];
-        float d = length_rn(p - f3{s0.cx, s0.cy, s0.cz}) - s0.radius;
-        for (int i = 1; i < f.n_spheres; ++i) {
-            const GSphere s = sv.spheres[f.first_sphere + i];
-            const float di = length_rn(p - f3{s.cx, s.cy, s.cz}) - s.radius;
-            const float h = fmaxf(f.k - fabsf(d - di), 0.0f) * (1.0f / f.k);
-            d = fminf(d, di) - h * h * f.k * 0.25f;
-        }
-        return d;
+        // March items are SDF-major, so the lanes of a wave normally evaluate one SDF: its spheres are then read at
+        // wave-uniform addresses and the loop is counted in scalar registers.
+        const int32_t first = __builtin_amdgcn_readfirstlane(f.first_sphere);
+        const int32_t n = __builtin_amdgcn_readfirstlane(f.n_spheres);
+        if (__all(f.first_sphere == first && f.n_spheres == n)) return smin_spheres(sv.spheres + first, n, f.k, p);
+        return smin_spheres(sv.spheres + f.first_sphere, f.n_spheres, f.k, p);
     }
     // Mandelbulb distance estimate
     const f3 p0 = p - f3{f.cx, f.cy, f.cz};
@@ -1241,13 +1270,16 @@ __global__ __launch_bounds__(BLOCK) void k_valu_peak(float *out, uint32_t iters,
     out[gid] = sum;
 }
 
-// Counts the bit patterns [first, first + count) on which sqrt_rn and sqrtf differ (NaNs compare equal).
+// Counts the bit patterns [first, first + count) on which sqrt_rn or sqrt_rn_wave differs from sqrtf (NaNs compare
+// equal).  A wave holds 64 consecutive patterns, so waves whose inputs are all at least 2^-96 check the unscaled
+// path of sqrt_rn_wave and the others its scaled one.
 __global__ __launch_bounds__(BLOCK) void k_sqrt_check(uint32_t first, uint32_t count, unsigned long long *mismatches) {
     uint32_t bad = 0;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += gridDim.x * BLOCK) {
         const float x = __uint_as_float(first + i);
-        const float a = sqrt_rn(x), b = sqrtf(x);
+        const float a = sqrt_rn(x), c = sqrt_rn_wave(x), b = sqrtf(x);
         if (__float_as_uint(a) != __float_as_uint(b) && !(a != a && b != b)) ++bad;
+        else if (__float_as_uint(c) != __float_as_uint(b) && !(c != c && b != b)) ++bad;
     }
     if (bad) atomicAdd(mismatches, (unsigned long long)bad);
 }

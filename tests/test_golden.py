@@ -23,7 +23,7 @@ def test_oracle_reproduces_golden(name):
 # (each such pixel must have a stated cause: tests/precision.py), and of primary hits whose distance may fall
 # outside 1e-5 (all at grazing incidence).  Measured on these fixtures: random spheres 4 of 2304 pixels (mirror
 # and glass chains), mixed primitives 3 of 1536 (paths that diverge), icosphere mesh 0 of 1296 (1 grazing hit of
-# 887), SDF + volume 28 of 576 (19 diverge, 9 meet an SDF first).
+# 887), SDF + volume 27 of 576 (20 diverge, 7 meet an SDF first).
 RADIANCE_OFF_MAX = {"random_spheres": 0.01, "mixed_primitives": 0.01, "triangle_mesh_s4": 0.002, "sdf_volume": 0.08}
 GRAZING_MAX = 0.005
 

@@ -1,7 +1,8 @@
 """The SDF sphere distances' square root (sqrt_rn in rayna_amd/csrc/kernels.hip: sqrtf's sequence without its final
-pass-through of zeros and infinities) equals the correctly rounded sqrtf on every float: both are evaluated on the
-GPU for all 2^32 bit patterns and compared bit for bit (NaNs compare equal).  The smooth-min and sphere SDFs of the march use it,
-and their bit-exact agreement with the oracle rests on this."""
+pass-through of zeros and infinities; sqrt_rn_wave, which scales small inputs only in waves that hold one) equals
+the correctly rounded sqrtf on every float: all are evaluated on the GPU for all 2^32 bit patterns and compared bit
+for bit (NaNs compare equal).  The smooth-min and sphere SDFs of the march use sqrt_rn_wave, and their bit-exact
+agreement with the oracle rests on this."""
 import pytest
 
 from rayna_amd import RayError
