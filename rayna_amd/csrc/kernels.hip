@@ -370,21 +370,25 @@ __device__ __forceinline__ float inv_sqrt_nr(float s) {
 // |z| and rho are rsqrt products (s * (1/sqrt s)) and the unit vectors use those reciprocals: two Newton
 // reciprocal square roots instead of two correctly rounded square roots and two divisions per iteration.  The
 // sums of products (|z|^2, dr, z^n + p0) are fused multiply-adds and u^n, w^n unit-circle powers.
-__device__ __forceinline__ bool bulb_iterate(const GSdf &f, f3 p0, f3 &z, float &dr, float &r) {
+// The SDF's bail-out radius and power come in registers (loaded once per estimate, not per iteration); POW is the
+// power when the caller knows it (8, the usual one) and 0 for the run-time `power`.
+template <int POW>
+__device__ __forceinline__ bool bulb_iterate(float bailout, int power, f3 p0, f3 &z, float &dr, float &r) {
+    const int n = POW ? POW : power;
     const float rho2 = fmaf(z.x, z.x, z.y * z.y);
     const float s2 = fmaf(z.z, z.z, rho2);  // |z|^2
     const float ir = inv_sqrt_nr(s2);
     r = s2 * ir;
-    if (r > f.bailout) return false;
+    if (r > bailout) return false;
     float rp1, zr;
-    bulb_powers(r, f.power, &rp1, &zr);
-    dr = fmaf(rp1 * (float)f.power, dr, 1.0f);
+    bulb_powers(r, n, &rp1, &zr);
+    dr = fmaf(rp1 * (float)n, dr, 1.0f);
     const float iq = inv_sqrt_nr(rho2);
     const float rho = rho2 * iq;
     cx u{1.0f, 0.0f}, w{1.0f, 0.0f};
     if (r > 0.0f) u = cx{z.z * ir, rho * ir};
     if (rho > 0.0f) w = cx{z.x * iq, z.y * iq};
-    const cx un = cpow_unit(u, f.power), wn = cpow_unit(w, f.power);
+    const cx un = cpow_unit(u, n), wn = cpow_unit(w, n);
     z = f3{fmaf(un.im * wn.re, zr, p0.x), fmaf(un.im * wn.im, zr, p0.y), fmaf(un.re, zr, p0.z)};
     return true;
 }
@@ -425,8 +429,15 @@ __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
     const f3 p0 = p - f3{f.cx, f.cy, f.cz};
     f3 z = p0;
     float dr = 1.0f, r = 0.0f;
-    for (int i = 0; i < f.bulb_iterations; ++i)
-        if (!bulb_iterate(f, p0, z, dr, r)) break;
+    const int iterations = f.bulb_iterations, power = f.power;
+    const float bailout = f.bailout;
+    if (__all(power == 8)) {
+        for (int i = 0; i < iterations; ++i)
+            if (!bulb_iterate<8>(bailout, power, p0, z, dr, r)) break;
+    } else {
+        for (int i = 0; i < iterations; ++i)
+            if (!bulb_iterate<0>(bailout, power, p0, z, dr, r)) break;
+    }
     return bulb_estimate(r, dr);
 }
 

@@ -24,6 +24,20 @@
 constexpr int MARCH_BLOCK = 256;
 constexpr uint32_t MARCH_ITERATE_LANES = 32;
 
+// Iterations of a lane's Mandelbulb estimate from iteration `it`: the lanes whose estimate goes on iterate again at
+// once while at least MARCH_ITERATE_LANES of the wave do; the rest of the pass (step starts and ends) waits for
+// fewer of them.  True when the estimate is complete (bail-out or the iteration limit).
+template <int POW>
+__device__ __forceinline__ bool bulb_run(float bailout, int power, int iterations, f3 p0, f3 &z, float &dr, float &r,
+                                         int &it) {
+    for (;;) {
+        bool done = it >= iterations;
+        if (!done) done = !bulb_iterate<POW>(bailout, power, p0, z, dr, r) || ++it >= iterations;
+        if (done) return true;
+        if ((uint32_t)__popcll(__ballot(1)) < MARCH_ITERATE_LANES) return false;
+    }
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams fp, PathBuffers pb,
                                                        const int32_t *queue, const uint32_t *count_ptr,
@@ -127,15 +141,15 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
             }
         }
         if (in_eval) {  // iterations of the Mandelbulb estimate (sdf_eval's loop)
+            // the SDF's parameters in registers before the loop, and the usual power 8 as a constant where the
+            // whole wave has it: the iterations then load nothing
+            const int iterations = f.bulb_iterations, power = f.power;
+            const float bailout = f.bailout;
             bool done;
-            for (;;) {
-                done = it >= f.bulb_iterations;
-                if (!done) done = !bulb_iterate(f, p0, z, dr, r) || ++it >= f.bulb_iterations;
-                if (done) break;
-                // the lanes whose estimate goes on iterate again at once while at least MARCH_ITERATE_LANES
-                // of the wave do; the rest of the pass (step starts and ends) waits for fewer of them
-                if ((uint32_t)__popcll(__ballot(1)) < MARCH_ITERATE_LANES) break;
-            }
+            if (__all(power == 8))
+                done = bulb_run<8>(bailout, power, iterations, p0, z, dr, r, it);
+            else
+                done = bulb_run<0>(bailout, power, iterations, p0, z, dr, r, it);
             if (done) {
                 dist = bulb_estimate(r, dr);
                 in_eval = false;
