@@ -62,7 +62,8 @@ inline void sincos_turns_f32(float r, float *s_out, float *c_out) {
 }
 
 // Natural logarithm.  x = m * 2^e with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1),
-// |s| <= 0.1716, series to s^11.  ln 2 is split in a 16-bit head and a tail so e*ln2_hi is exact.
+// |s| <= 0.1716, series to s^11 in Horner form with fmas (as the device evaluates it).  ln 2 is split in a 16-bit
+// head and a tail so e*ln2_hi is exact.
 inline float log_f32(float x) {
     if (x != x || x < 0.0f) return std::numeric_limits<float>::quiet_NaN();
     if (x == 0.0f) return -std::numeric_limits<float>::infinity();
@@ -83,11 +84,11 @@ inline float log_f32(float x) {
     const float s2 = s * s;
     const float L3 = 0x1.555556p-2f, L5 = 0x1.99999ap-3f, L7 = 0x1.24924ap-3f, L9 = 0x1.c71c72p-4f,
                 L11 = 0x1.745d18p-4f;
-    const float poly = s2 * (L3 + s2 * (L5 + s2 * (L7 + s2 * (L9 + s2 * L11))));
-    const float log_m = (s + s * poly) * 2.0f;
+    const float poly = s2 * std::fma(s2, std::fma(s2, std::fma(s2, std::fma(s2, L11, L9), L7), L5), L3);
+    const float log_m = std::fma(s, poly, s) * 2.0f;
     const float fe = (float)e;
     const float LN2_HI = 0x1.62e4p-1f, LN2_LO = 0x1.7f7d1cp-20f;
-    return fe * LN2_HI + (log_m + fe * LN2_LO);
+    return std::fma(fe, LN2_HI, std::fma(fe, LN2_LO, log_m));
 }
 
 // Exponential.  k = rint(x / ln 2), r = x - k ln2 (Cody-Waite, two parts), exp(r) by Taylor to r^7,

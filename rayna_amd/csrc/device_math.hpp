@@ -56,11 +56,11 @@ __device__ __forceinline__ float log_f(float x) {
     const float s2 = s * s;
     const float L3 = 0x1.555556p-2f, L5 = 0x1.99999ap-3f, L7 = 0x1.24924ap-3f, L9 = 0x1.c71c72p-4f,
                 L11 = 0x1.745d18p-4f;
-    const float poly = s2 * (L3 + s2 * (L5 + s2 * (L7 + s2 * (L9 + s2 * L11))));
-    const float log_m = (s + s * poly) * 2.0f;
+    const float poly = s2 * fmaf(s2, fmaf(s2, fmaf(s2, fmaf(s2, L11, L9), L7), L5), L3);
+    const float log_m = fmaf(s, poly, s) * 2.0f;
     const float fe = (float)e;
     const float LN2_HI = 0x1.62e4p-1f, LN2_LO = 0x1.7f7d1cp-20f;
-    return fe * LN2_HI + (log_m + fe * LN2_LO);
+    return fmaf(fe, LN2_HI, fmaf(fe, LN2_LO, log_m));
 }
 
 __device__ __forceinline__ float exp_f(float x) {

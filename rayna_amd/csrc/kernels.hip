@@ -412,9 +412,9 @@ __device__ __forceinline__ float smin_spheres(const GSphere *s, int n, float k, 
     return d;
 }
 
-__device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
+__device__ float sdf_eval(const GSphere *spheres, const GSdf &f, f3 p) {
     if (f.kind == RAYNA_SDF_SPHERE) {
-        const GSphere s = sv.spheres[f.first_sphere];
+        const GSphere s = spheres[f.first_sphere];
         return length_fma(p - f3{s.cx, s.cy, s.cz}) - s.radius;
     }
     if (f.kind == RAYNA_SDF_SMIN_SPHERES) {
@@ -422,8 +422,8 @@ __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
         // wave-uniform addresses and the loop is counted in scalar registers.
         const int32_t first = __builtin_amdgcn_readfirstlane(f.first_sphere);
         const int32_t n = __builtin_amdgcn_readfirstlane(f.n_spheres);
-        if (__all(f.first_sphere == first && f.n_spheres == n)) return smin_spheres(sv.spheres + first, n, f.k, p);
-        return smin_spheres(sv.spheres + f.first_sphere, f.n_spheres, f.k, p);
+        if (__all(f.first_sphere == first && f.n_spheres == n)) return smin_spheres(spheres + first, n, f.k, p);
+        return smin_spheres(spheres + f.first_sphere, f.n_spheres, f.k, p);
     }
     // Mandelbulb distance estimate
     const f3 p0 = p - f3{f.cx, f.cy, f.cz};
@@ -443,9 +443,9 @@ __device__ float sdf_eval(const SceneView &sv, const GSdf &f, f3 p) {
 
 __device__ f3 sdf_normal(const SceneView &sv, const GSdf &f, f3 p) {
     const float e = f.epsilon;
-    const f3 g{sdf_eval(sv, f, f3{p.x + e, p.y, p.z}) - sdf_eval(sv, f, f3{p.x - e, p.y, p.z}),
-               sdf_eval(sv, f, f3{p.x, p.y + e, p.z}) - sdf_eval(sv, f, f3{p.x, p.y - e, p.z}),
-               sdf_eval(sv, f, f3{p.x, p.y, p.z + e}) - sdf_eval(sv, f, f3{p.x, p.y, p.z - e})};
+    const f3 g{sdf_eval(sv.spheres, f, f3{p.x + e, p.y, p.z}) - sdf_eval(sv.spheres, f, f3{p.x - e, p.y, p.z}),
+               sdf_eval(sv.spheres, f, f3{p.x, p.y + e, p.z}) - sdf_eval(sv.spheres, f, f3{p.x, p.y - e, p.z}),
+               sdf_eval(sv.spheres, f, f3{p.x, p.y, p.z + e}) - sdf_eval(sv.spheres, f, f3{p.x, p.y, p.z - e})};
     return normalize(g);
 }
 
@@ -1423,13 +1423,13 @@ void launch_march(const SceneView &sv, const FrameParams &fp, const PathBuffers 
     if (n_sdf == 0) return;
     launch_fill_u32(sdf_count, 0u, 1, s);
     if (bv.reqs) {
-        hipLaunchKernelGGL(k_march<true>, dim3(grid), dim3(MARCH_BLOCK), 0, s, sv, fp, pb, queue, count, n_sdf, mt,
-                           mt_stride, work, tune, bv);
+        hipLaunchKernelGGL(k_march<true>, dim3(grid), dim3(MARCH_BLOCK), 0, s, fp, pb, sv.sdfs, sv.spheres, queue,
+                           count, n_sdf, mt, mt_stride, work, tune, bv);
         hipLaunchKernelGGL(k_march_merge<true>, dim3(merge_grid), dim3(BLOCK), 0, s, sv, fp, pb, queue, count,
                            n_sdf, mt, mt_stride, sdf_queue, sdf_count, bv);
     } else {
-        hipLaunchKernelGGL(k_march<false>, dim3(grid), dim3(MARCH_BLOCK), 0, s, sv, fp, pb, queue, count, n_sdf,
-                           mt, mt_stride, work, tune, bv);
+        hipLaunchKernelGGL(k_march<false>, dim3(grid), dim3(MARCH_BLOCK), 0, s, fp, pb, sv.sdfs, sv.spheres, queue,
+                           count, n_sdf, mt, mt_stride, work, tune, bv);
         hipLaunchKernelGGL(k_march_merge<false>, dim3(merge_grid), dim3(BLOCK), 0, s, sv, fp, pb, queue, count,
                            n_sdf, mt, mt_stride, sdf_queue, sdf_count, bv);
     }

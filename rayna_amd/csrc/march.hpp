@@ -38,8 +38,80 @@ __device__ __forceinline__ bool bulb_run(float bailout, int power, int iteration
     }
 }
 
+// The state of a lane's march (k_march): its item, ray and cut, the march distance and step, and a Mandelbulb
+// estimate in progress (iteration it, z, dr, r about p0).
+struct MarchLane {
+    uint32_t item;
+    DRay ray;
+    float t_cut, total;
+    int step;
+    bool in_eval;
+    int it;
+    f3 z, p0;
+    float dr, r;
+};
+
+// One pass of a lane's march against SDF f: a new step (cut and escape tests; a sphere or smooth-min estimate is
+// evaluated at once, a Mandelbulb estimate started), iterations of a Mandelbulb estimate, and the end of a step.
+// False once the march has ended (its outcome written to mt).
+__device__ __forceinline__ bool march_pass(const GSdf &f, const GSphere *spheres, MarchLane &L, float *mt,
+                                           uint64_t mt_stride) {
+    bool have_dist = false;
+    float dist = 0.0f;
+    if (!L.in_eval) {  // a new step of the march
+        const f3 point = at(L.ray, L.total);
+        const f3 qe = point - f3{f.ex, f.ey, f.ez};
+        if (L.total >= L.t_cut || (dot(qe, qe) > f.escape_r * f.escape_r && dot(qe, L.ray.d) > 0.0f)) {
+            mt[L.item] = __builtin_inff();
+            return false;
+        }
+        if (f.kind == RAYNA_SDF_MANDELBULB) {
+            L.p0 = point - f3{f.cx, f.cy, f.cz};
+            L.z = L.p0;
+            L.dr = 1.0f;
+            L.r = 0.0f;
+            L.it = 0;
+            L.in_eval = true;
+        } else {
+            dist = sdf_eval(spheres, f, point);
+            have_dist = true;
+        }
+    }
+    if (L.in_eval) {  // iterations of the Mandelbulb estimate (sdf_eval's loop)
+        // the SDF's parameters in registers before the loop, and the usual power 8 as a constant where the whole
+        // wave has it: the iterations then load nothing
+        const int iterations = f.bulb_iterations, power = f.power;
+        const float bailout = f.bailout;
+        bool done;
+        if (__all(power == 8))
+            done = bulb_run<8>(bailout, power, iterations, L.p0, L.z, L.dr, L.r, L.it);
+        else
+            done = bulb_run<0>(bailout, power, iterations, L.p0, L.z, L.dr, L.r, L.it);
+        if (done) {
+            dist = bulb_estimate(L.r, L.dr);
+            L.in_eval = false;
+            have_dist = true;
+        }
+    }
+    if (have_dist) {  // the end of a step
+        L.total = L.total + fabsf(dist);
+        if (fabsf(dist) < f.epsilon && T_MIN <= L.total && L.total <= FLT_MAX) {
+            mt[L.item] = signbit(dist) ? -L.total : L.total;
+            mt[mt_stride + L.item] = __int_as_float(L.step);  // Intersection::side (raymarched.rs:109)
+            return false;
+        }
+        if (L.step > f.max_iterations) {
+            mt[L.item] = __builtin_inff();
+            return false;
+        }
+        ++L.step;
+    }
+    return true;
+}
+
 template <bool BATCH>
-__global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams fp, PathBuffers pb,
+__global__ __launch_bounds__(MARCH_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_march(FrameParams fp, PathBuffers pb, const GSdf *__restrict__ sdfs,
+                                                       const GSphere *__restrict__ spheres,
                                                        const int32_t *queue, const uint32_t *count_ptr,
                                                        uint32_t n_sdf, float *mt, uint64_t mt_stride, uint32_t *work,
                                                        ExtendTuning tune, BatchView bv) {
@@ -49,16 +121,8 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
     const uint32_t lane = __lane_id();
     bool active = false, exhausted = false, last_claim = false;
     uint32_t batch_next = 0, batch_end = 0;
-    // per-lane march state
-    uint32_t item = 0;
-    int32_t j = 0;  // SDF index
-    DRay ray{};
-    float t_cut = 0.0f, total = 0.0f;
-    int step = 0;
-    bool in_eval = false;  // inside a Mandelbulb estimate
-    int it = 0;            // its iteration
-    f3 z{0.0f, 0.0f, 0.0f}, p0{0.0f, 0.0f, 0.0f};
-    float dr = 1.0f, r = 0.0f;
+    int32_t j = 0;  // the lane's SDF
+    MarchLane L{};
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         if (!exhausted && idle != 0ull) {
@@ -89,7 +153,6 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
             }
             if (last_claim && batch_next == batch_end) exhausted = true;
             if (!active && q != 0xffffffffu) {
-                item = q;
                 const uint32_t jj = q / count, qq = q % count;
                 const uint32_t slot = queue ? (uint32_t)queue[qq] : qq;
                 int32_t first = 0, n_own = (int32_t)n_sdf;
@@ -102,12 +165,13 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
                     mt[q] = __builtin_inff();  // the request's scene has fewer unbounded objects
                 } else {
                     j = first + (int32_t)jj;
-                    ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
-                                       f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
-                    t_cut = pb.hit_t[slot];
-                    total = T_MIN;
-                    step = 0;
-                    in_eval = false;
+                    L.item = q;
+                    L.ray = ray_from_dir(f3{pb.ox[slot], pb.oy[slot], pb.oz[slot]},
+                                         f3{pb.dx[slot], pb.dy[slot], pb.dz[slot]});
+                    L.t_cut = pb.hit_t[slot];
+                    L.total = T_MIN;
+                    L.step = 0;
+                    L.in_eval = false;
                     active = true;
                 }
             }
@@ -117,58 +181,13 @@ __global__ __launch_bounds__(MARCH_BLOCK) void k_march(SceneView sv, FrameParams
             continue;
         }
         if (!active) continue;
-        const GSdf &f = sv.sdfs[j];
-        bool have_dist = false;
-        float dist = 0.0f;
-        if (!in_eval) {  // a new step of the march
-            const f3 point = at(ray, total);
-            const f3 qe = point - f3{f.ex, f.ey, f.ez};
-            if (total >= t_cut || (dot(qe, qe) > f.escape_r * f.escape_r && dot(qe, ray.d) > 0.0f)) {
-                mt[item] = __builtin_inff();
-                active = false;
-                continue;
-            }
-            if (f.kind == RAYNA_SDF_MANDELBULB) {
-                p0 = point - f3{f.cx, f.cy, f.cz};
-                z = p0;
-                dr = 1.0f;
-                r = 0.0f;
-                it = 0;
-                in_eval = true;
-            } else {
-                dist = sdf_eval(sv, f, point);
-                have_dist = true;
-            }
-        }
-        if (in_eval) {  // iterations of the Mandelbulb estimate (sdf_eval's loop)
-            // the SDF's parameters in registers before the loop, and the usual power 8 as a constant where the
-            // whole wave has it: the iterations then load nothing
-            const int iterations = f.bulb_iterations, power = f.power;
-            const float bailout = f.bailout;
-            bool done;
-            if (__all(power == 8))
-                done = bulb_run<8>(bailout, power, iterations, p0, z, dr, r, it);
-            else
-                done = bulb_run<0>(bailout, power, iterations, p0, z, dr, r, it);
-            if (done) {
-                dist = bulb_estimate(r, dr);
-                in_eval = false;
-                have_dist = true;
-            }
-        }
-        if (have_dist) {  // the end of a step
-            total = total + fabsf(dist);
-            if (fabsf(dist) < f.epsilon && T_MIN <= total && total <= FLT_MAX) {
-                mt[item] = signbit(dist) ? -total : total;
-                mt[mt_stride + item] = __int_as_float(step);  // Intersection::side (raymarched.rs:109)
-                active = false;
-            } else if (step > f.max_iterations) {
-                mt[item] = __builtin_inff();
-                active = false;
-            } else {
-                ++step;
-            }
-        }
+        // The items a wave holds normally belong to one SDF: its record is then read at a wave-uniform address of a
+        // read-only argument (scalar loads), and the pass's branches on it are uniform.
+        const int32_t j0 = __builtin_amdgcn_readfirstlane(j);
+        if (__all(j == j0))
+            active = march_pass(sdfs[j0], spheres, L, mt, mt_stride);
+        else
+            active = march_pass(sdfs[j], spheres, L, mt, mt_stride);
     }
 }
 
