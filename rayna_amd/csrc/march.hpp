@@ -2,12 +2,11 @@
 // its anonymous namespace, after sdf_eval).
 //
 // Work item = (path, unbounded object), SDF-major (item = j * count + q), so the items a wave takes mostly
-// belong to one SDF.  A lane runs its march as a state machine: each pass of the persistent loop either starts
-// a march step (cut and escape tests; a sphere or smooth-min estimate is evaluated at once) or performs
-// iterations of the Mandelbulb estimate (one, then more while at least MARCH_ITERATE_LANES lanes of the wave
-// still iterate), so every lane of a wave executes the same iteration body whatever its march length or
-// iteration count.  A lane whose march ends takes the next item of its wave's claimed batch in
-// the next pass.  The operations of each march are exactly those of RaymarchedIsosurfaceMesh::intersect
+// belong to one SDF.  Each pass of the persistent loop takes every active lane one march step: cut and escape
+// tests, the estimate (a Mandelbulb estimate's iterations complete within the pass; lanes whose estimate bails out
+// early wait for the wave's longest, which costs less than handing the wave back to step starts and ends between
+// iterations), and the end of the step.  A lane whose march ends takes the next item of its wave's claimed batch
+// in the next pass.  The operations of each march are exactly those of RaymarchedIsosurfaceMesh::intersect
 // (raymarched.rs:73-122) and sdf_eval, in the same order, so the outcome is the same bit for bit: the march
 // starts at total = T_MIN; each step evaluates the estimate at the ray point of `total`, advances total by its
 // magnitude and hits when that magnitude is below epsilon, giving up after max_iterations + 2 estimates.  Two
@@ -22,99 +21,45 @@
 // hit wins.
 
 constexpr int MARCH_BLOCK = 256;
-constexpr uint32_t MARCH_ITERATE_LANES = 32;
 
-// Iterations of a lane's Mandelbulb estimate from iteration `it`: the lanes whose estimate goes on iterate again at
-// once while at least MARCH_ITERATE_LANES of the wave do; the rest of the pass (step starts and ends) waits for
-// fewer of them.  True when the estimate is complete (bail-out or the iteration limit).
-template <int POW>
-__device__ __forceinline__ bool bulb_run(float bailout, int power, int iterations, f3 p0, f3 &z, float &dr, float &r,
-                                         int &it) {
-    for (;;) {
-        bool done = it >= iterations;
-        if (!done) done = !bulb_iterate<POW>(bailout, power, p0, z, dr, r) || ++it >= iterations;
-        if (done) return true;
-        if ((uint32_t)__popcll(__ballot(1)) < MARCH_ITERATE_LANES) return false;
-    }
-}
-
-// The state of a lane's march (k_march): its item, ray and cut, the march distance and step, and a Mandelbulb
-// estimate in progress (iteration it, z, dr, r about p0).
+// The state of a lane's march (k_march): its item, ray and cut, the march distance and step.
 struct MarchLane {
     uint32_t item;
     DRay ray;
     float t_cut, total;
     int step;
-    bool in_eval;
-    int it;
-    f3 z, p0;
-    float dr, r;
 };
 
-// One pass of a lane's march against SDF f: a new step (cut and escape tests; a sphere or smooth-min estimate is
-// evaluated at once, a Mandelbulb estimate started), iterations of a Mandelbulb estimate, and the end of a step.
-// False once the march has ended (its outcome written to mt).
-__device__ __forceinline__ bool march_pass(const GSdf &f, const GSphere *spheres, MarchLane &L, float *mt,
+// One step of a lane's march against SDF f: the cut and escape tests, the estimate at the ray point of `total`, and
+// the step's end.  False once the march has ended (its outcome written to mt).
+__device__ __forceinline__ bool march_step(const GSdf &f, const GSphere *spheres, MarchLane &L, float *mt,
                                            uint64_t mt_stride) {
-    bool have_dist = false;
-    float dist = 0.0f;
-    if (!L.in_eval) {  // a new step of the march
-        const f3 point = at(L.ray, L.total);
-        const f3 qe = point - f3{f.ex, f.ey, f.ez};
-        if (L.total >= L.t_cut || (dot(qe, qe) > f.escape_r * f.escape_r && dot(qe, L.ray.d) > 0.0f)) {
-            mt[L.item] = __builtin_inff();
-            return false;
-        }
-        if (f.kind == RAYNA_SDF_MANDELBULB) {
-            L.p0 = point - f3{f.cx, f.cy, f.cz};
-            L.z = L.p0;
-            L.dr = 1.0f;
-            L.r = 0.0f;
-            L.it = 0;
-            L.in_eval = true;
-        } else {
-            dist = sdf_eval(spheres, f, point);
-            have_dist = true;
-        }
+    const f3 point = at(L.ray, L.total);
+    const f3 qe = point - f3{f.ex, f.ey, f.ez};
+    if (L.total >= L.t_cut || (dot(qe, qe) > f.escape_r * f.escape_r && dot(qe, L.ray.d) > 0.0f)) {
+        mt[L.item] = __builtin_inff();
+        return false;
     }
-    if (L.in_eval) {  // iterations of the Mandelbulb estimate (sdf_eval's loop)
-        // the SDF's parameters in registers before the loop, and the usual power 8 as a constant where the whole
-        // wave has it: the iterations then load nothing
-        const int iterations = f.bulb_iterations, power = f.power;
-        const float bailout = f.bailout;
-        bool done;
-        if (__all(power == 8))
-            done = bulb_run<8>(bailout, power, iterations, L.p0, L.z, L.dr, L.r, L.it);
-        else
-            done = bulb_run<0>(bailout, power, iterations, L.p0, L.z, L.dr, L.r, L.it);
-        if (done) {
-            dist = bulb_estimate(L.r, L.dr);
-            L.in_eval = false;
-            have_dist = true;
-        }
+    const float dist = sdf_eval(spheres, f, point);
+    L.total = L.total + fabsf(dist);
+    if (fabsf(dist) < f.epsilon && T_MIN <= L.total && L.total <= FLT_MAX) {
+        mt[L.item] = signbit(dist) ? -L.total : L.total;
+        mt[mt_stride + L.item] = __int_as_float(L.step);  // Intersection::side (raymarched.rs:109)
+        return false;
     }
-    if (have_dist) {  // the end of a step
-        L.total = L.total + fabsf(dist);
-        if (fabsf(dist) < f.epsilon && T_MIN <= L.total && L.total <= FLT_MAX) {
-            mt[L.item] = signbit(dist) ? -L.total : L.total;
-            mt[mt_stride + L.item] = __int_as_float(L.step);  // Intersection::side (raymarched.rs:109)
-            return false;
-        }
-        if (L.step > f.max_iterations) {
-            mt[L.item] = __builtin_inff();
-            return false;
-        }
-        ++L.step;
+    if (L.step > f.max_iterations) {
+        mt[L.item] = __builtin_inff();
+        return false;
     }
+    ++L.step;
     return true;
 }
 
 template <bool BATCH>
-__global__ __launch_bounds__(MARCH_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_march(FrameParams fp, PathBuffers pb, const GSdf *__restrict__ sdfs,
-                                                       const GSphere *__restrict__ spheres,
-                                                       const int32_t *queue, const uint32_t *count_ptr,
-                                                       uint32_t n_sdf, float *mt, uint64_t mt_stride, uint32_t *work,
-                                                       ExtendTuning tune, BatchView bv) {
+__global__ __launch_bounds__(MARCH_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_march(
+    FrameParams fp, PathBuffers pb, const GSdf *__restrict__ sdfs, const GSphere *__restrict__ spheres,
+    const int32_t *queue, const uint32_t *count_ptr, uint32_t n_sdf, float *mt, uint64_t mt_stride, uint32_t *work,
+    ExtendTuning tune, BatchView bv) {
     const uint32_t count = queue ? *count_ptr : fp.n_slots;
     const uint32_t n_items = count * n_sdf;
     const uint32_t refill_lanes = tune.refill_lanes, wave_batch = tune.wave_batch;
@@ -171,7 +116,6 @@ __global__ __launch_bounds__(MARCH_BLOCK) __attribute__((amdgpu_waves_per_eu(8))
                     L.t_cut = pb.hit_t[slot];
                     L.total = T_MIN;
                     L.step = 0;
-                    L.in_eval = false;
                     active = true;
                 }
             }
@@ -185,9 +129,9 @@ __global__ __launch_bounds__(MARCH_BLOCK) __attribute__((amdgpu_waves_per_eu(8))
         // read-only argument (scalar loads), and the pass's branches on it are uniform.
         const int32_t j0 = __builtin_amdgcn_readfirstlane(j);
         if (__all(j == j0))
-            active = march_pass(sdfs[j0], spheres, L, mt, mt_stride);
+            active = march_step(sdfs[j0], spheres, L, mt, mt_stride);
         else
-            active = march_pass(sdfs[j], spheres, L, mt, mt_stride);
+            active = march_step(sdfs[j], spheres, L, mt, mt_stride);
     }
 }
 
